@@ -1,0 +1,206 @@
+"""Benchmark of the MI355X path-tracing hot path.
+
+Headline (`value`): closest-hit Mrays/s on the raw-traversal workload -- a
+1M-triangle random soup (centroids U[0,1]^3, vertices c + N(0,0.01)^3, the
+reference median-split BVH) traced with two 8M-ray batches: 2828x2828 coherent
+pinhole rays and 8,000,000 incoherent rays (origins U[0,1]^3, directions
+uniform on the sphere); tmin 1e-4, tmax inf.  One step = one closest-hit
+traversal of both batches, rays resident in HBM.
+
+Also reported: any-hit Mrays/s of the same batches, the traversal kernel's
+roofline (algorithmic bytes per ray priced in the reference traversal order:
+24 + 8 + 24*box_tests + 8*node_visits + 36*triangle_tests), and the oracle's
+CPU restatement of BVH::rayHit timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "closest-hit Mrays/s + path samples/s at 1080p; adjoint:forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-tris", type=int, default=1000000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=1000000, help="rays per batch in the CPU sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    import raym0nade_amd as rm
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    log(f"[bench] generating soup ({args.n_tris} tris) and rays")
+    g = rm.generate(1, 1, n_tris=args.n_tris)
+    t0 = time.time()
+    S = rm.Scene(g)
+    log(f"[bench] scene: build {S.info.build_seconds:.2f}s upload {S.info.upload_seconds:.2f}s "
+        f"nodes {S.info.n_nodes} depth {S.info.depth}")
+    cam = g.args
+    n_coh = cam.width * cam.height
+    n_inc = 8000000
+    r_coh_h = rm.rays(0, n_coh, seed=1, cam=cam)
+    r_inc_h = rm.rays(1, n_inc, seed=1)
+    r_coh = torch.from_numpy(r_coh_h).to(dev)
+    r_inc = torch.from_numpy(r_inc_h).to(dev)
+    bufs = {}
+    for name, r in (("coh", r_coh), ("inc", r_inc)):
+        n = r.shape[0]
+        bufs[name] = (torch.empty(n, dtype=torch.float32, device=dev), torch.empty(n, dtype=torch.int32, device=dev),
+                      torch.empty(n, dtype=torch.uint8, device=dev))
+
+    # algorithmic bytes per batch from the instrumented traversal (untimed)
+    algo = {}
+    for name, r in (("coh", r_coh), ("inc", r_inc)):
+        for any_hit in (False, True):
+            c = S.trace_count(r, any_hit=any_hit).cpu().numpy().astype(np.float64)
+            interior, leaves, boxes, tris = c
+            n = r.shape[0]
+            out_b = 1 if any_hit else 8
+            algo[(name, any_hit)] = n * (24 + out_b) + 24 * boxes + 8 * (interior + leaves) + 36 * tris
+    log("[bench] algorithmic bytes/ray:", {f"{k[0]}{'_any' if k[1] else ''}": round(v / (n_coh if k[0] == 'coh' else n_inc), 1)
+                                           for k, v in algo.items()})
+
+    def step(times=None):
+        for name, r in (("coh", r_coh), ("inc", r_inc)):
+            t_out, id_out, _ = bufs[name]
+            if times is not None:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            S.trace_closest(r, t_out=t_out, id_out=id_out, stream=stream)
+            if times is not None:
+                e1.record(stream)
+                times[name].append((e0, e1))
+
+    def any_step(times):
+        for name, r in (("coh", r_coh), ("inc", r_inc)):
+            occ = bufs[name][2]
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            S.trace_any(r, occ=occ, stream=stream)
+            e1.record(stream)
+            times[name].append((e0, e1))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = {"coh": [], "inc": []}
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(times)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    k_ms = {k: np.array([a.elapsed_time(b) for a, b in v]) for k, v in times.items()}
+    any_times = {"coh": [], "inc": []}
+    for _ in range(max(3, args.steps // 2)):
+        any_step(any_times)
+    torch.cuda.synchronize()
+    a_ms = {k: np.array([a.elapsed_time(b) for a, b in v]) for k, v in any_times.items()}
+
+    n_rays = n_coh + n_inc
+    ms_per_step = wall * 1e3 / args.steps
+    kern_ms = k_ms["coh"].mean() + k_ms["inc"].mean()
+    value = n_rays / (kern_ms * 1e-3) / 1e6
+    mrays = {f"closest_{k}": (n_coh if k == "coh" else n_inc) / (k_ms[k].mean() * 1e-3) / 1e6 for k in k_ms}
+    mrays.update({f"any_{k}": (n_coh if k == "coh" else n_inc) / (a_ms[k].mean() * 1e-3) / 1e6 for k in a_ms})
+    gbs = {f"closest_{k}": algo[(k, False)] / (k_ms[k].mean() * 1e-3) / 1e9 for k in k_ms}
+    gbs.update({f"any_{k}": algo[(k, True)] / (a_ms[k].mean() * 1e-3) / 1e9 for k in a_ms})
+    # dominant kernel: the closest-hit traversal kernel over both batches
+    bytes_per_launch = (algo[("coh", False)] + algo[("inc", False)]) / 2
+    dur_per_launch = (k_ms["coh"].mean() + k_ms["inc"].mean()) / 2 * 1e-3
+    achieved = bytes_per_launch / dur_per_launch / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_trace_closest.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_trace_closest", "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                "avg_launch_ms": round(dur_per_launch * 1e3, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else O.hw_threads()
+        threads = max(1, min(threads, 16))
+        log(f"[bench] CPU baseline: oracle BVH build + BVH::rayHit on {threads} threads")
+        OS = O.OracleScene(g.desc)
+        k = max(1, min(args.cpu_sample, n_coh, n_inc))
+        sc = r_coh_h[:: max(1, n_coh // k)][:k]
+        si = r_inc_h[:k]
+        OS.closest(si[:10000], threads=threads)  # warm-up
+        c0 = time.perf_counter()
+        OS.closest(sc, threads=threads)
+        OS.closest(si, threads=threads)
+        cdt = time.perf_counter() - c0
+        cpu = {"value": round((sc.shape[0] + si.shape[0]) / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{sc.shape[0]} coherent (every {max(1, n_coh // k)}th) + {si.shape[0]} incoherent rays of "
+                         f"the same batches, closest hit, {cdt:.1f} s; CPU {cpu_info()}; g++ -O2 -ffp-contract=off"}
+        log(f"[bench] CPU: {cpu['value']} Mrays/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "raw traversal: 1M-tri random soup, reference median-split BVH; 2828x2828 coherent + "
+                               "8M incoherent rays, closest hit, tmin 1e-4, tmax inf",
+                   "n_tris": args.n_tris, "rays_per_step": n_rays, "parallelism": "single-gpu"},
+        "mrays_per_s": {k: round(v, 1) for k, v in mrays.items()},
+        "algorithmic_gbs": {k: round(v, 1) for k, v in gbs.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

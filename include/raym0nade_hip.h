@@ -1,0 +1,236 @@
+/*
+ * raym0nade_hip.h -- C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * The reference renderer (lemonchu/Raym0nade) has no plugin or FFI layer; its
+ * hot path sits behind in-process C++ calls.  Each entry point below names the
+ * reference interface it replaces (file:line in the reference tree).
+ *
+ * Conventions
+ *   - Every function returns an rm_status; on failure rm_last_error() returns
+ *     a thread-local message.  Nothing aborts the process.
+ *   - Arguments named *_dev are device pointers (HBM), caller-owned.  Host
+ *     arrays are only read during the call.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Functions that take a stream are asynchronous unless stated otherwise.
+ *   - A scene is immutable after creation; concurrent trace/render calls on
+ *     distinct streams and distinct frames are safe.
+ */
+#ifndef RAYM0NADE_HIP_H
+#define RAYM0NADE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum rm_status {
+    RM_OK = 0,
+    RM_ERR_INVALID = 1,     /* bad argument */
+    RM_ERR_OOM = 2,         /* device or host allocation failed */
+    RM_ERR_HIP = 3,         /* HIP runtime error (message has the HIP string) */
+    RM_ERR_EMPTY = 4,       /* zero-triangle scene (undefined behaviour in the reference, src/bvh.cpp:51) */
+    RM_ERR_UNSUPPORTED = 5, /* feature outside this build */
+    RM_ERR_NO_DEVICE = 6    /* no HIP device visible */
+};
+
+/* Thread-local description of the last failure in this thread. */
+const char *rm_last_error(void);
+/* ABI version, bumped on any layout change of the structs below. */
+int32_t rm_abi_version(void);
+
+/* ------------------------------------------------------------------ scene */
+
+/* Material.  Replaces the reference's Material (include/material.h:29-52);
+ * fields with a reference counterpart keep its defaults, the rest are
+ * extensions used when no texture is bound:
+ *   base_color   constant diffuse albedo (reference: white without a diffuse map,
+ *                src/material.cpp:349-351)
+ *   emission     constant emitted radiance (reference: emissive texture only,
+ *                src/material.cpp:365-372)
+ *   metallic     constant metallic (reference: 0 without a specular map,
+ *                src/material.cpp:374-378)
+ *   opacity/ior/roughness/transmitting_color as Material (src/material.cpp:109-111,300-335)
+ *   albedo_tex   index of an fp32 linear RGB texture (rm_texture) or -1 */
+typedef struct rm_material {
+    float base_color[3];
+    float emission[3];
+    float transmitting_color[3];
+    float roughness;
+    float metallic;
+    float opacity;
+    float ior;
+    int32_t albedo_tex;
+} rm_material;
+
+/* fp32 linear RGB texture, row-major [height][width][3].  A mip chain of 2x2
+ * box averages is built like ImageData::generateMipmaps (src/material.cpp:113-148). */
+typedef struct rm_texture {
+    int32_t width, height;
+    const float *rgb;
+} rm_texture;
+
+/* Flat scene.  Replaces what Model's constructor assembles from assimp
+ * (src/model.cpp:88-123,172-215): faces with per-vertex uv/normal and a
+ * material, grouped into meshes (light objects are built per mesh, in mesh
+ * order, like Model::checkLightObject, src/model.cpp:44-82), and an optional
+ * HDR sky (SkyBox, src/component.cpp:54-67,120-140; with a sky, emissive
+ * materials are ignored exactly as the reference skips emissive textures,
+ * src/model.cpp:121,140). */
+typedef struct rm_scene_desc {
+    int64_t n_tris;
+    const float *verts;      /* [n][3][3] */
+    const float *uvs;        /* [n][3][2] or NULL (all zero) */
+    const float *normals;    /* [n][3][3] or NULL (geometric normal) */
+    const int32_t *material; /* [n] */
+    const int32_t *mesh;     /* [n] mesh id, meshes contiguous; NULL = one mesh */
+    int32_t n_materials;
+    const rm_material *materials;
+    int32_t n_textures;
+    const rm_texture *textures;
+    int32_t sky_width, sky_height;
+    const float *sky_rgb;    /* [h][w][3] radiance or NULL */
+} rm_scene_desc;
+
+typedef struct rm_scene rm_scene;
+
+typedef struct rm_scene_info {
+    int64_t n_tris;
+    int64_t n_nodes;        /* nodeCount(1,n)+1, src/bvh.cpp:46-50 */
+    int32_t depth;          /* levels of the implicit heap */
+    int32_t n_light_objects;
+    int32_t n_light_faces;
+    int32_t has_sky;
+    int64_t device_bytes;   /* HBM held by the scene */
+    double build_seconds;   /* host BVH build + flatten */
+    double upload_seconds;
+} rm_scene_info;
+
+/* Builds the reference BVH (variance-axis median split, <=10 triangles per
+ * leaf, implicit heap: BVH::build, src/bvh.cpp:18-54) on the host, flattens it
+ * into the HBM layout and uploads it together with shading data, light
+ * objects and the sky CDF.  Replaces Model::Model + BVH::build. */
+int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out);
+int32_t rm_scene_destroy(rm_scene *scene);
+int32_t rm_scene_get_info(const rm_scene *scene, rm_scene_info *info);
+/* perm[slot] = original triangle id after the build's in-place permutation
+ * (the reference permutes `faces`, src/bvh.cpp:38). Host array of n_tris. */
+int32_t rm_scene_get_perm(const rm_scene *scene, int32_t *perm);
+
+/* Host-only BVH build (BVH::build, src/bvh.cpp:48-54) of verts[n][3][3]:
+ * writes the permutation perm[n] (slot -> original id), the heap size
+ * nodeCount(1,n)+1 and the depth.  Needs no device. */
+int32_t rm_bvh_build(const float *verts, int64_t n, int32_t *perm, int64_t *n_nodes, int32_t *depth);
+
+/* ------------------------------------------------------------- traversal */
+
+/* Closest hit.  Replaces BVH::rayHit (src/bvh.cpp:90-92) called as in
+ * Model::rayHit without alpha textures (src/model.cpp:332-341): search
+ * (tmin, tmax), strict on both ends.  rays_dev: [n][6] = origin xyz, direction
+ * xyz.  t_out_dev: final t_max (tmax when missed).  id_out_dev: ORIGINAL
+ * triangle id or -1.  Triangle ids are bit-exact with the reference
+ * traversal; t is bit-exact as well. */
+int32_t rm_trace_closest(const rm_scene *scene, const float *rays_dev, int64_t n,
+                         float tmin, float tmax, float *t_out_dev, int32_t *id_out_dev,
+                         void *stream);
+
+/* Occlusion.  Replaces Model::rayHit_test (src/model.cpp:343-354): true iff the
+ * closest hit on (tmin, aim+1e-4) has t < aim.  aim_dev (per ray, may be NULL)
+ * overrides aim_depth.  Exits as soon as the running closest t drops below
+ * aim, which gives the same boolean.  occluded_dev: 0/1 bytes. */
+int32_t rm_trace_any(const rm_scene *scene, const float *rays_dev, int64_t n,
+                     float tmin, float aim_depth, const float *aim_dev,
+                     uint8_t *occluded_dev, void *stream);
+
+/* Instrumented closest/any traversal (same results) that also counts, in the
+ * reference traversal order, interior visits, leaf visits, box tests and
+ * triangle tests: counts_dev = uint64[4].  Used to price algorithmic bytes. */
+int32_t rm_trace_count(const rm_scene *scene, const float *rays_dev, int64_t n,
+                       float tmin, float tmax, int32_t any_hit,
+                       uint64_t *counts_dev, void *stream);
+
+/* ------------------------------------------------------------------ render */
+
+/* RenderArgs (include/render.h:8-15) plus the extensions the workloads need.
+ * seed: counter-RNG seed.  max_depth: maxRayDepth (16 in the reference,
+ * src/render.cpp:125; 1..16).  rr_min_depth: first depth at which NEE/RR may
+ * terminate a path (1 reproduces the reference, src/render.cpp:147-150).
+ * paths_per_round: indirect samples per pixel launched per round (0 = auto). */
+typedef struct rm_render_args {
+    float position[3], direction[3], up[3], right[3];
+    float accuracy, focus, coc, exposure, p_direct;
+    int32_t width, height, spp, threads;
+    uint32_t seed;
+    int32_t max_depth;
+    int32_t rr_min_depth;
+    int32_t paths_per_round;
+} rm_render_args;
+
+typedef struct rm_frame rm_frame;
+
+/* Frame buffers (device), the GPU form of Photo (include/image.h:15-67). */
+enum rm_buffer {
+    RM_BUF_POSITION = 0,       /* G-buffer position, NaN on miss      [h][w][3] */
+    RM_BUF_SHAPE_NORMAL = 1,   /*                                      [h][w][3] */
+    RM_BUF_SURFACE_NORMAL = 2, /*                                      [h][w][3] */
+    RM_BUF_EMISSION = 3,       /*                                      [h][w][3] */
+    RM_BUF_BASE_COLOR = 4,     /*                                      [h][w][3] */
+    RM_BUF_DD = 5,             /* radiance_Dd: rgb + Var               [h][w][4] */
+    RM_BUF_DS = 6,
+    RM_BUF_ID = 7,
+    RM_BUF_IS = 8,
+    RM_BUF_PIXELS = 9          /* last post-processed image           [h][w][3] */
+};
+
+int32_t rm_frame_create(int32_t width, int32_t height, rm_frame **out);
+int32_t rm_frame_destroy(rm_frame *frame);
+/* Device pointer of one buffer (stable for the frame's lifetime). */
+int32_t rm_frame_buffer(rm_frame *frame, int32_t which, float **dev_ptr);
+/* Synchronous copy of one buffer to host memory. */
+int32_t rm_frame_download(const rm_frame *frame, int32_t which, float *host);
+
+/* Renders one frame.  Replaces render_multiThread's pixel loop
+ * (src/render.cpp:593-633 -> renderPixel, src/render.cpp:448-551). */
+int32_t rm_render(const rm_scene *scene, const rm_render_args *args, rm_frame *frame,
+                  void *stream);
+
+/* Photo::postProcessing (src/image.cpp:470-479) without bloom / depth of field:
+ * shade(options) -> gammaCorrection -> FXAA if options & 512.  Options are
+ * bit-compatible with Photo::ShadeOption (include/image.h:17-34).  Writes
+ * RM_BUF_PIXELS. */
+int32_t rm_postprocess(rm_frame *frame, int32_t shade_options, float exposure, void *stream);
+
+/* ------------------------------------------------------ synthetic inputs */
+
+/* Seeded procedural workloads (no reference counterpart: the reference only
+ * loads assimp models).  config 0 = Cornell box, 1 = triangle soup,
+ * 2 = displaced geodesic spheres under an HDR sky, 4 = textured Cornell box. */
+typedef struct rm_gen_options {
+    int64_t n_tris;     /* config 1 triangle count (0 -> 1,000,000) */
+    int32_t grid;       /* config 2 spheres per side (0 -> 8) */
+    int32_t freq;       /* config 2 geodesic frequency (0 -> 90: 162000 triangles per sphere) */
+    int32_t sky_width;  /* config 2 sky (0 -> 2048x1024, -1 -> no sky) */
+    int32_t sky_height;
+    int32_t tex_size;   /* config 4 texture side (0 -> 256) */
+} rm_gen_options;
+
+typedef struct rm_generated rm_generated;
+int32_t rm_generate_scene(int32_t config, uint32_t seed, const rm_gen_options *opt, rm_generated **out);
+/* Borrowed views of the generated arrays (valid until rm_generated_destroy)
+ * and the workload's RenderArgs. */
+int32_t rm_generated_desc(const rm_generated *gen, rm_scene_desc *desc, rm_render_args *args);
+int32_t rm_generated_destroy(rm_generated *gen);
+/* Ray batch into host memory rays[n][6]: kind 0 = pinhole primary rays of
+ * `cam` (n == width*height, formed as renderPixel does, src/render.cpp:466-476);
+ * kind 1 = origins U[0,1]^3, directions uniform on the sphere. */
+int32_t rm_generate_rays(int32_t kind, uint32_t seed, int64_t n, const rm_render_args *cam, float *rays);
+
+/* ------------------------------------------------------------ utilities */
+
+/* Philox-4x32-10 block (the counter RNG shared by host and device code). */
+void rm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RAYM0NADE_HIP_H */

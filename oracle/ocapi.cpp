@@ -1,0 +1,408 @@
+// ORACLE -- test infrastructure only (see oracle.h).
+//
+// C entry points that let the tests and the CPU-baseline leg drive the
+// restatement: scene assembly (what Model::Model does after assimp,
+// src/model.cpp:88-123,172-215, incl. light objects src/model.cpp:23-82 and
+// the sky, src/component.cpp:54-67), threaded ray batches through
+// BVH::rayHit / Model::rayHit_test, single-primitive probes, and rendering.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <thread>
+#include "../include/raym0nade_hip.h"
+#include "oracle.h"
+
+using namespace ref;
+
+struct oracle_scene {
+    Model model;
+    std::vector<int32_t> perm;
+};
+
+namespace {
+
+// Model::checkLightObject (src/model.cpp:44-82) for one mesh
+void checkLightObject(Model &M, const std::vector<Face> &meshFaces, const Material &material) {
+    LightObject lo;
+    vec3 color(0.0f);
+    std::vector<float> faceWeights;
+    for (const Face &face : meshFaces) {
+        // getAverageEmissiveColor (src/model.cpp:23-42) on a constant emission
+        vec3 averageColor(0.0f);
+        for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j) averageColor += material.getEmissiveColor(0, 0, NAN);
+        vec3 textureColor = averageColor / static_cast<float>(64);
+        float Clum = dot(textureColor, RGB_Weight),
+              area = length(cross(face.v[1] - face.v[0], face.v[2] - face.v[0])) / 2.0f;
+        if (Clum < eps_zero || area < eps_zero) continue;
+        color += textureColor * area;
+        float power = area * Clum;
+        lo.faces.emplace_back(face);
+        faceWeights.emplace_back(power);
+    }
+    if (lo.faces.empty()) return;
+    lo.color = color / dot(color, RGB_Weight);
+    for (size_t j = 0; j < lo.faces.size(); j++) {
+        lo.power += faceWeights[j];
+        lo.center += lo.faces[j].center() * faceWeights[j];
+    }
+    lo.faceDist.Init(faceWeights);
+    lo.center /= lo.power;
+    M.lightObjects.push_back(std::move(lo));
+}
+
+template <class F>
+void parallel_for(int64_t n, int threads, F f) {
+    if (threads <= 1 || n < 1024) {
+        for (int64_t i = 0; i < n; ++i) f(i, 0);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    const int64_t chunk = 1024;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] {
+            while (true) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                int64_t e = std::min(n, b + chunk);
+                for (int64_t i = b; i < e; ++i) f(i, t);
+            }
+        });
+    for (auto &th : pool) th.join();
+}
+
+Ray load_ray(const float *rays, int64_t i) {
+    const float *p = rays + i * 6;
+    return Ray{vec3(p[0], p[1], p[2]), vec3(p[3], p[4], p[5])};
+}
+
+}  // namespace
+
+extern "C" {
+
+// Returns NULL on an invalid or empty scene.
+oracle_scene *oracle_scene_create(const rm_scene_desc *d) {
+    if (!d || d->n_tris <= 0 || !d->verts || !d->material || d->n_materials <= 0) return nullptr;
+    oracle_scene *S = new oracle_scene();
+    Model &M = S->model;
+    const int64_t n = d->n_tris;
+    const bool has_sky = d->sky_rgb && d->sky_width > 0 && d->sky_height > 0;
+    M.materials.resize(d->n_materials);
+    for (int m = 0; m < d->n_materials; ++m) {
+        const rm_material &R = d->materials[m];
+        Material &X = M.materials[m];
+        X.id = m;
+        X.baseColor = vec3(R.base_color[0], R.base_color[1], R.base_color[2]);
+        X.emission = has_sky ? vec3(0.0f) : vec3(R.emission[0], R.emission[1], R.emission[2]);
+        X.transmittingColor = vec3(R.transmitting_color[0], R.transmitting_color[1], R.transmitting_color[2]);
+        X.roughness = R.roughness;
+        X.metallic = R.metallic;
+        X.opacity = R.opacity;
+        X.ior = R.ior;
+        if (R.albedo_tex >= 0 && R.albedo_tex < d->n_textures) {
+            const rm_texture &T = d->textures[R.albedo_tex];
+            X.diffuse.width = T.width;
+            X.diffuse.height = T.height;
+            X.diffuse.data[0].assign(T.rgb, T.rgb + (size_t)T.width * T.height * 3);
+            X.diffuse.generateMipmaps();
+        }
+    }
+    M.vertexDatas.reserve(n * 3);
+    M.faces.reserve(n);
+    for (int64_t i = 0; i < n; ++i) {
+        Face f;
+        const float *v = d->verts + i * 9;
+        for (int k = 0; k < 3; ++k) f.v[k] = vec3(v[k * 3], v[k * 3 + 1], v[k * 3 + 2]);
+        vec3 g = normalize(cross(f.v[1] - f.v[0], f.v[2] - f.v[0]));
+        for (int k = 0; k < 3; ++k) {
+            VertexData vd;
+            vd.uv = d->uvs ? vec2(d->uvs[i * 6 + k * 2], d->uvs[i * 6 + k * 2 + 1]) : vec2(0.0f);
+            vd.normal = d->normals ? vec3(d->normals[i * 9 + k * 3], d->normals[i * 9 + k * 3 + 1],
+                                          d->normals[i * 9 + k * 3 + 2])
+                                   : g;
+            M.vertexDatas.push_back(vd);
+            f.data[k] = &M.vertexDatas.back();
+        }
+        f.material = &M.materials[d->material[i]];
+        f.id = (int32_t)i;
+        M.faces.push_back(f);
+    }
+    // light objects per mesh, before the BVH permutes `faces` (src/model.cpp:121,214)
+    if (!has_sky) {
+        int64_t i = 0;
+        while (i < n) {
+            int64_t j = i;
+            int32_t mesh = d->mesh ? d->mesh[i] : 0;
+            while (j < n && (d->mesh ? d->mesh[j] : 0) == mesh) ++j;
+            const Material &mat = M.materials[d->material[i]];
+            if (mat.hasEmission()) {
+                std::vector<Face> mf(M.faces.begin() + i, M.faces.begin() + j);
+                checkLightObject(M, mf, mat);
+            }
+            i = j;
+        }
+    }
+    if (has_sky) {
+        SkyBox &K = M.skyMap;
+        K.width = d->sky_width;
+        K.height = d->sky_height;
+        K.data.resize((size_t)K.width * K.height);
+        for (size_t p = 0; p < K.data.size(); ++p)
+            K.data[p] = vec3(d->sky_rgb[p * 3], d->sky_rgb[p * 3 + 1], d->sky_rgb[p * 3 + 2]);
+        K.Init();
+    }
+    M.bvh.build(M.faces);
+    S->perm.resize(n);
+    for (int64_t s = 0; s < n; ++s) S->perm[s] = M.faces[s].id;
+    return S;
+}
+
+void oracle_scene_destroy(oracle_scene *S) { delete S; }
+
+int64_t oracle_scene_node_count(const oracle_scene *S) { return S->model.bvh.size; }
+int32_t oracle_scene_light_count(const oracle_scene *S) { return (int32_t)S->model.lightObjects.size(); }
+void oracle_scene_get_perm(const oracle_scene *S, int32_t *perm) {
+    memcpy(perm, S->perm.data(), S->perm.size() * sizeof(int32_t));
+}
+// Node u: box (6 floats) and face range; used by the BVH invariant tests.
+void oracle_scene_get_nodes(const oracle_scene *S, float *boxes, int32_t *ranges) {
+    const BVH &B = S->model.bvh;
+    for (int u = 0; u < B.size; ++u) {
+        const BVH_Node &N = B.node[u];
+        float b[6] = {N.box.v0.x, N.box.v0.y, N.box.v0.z, N.box.v1.x, N.box.v1.y, N.box.v1.z};
+        memcpy(boxes + (size_t)u * 6, b, sizeof(b));
+        ranges[u * 2] = N.faceL;
+        ranges[u * 2 + 1] = N.faceR;
+    }
+}
+// Light object k: center, color, power, face count, first face CDF entries.
+void oracle_scene_get_light(const oracle_scene *S, int32_t k, float *out7, float *cdf) {
+    const LightObject &L = S->model.lightObjects[k];
+    float v[7] = {L.center.x, L.center.y, L.center.z, L.color.x, L.color.y, L.color.z, L.power};
+    memcpy(out7, v, sizeof(v));
+    if (cdf) memcpy(cdf, L.faceDist.prefixSums.data(), L.faceDist.prefixSums.size() * sizeof(float));
+}
+
+// Closest hit for a batch (BVH::rayHit on HitRecord(tmin, tmax)).  counts:
+// uint64[4] = interior visits, leaf visits, box tests, triangle tests.
+void oracle_trace_closest(const oracle_scene *S, const float *rays, int64_t n, float tmin, float tmax, float *t_out,
+                          int32_t *id_out, int32_t threads, uint64_t *counts) {
+    std::vector<Counters> per(std::max(1, threads));
+    parallel_for(n, threads, [&](int64_t i, int t) {
+        HitRecord hit(tmin, tmax);
+        S->model.bvh.rayHit(load_ray(rays, i), hit, counts ? &per[t] : nullptr);
+        t_out[i] = hit.t_max;
+        id_out[i] = hit.face ? hit.face->id : -1;
+    });
+    if (counts) {
+        memset(counts, 0, 4 * sizeof(uint64_t));
+        for (auto &c : per) { counts[0] += c.interior; counts[1] += c.leaves; counts[2] += c.boxes; counts[3] += c.tris; }
+    }
+}
+
+// Model::rayHit_test for a batch with t_min = tmin (the reference uses 1e-4).
+void oracle_trace_any(const oracle_scene *S, const float *rays, int64_t n, float tmin, float aim_depth,
+                      const float *aim, uint8_t *occ, int32_t threads, uint64_t *counts) {
+    std::vector<Counters> per(std::max(1, threads));
+    parallel_for(n, threads, [&](int64_t i, int t) {
+        float a = aim ? aim[i] : aim_depth;
+        HitRecord hit(tmin, a + eps_zero);
+        S->model.bvh.rayHit(load_ray(rays, i), hit, counts ? &per[t] : nullptr);
+        occ[i] = hit.t_max >= a ? 0 : 1;
+    });
+    if (counts) {
+        memset(counts, 0, 4 * sizeof(uint64_t));
+        for (auto &c : per) { counts[0] += c.interior; counts[1] += c.leaves; counts[2] += c.boxes; counts[3] += c.tris; }
+    }
+}
+
+// Brute force over all faces in original order with the same triangle test.
+void oracle_brute_closest(const oracle_scene *S, const float *rays, int64_t n, float tmin, float tmax, float *t_out,
+                          int32_t *id_out, int32_t threads) {
+    const Model &M = S->model;
+    std::vector<const Face *> byId(M.faces.size());
+    for (const Face &f : M.faces) byId[f.id] = &f;
+    parallel_for(n, threads, [&](int64_t i, int) {
+        Ray r = load_ray(rays, i);
+        float best = tmax;
+        int32_t id = -1;
+        for (const Face *f : byId) {
+            float t = RayTriangleIntersection(r, f->v[0], f->v[1], f->v[2]);
+            if (tmin < t && t < best) { best = t; id = f->id; }
+        }
+        t_out[i] = best;
+        id_out[i] = id;
+    });
+}
+
+// Single-primitive probes.
+void oracle_ray_in_box(const float *ray6, const float *box6, float *tL, float *tR) {
+    Ray r{vec3(ray6[0], ray6[1], ray6[2]), vec3(ray6[3], ray6[4], ray6[5])};
+    Box b(vec3(box6[0], box6[1], box6[2]), vec3(box6[3], box6[4], box6[5]));
+    rayInBox(r, b, *tL, *tR);
+}
+float oracle_ray_triangle(const float *ray6, const float *tri9) {
+    Ray r{vec3(ray6[0], ray6[1], ray6[2]), vec3(ray6[3], ray6[4], ray6[5])};
+    return RayTriangleIntersection(r, vec3(tri9[0], tri9[1], tri9[2]), vec3(tri9[3], tri9[4], tri9[5]),
+                                   vec3(tri9[6], tri9[7], tri9[8]));
+}
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox4x32_10(ctr, key, out); }
+float oracle_gen(uint32_t seed, uint32_t tag, uint32_t pix, uint32_t smp, int32_t skip) {
+    Generator g(seed, tag, pix, smp);
+    for (int i = 0; i < skip; ++i) g();
+    return g();
+}
+
+int32_t oracle_hw_threads(void) { return (int32_t)std::max(1u, std::thread::hardware_concurrency()); }
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- render
+namespace {
+RenderArgs to_args(const rm_render_args *a) {
+    RenderArgs r;
+    r.position = vec3(a->position[0], a->position[1], a->position[2]);
+    r.direction = vec3(a->direction[0], a->direction[1], a->direction[2]);
+    r.up = vec3(a->up[0], a->up[1], a->up[2]);
+    r.right = vec3(a->right[0], a->right[1], a->right[2]);
+    r.accuracy = a->accuracy;
+    r.focus = a->focus;
+    r.CoC = a->coc;
+    r.exposure = a->exposure;
+    r.P_Direct = a->p_direct;
+    r.width = a->width;
+    r.height = a->height;
+    r.spp = a->spp;
+    r.threads = a->threads;
+    r.seed = a->seed;
+    r.max_depth = a->max_depth > 0 ? a->max_depth : 16;
+    r.rr_min_depth = a->rr_min_depth > 0 ? a->rr_min_depth : 1;
+    return r;
+}
+void put3(float *p, const vec3 &v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+vec3 get3(const float *p) { return vec3(p[0], p[1], p[2]); }
+}  // namespace
+
+extern "C" {
+
+// Renders the rows listed in `rows` (all rows if NULL) with `threads` workers.
+// gbuf: [h][w][15] = position, shapeNormal, surfaceNormal, emission, baseColor.
+// rad:  [4][h][w][4] = Dd, Ds, Id, Is as (rgb, Var).  Rows not rendered keep
+// the defaults of a fresh Photo.
+int32_t oracle_render(const oracle_scene *S, const rm_render_args *a, int32_t threads, const int32_t *rows,
+                      int32_t n_rows, float *gbuf, float *rad) {
+    if (!S || !a || a->width <= 0 || a->height <= 0 || a->max_depth > 16) return 1;
+    RenderArgs args = to_args(a);
+    Photo photo(args.width, args.height);
+    photo.exposure = args.exposure;
+    std::vector<int> rs;
+    if (rows)
+        for (int i = 0; i < n_rows; ++i) rs.push_back(rows[i]);
+    render_multiThread(S->model, args, photo, threads, rows ? &rs : nullptr);
+    const size_t np = (size_t)args.width * args.height;
+    if (gbuf)
+        for (size_t i = 0; i < np; ++i) {
+            const HitInfo &G = photo.Gbuffer[i];
+            put3(gbuf + i * 15 + 0, G.position);
+            put3(gbuf + i * 15 + 3, G.shapeNormal);
+            put3(gbuf + i * 15 + 6, G.surfaceNormal);
+            put3(gbuf + i * 15 + 9, G.emission);
+            put3(gbuf + i * 15 + 12, G.baseColor);
+        }
+    if (rad) {
+        const std::vector<RadianceData> *bufs[4] = {&photo.radiance_Dd, &photo.radiance_Ds, &photo.radiance_Id,
+                                                    &photo.radiance_Is};
+        for (int b = 0; b < 4; ++b)
+            for (size_t i = 0; i < np; ++i) {
+                const RadianceData &R = (*bufs[b])[i];
+                float *p = rad + ((size_t)b * np + i) * 4;
+                put3(p, R.radiance);
+                p[3] = R.Var;
+            }
+    }
+    return 0;
+}
+
+// Photo::postProcessing on given buffers (same layouts as oracle_render).
+void oracle_postprocess(int32_t w, int32_t h, const float *gbuf, const float *rad, int32_t options, float exposure,
+                        float *pixels) {
+    Photo photo(w, h);
+    photo.exposure = exposure;
+    const size_t np = (size_t)w * h;
+    for (size_t i = 0; i < np; ++i) {
+        HitInfo &G = photo.Gbuffer[i];
+        G.position = get3(gbuf + i * 15);
+        G.shapeNormal = get3(gbuf + i * 15 + 3);
+        G.surfaceNormal = get3(gbuf + i * 15 + 6);
+        G.emission = get3(gbuf + i * 15 + 9);
+        G.baseColor = get3(gbuf + i * 15 + 12);
+    }
+    std::vector<RadianceData> *bufs[4] = {&photo.radiance_Dd, &photo.radiance_Ds, &photo.radiance_Id, &photo.radiance_Is};
+    for (int b = 0; b < 4; ++b)
+        for (size_t i = 0; i < np; ++i) {
+            const float *p = rad + ((size_t)b * np + i) * 4;
+            (*bufs[b])[i].radiance = get3(p);
+            (*bufs[b])[i].Var = p[3];
+        }
+    photo.postProcessing(options);
+    for (size_t i = 0; i < np; ++i) put3(pixels + i * 3, photo.pixelarray[i]);
+}
+
+// accumulateInwardRadiance on one sample, for its invariant tests.
+void oracle_accumulate(const float *base3, const float *bsdf3, const float *light3, float weight, float *d4, float *s4) {
+    RadianceData d, s;
+    d.radiance = get3(d4); d.Var = d4[3];
+    s.radiance = get3(s4); s.Var = s4[3];
+    accumulateInwardRadiance(get3(base3), LightSample(get3(bsdf3), get3(light3), weight), d, s);
+    put3(d4, d.radiance); d4[3] = d.Var;
+    put3(s4, s.radiance); s4[3] = s.Var;
+}
+
+// BSDF::getBRDF / getBTDF at a hand-chosen configuration: params =
+// {N(3), shapeN(3), V(3), base(3), roughness, metallic, opacity, eta, entering}.
+void oracle_bsdf_eval(const float *params, const float *L3, int32_t which, float *out3) {
+    HitInfo h;
+    h.surfaceNormal = get3(params);
+    h.shapeNormal = get3(params + 3);
+    h.baseColor = get3(params + 9);
+    h.roughness = params[12];
+    h.metallic = params[13];
+    h.opacity = params[14];
+    h.eta = params[15];
+    h.entering = params[16] != 0.0f;
+    BSDF b(get3(params + 6), h);
+    vec3 r = which == 0 ? b.getBRDF(get3(L3)) : (which == 1 ? b.getBTDF(get3(L3)) : b.getBSDF(get3(L3)));
+    put3(out3, r);
+}
+
+// One BSDF sample from stream (seed, tag, pix, smp): which 0 = sampleCos,
+// 1 = sampleBRDF (GGX), 2 = sampleReflection, 3 = sampleBTDF.
+// out: dir(3), bsdfPdf(3), pdf, fails.
+void oracle_bsdf_sample(const float *params, int32_t which, uint32_t seed, uint32_t pix, uint32_t smp, float *out8) {
+    HitInfo h;
+    h.surfaceNormal = get3(params);
+    h.shapeNormal = get3(params + 3);
+    h.baseColor = get3(params + 9);
+    h.roughness = params[12];
+    h.metallic = params[13];
+    h.opacity = params[14];
+    h.eta = params[15];
+    h.entering = params[16] != 0.0f;
+    BSDF b(get3(params + 6), h);
+    Generator g(seed, TAG_INDIRECT, pix, smp);
+    vec3 dir, pdf3;
+    float pdf = 0.0f;
+    int fails = 0;
+    if (which == 0) b.sampleCos(g, dir, pdf3, pdf, fails);
+    else if (which == 1) b.sampleBRDF(g, dir, pdf3, pdf, fails);
+    else if (which == 2) b.sampleReflection(g, dir, pdf3, fails);
+    else b.sampleBTDF(g, dir, pdf3, fails);
+    put3(out8, dir);
+    put3(out8 + 3, pdf3);
+    out8[6] = pdf;
+    out8[7] = (float)fails;
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// ORACLE -- test infrastructure only (see oracle.h).
+// Restates the material sampling of src/material.cpp:26-148,337-383 and the
+// shading-point code of src/model.cpp:23-82,232-328.
+#include <algorithm>
+#include <cmath>
+#include "oracle.h"
+
+namespace ref {
+
+// src/material.cpp:113-148 on fp32 texels (EXT: no integer truncation).
+void ImageData::generateMipmaps() {
+    map_depth = 8;
+    for (int level = 1; level < 8; ++level) {
+        int prevWidth = width >> (level - 1), prevHeight = height >> (level - 1);
+        int currWidth = width >> level, currHeight = height >> level;
+        if (currWidth == 0 || currHeight == 0) {
+            map_depth = level;
+            break;
+        }
+        data[level].resize((size_t)currWidth * currHeight * 3);
+        for (int y = 0; y < currHeight; ++y)
+            for (int x = 0; x < currWidth; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    float sum = 0.0f;
+                    for (int dy = 0; dy < 2; ++dy)
+                        for (int dx = 0; dx < 2; ++dx) {
+                            int px = (x * 2 + dx) % prevWidth;
+                            int py = (y * 2 + dy) % prevHeight;
+                            sum += data[level - 1][((size_t)py * prevWidth + px) * 3 + c];
+                        }
+                    data[level][((size_t)y * currWidth + x) * 3 + c] = sum / 4.0f;
+                }
+    }
+}
+
+// src/material.cpp:48-79 (wrap addressing, bilinear)
+static vec3 get_bilinear(const std::vector<float> &data, int width, int height, float u, float v) {
+    auto mod = [](int &x, int m) {
+        if (x < 0 || x >= m) {
+            x %= m;
+            if (x < 0) x += m;
+        }
+    };
+    float x = u * float(width), y = v * float(height);
+    int x0 = int(std::floor(x)), y0 = int(std::floor(y));
+    float dx = x - float(x0), dy = y - float(y0);
+    mod(x0, width);
+    mod(y0, height);
+    int x1 = (x0 + 1) % width, y1 = (y0 + 1) % height;
+    auto at = [&](int xx, int yy) {
+        size_t i = ((size_t)yy * width + xx) * 3;
+        return vec3(data[i], data[i + 1], data[i + 2]);
+    };
+    vec3 c00 = at(x0, y0), c01 = at(x1, y0), c10 = at(x0, y1), c11 = at(x1, y1);
+    vec3 c0 = c00 * (1.0f - dx) + c01 * dx;
+    vec3 c1 = c10 * (1.0f - dx) + c11 * dx;
+    return c0 * (1.0f - dy) + c1 * dy;
+}
+
+// src/material.cpp:81-94
+vec3 ImageData::get(float u, float v, float depth) const {
+    v = 1.0f - v;
+    depth = std::max(std::min(depth, static_cast<float>(map_depth - 1)), 0.0f);
+    int level = static_cast<int>(depth);
+    int nextLevel = std::min(level + 1, map_depth - 1);
+    float levelBlend = depth - float(level);
+    vec3 ret1 = get_bilinear(data[level], width >> level, height >> level, u, v);
+    vec3 ret2 = get_bilinear(data[nextLevel], width >> nextLevel, height >> nextLevel, u, v);
+    return ret1 * (1.0f - levelBlend) + ret2 * levelBlend;
+}
+
+// src/material.cpp:349-356.  EXT: constant albedo without a texture; fp32
+// textures are linear, so no gamma.
+vec4 Material::getDiffuseColor(float u, float v, float duv) const {
+    if (diffuse.empty()) return vec4(baseColor.x, baseColor.y, baseColor.z, 1.0f);
+    float map_depth = std::isnan(duv) ? 0.0f : std::log2(duv * float(diffuse.width));
+    vec3 c = diffuse.get(u, v, map_depth);
+    return vec4(c.x, c.y, c.z, 1.0f);
+}
+
+// src/material.cpp:358-363 (no normal maps in this build's scenes)
+vec3 Material::getNormal(float, float, float) const { return vec3(0.0f); }
+
+// src/material.cpp:365-372.  EXT: constant emission.
+vec3 Material::getEmissiveColor(float, float, float) const { return emission; }
+
+// src/material.cpp:374-383.  EXT: constant metallic without a specular map.
+void Material::getSurfaceData(float, float, float &r, float &m) const {
+    m = metallic;
+    r = roughness;
+}
+
+// src/model.cpp:232-238
+static bool reverseFix(vec3 &v, const vec3 &Dir) {
+    if (dot(v, Dir) < 0.0f) {
+        v *= -1.0f;
+        return false;
+    }
+    return true;
+}
+
+// src/model.cpp:240-270
+void getHitNormals(const Face &face, const vec3 &inDir, const vec3 &baryCoords, vec3 &shapeNormal,
+                   vec3 &surfaceNormal_raw, bool &entering) {
+    vec3 crossV0 = cross(face.v[1] - face.v[0], face.v[2] - face.v[0]);
+    shapeNormal = normalize(crossV0);
+    entering = reverseFix(shapeNormal, -inDir);
+    surfaceNormal_raw = shapeNormal;
+    float area = length(crossV0) / 2.0f;
+    const float areaThreshold = 1e-2f;
+    if (area > areaThreshold) return;
+    vec3 normalV0 = face.data[0]->normal, normalV1 = face.data[1]->normal, normalV2 = face.data[2]->normal;
+    reverseFix(normalV0, shapeNormal);
+    reverseFix(normalV1, shapeNormal);
+    reverseFix(normalV2, shapeNormal);
+    const float dotThreshold = 0.85f;
+    surfaceNormal_raw = normalize(baryCoords[0] * (dot(normalV0, shapeNormal) > dotThreshold ? normalV0 : shapeNormal) +
+                                  baryCoords[1] * (dot(normalV1, shapeNormal) > dotThreshold ? normalV1 : shapeNormal) +
+                                  baryCoords[2] * (dot(normalV2, shapeNormal) > dotThreshold ? normalV2 : shapeNormal));
+    if (!isfinite(surfaceNormal_raw)) surfaceNormal_raw = shapeNormal;
+}
+
+// src/model.cpp:272-277
+static vec2 getDuv(const Face &face, const vec3 &hit_dPdx) {
+    vec3 b = barycentric(face.v[0], face.v[1], face.v[2], face.v[0] + hit_dPdx);
+    return (b[0] - 1.0f) * face.data[0]->uv + b[1] * face.data[1]->uv + b[2] * face.data[2]->uv;
+}
+
+// src/model.cpp:279-298
+static void calcSurfaceNormal(const Face &face, const vec3 &normalMap, const vec3 &shapeNormal, vec3 &surfaceNormal) {
+    vec3 edge1 = face.v[1] - face.v[0], edge2 = face.v[2] - face.v[0];
+    vec2 deltaUV1 = face.data[1]->uv - face.data[0]->uv, deltaUV2 = face.data[2]->uv - face.data[0]->uv;
+    float f = 1.0f / (deltaUV1.x * deltaUV2.y - deltaUV2.x * deltaUV1.y);
+    vec3 tangentBasisU = f * (deltaUV2.y * edge1 - deltaUV1.y * edge2),
+         tangentBasisV = f * (-deltaUV2.x * edge1 + deltaUV1.x * edge2);
+    vec3 tangent = normalize(tangentBasisU - shapeNormal * dot(shapeNormal, tangentBasisU)),
+         bitangent = normalize(tangentBasisV - shapeNormal * dot(shapeNormal, tangentBasisV) -
+                               tangent * dot(tangent, tangentBasisV));
+    vec3 sav = surfaceNormal;
+    surfaceNormal = normalize(tangent * normalMap.x + bitangent * normalMap.y + surfaceNormal);
+    if (!isfinite(surfaceNormal)) surfaceNormal = sav;
+}
+
+// src/model.cpp:300-328
+void getHitMaterial(const Face &face, const vec3 &baryCoords, const vec3 &hit_dPdx, const vec3 &hit_dPdy,
+                    HitInfo &hitInfo) {
+    vec2 texUV = baryCoords[0] * face.data[0]->uv + baryCoords[1] * face.data[1]->uv + baryCoords[2] * face.data[2]->uv;
+    const Material &material = *face.material;
+    hitInfo.id = material.id;
+    material.getSurfaceData(texUV[0], texUV[1], hitInfo.roughness, hitInfo.metallic);
+    hitInfo.opacity = material.opacity;
+    hitInfo.eta = material.ior;
+    if (hitInfo.opacity > 1.0f - eps_zero) hitInfo.entering = true;
+    vec2 dUVdx = getDuv(face, hit_dPdx), dUVdy = getDuv(face, hit_dPdy);
+    float duv = (isfinite(dUVdx) && isfinite(dUVdx)) ? (length(dUVdx) + length(dUVdy)) / 2.0f : NAN;
+    if (hitInfo.opacity < eps_zero)
+        hitInfo.baseColor = material.transmittingColor;
+    else {
+        vec4 c = material.getDiffuseColor(texUV[0], texUV[1], duv);
+        hitInfo.baseColor = vec3(c.x, c.y, c.z);
+    }
+    hitInfo.emission = material.getEmissiveColor(texUV[0], texUV[1], duv);
+    vec3 normalMap = material.getNormal(texUV[0], texUV[1], duv);
+    calcSurfaceNormal(face, normalMap, hitInfo.shapeNormal, hitInfo.surfaceNormal);
+}
+
+}  // namespace ref

@@ -1,0 +1,188 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes access to liboracle.so, the CPU restatement of the reference renderer.
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import this
+module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+u8p = C.POINTER(C.c_uint8)
+u64p = C.POINTER(C.c_uint64)
+
+_lib = None
+
+
+def build() -> str:
+    r = subprocess.run(["make", "-C", HERE, "-B", "liboracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        sig = {
+            "oracle_scene_create": (vp, [vp]),
+            "oracle_scene_destroy": (None, [vp]),
+            "oracle_scene_node_count": (C.c_int64, [vp]),
+            "oracle_scene_light_count": (C.c_int32, [vp]),
+            "oracle_scene_get_perm": (None, [vp, i32p]),
+            "oracle_scene_get_nodes": (None, [vp, f32p, i32p]),
+            "oracle_scene_get_light": (None, [vp, C.c_int32, f32p, f32p]),
+            "oracle_trace_closest": (None, [vp, f32p, C.c_int64, C.c_float, C.c_float, f32p, i32p, C.c_int32, u64p]),
+            "oracle_trace_any": (None, [vp, f32p, C.c_int64, C.c_float, C.c_float, f32p, u8p, C.c_int32, u64p]),
+            "oracle_brute_closest": (None, [vp, f32p, C.c_int64, C.c_float, C.c_float, f32p, i32p, C.c_int32]),
+            "oracle_ray_in_box": (None, [f32p, f32p, f32p, f32p]),
+            "oracle_ray_triangle": (C.c_float, [f32p, f32p]),
+            "oracle_philox": (None, [C.POINTER(C.c_uint32)] * 3),
+            "oracle_gen": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32]),
+            "oracle_hw_threads": (C.c_int32, []),
+            "oracle_render": (C.c_int32, [vp, vp, C.c_int32, i32p, C.c_int32, f32p, f32p]),
+            "oracle_postprocess": (None, [C.c_int32, C.c_int32, f32p, f32p, C.c_int32, C.c_float, f32p]),
+            "oracle_accumulate": (None, [f32p, f32p, f32p, C.c_float, f32p, f32p]),
+            "oracle_bsdf_eval": (None, [f32p, f32p, C.c_int32, f32p]),
+            "oracle_bsdf_sample": (None, [f32p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, f32p]),
+        }
+        for k, (res, args) in sig.items():
+            fn = getattr(L, k)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def fp(a):
+    return a.ctypes.data_as(f32p)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class OracleScene:
+    def __init__(self, desc):
+        """desc: an RmSceneDesc (ctypes) from raym0nade_amd._lib."""
+        self._keep = desc
+        self.h = lib().oracle_scene_create(C.byref(desc))
+        if not self.h:
+            raise ValueError("oracle rejected the scene")
+        self.n = desc.n_tris
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_scene_destroy(self.h)
+            self.h = None
+
+    def node_count(self):
+        return lib().oracle_scene_node_count(self.h)
+
+    def perm(self):
+        p = np.empty(self.n, dtype=np.int32)
+        lib().oracle_scene_get_perm(self.h, p.ctypes.data_as(i32p))
+        return p
+
+    def nodes(self):
+        m = self.node_count()
+        boxes = np.empty((m, 6), dtype=np.float32)
+        ranges = np.empty((m, 2), dtype=np.int32)
+        lib().oracle_scene_get_nodes(self.h, fp(boxes), ranges.ctypes.data_as(i32p))
+        return boxes, ranges
+
+    def lights(self):
+        out = []
+        for k in range(lib().oracle_scene_light_count(self.h)):
+            v = np.empty(7, dtype=np.float32)
+            lib().oracle_scene_get_light(self.h, k, fp(v), None)
+            out.append(v)
+        return out
+
+    def closest(self, rays, tmin=1e-4, tmax=np.inf, threads=0, counts=False):
+        rays = f32(rays)
+        n = rays.shape[0]
+        t = np.empty(n, dtype=np.float32)
+        ids = np.empty(n, dtype=np.int32)
+        c = np.zeros(4, dtype=np.uint64)
+        lib().oracle_trace_closest(self.h, fp(rays), n, tmin, tmax, fp(t), ids.ctypes.data_as(i32p),
+                                   threads or lib().oracle_hw_threads(), c.ctypes.data_as(u64p) if counts else None)
+        return (t, ids, c) if counts else (t, ids)
+
+    def any(self, rays, tmin=1e-4, aim=np.inf, aims=None, threads=0, counts=False):
+        rays = f32(rays)
+        n = rays.shape[0]
+        occ = np.empty(n, dtype=np.uint8)
+        c = np.zeros(4, dtype=np.uint64)
+        a = f32(aims) if aims is not None else None
+        lib().oracle_trace_any(self.h, fp(rays), n, tmin, aim, fp(a) if a is not None else None,
+                               occ.ctypes.data_as(u8p), threads or lib().oracle_hw_threads(),
+                               c.ctypes.data_as(u64p) if counts else None)
+        return (occ, c) if counts else occ
+
+    def brute(self, rays, tmin=1e-4, tmax=np.inf, threads=0):
+        rays = f32(rays)
+        n = rays.shape[0]
+        t = np.empty(n, dtype=np.float32)
+        ids = np.empty(n, dtype=np.int32)
+        lib().oracle_brute_closest(self.h, fp(rays), n, tmin, tmax, fp(t), ids.ctypes.data_as(i32p),
+                                   threads or lib().oracle_hw_threads())
+        return t, ids
+
+    def render(self, args, rows=None, threads=0):
+        """Returns (gbuf [h][w][15], rad [4][h][w][4]) for RmRenderArgs `args`."""
+        w, h = args.width, args.height
+        g = np.empty((h, w, 15), dtype=np.float32)
+        r = np.empty((4, h, w, 4), dtype=np.float32)
+        rr = None
+        nr = 0
+        if rows is not None:
+            rr = np.ascontiguousarray(rows, dtype=np.int32)
+            nr = rr.size
+        st = lib().oracle_render(self.h, C.byref(args), threads or lib().oracle_hw_threads(),
+                                 rr.ctypes.data_as(i32p) if rr is not None else None, nr, fp(g), fp(r))
+        if st != 0:
+            raise ValueError("oracle_render rejected the arguments")
+        return g, r
+
+
+def ray_in_box(ray, box, tL, tR):
+    a, b = C.c_float(tL), C.c_float(tR)
+    lib().oracle_ray_in_box(fp(f32(ray)), fp(f32(box)), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def ray_triangle(ray, tri):
+    return lib().oracle_ray_triangle(fp(f32(ray)), fp(f32(tri)))
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox(c, k, o)
+    return tuple(o)
+
+
+def postprocess(gbuf, rad, options, exposure=1.0):
+    h, w = gbuf.shape[:2]
+    out = np.empty((h, w, 3), dtype=np.float32)
+    lib().oracle_postprocess(w, h, fp(f32(gbuf)), fp(f32(rad)), options, exposure, fp(out))
+    return out
+
+
+def hw_threads():
+    return lib().oracle_hw_threads()

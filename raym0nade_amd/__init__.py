@@ -1,0 +1,169 @@
+"""raym0nade_amd -- MI355X (gfx950) path-tracing hot path of the Raym0nade renderer.
+
+Python host API over the C ABI in include/raym0nade_hip.h.  Device buffers are
+torch tensors on the current HIP device (torch is used for memory and streams
+only); every computation happens in the library's own gfx950 kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (RM_BUF_BASE_COLOR, RM_BUF_DD, RM_BUF_DS, RM_BUF_EMISSION, RM_BUF_ID, RM_BUF_IS, RM_BUF_PIXELS,
+                   RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RmError, RmGenOptions, RmRenderArgs,
+                   RmSceneDesc, RmSceneInfo, check, lib)
+
+__all__ = ["GeneratedScene", "Scene", "Frame", "generate", "rays", "philox", "RmError", "RmRenderArgs"]
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class GeneratedScene:
+    """A procedural workload (rm_generate_scene); arrays are borrowed numpy views."""
+
+    def __init__(self, config: int, seed: int = 1, **opts):
+        o = RmGenOptions()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        h = C.c_void_p()
+        check(lib().rm_generate_scene(config, seed, C.byref(o), C.byref(h)))
+        self._h = h
+        self.desc = RmSceneDesc()
+        self.args = RmRenderArgs()
+        check(lib().rm_generated_desc(h, C.byref(self.desc), C.byref(self.args)))
+        n = self.desc.n_tris
+        self.n_tris = n
+        self.verts = self._view(self.desc.verts, C.c_float, n * 9).reshape(n, 3, 3)
+        self.material = self._view(self.desc.material, C.c_int32, n)
+
+    def _view(self, ptr, ctype, count):
+        # the ctypes array keeps this object alive for as long as the view exists
+        arr = (ctype * count).from_address(C.cast(ptr, C.c_void_p).value)
+        arr._owner = self
+        return np.ctypeslib.as_array(arr)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().rm_generated_destroy(h)
+            self._h = None
+
+
+def generate(config: int, seed: int = 1, **opts) -> GeneratedScene:
+    return GeneratedScene(config, seed, **opts)
+
+
+def rays(kind: int, n: int, seed: int = 1, cam: RmRenderArgs | None = None) -> np.ndarray:
+    """Host ray batch [n][6] (kind 0: pinhole rays of cam; kind 1: uniform)."""
+    out = np.empty((n, 6), dtype=np.float32)
+    check(lib().rm_generate_rays(kind, seed, n, C.byref(cam) if cam is not None else None, _fp(out)))
+    return out
+
+
+def philox(ctr, key) -> tuple:
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().rm_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+class Scene:
+    """Device scene (rm_scene_create): reference BVH, flattened, in HBM."""
+
+    def __init__(self, desc_or_gen):
+        desc = desc_or_gen.desc if isinstance(desc_or_gen, GeneratedScene) else desc_or_gen
+        self._keep = desc_or_gen
+        h = C.c_void_p()
+        check(lib().rm_scene_create(C.byref(desc), C.byref(h)))
+        self._h = h
+        self.info = RmSceneInfo()
+        check(lib().rm_scene_get_info(h, C.byref(self.info)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def perm(self) -> np.ndarray:
+        p = np.empty(self.info.n_tris, dtype=np.int32)
+        check(lib().rm_scene_get_perm(self._h, p.ctypes.data_as(C.POINTER(C.c_int32))))
+        return p
+
+    def trace_closest(self, rays_dev, tmin=1e-4, tmax=float("inf"), t_out=None, id_out=None, stream=None):
+        import torch
+        n = rays_dev.shape[0]
+        if t_out is None:
+            t_out = torch.empty(n, dtype=torch.float32, device=rays_dev.device)
+        if id_out is None:
+            id_out = torch.empty(n, dtype=torch.int32, device=rays_dev.device)
+        assert rays_dev.is_contiguous() and rays_dev.dtype == torch.float32 and rays_dev.shape[1] == 6
+        check(lib().rm_trace_closest(self._h, _ptr(rays_dev), n, tmin, tmax, _ptr(t_out), _ptr(id_out), _stream(stream)))
+        return t_out, id_out
+
+    def trace_any(self, rays_dev, tmin=1e-4, aim=float("inf"), aim_dev=None, occ=None, stream=None):
+        import torch
+        n = rays_dev.shape[0]
+        if occ is None:
+            occ = torch.empty(n, dtype=torch.uint8, device=rays_dev.device)
+        assert rays_dev.is_contiguous() and rays_dev.dtype == torch.float32 and rays_dev.shape[1] == 6
+        check(lib().rm_trace_any(self._h, _ptr(rays_dev), n, tmin, aim, _ptr(aim_dev) if aim_dev is not None else None,
+                                 _ptr(occ), _stream(stream)))
+        return occ
+
+    def trace_count(self, rays_dev, tmin=1e-4, tmax=float("inf"), any_hit=False, stream=None):
+        import torch
+        counts = torch.zeros(4, dtype=torch.int64, device=rays_dev.device)
+        check(lib().rm_trace_count(self._h, _ptr(rays_dev), rays_dev.shape[0], tmin, tmax, int(any_hit), _ptr(counts),
+                                   _stream(stream)))
+        return counts
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().rm_scene_destroy(h)
+            self._h = None
+
+
+class Frame:
+    """Device frame buffers (rm_frame_create)."""
+
+    CHANNELS = {RM_BUF_POSITION: 3, RM_BUF_SHAPE_NORMAL: 3, RM_BUF_SURFACE_NORMAL: 3, RM_BUF_EMISSION: 3,
+                RM_BUF_BASE_COLOR: 3, RM_BUF_DD: 4, RM_BUF_DS: 4, RM_BUF_ID: 4, RM_BUF_IS: 4, RM_BUF_PIXELS: 3}
+
+    def __init__(self, width: int, height: int):
+        h = C.c_void_p()
+        check(lib().rm_frame_create(width, height, C.byref(h)))
+        self._h = h
+        self.width, self.height = width, height
+
+    def render(self, scene: Scene, args: RmRenderArgs, stream=None):
+        check(lib().rm_render(scene.handle, C.byref(args), self._h, _stream(stream)))
+
+    def postprocess(self, options: int, exposure: float = 1.0, stream=None):
+        check(lib().rm_postprocess(self._h, options, exposure, _stream(stream)))
+
+    def download(self, which: int) -> np.ndarray:
+        ch = self.CHANNELS[which]
+        out = np.empty((self.height, self.width, ch), dtype=np.float32)
+        check(lib().rm_frame_download(self._h, which, _fp(out)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().rm_frame_destroy(h)
+            self._h = None

@@ -1,0 +1,15 @@
+// Frame object shared by the render and post-processing translation units.
+#pragma once
+#include "rm_internal.h"
+
+constexpr int RM_NUM_BUFFERS = 10;
+
+inline int frame_channels(int which) { return (which >= RM_BUF_DD && which <= RM_BUF_IS) ? 4 : 3; }
+
+struct rm_frame {
+    int32_t w = 0, h = 0;
+    rm::DevBuf buf[RM_NUM_BUFFERS];
+    // integrator scratch (path queues, per-pixel accumulators), grown on demand
+    rm::DevBuf work;
+    size_t work_bytes = 0;
+};

@@ -1,0 +1,91 @@
+// Internal declarations shared by the host translation units of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/raym0nade_hip.h"
+#include "rm_layout.h"
+
+namespace rm {
+
+// Records a failure for rm_last_error() and returns `code`.
+int32_t fail(int32_t code, const char *fmt, ...);
+// Maps a HIP error to RM_ERR_HIP / RM_ERR_OOM with context.
+int32_t hip_fail(hipError_t e, const char *what);
+
+#define RM_HIP(call)                                        \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return ::rm::hip_fail(e_, #call); \
+    } while (0)
+
+// Ensures a device is present and current; caches the answer.
+int32_t ensure_device();
+
+// Owned device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int32_t alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) { p = nullptr; return hip_fail(e, "hipMalloc"); }
+        bytes = n;
+        return RM_OK;
+    }
+    int32_t upload(const void *src, size_t n) {
+        int32_t s = alloc(n);
+        if (s) return s;
+        if (n) {
+            hipError_t e = hipMemcpy(p, src, n, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpy H2D");
+        }
+        return RM_OK;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Host-built, flattened scene before upload (also used by host-only tests of
+// the builder through rm_scene_create's info).
+struct HostScene {
+    int64_t n = 0;
+    int32_t depth = 0;
+    int64_t n_nodes = 0;      // nodeCount(1,n)+1
+    uint32_t root_leaf = 0;
+    std::vector<int32_t> perm;          // slot -> original id
+    std::vector<PairNode> pairs;        // indexed by interior node u
+    std::vector<TriRec> tris;           // slot order
+    std::vector<TriShade> shade;        // slot order
+    std::vector<MatRec> mats;
+    std::vector<TexRec> texs;
+    std::vector<float> texels;
+    std::vector<LightRec> lights;
+    std::vector<TriShade> light_faces;
+    std::vector<float> light_cdf;
+    std::vector<float> sky;             // premultiplied by solid angle
+    std::vector<float> sky_cdf;
+    int32_t sky_w = 0, sky_h = 0;
+};
+
+int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs);
+
+}  // namespace rm
+
+struct rm_scene {
+    rm::HostScene host;  // perm + sizes kept; big arrays released after upload
+    rm::SceneView view{};
+    rm::DevBuf nodes, tris, shade, mats, texs, texels, lights, light_faces, light_cdf, sky, sky_cdf;
+    rm_scene_info info{};
+};

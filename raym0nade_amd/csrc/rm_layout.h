@@ -1,0 +1,113 @@
+// HBM layout of a flattened scene, shared by the host builder and the kernels.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#endif
+
+namespace rm {
+
+// The reference stores BVH_Node{Box; faceL, faceR} (32 B) in an implicit heap
+// (children of u at 2u, 2u+1; src/bvh.cpp:44-50, include/bvh.h:7-11) and tests
+// BOTH child boxes when it visits an interior node (src/bvh.cpp:71-77).  The
+// GPU keeps the heap numbering but stores, per interior node u, the two child
+// boxes together plus a leaf code per child, so one visit is one aligned 64-B
+// read (half a 128-B cache line).
+//   a = child0.lo.xyz, child0.hi.x
+//   b = child0.hi.yz,  child1.lo.xy
+//   c = child1.lo.z,   child1.hi.xyz
+//   leaf[k] = 0 if child k is interior (its pair record is at index 2u+k),
+//             else (first_face << 4) | face_count   (count in 1..10)
+struct alignas(16) PairNode {
+    float a[4];
+    float b[4];
+    float c[4];
+    uint32_t leaf[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(PairNode) == 64, "PairNode is 64 B");
+
+// Intersection record of one triangle in BVH (leaf) order, 48 B:
+//   p = v0.xyz, thr      thr: |a| below which Moller-Trumbore rejects, exact
+//                        stand-in for |a| / length(edge1) < 1e-4
+//                        (src/geometry.cpp:70); see flatten code.
+//   e1 = v1 - v0, orig   orig: original triangle id as int bits
+//   e2 = v2 - v0, 0
+// Edges are the same fp32 differences the reference computes per test
+// (src/geometry.cpp:65-66), so every test result is bit-identical.
+struct alignas(16) TriRec {
+    float p[4];
+    float e1[4];
+    float e2[4];
+};
+static_assert(sizeof(TriRec) == 48, "TriRec is 48 B");
+
+// Leaf-code helpers.
+constexpr uint32_t LEAF_SHIFT = 4;
+constexpr uint32_t LEAF_MASK = 15;
+constexpr int LEAF_BAG = 10;  // LeafBagSize, src/bvh.cpp:16
+
+// Shading data of one triangle in BVH order (what the reference reaches
+// through Face::data[] and Face::material, include/component.h:15-22).
+struct alignas(16) TriShade {
+    float v[3][3];   // vertices
+    float n[3][3];   // vertex normals
+    float uv[3][2];  // texture coordinates
+    int32_t material;
+    int32_t mesh;
+    int32_t pad[2];
+};
+static_assert(sizeof(TriShade) % 16 == 0, "TriShade aligned");
+
+// Material as the kernels read it.
+struct alignas(16) MatRec {
+    float base[3];
+    float roughness;
+    float emission[3];
+    float metallic;
+    float trans[3];
+    float opacity;
+    float ior;
+    int32_t tex;       // -1 or texture index
+    int32_t pad[2];
+};
+
+// fp32 texture with a mip chain (level l at offset[l], size w>>l x h>>l).
+struct TexRec {
+    int32_t width, height, levels, pad;
+    int64_t offset[8];  // in floats, into the scene's texel pool
+};
+
+// One light object (LightObject, include/component.h:47-54).
+struct LightRec {
+    float center[3];
+    float power;
+    float color[3];
+    int32_t first_face;  // into light_faces / light_cdf
+    int32_t n_faces;
+    int32_t pad[3];
+};
+
+// Device view of a scene (all pointers are device pointers).
+struct SceneView {
+    const PairNode *nodes;
+    const TriRec *tris;
+    const TriShade *shade;
+    const MatRec *mats;
+    const TexRec *texs;
+    const float *texels;
+    const LightRec *lights;
+    const TriShade *light_faces;  // copies of light faces (reference copies Face into LightObject)
+    const float *light_cdf;       // per light object, prefix sums of face weights
+    const float *sky;             // premultiplied by pixel solid angle, [h][w][3]
+    const float *sky_cdf;         // prefix sums of luminance weights
+    int64_t n_tris;
+    int32_t root_leaf;            // 0, or leaf code when the whole scene is one leaf
+    int32_t depth;
+    int32_t n_lights;
+    int32_t sky_w, sky_h;
+    int32_t n_mats;
+};
+
+}  // namespace rm

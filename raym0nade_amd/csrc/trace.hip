@@ -1,0 +1,143 @@
+// Ray-batch traversal kernels behind rm_trace_closest / rm_trace_any /
+// rm_trace_count.
+//
+// One lane per ray; rays arrive as [n][6] fp32 (origin, direction) in HBM.
+// Blocks are renumbered so that each XCD (blocks are dealt round-robin over
+// the 8 XCDs) works on one contiguous range of rays: neighbouring rays of a
+// coherent batch then share that XCD's L2.
+#include <hip/hip_runtime.h>
+#include "rm_internal.h"
+#include "traverse.h"
+
+namespace rm {
+
+constexpr int TRACE_BLOCK = 256;
+
+// Bijection blockIdx -> logical block with contiguous ranges per XCD.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
+    uint32_t q = G >> 3, r = G & 7u, x = b & 7u;
+    uint32_t start = x * q + (x < r ? x : r);
+    return start + (b >> 3);
+}
+
+__device__ __forceinline__ void load_ray(const float *rays, int64_t i, V3 &o, V3 &d) {
+    const float2 *p = reinterpret_cast<const float2 *>(rays + i * 6);
+    float2 a = p[0], b = p[1], c = p[2];
+    o = V3{a.x, a.y, b.x};
+    d = V3{b.y, c.x, c.y};
+}
+
+// KIND only distinguishes instantiations by name in profiles (0 = generic).
+template <int KIND>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_closest(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                               float tmin, float tmax0, float *__restrict__ t_out,
+                                                               int32_t *__restrict__ id_out) {
+    int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    V3 o, d;
+    load_ray(rays, i, o, d);
+    RayPre r = ray_pre(o, d);
+    float tmax = tmax0;
+    TraceCounts c{};
+    int32_t best = trace<false, false>(S, r, tmin, tmax, -INFINITY, c);
+    int32_t id = -1;
+    if (best >= 0) id = __float_as_int(S.tris[best].e1[3]);
+    t_out[i] = tmax;
+    id_out[i] = id;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_any(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                           float tmin, float aim0, const float *__restrict__ aim_arr,
+                                                           uint8_t *__restrict__ occ) {
+    int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    V3 o, d;
+    load_ray(rays, i, o, d);
+    RayPre r = ray_pre(o, d);
+    float aim = aim_arr ? aim_arr[i] : aim0;
+    // HitRecord hit(eps_zero, aimDepth + eps_zero) and `t_max >= aimDepth -> false`
+    float tmax = aim + T_EPS;
+    TraceCounts c{};
+    trace<true, false>(S, r, tmin, tmax, aim, c);
+    occ[i] = (tmax < aim) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_count(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                             float tmin, float tmax0, int any_hit,
+                                                             unsigned long long *__restrict__ counts) {
+    int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    TraceCounts c{};
+    if (i < n) {
+        V3 o, d;
+        load_ray(rays, i, o, d);
+        RayPre r = ray_pre(o, d);
+        if (any_hit) {
+            float aim = tmax0;
+            float tmax = aim + T_EPS;
+            trace<true, true>(S, r, tmin, tmax, aim, c);
+        } else {
+            float tmax = tmax0;
+            trace<false, true>(S, r, tmin, tmax, -INFINITY, c);
+        }
+    }
+    // wave reduction then one atomic per wave
+    unsigned long long v[4] = {c.interior, c.leaves, c.boxes, c.tris};
+    for (int k = 0; k < 4; ++k) {
+        unsigned long long x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd(counts + k, x);
+    }
+}
+
+}  // namespace rm
+
+using namespace rm;
+
+static int32_t check_trace_args(const rm_scene *s, const float *rays, int64_t n, const void *out1, const char *fn) {
+    if (!s) return fail(RM_ERR_INVALID, "%s: scene is NULL", fn);
+    if (n < 0) return fail(RM_ERR_INVALID, "%s: negative ray count", fn);
+    if (n > 0 && (!rays || !out1)) return fail(RM_ERR_INVALID, "%s: NULL buffer", fn);
+    if ((int64_t)((n + TRACE_BLOCK - 1) / TRACE_BLOCK) > 0x7fffffffLL) return fail(RM_ERR_INVALID, "%s: too many rays", fn);
+    if (s->view.depth > MAX_STACK) return fail(RM_ERR_UNSUPPORTED, "%s: tree depth %d exceeds %d", fn, s->view.depth, MAX_STACK);
+    return RM_OK;
+}
+
+extern "C" int32_t rm_trace_closest(const rm_scene *s, const float *rays, int64_t n, float tmin, float tmax,
+                                    float *t_out, int32_t *id_out, void *stream) {
+    int32_t st = check_trace_args(s, rays, n, t_out, "rm_trace_closest");
+    if (st) return st;
+    if (n > 0 && !id_out) return fail(RM_ERR_INVALID, "rm_trace_closest: id_out is NULL");
+    if (n == 0) return RM_OK;
+    unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
+    hipLaunchKernelGGL(k_trace_closest<0>, dim3(grid), dim3(TRACE_BLOCK), 0, (hipStream_t)stream, s->view, rays, n, tmin,
+                       tmax, t_out, id_out);
+    RM_HIP(hipGetLastError());
+    return RM_OK;
+}
+
+extern "C" int32_t rm_trace_any(const rm_scene *s, const float *rays, int64_t n, float tmin, float aim_depth,
+                                const float *aim_dev, uint8_t *occ, void *stream) {
+    int32_t st = check_trace_args(s, rays, n, occ, "rm_trace_any");
+    if (st) return st;
+    if (n == 0) return RM_OK;
+    unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
+    hipLaunchKernelGGL(k_trace_any<0>, dim3(grid), dim3(TRACE_BLOCK), 0, (hipStream_t)stream, s->view, rays, n, tmin,
+                       aim_depth, aim_dev, occ);
+    RM_HIP(hipGetLastError());
+    return RM_OK;
+}
+
+extern "C" int32_t rm_trace_count(const rm_scene *s, const float *rays, int64_t n, float tmin, float tmax,
+                                  int32_t any_hit, uint64_t *counts, void *stream) {
+    int32_t st = check_trace_args(s, rays, n, counts, "rm_trace_count");
+    if (st) return st;
+    if (!counts) return fail(RM_ERR_INVALID, "rm_trace_count: counts is NULL");
+    RM_HIP(hipMemsetAsync(counts, 0, 4 * sizeof(uint64_t), (hipStream_t)stream));
+    if (n == 0) return RM_OK;
+    unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
+    hipLaunchKernelGGL(k_trace_count, dim3(grid), dim3(TRACE_BLOCK), 0, (hipStream_t)stream, s->view, rays, n, tmin, tmax,
+                       (int)any_hit, (unsigned long long *)counts);
+    RM_HIP(hipGetLastError());
+    return RM_OK;
+}

@@ -1,0 +1,186 @@
+// gfx950 BVH traversal that reproduces the reference's recursive walk
+// (BVH::dfs_rayHit, src/bvh.cpp:56-88) result for result.
+//
+// Reference order, restated iteratively:
+//   visit(u interior): tL_k,tR_k = rayInBox(child k) on [t_min, t_max(now)];
+//     first = child0 if tL0 < tL1 else child1 (ties -> child1);
+//     visit(first)  if tL_first  < tR_first;
+//     visit(second) if tL_second < tR_second && tL_second < t_max(after first).
+//   visit(leaf): Moller-Trumbore on faces in order, accept t_min < t < t_max.
+// The second child is pushed with tL_second when tL_second < tR_second holds and
+// the t_max test is made when it is popped, after the whole first subtree: the
+// same sequence of box tests, triangle tests and t_max updates as the recursion.
+//
+// Exactness: every operation is the reference's fp32 operation in glm order
+// (see rm_vec.h), built with -ffp-contract=off and IEEE division/sqrt.  Three
+// values the reference recomputes per test are precomputed per ray or per
+// triangle with the identical operations (1/d_i, v1-v0, v2-v0), and
+// |a|/length(e1) < 1e-4 is replaced by the exactly equivalent |a| < thr.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rm_layout.h"
+#include "rm_vec.h"
+
+namespace rm {
+
+constexpr float T_EPS = 1e-4f;   // eps_zero
+constexpr int MAX_STACK = 32;    // pending entries <= tree depth - 1 (2^27 faces -> 25)
+
+struct RayPre {
+    V3 o, d;
+    float inv[3];
+    bool par[3];
+};
+
+__device__ __forceinline__ RayPre ray_pre(V3 o, V3 d) {
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    r.par[0] = fabsf(d.x) < T_EPS;
+    r.par[1] = fabsf(d.y) < T_EPS;
+    r.par[2] = fabsf(d.z) < T_EPS;
+    r.inv[0] = 1.0f / d.x;
+    r.inv[1] = 1.0f / d.y;
+    r.inv[2] = 1.0f / d.z;
+    return r;
+}
+
+// rayInBox (src/geometry.cpp:40-61) on one slab.  Returns false when the
+// reference would return early (tR=-1 or tL>tR), leaving tL/tR as it does.
+__device__ __forceinline__ bool slab(float o, float inv, bool par, float lo, float hi, float &tL, float &tR) {
+    if (par) {
+        if (o < lo || o > hi) {
+            tR = -1.0f;
+            return false;
+        }
+        return true;
+    }
+    float a = (lo - o) * inv, b = (hi - o) * inv;
+    float en = inv >= 0.0f ? a : b, ex = inv >= 0.0f ? b : a;
+    // std::fmax / std::fmin; all operands are finite or +-inf here, where the
+    // hardware max/min agree with them up to the sign of a zero, which no
+    // later comparison can observe.
+    tL = fmaxf(tL, en);
+    tR = fminf(tR, ex);
+    tR += T_EPS;
+    return !(tL > tR);
+}
+
+__device__ __forceinline__ void box_test(const RayPre &r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                         float &tL, float &tR) {
+    if (!slab(r.o.x, r.inv[0], r.par[0], lx, hx, tL, tR)) return;
+    if (!slab(r.o.y, r.inv[1], r.par[1], ly, hy, tL, tR)) return;
+    slab(r.o.z, r.inv[2], r.par[2], lz, hz, tL, tR);
+}
+
+// RayTriangleIntersection (src/geometry.cpp:63-87).
+__device__ __forceinline__ float tri_test(const RayPre &r, const TriRec &T) {
+    V3 v0{T.p[0], T.p[1], T.p[2]};
+    V3 e1{T.e1[0], T.e1[1], T.e1[2]};
+    V3 e2{T.e2[0], T.e2[1], T.e2[2]};
+    V3 h = cross(r.d, e2);
+    float a = dot(e1, h);
+    if (fabsf(a) < T.p[3]) return INFINITY;
+    float f = 1.0f / a;
+    V3 s = r.o - v0;
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return INFINITY;
+    V3 q = cross(s, e1);
+    float v = f * dot(r.d, q);
+    if (v < 0.0f || u + v > 1.0f) return INFINITY;
+    return f * dot(e2, q);
+}
+
+__device__ __forceinline__ TriRec load_tri(const TriRec *tris, uint32_t i) {
+    TriRec T;
+    const float4 *p = reinterpret_cast<const float4 *>(tris + i);
+    float4 x = p[0], y = p[1], z = p[2];
+    T.p[0] = x.x; T.p[1] = x.y; T.p[2] = x.z; T.p[3] = x.w;
+    T.e1[0] = y.x; T.e1[1] = y.y; T.e1[2] = y.z; T.e1[3] = y.w;
+    T.e2[0] = z.x; T.e2[1] = z.y; T.e2[2] = z.z; T.e2[3] = z.w;
+    return T;
+}
+
+struct TraceCounts {
+    uint32_t interior, leaves, boxes, tris;
+};
+
+// Leaf test: faces [first, first+count) in order, strict acceptance
+// (src/bvh.cpp:58-67).
+template <bool COUNT>
+__device__ __forceinline__ void leaf_test(const SceneView &S, const RayPre &r, uint32_t code, float tmin, float &tmax,
+                                          int32_t &best, TraceCounts &c) {
+    uint32_t first = code >> LEAF_SHIFT, cnt = code & LEAF_MASK;
+    if (COUNT) { c.leaves++; c.tris += cnt; }
+    for (uint32_t i = 0; i < cnt; ++i) {
+        TriRec T = load_tri(S.tris, first + i);
+        float t = tri_test(r, T);
+        if (tmin < t && t < tmax) {
+            tmax = t;
+            best = (int32_t)(first + i);
+        }
+    }
+}
+
+// Closest hit on (tmin, tmax).  With ANY, returns as soon as tmax < aim
+// (Model::rayHit_test's boolean is decided then).  Returns the slot of the
+// closest accepted triangle (BVH order) or -1; tmax holds its t.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ int32_t trace(const SceneView &S, const RayPre &r, float tmin, float &tmax, float aim,
+                                         TraceCounts &c) {
+    int32_t best = -1;
+    if (S.root_leaf) {
+        leaf_test<COUNT>(S, r, (uint32_t)S.root_leaf, tmin, tmax, best, c);
+        return best;
+    }
+    uint32_t stk_ref[MAX_STACK];
+    float stk_tl[MAX_STACK];
+    int sp = 0;
+    uint32_t u = 1;
+    while (true) {
+        // ---- interior visit
+        if (COUNT) { c.interior++; c.boxes += 2; }
+        const float4 *P = reinterpret_cast<const float4 *>(S.nodes + u);
+        float4 A = P[0], Bq = P[1], C = P[2];
+        uint2 Lf = *reinterpret_cast<const uint2 *>(&P[3]);
+        float tL0 = tmin, tR0 = tmax, tL1 = tmin, tR1 = tmax;
+        box_test(r, A.x, A.y, A.z, A.w, Bq.x, Bq.y, tL0, tR0);
+        box_test(r, Bq.z, Bq.w, C.x, C.y, C.z, C.w, tL1, tR1);
+        uint32_t ref0 = Lf.x ? (Lf.x | 0x80000000u) : (u << 1);
+        uint32_t ref1 = Lf.y ? (Lf.y | 0x80000000u) : (u << 1 | 1u);
+        bool f0 = tL0 < tL1;
+        float tLf = f0 ? tL0 : tL1, tRf = f0 ? tR0 : tR1;
+        float tLs = f0 ? tL1 : tL0, tRs = f0 ? tR1 : tR0;
+        uint32_t rf = f0 ? ref0 : ref1, rs = f0 ? ref1 : ref0;
+        if (tLs < tRs) {
+            stk_ref[sp] = rs;
+            stk_tl[sp] = tLs;
+            ++sp;
+        }
+        uint32_t next = (tLf < tRf) ? rf : 0u;
+        // ---- resolve leaves and pops until the next interior node
+        while (true) {
+            if (next == 0u) {
+                while (sp > 0) {
+                    --sp;
+                    if (stk_tl[sp] < tmax) {
+                        next = stk_ref[sp];
+                        break;
+                    }
+                }
+                if (next == 0u) return best;
+            }
+            if (next & 0x80000000u) {
+                if (COUNT) {}
+                leaf_test<COUNT>(S, r, next & 0x7fffffffu, tmin, tmax, best, c);
+                if (ANY && tmax < aim) return best;
+                next = 0u;
+                continue;
+            }
+            u = next;
+            break;
+        }
+    }
+}
+
+}  // namespace rm

@@ -1,0 +1,161 @@
+"""GPU traversal parity: rm_trace_closest / rm_trace_any / rm_trace_count against
+the oracle's BVH::rayHit / Model::rayHit_test (src/bvh.cpp:56-92,
+src/model.cpp:332-354).  Bar: triangle ids, t and occlusion bits bit-exact;
+counters (interior visits, leaves, box and triangle tests) exactly equal,
+since the kernel walks the reference order."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_py as O
+import raym0nade_amd as rm
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def run_closest(S, r, dev, tmin=1e-4, tmax=np.inf):
+    t, ids = S.trace_closest(gpu(r, dev), tmin=tmin, tmax=tmax)
+    torch.cuda.synchronize()
+    return t.cpu().numpy(), ids.cpu().numpy()
+
+
+def run_any(S, r, dev, aim=np.inf, aims=None):
+    occ = S.trace_any(gpu(r, dev), aim=aim, aim_dev=gpu(aims, dev) if aims is not None else None)
+    torch.cuda.synchronize()
+    return occ.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def soup(dev):
+    print("[trace] soup 100k tris", flush=True)
+    g = rm.generate(1, 2, n_tris=100000)
+    return g, rm.Scene(g), O.OracleScene(g.desc)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_closest_bitexact_soup(soup, dev, kind):
+    g, S, OS = soup
+    if kind == 0:
+        cam = g.args
+        cam.width = cam.height = 400
+        cam.accuracy = np.float32(2 * np.tan(np.radians(20)) / 400)
+        r = rm.rays(0, 400 * 400, cam=cam)
+    else:
+        r = rm.rays(1, 200000, seed=9)
+    t, ids = run_closest(S, r, dev)
+    to, io = OS.closest(r)
+    assert np.array_equal(ids, io)
+    assert np.array_equal(t.view(np.uint32), to.view(np.uint32))
+
+
+def test_perm_matches_oracle(soup):
+    g, S, OS = soup
+    assert np.array_equal(S.perm(), OS.perm())
+    assert S.info.n_nodes == OS.node_count()
+
+
+def test_any_bitexact_soup(soup, dev):
+    g, S, OS = soup
+    r = rm.rays(1, 200000, seed=10)
+    for aim in [np.inf, 0.3, 0.02]:
+        assert np.array_equal(run_any(S, r, dev, aim=aim), OS.any(r, aim=aim))
+    aims = np.random.default_rng(1).uniform(0.0, 1.5, r.shape[0]).astype(np.float32)
+    assert np.array_equal(run_any(S, r, dev, aims=aims), OS.any(r, aims=aims))
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_counts_equal_oracle(soup, dev, any_hit):
+    g, S, OS = soup
+    r = rm.rays(1, 50000, seed=11)
+    c = S.trace_count(gpu(r, dev), any_hit=any_hit).cpu().numpy().astype(np.uint64)
+    if any_hit:
+        # the oracle runs the full closest traversal; the kernel stops once
+        # occlusion is decided, so it may only do less work
+        _, co = OS.any(r, counts=True)
+        assert (c <= co).all() and c[0] > 0
+    else:
+        _, _, co = OS.closest(r, counts=True)
+        assert np.array_equal(c, co)
+
+
+def test_tmin_tmax_window(soup, dev):
+    g, S, OS = soup
+    r = rm.rays(1, 50000, seed=12)
+    for tmin, tmax in [(0.05, 0.4), (1e-4, 0.1), (0.2, np.inf)]:
+        t, ids = run_closest(S, r, dev, tmin, tmax)
+        to, io = OS.closest(r, tmin, tmax)
+        assert np.array_equal(ids, io) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+
+
+def test_axis_parallel_rays(soup, dev):
+    # directions with exact zero / tiny components take the |d_i| < 1e-4 branch
+    rng = np.random.default_rng(5)
+    n = 30000
+    o = rng.uniform(-0.2, 1.2, (n, 3)).astype(np.float32)
+    d = np.zeros((n, 3), np.float32)
+    ax = rng.integers(0, 3, n)
+    d[np.arange(n), ax] = rng.choice([-1.0, 1.0], n)
+    tiny = rng.uniform(-9e-5, 9e-5, (n, 3)).astype(np.float32)
+    mask = rng.uniform(size=(n, 3)) < 0.5
+    d = np.where(mask & (d == 0), tiny, d).astype(np.float32)
+    r = np.concatenate([o, d], 1)
+    t, ids = run_closest(S := soup[1], r, dev)
+    to, io = soup[2].closest(r)
+    assert np.array_equal(ids, io) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+
+
+def test_cornell_bitexact(dev):
+    g = rm.generate(0, 1)
+    S, OS = rm.Scene(g), O.OracleScene(g.desc)
+    r = rm.rays(0, 512 * 512, cam=g.args)
+    t, ids = run_closest(S, r, dev)
+    to, io = OS.closest(r)
+    assert np.array_equal(ids, io) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 10, 11])
+def test_tiny_scenes(dev, n):
+    # n <= 10: the whole scene is one leaf (root visited as a leaf)
+    g = rm.generate(1, 40 + n, n_tris=n)
+    S, OS = rm.Scene(g), O.OracleScene(g.desc)
+    r = rm.rays(1, 20000, seed=n)
+    r[:, :3] = r[:, :3] * 0.2 + 0.4
+    t, ids = run_closest(S, r, dev)
+    to, io = OS.closest(r)
+    assert np.array_equal(ids, io) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
+    assert np.array_equal(run_any(S, r, dev), OS.any(r))
+
+
+def test_zero_rays(soup, dev):
+    g, S, OS = soup
+    r = torch.empty((0, 6), dtype=torch.float32, device=dev)
+    t, ids = S.trace_closest(r)
+    assert t.numel() == 0
+
+
+def test_full_size_c1(dev):
+    """1M-triangle soup, 2828^2 coherent + 8M incoherent rays.  The GPU runs
+    every ray; the oracle checks a 100k-ray sample of each batch bit for bit,
+    and the GPU's own counters over the sample equal the oracle's."""
+    print("[trace] full C1: 1M tris", flush=True)
+    g = rm.generate(1, 1, n_tris=1000000)
+    S, OS = rm.Scene(g), O.OracleScene(g.desc)
+    assert S.info.n_nodes == 262144 and S.info.depth == 18
+    rng = np.random.default_rng(0)
+    for kind in (0, 1):
+        n = 2828 * 2828 if kind == 0 else 8000000
+        r = rm.rays(kind, n, seed=1, cam=g.args)
+        t, ids = run_closest(S, r, dev)
+        occ = run_any(S, r, dev)
+        assert (occ.astype(bool) == (ids >= 0)).all()   # aim = inf: occluded iff any hit
+        idx = np.sort(rng.choice(n, 100000, replace=False))
+        to, io = OS.closest(r[idx])
+        assert np.array_equal(ids[idx], io)
+        assert np.array_equal(t[idx].view(np.uint32), to.view(np.uint32))
+        c = S.trace_count(gpu(r[idx], dev)).cpu().numpy().astype(np.uint64)
+        _, _, co = OS.closest(r[idx], counts=True)
+        assert np.array_equal(c, co)
