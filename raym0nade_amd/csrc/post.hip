@@ -1,6 +1,160 @@
-// Post-processing kernels (work in progress).
+// Post-processing behind rm_postprocess: Photo::shade (src/image.cpp:215-246),
+// Photo::gammaCorrection (:454-468) and Photo::FXAA (:358-452), one lane per
+// pixel.  shade and the tone curve are fused in one pass; FXAA reads the
+// tone-mapped image from a second buffer because every output pixel looks at
+// up to 12 neighbours along the edge normal.
+#include <hip/hip_runtime.h>
 #include "rm_frame.h"
+#include "rm_vec.h"
+
+namespace rm {
+
+constexpr int PB = 256;
+enum : int {
+    O_BaseColor = 1, O_Emission = 2, O_Direct = 4, O_Indirect = 8, O_Diffuse = 16, O_Specular = 32,
+    O_ShapeNormal = 64, O_SurfaceNormal = 128, O_Bloom = 256, O_FXAA = 512, O_DoF = 1024
+};
+
+__device__ __forceinline__ V3 ld3p(const float *p, int64_t i) { return V3{p[i * 3], p[i * 3 + 1], p[i * 3 + 2]}; }
+__device__ __forceinline__ V3 ld4p(const float *p, int64_t i) { return V3{p[i * 4], p[i * 4 + 1], p[i * 4 + 2]}; }
+
+struct PostIn {
+    const float *shapeN, *surfN, *emis, *base, *Dd, *Ds, *Id, *Is;
+};
+
+__global__ __launch_bounds__(PB) void k_shade_gamma(PostIn I, int64_t n, int opt, float exposure, float *out) {
+    int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (i >= n) return;
+    V3 px;
+    if (opt & O_ShapeNormal) {
+        px = (ld3p(I.shapeN, i) + v3(1.0f)) / 2.0f;
+    } else if (opt & O_SurfaceNormal) {
+        px = (ld3p(I.surfN, i) + v3(1.0f)) / 2.0f;
+    } else {
+        V3 rd = v3(0.0f), rs = v3(0.0f);
+        if (opt & O_Direct) {
+            if (opt & O_Diffuse) rd += ld4p(I.Dd, i);
+            if (opt & O_Specular) rs += ld4p(I.Ds, i);
+        }
+        if (opt & O_Indirect) {
+            if (opt & O_Diffuse) rd += ld4p(I.Id, i);
+            if (opt & O_Specular) rs += ld4p(I.Is, i);
+        }
+        if (!(opt & (O_Direct | O_Indirect))) rd = v3(1.0f);
+        V3 dc = (opt & O_BaseColor) ? ld3p(I.base, i) : v3(1.0f);
+        px = dc * rd + rs;
+        if (opt & O_Emission) px += ld3p(I.emis, i) * exposure;
+    }
+    // gammaCorrection: max(0), tanh luminance soft clip above 0.75, min(1), ^(1/2.2)
+    px = vmax(px, v3(0.0f));
+    float c = dot(px, V3{0.3f, 0.6f, 0.1f});
+    if (c > 0.75f) px = px / c * (tanhf(3.0f * (c - 0.75f)) / 3.0f + 0.75f);
+    px = vmin(px, v3(1.0f));
+    const float e = 1.0f / 2.2f;
+    px = V3{powf(px.x, e), powf(px.y, e), powf(px.z, e)};
+    out[i * 3] = px.x;
+    out[i * 3 + 1] = px.y;
+    out[i * 3 + 2] = px.z;
+}
+
+__device__ __forceinline__ float luma(const float *d, int64_t i) {
+    return dot(V3{d[i * 3], d[i * 3 + 1], d[i * 3 + 2]}, V3{0.3f, 0.6f, 0.1f});
+}
+
+__global__ __launch_bounds__(PB) void k_fxaa(const float *__restrict__ d, int w, int h, float *__restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (i >= (int64_t)w * h) return;
+    int x = (int)(i % w), y = (int)(i / w);
+    const float EDGE_MIN = 0.0312f, EDGE_MAX = 0.125f, SUBPIX = 0.75f;
+    float M = luma(d, i);
+    float N = y > 0 ? luma(d, i - w) : M;
+    float S = y < h - 1 ? luma(d, i + w) : M;
+    float E = x < w - 1 ? luma(d, i + 1) : M;
+    float Wl = x > 0 ? luma(d, i - 1) : M;
+    // std::min / std::max over an initializer list keep the first extreme
+    float mn = N, mx = N;
+    if (S < mn) mn = S;
+    if (E < mn) mn = E;
+    if (Wl < mn) mn = Wl;
+    if (mx < S) mx = S;
+    if (mx < E) mx = E;
+    if (mx < Wl) mx = Wl;
+    float range = mx - mn;
+    float thr = EDGE_MIN < mx * EDGE_MAX ? mx * EDGE_MAX : EDGE_MIN;  // std::max(EDGE_MIN, mx*EDGE_MAX)
+    V3 c0{d[i * 3], d[i * 3 + 1], d[i * 3 + 2]};
+    if (range < thr) {
+        out[i * 3] = c0.x;
+        out[i * 3 + 1] = c0.y;
+        out[i * 3 + 2] = c0.z;
+        return;
+    }
+    float NW = (y > 0 && x > 0) ? luma(d, i - w - 1) : M;
+    float NE = (y > 0 && x < w - 1) ? luma(d, i - w + 1) : M;
+    float SW = (y < h - 1 && x > 0) ? luma(d, i + w - 1) : M;
+    float SE = (y < h - 1 && x < w - 1) ? luma(d, i + w + 1) : M;
+    float eh = fabsf((NW + Wl + SW) - (NE + E + SE)) * (1.0f / 3.0f);
+    float ev = fabsf((NW + N + NE) - (SW + S + SE)) * (1.0f / 3.0f);
+    bool horiz = eh >= ev;
+    float step = horiz ? 1.0f / float(w) : 1.0f / float(h);
+    float gs = (horiz ? eh : ev) / range;
+    gs = gs < -2.0f ? -2.0f : (gs > 2.0f ? 2.0f : gs);  // std::clamp
+    float u = float(x) / float(w), v = float(y) / float(h);
+    V3 fin = c0;
+    float best = 0.0f;
+    for (int k = 0; k < 12; ++k) {
+        float o = gs * step * float(k + 1);
+        float su = u + (horiz ? 0.0f : o), sv = v + (horiz ? o : 0.0f);
+        if (su < 0.0f || su > 1.0f || sv < 0.0f || sv > 1.0f) continue;
+        int sx = (int)(su * float(w)), sy = (int)(sv * float(h));
+        sx = sx < 0 ? 0 : (sx > w - 1 ? w - 1 : sx);
+        sy = sy < 0 ? 0 : (sy > h - 1 ? h - 1 : sy);
+        int64_t j = (int64_t)sy * w + sx;
+        float dl = fabsf(luma(d, j) - M);
+        if (dl > best) {
+            best = dl;
+            fin = V3{d[j * 3], d[j * 3 + 1], d[j * 3 + 2]};
+        }
+    }
+    float sub = (fabsf(N + S - 2.0f * M) * 2.0f + fabsf(E + Wl - 2.0f * M)) * 0.25f;
+    sub = 1.0f < sub ? 1.0f : sub;  // std::min(sub, 1)
+    float t = sub * SUBPIX;
+    V3 r = c0 * (1.0f - t) + fin * t;
+    out[i * 3] = r.x;
+    out[i * 3 + 1] = r.y;
+    out[i * 3 + 2] = r.z;
+}
+
+}  // namespace rm
+
 using namespace rm;
-extern "C" int32_t rm_postprocess(rm_frame *, int32_t, float, void *) {
-    return fail(RM_ERR_UNSUPPORTED, "rm_postprocess: not built yet");
+
+extern "C" int32_t rm_postprocess(rm_frame *f, int32_t opt, float exposure, void *stream_) {
+    if (!f) return fail(RM_ERR_INVALID, "rm_postprocess: frame is NULL");
+    if (opt & (O_Bloom | O_DoF)) return fail(RM_ERR_UNSUPPORTED, "rm_postprocess: bloom and depth of field are not built");
+    hipStream_t st = (hipStream_t)stream_;
+    const int64_t n = (int64_t)f->w * f->h;
+    PostIn I;
+    I.shapeN = f->buf[RM_BUF_SHAPE_NORMAL].as<float>();
+    I.surfN = f->buf[RM_BUF_SURFACE_NORMAL].as<float>();
+    I.emis = f->buf[RM_BUF_EMISSION].as<float>();
+    I.base = f->buf[RM_BUF_BASE_COLOR].as<float>();
+    I.Dd = f->buf[RM_BUF_DD].as<float>();
+    I.Ds = f->buf[RM_BUF_DS].as<float>();
+    I.Id = f->buf[RM_BUF_ID].as<float>();
+    I.Is = f->buf[RM_BUF_IS].as<float>();
+    float *pixels = f->buf[RM_BUF_PIXELS].as<float>();
+    unsigned g = (unsigned)((n + PB - 1) / PB);
+    if (opt & O_FXAA) {
+        if (f->post.bytes < (size_t)n * 12) {
+            int32_t rc = f->post.alloc((size_t)n * 12);
+            if (rc) return rc;
+        }
+        float *tmp = f->post.as<float>();
+        hipLaunchKernelGGL(k_shade_gamma, dim3(g), dim3(PB), 0, st, I, n, (int)opt, exposure, tmp);
+        hipLaunchKernelGGL(k_fxaa, dim3(g), dim3(PB), 0, st, (const float *)tmp, f->w, f->h, pixels);
+    } else {
+        hipLaunchKernelGGL(k_shade_gamma, dim3(g), dim3(PB), 0, st, I, n, (int)opt, exposure, pixels);
+    }
+    RM_HIP(hipGetLastError());
+    return RM_OK;
 }

@@ -1,6 +1,822 @@
-// Wavefront integrator (work in progress).
+// Wavefront integrator behind rm_render: the reference's per-pixel recursion
+// (renderPixel -> sampleIndirectLightFromFirstIntersection -> sampleRay,
+// src/render.cpp:121-551) restated as kernels over path queues.
+//
+//   k_primary   one lane per pixel: camera ray, closest hit, G-buffer, red
+//               nudge, sample budget (renderPixel, :448-501)
+//   k_direct    one lane per pixel: spp_direct light samples at the primary
+//               vertex with their shadow rays (:425-437,518-522)
+//   per round j of indirect samples:
+//     k_spawn   one lane per accumulator set: the depth-0 vertex of one
+//               indirect sample (:314-397) -> compacted path queue
+//     k_bounce  one lane per live path, once per depth: closest hit and the
+//               whole sampleRay vertex (:121-312) -> next queue, or the path's
+//               light samples straight into its accumulator set
+//   k_finish    one lane per pixel: merge the pixel's sets, drop the outlier
+//               sample, calcVar (:529-550)
+//
+// Sample streams: indirect sample s of pixel p draws from Philox stream
+// (seed, TAG_INDIRECT, p, s), primary light sample T from (seed, TAG_DIRECT,
+// p, T), in the reference's draw order.
+//
+// Accumulation: in round j pixel p runs M_p paths (K for opaque hits, 16K for
+// transparent ones, whose budget is 16x: :498-501), sample s = j*M_p + m into
+// accumulator set m.  A set receives at most one path per round, so sets are
+// updated without atomics and the result is deterministic.  The reference
+// drops a sample iff Clum/(sum-Clum+1e-4) > 16 (:534-547); only the sample with
+// the largest Clum can satisfy that (Clum >= 0), so each set tracks its
+// maximum and its increments, and k_finish subtracts the global maximum's
+// increments when it is dropped.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <vector>
 #include "rm_frame.h"
+#include "shade.h"
+
+namespace rm {
+
+constexpr int RB = 256;      // block size of the render kernels
+constexpr int NCOUNT = 32;   // queue counters per round (>= max_depth + 1)
+
+// ---------------------------------------------------------------- medium
+// Medium (src/render.cpp:13-42): a multimap id -> (ior, absorb) holding air
+// (-1, 1.0, white) plus the entered dielectrics, kept sorted by id with equal
+// ids in insertion order.  Air contributes exact factors, so only the entered
+// media are stored (at most MED_CAP; deeper nesting is dropped).
+constexpr int MED_CAP = 3;
+struct Med {
+    int n;
+    int id[MED_CAP];
+    float ior[MED_CAP];
+    V3 ab[MED_CAP];
+};
+__device__ __forceinline__ float med_ior(const Med &m) {
+    float r = 1.0f;
+    for (int i = 0; i < m.n; ++i) r = gmax(r, m.ior[i]);  // std::max(ior, medium.ior)
+    return r;
+}
+__device__ __forceinline__ V3 med_absorb(const Med &m) {
+    V3 a = v3(1.0f);
+    for (int i = 0; i < m.n; ++i) a *= m.ab[i];
+    return a;
+}
+__device__ __forceinline__ void med_erase(Med &m, int id) {
+    int k = 0;
+    for (int i = 0; i < m.n; ++i)
+        if (m.id[i] != id) {
+            m.id[k] = m.id[i]; m.ior[k] = m.ior[i]; m.ab[k] = m.ab[i]; ++k;
+        }
+    m.n = k;
+}
+__device__ __forceinline__ void med_insert(Med &m, int id, float ior, V3 ab) {
+    if (m.n >= MED_CAP) return;
+    int pos = m.n;
+    while (pos > 0 && m.id[pos - 1] > id) {
+        m.id[pos] = m.id[pos - 1]; m.ior[pos] = m.ior[pos - 1]; m.ab[pos] = m.ab[pos - 1];
+        --pos;
+    }
+    m.id[pos] = id; m.ior[pos] = ior; m.ab[pos] = ab;
+    ++m.n;
+}
+
+// getAbsorb (src/render.cpp:83-87)
+__device__ __forceinline__ V3 get_absorb(V3 a, float dis) {
+    float c = dot(a, W_RGB());
+    if (!(c < 1.0f - T_EPS)) return v3(1.0f);
+    float k = 32.0f * dis;
+    V3 b{a.x < 0.0f ? 0.0f : a.x, a.y < 0.0f ? 0.0f : a.y, a.z < 0.0f ? 0.0f : a.z};
+    return V3{powf(b.x, k), powf(b.y, k), powf(b.z, k)};
+}
+
+// calcEta (src/render.cpp:89-99)
+__device__ __forceinline__ void calc_eta(Med &m, Surf &s) {
+    float e1 = med_ior(m), e2;
+    if (s.entering) {
+        e2 = gmax(e1, s.eta);
+    } else {
+        med_erase(m, s.id);
+        e2 = med_ior(m);
+        med_insert(m, s.id, s.eta, s.base);
+    }
+    s.eta = e1 / e2;
+}
+
+// ---------------------------------------------------------------- buffers
+struct PathQ {
+    float4 *f[12];
+};
+// f0 o.xyz | pixel      f1 d.xyz | sample     f2 T.xyz | W        f3 p0.xyz | rf
+// f4 dPdx  | rng n      f5 dPdy  | set        f6 dDdx  | depth+exclude<<8
+// f7 dDdy  | medium n   f8-f11 medium entries (id, ior, absorb) x 3
+
+struct Path {
+    V3 o, d, T, p0;
+    float W, rf;
+    uint32_t pix, smp, n, set;
+    int depth;
+    bool exclude;
+    Diff diff;
+    Med med;
+};
+
+__device__ __forceinline__ float ub(uint32_t x) { return __uint_as_float(x); }
+__device__ __forceinline__ uint32_t bu(float x) { return __float_as_uint(x); }
+
+__device__ __forceinline__ void store_path(const PathQ &Q, uint32_t q, const Path &P) {
+    Q.f[0][q] = make_float4(P.o.x, P.o.y, P.o.z, ub(P.pix));
+    Q.f[1][q] = make_float4(P.d.x, P.d.y, P.d.z, ub(P.smp));
+    Q.f[2][q] = make_float4(P.T.x, P.T.y, P.T.z, P.W);
+    Q.f[3][q] = make_float4(P.p0.x, P.p0.y, P.p0.z, P.rf);
+    Q.f[4][q] = make_float4(P.diff.dPdx.x, P.diff.dPdx.y, P.diff.dPdx.z, ub(P.n));
+    Q.f[5][q] = make_float4(P.diff.dPdy.x, P.diff.dPdy.y, P.diff.dPdy.z, ub(P.set));
+    Q.f[6][q] = make_float4(P.diff.dDdx.x, P.diff.dDdx.y, P.diff.dDdx.z, ub((uint32_t)P.depth | (P.exclude ? 256u : 0u)));
+    Q.f[7][q] = make_float4(P.diff.dDdy.x, P.diff.dDdy.y, P.diff.dDdy.z, ub((uint32_t)P.med.n));
+    const Med &m = P.med;
+    Q.f[8][q] = make_float4(ub((uint32_t)m.id[0]), m.ior[0], m.ab[0].x, m.ab[0].y);
+    Q.f[9][q] = make_float4(m.ab[0].z, ub((uint32_t)m.id[1]), m.ior[1], m.ab[1].x);
+    Q.f[10][q] = make_float4(m.ab[1].y, m.ab[1].z, ub((uint32_t)m.id[2]), m.ior[2]);
+    Q.f[11][q] = make_float4(m.ab[2].x, m.ab[2].y, m.ab[2].z, 0.0f);
+}
+
+__device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
+    Path P;
+    float4 a = Q.f[0][q], b = Q.f[1][q], c = Q.f[2][q], d = Q.f[3][q];
+    float4 e = Q.f[4][q], f = Q.f[5][q], g = Q.f[6][q], h = Q.f[7][q];
+    P.o = V3{a.x, a.y, a.z}; P.pix = bu(a.w);
+    P.d = V3{b.x, b.y, b.z}; P.smp = bu(b.w);
+    P.T = V3{c.x, c.y, c.z}; P.W = c.w;
+    P.p0 = V3{d.x, d.y, d.z}; P.rf = d.w;
+    P.diff.dPdx = V3{e.x, e.y, e.z}; P.n = bu(e.w);
+    P.diff.dPdy = V3{f.x, f.y, f.z}; P.set = bu(f.w);
+    P.diff.dDdx = V3{g.x, g.y, g.z};
+    uint32_t dx = bu(g.w);
+    P.depth = (int)(dx & 255u);
+    P.exclude = (dx & 256u) != 0;
+    P.diff.dDdy = V3{h.x, h.y, h.z};
+    P.med.n = (int)bu(h.w);
+    if (P.med.n > 0) {
+        float4 m0 = Q.f[8][q], m1 = Q.f[9][q], m2 = Q.f[10][q], m3 = Q.f[11][q];
+        P.med.id[0] = (int)bu(m0.x); P.med.ior[0] = m0.y; P.med.ab[0] = V3{m0.z, m0.w, m1.x};
+        P.med.id[1] = (int)bu(m1.y); P.med.ior[1] = m1.z; P.med.ab[1] = V3{m1.w, m2.x, m2.y};
+        P.med.id[2] = (int)bu(m2.z); P.med.ior[2] = m2.w; P.med.ab[2] = V3{m3.x, m3.y, m3.z};
+    }
+    return P;
+}
+
+// Per-pixel primary-vertex state kept between kernels.
+//   pi0 = nudged base colour, roughness
+//   pi1 = metallic, opacity, eta (absolute), id | entering << 30
+//   pi2 = saved base colour, n_indirect (int bits)
+struct PixBufs {
+    float4 *pi0, *pi1, *pi2;
+    int32_t *M;        // sets per pixel (0: no indirect work)
+    int32_t *set_off;  // exclusive prefix of M
+    int32_t *set_pix;  // set -> pixel
+};
+
+// Accumulator set: 5 float4 = Id, Is, (S, maxC, count, -), maxD, maxS.
+struct AccBufs {
+    float4 *a[5];
+};
+
+struct FrameView {
+    float *pos, *shapeN, *surfN, *emis, *base, *Dd, *Ds, *Id, *Is;
+};
+
+struct RArgs {
+    V3 pos, dir, up, right;
+    float acc, exposure, p_direct;
+    int w, h, spp, max_depth, rr_min, K;
+    uint32_t seed;
+};
+
+__device__ __forceinline__ void st3(float *p, int64_t i, V3 v) {
+    p[i * 3] = v.x;
+    p[i * 3 + 1] = v.y;
+    p[i * 3 + 2] = v.z;
+}
+
+// renderPixel's camera ray and initRayDiff (src/render.cpp:439-477)
+__device__ __forceinline__ void camera_ray(const RArgs &A, int x, int y, V3 &Dir, Diff &diff) {
+    float rx = float(x) - float(A.w) / 2.0f, ry = float(y) - float(A.h) / 2.0f;
+    V3 d = A.dir + A.acc * (rx * A.right + ry * A.up);
+    diff.dPdx = v3(0.0f);
+    diff.dPdy = v3(0.0f);
+    V3 ddx = A.acc * A.right, ddy = A.acc * A.up;
+    float dd = dot(d, d), dx = dot(d, ddx), dy = dot(d, ddy);
+    diff.dDdx = (dd * ddx - d * dx) / (sqrtf(dd) * dd);
+    diff.dDdy = (dd * ddy - d * dy) / (sqrtf(dd) * dd);
+    Dir = normalize(d);
+}
+
+__device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F, int64_t p) {
+    Surf s = surf_default();
+    float4 a = P.pi0[p], b = P.pi1[p];
+    s.pos = ld3(F.pos + p * 3);
+    s.shapeN = ld3(F.shapeN + p * 3);
+    s.surfN = ld3(F.surfN + p * 3);
+    s.emission = ld3(F.emis + p * 3);
+    s.base = V3{a.x, a.y, a.z};
+    s.roughness = a.w;
+    s.metallic = b.x;
+    s.opacity = b.y;
+    s.eta = b.z;
+    uint32_t bits = bu(b.w);
+    s.id = (int)(bits & 0x3fffffffu);
+    s.entering = (bits >> 30) & 1u;
+    return s;
+}
+
+__global__ __launch_bounds__(RB) void k_primary(SceneView S, RArgs A, FrameView F, PixBufs P) {
+    int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (p >= (int64_t)A.w * A.h) return;
+    int x = (int)(p % A.w), y = (int)(p / A.w);
+    V3 Dir;
+    Diff base;
+    camera_ray(A, x, y, Dir, base);
+    RayPre r = ray_pre(A.pos, Dir);
+    float t = INFINITY;
+    TraceCounts tc{};
+    int32_t slot = trace<false, false>(S, r, T_EPS, t, -INFINITY, tc);
+    Surf s = surf_default();
+    float4 zero = make_float4(0, 0, 0, 0);
+    if (slot < 0) {
+        // miss: position NaN, sky emission; nothing else is computed (:479-485)
+        st3(F.pos, p, vnan());
+        st3(F.shapeN, p, vnan());
+        st3(F.surfN, p, vnan());
+        st3(F.emis, p, sky_get(S, Dir));
+        st3(F.base, p, v3(0.0f));
+        P.pi0[p] = zero;
+        P.pi1[p] = zero;
+        P.pi2[p] = zero;
+        P.M[p] = 0;
+        return;
+    }
+    s.pos = A.pos + t * Dir;
+    V3 dPdx, dPdy;
+    hit_info(S, slot, Dir, t, base, dPdx, dPdy, s);
+    bool opaque = s.opacity > 1.0f - T_EPS;
+    V3 sav = s.base;
+    float c0 = dot(s.base, W_RGB());
+    // the reference compares against double literals and adds 4e-2 in double
+    if ((double)c0 < 2e-2 || (c0 < 0.8f && (double)length(s.base / c0 - v3(1.0f)) < 2e-2))
+        s.base.x = (float)((double)s.base.x + 4e-2);
+    int spp_direct = (int)(float(A.spp) * A.p_direct);
+    int n_ind = (A.spp - spp_direct) * (opaque ? 1 : 16);
+    st3(F.pos, p, s.pos);
+    st3(F.shapeN, p, s.shapeN);
+    st3(F.surfN, p, s.surfN);
+    st3(F.emis, p, s.emission);
+    st3(F.base, p, s.base);  // the nudged colour stays when no indirect sample exists (:529-530)
+    P.pi0[p] = make_float4(s.base.x, s.base.y, s.base.z, s.roughness);
+    P.pi1[p] = make_float4(s.metallic, s.opacity, s.eta, ub((uint32_t)s.id | (s.entering ? (1u << 30) : 0u)));
+    P.pi2[p] = make_float4(sav.x, sav.y, sav.z, ub((uint32_t)n_ind));
+    bool indirect = !(length(s.emission) > 0.0f) && n_ind > 0;
+    P.M[p] = indirect ? (opaque ? A.K : 16 * A.K) : 0;
+}
+
+__device__ __forceinline__ void calc_var(Acc4 &a, float e) {
+    a.r *= e; a.g *= e; a.b *= e;
+    a.v *= e * e;
+    a.v -= dot(V3{a.r, a.g, a.b}, V3{a.r, a.g, a.b});
+    if (a.v < 0.0f) a.v = 0.0f;
+}
+
+__global__ __launch_bounds__(RB) void k_direct(SceneView S, RArgs A, FrameView F, PixBufs P) {
+    int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (p >= (int64_t)A.w * A.h) return;
+    Acc4 d{0, 0, 0, 0}, sp{0, 0, 0, 0};
+    V3 pos = ld3(F.pos + p * 3);
+    V3 em = ld3(F.emis + p * 3);
+    if (any_finite(pos) && !(length(em) > 0.0f)) {
+        Surf s = pixel_surf(P, F, p);
+        V3 in = normalize(s.pos - A.pos);
+        V3 V = -in;
+        int cnt = (int)(float(A.spp) * A.p_direct);
+        float total = direct_total(S, s, V);
+        if (total != 0.0f) {
+            TraceCounts tc{};
+            for (int T = 0; T < cnt; ++T) {
+                Stream g{A.seed, TAG_DIRECT, (uint32_t)p, (uint32_t)T, 0u};
+                LSample ls;
+                if (direct_sample<false>(S, s, V, total, cnt, g, ls, tc)) accumulate(s.base, ls, d, sp);
+            }
+        }
+    }
+    calc_var(d, A.exposure);
+    calc_var(sp, A.exposure);
+    reinterpret_cast<float4 *>(F.Dd)[p] = make_float4(d.r, d.g, d.b, d.v);
+    reinterpret_cast<float4 *>(F.Ds)[p] = make_float4(sp.r, sp.g, sp.b, sp.v);
+}
+
+// Wave-aggregated append: one atomic per wave.
+__device__ __forceinline__ uint32_t append(uint32_t *counter, bool want) {
+    uint64_t mask = __ballot(want);
+    if (!want) return 0xffffffffu;
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader, 64);
+    return base + rank;
+}
+
+// The depth-0 vertex (sampleIndirectLightFromFirstIntersection, :314-397).
+__global__ __launch_bounds__(RB) void k_spawn(SceneView S, RArgs A, FrameView F, PixBufs P, int round, uint32_t total_sets,
+                                              PathQ Q, uint32_t *count) {
+    uint32_t g = blockIdx.x * RB + threadIdx.x;
+    bool want = false;
+    Path path;
+    if (g < total_sets) {
+        int64_t p = P.set_pix[g];
+        int M = P.M[p];
+        int m = (int)(g - (uint32_t)P.set_off[p]);
+        uint32_t n_ind = bu(P.pi2[p].w);
+        uint32_t s_idx = (uint32_t)round * (uint32_t)M + (uint32_t)m;
+        if (s_idx < n_ind) {
+            Surf s = pixel_surf(P, F, p);
+            int x = (int)(p % A.w), y = (int)(p / A.w);
+            V3 Dir;
+            Diff cam;
+            camera_ray(A, x, y, Dir, cam);
+            V3 in = normalize(s.pos - A.pos);
+            float hit_t = length(s.pos - A.pos);
+            V3 V = -in;
+            float rf = s.roughness * 1.0f;
+            float ior = s.eta;
+            s.eta = 1.0f / s.eta;
+            float P_reflect = 1.0f, Fr = 0.0f;
+            if (s.opacity < T_EPS) {
+                V3 rd;
+                precise_refraction(s, V, rd, Fr);
+                P_reflect = 0.24f + (1.0f - 0.24f) * Fr;
+            }
+            Stream rng{A.seed, TAG_INDIRECT, (uint32_t)p, s_idx, 0u};
+            Med med;
+            med.n = 0;
+            V3 nd, bp;
+            int fails = 0;
+            Diff next;
+            if (rng.next() < P_reflect) {
+                calc_dPdxy(in, hit_t, s.shapeN, cam, next.dPdx, next.dPdy);
+                calc_dDdxy(in, s.surfN, cam, next.dDdx, next.dDdy);
+                sample_reflection(s, V, rng, nd, bp, fails);
+                bp /= P_reflect;
+            } else {
+                sample_btdf(s, V, rng, nd, bp, fails);
+                bp *= (1.0f - Fr);
+                bp /= (1.0f - P_reflect);
+                next = cam;
+                if (s.entering)
+                    med_insert(med, s.id, ior, s.base);
+                else
+                    med_erase(med, s.id);
+            }
+            if (any_finite(nd)) {
+                want = true;
+                path.o = s.pos;
+                path.d = nd;
+                path.T = v3(1.0f);
+                path.p0 = bp;
+                path.W = fails > 0 ? 1.0f / (float)(1 + fails) : 1.0f;
+                path.rf = rf;
+                path.pix = (uint32_t)p;
+                path.smp = s_idx;
+                path.n = rng.n;
+                path.set = g;
+                path.depth = 1;
+                path.exclude = true;
+                path.diff = next;
+                path.med = med;
+            }
+        }
+    }
+    uint32_t q = append(count, want);
+    if (want) store_path(Q, q, path);
+}
+
+// Light sample of a finished path -> its accumulator set.
+struct SetAcc {
+    Acc4 d, s;
+    float S, maxC, cnt;
+    Acc4 md, ms;
+};
+
+__device__ __forceinline__ void add_sample(SetAcc &a, V3 base, V3 p0, V3 light, float w) {
+    float c = dot(p0 * light, W_RGB()) * w;
+    Acc4 dd{0, 0, 0, 0}, ds{0, 0, 0, 0};
+    accumulate(base, LSample{p0, light, w}, dd, ds);
+    a.d.r += dd.r; a.d.g += dd.g; a.d.b += dd.b; a.d.v += dd.v;
+    a.s.r += ds.r; a.s.g += ds.g; a.s.b += ds.b; a.s.v += ds.v;
+    a.S += c;
+    a.cnt += 1.0f;
+    if (c > a.maxC) {
+        a.maxC = c;
+        a.md = dd;
+        a.ms = ds;
+    }
+}
+
+__device__ __forceinline__ SetAcc load_set(const AccBufs &B, uint32_t g) {
+    SetAcc a;
+    float4 x0 = B.a[0][g], x1 = B.a[1][g], x2 = B.a[2][g], x3 = B.a[3][g], x4 = B.a[4][g];
+    a.d = Acc4{x0.x, x0.y, x0.z, x0.w};
+    a.s = Acc4{x1.x, x1.y, x1.z, x1.w};
+    a.S = x2.x; a.maxC = x2.y; a.cnt = x2.z;
+    a.md = Acc4{x3.x, x3.y, x3.z, x3.w};
+    a.ms = Acc4{x4.x, x4.y, x4.z, x4.w};
+    return a;
+}
+__device__ __forceinline__ void store_set(const AccBufs &B, uint32_t g, const SetAcc &a) {
+    B.a[0][g] = make_float4(a.d.r, a.d.g, a.d.b, a.d.v);
+    B.a[1][g] = make_float4(a.s.r, a.s.g, a.s.b, a.s.v);
+    B.a[2][g] = make_float4(a.S, a.maxC, a.cnt, 0.0f);
+    B.a[3][g] = make_float4(a.md.r, a.md.g, a.md.b, a.md.v);
+    B.a[4][g] = make_float4(a.ms.r, a.ms.g, a.ms.b, a.ms.v);
+}
+
+__constant__ int c_sample_count[17] = {0, 1, 2, 2, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 5, 6};  // src/render.cpp:126
+
+// One sampleRay vertex (src/render.cpp:121-312) for every live path.
+__global__ __launch_bounds__(RB) void k_bounce(SceneView S, RArgs A, PixBufs P, AccBufs AB, PathQ Qin,
+                                               const uint32_t *count_in, PathQ Qout, uint32_t *count_out) {
+    uint32_t q = blockIdx.x * RB + threadIdx.x;
+    uint32_t n_in = *count_in;
+    bool live = q < n_in;
+    bool cont = false;
+    Path path;
+    if (live) {
+        path = load_path(Qin, q);
+        Stream rng{A.seed, TAG_INDIRECT, path.pix, path.smp, path.n};
+        const int depth = path.depth;
+        const int maxD = A.max_depth;
+        RayPre r = ray_pre(path.o, path.d);
+        float t = INFINITY;
+        TraceCounts tc{};
+        int32_t slot = trace<false, false>(S, r, T_EPS, t, -INFINITY, tc);
+        // light samples produced at this vertex (<= 6)
+        LSample ls[6];
+        int nls = 0;
+        V3 extra = v3(1.0f);  // refract-side NEE multiplies by absorb after the samples (passBsdf)
+        bool extra_on = false;
+        if (slot < 0) {
+            if (!(path.exclude || !S.sky)) ls[nls++] = LSample{v3(1.0f), sky_get(S, path.d), 1.0f};
+        } else {
+            Surf s = surf_default();
+            s.pos = path.o + path.d * t;
+            V3 dPdx, dPdy;
+            hit_info(S, slot, path.d, t, path.diff, dPdx, dPdy, s);
+            V3 V = -path.d;
+            float ior = s.eta;
+            path.rf = gmax(path.rf, 1.0f * s.roughness);
+            s.roughness = gmax(s.roughness, path.rf);
+            const float P_RR = 1.0f + (0.5f - 1.0f) * sqrtf(s.roughness);
+            bool doNEE = P_RR < 0.9f && depth >= A.rr_min;
+            V3 absorb = get_absorb(med_absorb(path.med), t);
+            if (length(s.emission) > T_EPS) {
+                if (!path.exclude) ls[nls++] = LSample{absorb, s.emission, 1.0f};
+            } else {
+                float P_reflect = 1.0f, Fr = 0.0f;
+                if (s.opacity < T_EPS) {
+                    calc_eta(path.med, s);
+                    V3 rd;
+                    precise_refraction(s, V, rd, Fr);
+                    P_reflect = path.med.n == 0 ? 0.24f + (1.0f - 0.24f) * Fr : Fr;
+                    P_reflect = gmax(P_reflect - 1e-3f, 0.0f);
+                }
+                V3 nd = vnan(), bp = vnan();
+                int fails = 0;
+                Diff next;
+                bool nee = false;
+                float factor = 1.0f;
+                if (rng.next() <= P_reflect) {
+                    doNEE = doNEE && s.entering && path.med.n == 0;
+                    if (doNEE && (depth >= maxD || rng.next() > P_RR)) {
+                        nee = true;
+                        factor = 1.0f / P_reflect;
+                        if (depth < maxD) factor /= (1.0f - P_RR);
+                    } else {
+                        sample_reflection(s, V, rng, nd, bp, fails);
+                        bp /= P_reflect;
+                        if (doNEE) bp /= P_RR;
+                        next.dPdx = dPdx;
+                        next.dPdy = dPdy;
+                        calc_dDdxy(path.d, s.surfN, path.diff, next.dDdx, next.dDdy);
+                    }
+                } else {
+                    doNEE = doNEE && !s.entering && path.med.n == 1;
+                    if (doNEE && (depth >= maxD || rng.next() > P_RR)) {
+                        nee = true;
+                        factor = 1.0f / (1.0f - P_reflect);
+                        if (depth < maxD) factor /= (1.0f - P_RR);
+                        extra = absorb;
+                        extra_on = true;
+                    } else {
+                        if (fabsf(s.eta - 1.0f) < T_EPS) {
+                            nd = path.d;
+                            bp = v3(1.0f);
+                        } else {
+                            sample_btdf(s, V, rng, nd, bp, fails);
+                            bp *= (1.0f - Fr);
+                        }
+                        next = path.diff;
+                        bp /= (1.0f - P_reflect);
+                        if (doNEE) bp /= P_RR;
+                        if (s.entering)
+                            med_insert(path.med, s.id, ior, s.base);
+                        else
+                            med_erase(path.med, s.id);
+                    }
+                }
+                if (nee) {
+                    int cnt = c_sample_count[depth];
+                    float total = direct_total(S, s, V);
+                    if (total != 0.0f) {
+                        for (int T = 0; T < cnt; ++T) {
+                            LSample x;
+                            if (direct_sample<false>(S, s, V, total, cnt, rng, x, tc)) {
+                                x.bsdf *= factor;  // scaling(samples, factor)
+                                ls[nls++] = x;
+                            }
+                        }
+                    }
+                } else if (any_finite(nd) && depth != maxD) {
+                    bp *= absorb;
+                    cont = true;
+                    path.T = path.T * bp;
+                    if (fails > 0) path.W *= 1.0f / (float)(1 + fails);
+                    path.exclude = doNEE;
+                    path.depth = depth + 1;
+                    path.o = s.pos;
+                    path.d = nd;
+                    path.diff = next;
+                    path.n = rng.n;
+                }
+            }
+        }
+        if (nls > 0) {
+            // passBsdf up to the first vertex: light *= terminal bsdf (and, on the
+            // refraction side, the medium's absorb) and every deeper bsdfPdf
+            uint32_t p = path.pix;
+            float4 b0 = P.pi0[p], b1 = P.pi1[p];
+            V3 base = b1.y < T_EPS ? v3(0.0f) : V3{b0.x, b0.y, b0.z};
+            float inv = 1.0f / (float)bu(P.pi2[p].w);
+            SetAcc a = load_set(AB, path.set);
+            for (int k = 0; k < nls; ++k) {
+                V3 L = ls[k].light * ls[k].bsdf;
+                if (extra_on) L = L * extra;
+                L = L * path.T;
+                add_sample(a, base, path.p0, L, ls[k].weight * path.W * inv);
+            }
+            store_set(AB, path.set, a);
+        } else if (!cont) {
+            // a path that ends without light samples still counts toward
+            // `samples.empty()` only through other samples: nothing to do
+        }
+    }
+    uint32_t slot_out = append(count_out, cont);
+    if (cont) store_path(Qout, slot_out, path);
+}
+
+__global__ __launch_bounds__(RB) void k_clear_sets(AccBufs AB, uint32_t n) {
+    uint32_t g = blockIdx.x * RB + threadIdx.x;
+    if (g >= n) return;
+    float4 z = make_float4(0, 0, 0, 0);
+    AB.a[0][g] = z;
+    AB.a[1][g] = z;
+    AB.a[2][g] = make_float4(0.0f, -1.0f, 0.0f, 0.0f);
+    AB.a[3][g] = z;
+    AB.a[4][g] = z;
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(RB) void k_finish(RArgs A, FrameView F, PixBufs P, AccBufs AB) {
+    int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (p >= (int64_t)A.w * A.h) return;
+    int M = P.M[p];
+    Acc4 d{0, 0, 0, 0}, s{0, 0, 0, 0};
+    if (M > 0) {
+        int off = P.set_off[p];
+        float S = 0.0f, cnt = 0.0f, maxC = -1.0f;
+        Acc4 md{0, 0, 0, 0}, ms{0, 0, 0, 0};
+        for (int k = 0; k < M; ++k) {
+            SetAcc a = load_set(AB, (uint32_t)(off + k));
+            d.r += a.d.r; d.g += a.d.g; d.b += a.d.b; d.v += a.d.v;
+            s.r += a.s.r; s.g += a.s.g; s.b += a.s.b; s.v += a.s.v;
+            S += a.S;
+            cnt += a.cnt;
+            if (a.maxC > maxC) { maxC = a.maxC; md = a.md; ms = a.ms; }
+        }
+        if (cnt > 0.0f) {
+            if (maxC >= 0.0f && maxC / (S - maxC + T_EPS) > 16.0f) {
+                d.r -= md.r; d.g -= md.g; d.b -= md.b; d.v -= md.v;
+                s.r -= ms.r; s.g -= ms.g; s.b -= ms.b; s.v -= ms.v;
+            }
+            calc_var(d, A.exposure);
+            calc_var(s, A.exposure);
+            float4 sv = P.pi2[p];
+            st3(F.base, p, V3{sv.x, sv.y, sv.z});
+        }
+    }
+    reinterpret_cast<float4 *>(F.Id)[p] = make_float4(d.r, d.g, d.b, d.v);
+    reinterpret_cast<float4 *>(F.Is)[p] = make_float4(s.r, s.g, s.b, s.v);
+}
+
+// Exclusive prefix of M over pixels, two passes with the block totals scanned
+// on the host, then the set -> pixel map.
+constexpr int SB = 1024;
+__global__ __launch_bounds__(SB) void k_block_sums(const int32_t *M, int64_t n, int64_t *sums) {
+    __shared__ int64_t sh[SB / 64];
+    int64_t i = (int64_t)blockIdx.x * SB + threadIdx.x;
+    int64_t v = i < n ? M[i] : 0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int k = 0; k < SB / 64; ++k) t += sh[k];
+        sums[blockIdx.x] = t;
+    }
+}
+__global__ __launch_bounds__(SB) void k_block_scan(const int32_t *M, int64_t n, const int64_t *block_off, int32_t *off,
+                                                   int32_t *set_pix) {
+    __shared__ int32_t sh[SB];
+    int64_t i = (int64_t)blockIdx.x * SB + threadIdx.x;
+    int32_t v = i < n ? M[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < SB; s <<= 1) {
+        int32_t add = threadIdx.x >= (unsigned)s ? sh[threadIdx.x - s] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    if (i < n) {
+        int32_t o = (int32_t)block_off[blockIdx.x] + sh[threadIdx.x] - v;
+        off[i] = o;
+        for (int k = 0; k < v; ++k) set_pix[o + k] = (int32_t)i;
+    }
+}
+
+template <class T>
+T *carve(char *&p, size_t count) {
+    T *r = reinterpret_cast<T *>(p);
+    p += (count * sizeof(T) + 255) & ~(size_t)255;
+    return r;
+}
+
+}  // namespace rm
+
 using namespace rm;
-extern "C" int32_t rm_render(const rm_scene *, const rm_render_args *, rm_frame *, void *) {
-    return fail(RM_ERR_UNSUPPORTED, "rm_render: not built yet");
+
+static int32_t ensure_work(rm_frame *f, size_t bytes) {
+    if (f->work.bytes >= bytes) return RM_OK;
+    return f->work.alloc(bytes);
+}
+
+extern "C" int32_t rm_render(const rm_scene *scene, const rm_render_args *a, rm_frame *f, void *stream_) {
+    if (!scene || !a || !f) return fail(RM_ERR_INVALID, "rm_render: NULL argument");
+    if (a->width != f->w || a->height != f->h)
+        return fail(RM_ERR_INVALID, "rm_render: frame is %dx%d, args ask %dx%d", f->w, f->h, a->width, a->height);
+    if (a->spp < 0) return fail(RM_ERR_INVALID, "rm_render: negative spp");
+    int max_depth = a->max_depth > 0 ? a->max_depth : 16;
+    if (max_depth > 16) return fail(RM_ERR_INVALID, "rm_render: max_depth %d exceeds 16 (sampleCount table)", max_depth);
+    if (!(a->p_direct >= 0.0f && a->p_direct <= 1.0f)) return fail(RM_ERR_INVALID, "rm_render: p_direct outside [0,1]");
+    if (scene->view.depth > MAX_STACK) return fail(RM_ERR_UNSUPPORTED, "rm_render: tree too deep");
+    hipStream_t st = (hipStream_t)stream_;
+    const int64_t np = (int64_t)f->w * f->h;
+    RArgs A;
+    A.pos = V3{a->position[0], a->position[1], a->position[2]};
+    A.dir = V3{a->direction[0], a->direction[1], a->direction[2]};
+    A.up = V3{a->up[0], a->up[1], a->up[2]};
+    A.right = V3{a->right[0], a->right[1], a->right[2]};
+    A.acc = a->accuracy;
+    A.exposure = a->exposure;
+    A.p_direct = a->p_direct;
+    A.w = a->width;
+    A.h = a->height;
+    A.spp = a->spp;
+    A.max_depth = max_depth;
+    A.rr_min = a->rr_min_depth > 0 ? a->rr_min_depth : 1;
+    A.seed = a->seed;
+    int K = a->paths_per_round;
+    if (K <= 0) K = (int)std::min<int64_t>(16, std::max<int64_t>(1, (int64_t)(4 << 20) / std::max<int64_t>(1, np)));
+    A.K = K;
+    const int spp_direct = (int)(float(a->spp) * a->p_direct);
+    const int n_opaque = a->spp - spp_direct;
+    const int rounds = n_opaque > 0 ? (n_opaque + K - 1) / K : 0;
+
+    FrameView F;
+    F.pos = f->buf[RM_BUF_POSITION].as<float>();
+    F.shapeN = f->buf[RM_BUF_SHAPE_NORMAL].as<float>();
+    F.surfN = f->buf[RM_BUF_SURFACE_NORMAL].as<float>();
+    F.emis = f->buf[RM_BUF_EMISSION].as<float>();
+    F.base = f->buf[RM_BUF_BASE_COLOR].as<float>();
+    F.Dd = f->buf[RM_BUF_DD].as<float>();
+    F.Ds = f->buf[RM_BUF_DS].as<float>();
+    F.Id = f->buf[RM_BUF_ID].as<float>();
+    F.Is = f->buf[RM_BUF_IS].as<float>();
+
+    // per-pixel part of the work buffer
+    const int64_t nblk = (np + SB - 1) / SB;
+    size_t pix_bytes = 0;
+    {
+        size_t s = 0;
+        auto add = [&](size_t b) { s += (b + 255) & ~(size_t)255; };
+        add(np * 16); add(np * 16); add(np * 16);
+        add(np * 4); add(np * 4); add(nblk * 8); add(NCOUNT * 4);
+        pix_bytes = s;
+    }
+    int32_t rc = ensure_work(f, pix_bytes);
+    if (rc) return rc;
+    char *cur = f->work.as<char>();
+    PixBufs P;
+    P.pi0 = carve<float4>(cur, np);
+    P.pi1 = carve<float4>(cur, np);
+    P.pi2 = carve<float4>(cur, np);
+    P.M = carve<int32_t>(cur, np);
+    P.set_off = carve<int32_t>(cur, np);
+    int64_t *bsum = carve<int64_t>(cur, nblk);
+    // counters[d] = live paths entering depth d+1 (d = 0..max_depth), max_depth <= 16
+    uint32_t *counters = carve<uint32_t>(cur, NCOUNT);
+
+    unsigned gp = (unsigned)((np + RB - 1) / RB);
+    hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), 0, st, scene->view, A, F, P);
+    RM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, np, bsum);
+    std::vector<int64_t> hs(nblk);
+    RM_HIP(hipMemcpyAsync(hs.data(), bsum, nblk * 8, hipMemcpyDeviceToHost, st));
+    RM_HIP(hipStreamSynchronize(st));
+    int64_t total = 0;
+    for (int64_t b = 0; b < nblk; ++b) {
+        int64_t v = hs[b];
+        hs[b] = total;
+        total += v;
+    }
+    if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "rm_render: %lld path slots per round", (long long)total);
+    // second part: set map, accumulators, two path queues
+    size_t set_bytes = 0;
+    {
+        size_t s = 0;
+        auto add = [&](size_t b) { s += (b + 255) & ~(size_t)255; };
+        add(total * 4);
+        for (int k = 0; k < 5; ++k) add(total * 16);
+        for (int k = 0; k < 24; ++k) add(total * 16);
+        set_bytes = s;
+    }
+    if (f->work.bytes < pix_bytes + set_bytes) {
+        // grow and re-run the primary pass (the buffer holds its results)
+        rc = f->work.alloc(pix_bytes + set_bytes);
+        if (rc) return rc;
+        return rm_render(scene, a, f, stream_);
+    }
+    RM_HIP(hipMemcpyAsync(bsum, hs.data(), nblk * 8, hipMemcpyHostToDevice, st));
+    P.set_pix = carve<int32_t>(cur, total);
+    AccBufs AB;
+    for (int k = 0; k < 5; ++k) AB.a[k] = carve<float4>(cur, total);
+    PathQ Q[2];
+    for (int qi = 0; qi < 2; ++qi)
+        for (int k = 0; k < 12; ++k) Q[qi].f[k] = carve<float4>(cur, total);
+    hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, np, bsum, P.set_off, P.set_pix);
+    RM_HIP(hipGetLastError());
+
+    hipLaunchKernelGGL(k_direct, dim3(gp), dim3(RB), 0, st, scene->view, A, F, P);
+    RM_HIP(hipGetLastError());
+
+    const uint32_t nsets = (uint32_t)total;
+    if (nsets > 0) {
+        unsigned gs = (nsets + RB - 1) / RB;
+        hipLaunchKernelGGL(k_clear_sets, dim3(gs), dim3(RB), 0, st, AB, nsets);
+        uint32_t *hcount = nullptr;
+        RM_HIP(hipHostMalloc((void **)&hcount, 64, hipHostMallocDefault));
+        for (int j = 0; j < rounds; ++j) {
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counters, NCOUNT);
+            hipLaunchKernelGGL(k_spawn, dim3(gs), dim3(RB), 0, st, scene->view, A, F, P, j, nsets, Q[0], counters);
+            RM_HIP(hipGetLastError());
+            uint32_t live = nsets;
+            int cur_q = 0;
+            for (int depth = 1; depth <= max_depth; ++depth) {
+                RM_HIP(hipMemcpyAsync(hcount, counters + (depth - 1), 4, hipMemcpyDeviceToHost, st));
+                RM_HIP(hipStreamSynchronize(st));
+                live = *hcount;
+                if (live == 0) break;
+                unsigned gb = (live + RB - 1) / RB;
+                hipLaunchKernelGGL(k_bounce, dim3(gb), dim3(RB), 0, st, scene->view, A, P, AB, Q[cur_q],
+                                   counters + (depth - 1), Q[cur_q ^ 1], counters + depth);
+                RM_HIP(hipGetLastError());
+                cur_q ^= 1;
+            }
+        }
+        RM_HIP(hipStreamSynchronize(st));
+        (void)hipHostFree(hcount);
+    }
+    hipLaunchKernelGGL(k_finish, dim3(gp), dim3(RB), 0, st, A, F, P, AB);
+    RM_HIP(hipGetLastError());
+    return RM_OK;
 }

@@ -12,4 +12,5 @@ struct rm_frame {
     // integrator scratch (path queues, per-pixel accumulators), grown on demand
     rm::DevBuf work;
     size_t work_bytes = 0;
+    rm::DevBuf post;  // tone-mapped image read by FXAA
 };

@@ -111,7 +111,7 @@ void camera(rm_render_args &a, const double pos[3], const double target[3], int 
         a.position[j] = (float)pos[j];
         a.direction[j] = (float)d[j];
         a.right[j] = (float)r[j];
-        a.up[j] = (float)(-u[j]);
+        a.up[j] = (float)u[j];
     }
     a.width = w;
     a.height = h;

@@ -1,0 +1,611 @@
+// gfx950 shading: hit information, texture fetch, the reference BSDF and its
+// samplers, and next-event estimation.  Each function follows the reference
+// function it names, in the same fp32 operation order (built with
+// -ffp-contract=off); only transcendental functions (sin, cos, log, pow, acos,
+// atan2, tanh) come from the ROCm device library instead of glibc.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rm_layout.h"
+#include "rm_vec.h"
+#include "traverse.h"
+
+namespace rm {
+
+constexpr float PI_F = 3.14159265358979323846f;
+__device__ __forceinline__ V3 W_RGB() { return V3{0.3f, 0.6f, 0.1f}; }  // RGB_Weight
+__device__ __forceinline__ V3 ld3(const float *p) { return V3{p[0], p[1], p[2]}; }
+__device__ __forceinline__ V3 vnan() { return V3{NAN, NAN, NAN}; }
+
+struct V2 { float x, y; };
+__device__ __forceinline__ V2 operator+(V2 a, V2 b) { return V2{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ V2 operator-(V2 a, V2 b) { return V2{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ V2 operator*(float s, V2 a) { return V2{s * a.x, s * a.y}; }
+__device__ __forceinline__ float len2d(V2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+__device__ __forceinline__ bool any_finite2(V2 v) { return isfinite(v.x) || isfinite(v.y); }
+
+// Shading state at a vertex (HitInfo, include/model.h:12-18, defaults from
+// src/model.cpp:4-7).
+struct Surf {
+    V3 shapeN, surfN, emission, base, pos;
+    float specular, roughness, metallic, opacity, eta;
+    int id;
+    bool entering;
+};
+
+__device__ __forceinline__ Surf surf_default() {
+    Surf s;
+    s.pos = vnan();
+    s.shapeN = vnan();
+    s.surfN = vnan();
+    s.opacity = 1.0f;
+    s.specular = 0.04f;
+    s.roughness = 0.8f;
+    s.metallic = 0.0f;
+    s.eta = 1.0f;
+    s.emission = v3(0.0f);
+    s.base = v3(0.0f);
+    s.entering = true;
+    s.id = 0;
+    return s;
+}
+
+struct Diff { V3 dPdx, dPdy, dDdx, dDdy; };  // RayDifferential
+
+// barycentric (src/geometry.cpp:89-103)
+__device__ __forceinline__ V3 barycentric(V3 v0, V3 v1, V3 v2, V3 P) {
+    V3 a = v1 - v0, b = v2 - v0;
+    V3 n = cross(a, b);
+    float denom = dot(n, n);
+    V3 p = P - v0;
+    float alpha = dot(cross(p, b), n) / denom;
+    float beta = dot(cross(a, p), n) / denom;
+    float gamma = 1.0f - alpha - beta;
+    return V3{gamma, alpha, beta};
+}
+
+// getTangentSpace / getTangentSpaceWithInDir (src/geometry.cpp:105-123)
+__device__ __forceinline__ void tangent_space(V3 n, V3 &t, V3 &b) {
+    V3 v0 = fabsf(n.x) < 0.8f ? V3{1.0f, 0.0f, 0.0f} : V3{0.0f, 1.0f, 0.0f};
+    t = normalize(cross(v0, n));
+    b = cross(n, t);
+}
+__device__ __forceinline__ void tangent_space_in(V3 n, V3 in, V3 &t, V3 &b) {
+    float c = dot(n, in);
+    if (c < -1.0f + 1e-3f) {
+        tangent_space(n, t, b);
+        return;
+    }
+    b = normalize(cross(n, in));
+    t = cross(b, n);
+}
+
+// reverseFix (src/model.cpp:232-238)
+__device__ __forceinline__ bool reverse_fix(V3 &v, V3 dir) {
+    if (dot(v, dir) < 0.0f) {
+        v = v * -1.0f;
+        return false;
+    }
+    return true;
+}
+
+// getHitNormals (src/model.cpp:240-270)
+__device__ __forceinline__ void hit_normals(const TriShade &F, V3 in_dir, V3 bc, V3 &shapeN, V3 &surfN, bool &entering) {
+    V3 v0 = ld3(F.v[0]), v1 = ld3(F.v[1]), v2 = ld3(F.v[2]);
+    V3 c = cross(v1 - v0, v2 - v0);
+    shapeN = normalize(c);
+    entering = reverse_fix(shapeN, -in_dir);
+    surfN = shapeN;
+    float area = length(c) / 2.0f;
+    if (area > 1e-2f) return;
+    V3 n0 = ld3(F.n[0]), n1 = ld3(F.n[1]), n2 = ld3(F.n[2]);
+    reverse_fix(n0, shapeN);
+    reverse_fix(n1, shapeN);
+    reverse_fix(n2, shapeN);
+    V3 a = bc.x * (dot(n0, shapeN) > 0.85f ? n0 : shapeN);
+    V3 b = bc.y * (dot(n1, shapeN) > 0.85f ? n1 : shapeN);
+    V3 d = bc.z * (dot(n2, shapeN) > 0.85f ? n2 : shapeN);
+    surfN = normalize(a + b + d);
+    if (!any_finite(surfN)) surfN = shapeN;
+}
+
+// ImageData::get_bilinear + get (src/material.cpp:48-94) on fp32 texels
+__device__ __forceinline__ V3 tex_bilinear(const float *data, int w, int h, float u, float v) {
+    float x = u * float(w), y = v * float(h);
+    int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    float dx = x - float(x0), dy = y - float(y0);
+    if (x0 < 0 || x0 >= w) { x0 %= w; if (x0 < 0) x0 += w; }
+    if (y0 < 0 || y0 >= h) { y0 %= h; if (y0 < 0) y0 += h; }
+    int x1 = (x0 + 1) % w, y1 = (y0 + 1) % h;
+    V3 c00 = ld3(data + ((int64_t)y0 * w + x0) * 3), c01 = ld3(data + ((int64_t)y0 * w + x1) * 3);
+    V3 c10 = ld3(data + ((int64_t)y1 * w + x0) * 3), c11 = ld3(data + ((int64_t)y1 * w + x1) * 3);
+    V3 c0 = c00 * (1.0f - dx) + c01 * dx;
+    V3 c1 = c10 * (1.0f - dx) + c11 * dx;
+    return c0 * (1.0f - dy) + c1 * dy;
+}
+
+__device__ __forceinline__ V3 tex_get(const SceneView &S, const TexRec &T, float u, float v, float depth) {
+    v = 1.0f - v;
+    float hi = float(T.levels - 1);
+    depth = gmax(gmin(depth, hi), 0.0f);  // std::max(std::min(depth, hi), 0)
+    int level = (int)depth;
+    int next = level + 1 < T.levels - 1 ? level + 1 : T.levels - 1;
+    float blend = depth - float(level);
+    V3 a = tex_bilinear(S.texels + T.offset[level], T.width >> level, T.height >> level, u, v);
+    V3 b = tex_bilinear(S.texels + T.offset[next], T.width >> next, T.height >> next, u, v);
+    return a * (1.0f - blend) + b * blend;
+}
+
+// getDuv (src/model.cpp:272-277)
+__device__ __forceinline__ V2 get_duv(const TriShade &F, V3 dP) {
+    V3 v0 = ld3(F.v[0]);
+    V3 b = barycentric(v0, ld3(F.v[1]), ld3(F.v[2]), v0 + dP);
+    V2 u0{F.uv[0][0], F.uv[0][1]}, u1{F.uv[1][0], F.uv[1][1]}, u2{F.uv[2][0], F.uv[2][1]};
+    return ((b.x - 1.0f) * u0 + b.y * u1) + b.z * u2;
+}
+
+// calcSurfaceNormal (src/model.cpp:279-298) with a zero normal map
+__device__ __forceinline__ void calc_surface_normal(const TriShade &F, V3 shapeN, V3 &surfN) {
+    V3 v0 = ld3(F.v[0]);
+    V3 e1 = ld3(F.v[1]) - v0, e2 = ld3(F.v[2]) - v0;
+    V2 d1{F.uv[1][0] - F.uv[0][0], F.uv[1][1] - F.uv[0][1]};
+    V2 d2{F.uv[2][0] - F.uv[0][0], F.uv[2][1] - F.uv[0][1]};
+    float f = 1.0f / (d1.x * d2.y - d2.x * d1.y);
+    V3 tu = f * (d2.y * e1 - d1.y * e2), tv = f * (-d2.x * e1 + d1.x * e2);
+    V3 t = normalize(tu - shapeN * dot(shapeN, tu));
+    V3 b = normalize(tv - shapeN * dot(shapeN, tv) - t * dot(t, tv));
+    V3 sav = surfN;
+    surfN = normalize(t * 0.0f + b * 0.0f + surfN);
+    if (!any_finite(surfN)) surfN = sav;
+}
+
+// getHitMaterial (src/model.cpp:300-328)
+__device__ __forceinline__ void hit_material(const SceneView &S, const TriShade &F, V3 bc, V3 dPdx, V3 dPdy, Surf &h) {
+    V2 u0{F.uv[0][0], F.uv[0][1]}, u1{F.uv[1][0], F.uv[1][1]}, u2{F.uv[2][0], F.uv[2][1]};
+    V2 uv = (bc.x * u0 + bc.y * u1) + bc.z * u2;
+    const MatRec &M = S.mats[F.material];
+    h.id = F.material;
+    h.roughness = M.roughness;  // getSurfaceData without a specular map
+    h.metallic = M.metallic;
+    h.opacity = M.opacity;
+    h.eta = M.ior;
+    if (h.opacity > 1.0f - T_EPS) h.entering = true;
+    V2 ddx = get_duv(F, dPdx), ddy = get_duv(F, dPdy);
+    float duv = any_finite2(ddx) ? (len2d(ddx) + len2d(ddy)) / 2.0f : NAN;
+    if (h.opacity < T_EPS) {
+        h.base = ld3(M.trans);
+    } else if (M.tex >= 0) {
+        const TexRec &T = S.texs[M.tex];
+        float depth = isnan(duv) ? 0.0f : log2f(duv * float(T.width));
+        h.base = tex_get(S, T, uv.x, uv.y, depth);
+    } else {
+        h.base = ld3(M.base);
+    }
+    h.emission = ld3(M.emission);
+    calc_surface_normal(F, h.shapeN, h.surfN);
+}
+
+// calc_dPdxy (src/render.cpp:44-50)
+__device__ __forceinline__ void calc_dPdxy(V3 dir, float t, V3 n, const Diff &b, V3 &dPdx, V3 &dPdy) {
+    float dtdx = -dot(b.dPdx + t * b.dDdx, n) / dot(dir, n);
+    float dtdy = -dot(b.dPdy + t * b.dDdy, n) / dot(dir, n);
+    dPdx = (b.dPdx + dtdx * dir) + t * b.dDdx;
+    dPdy = (b.dPdy + dtdy * dir) + t * b.dDdy;
+}
+
+// calc_dDdxy (src/render.cpp:52-60); the normal derivatives are zero
+__device__ __forceinline__ void calc_dDdxy(V3 dir, V3 n, const Diff &b, V3 &dDdx, V3 &dDdy) {
+    V3 z = v3(0.0f);
+    float ax = dot(z, dir) + dot(b.dDdx, n);
+    float ay = dot(z, dir) + dot(b.dDdy, n);
+    dDdx = b.dDdx - 2.0f * (dot(dir, n) * z + ax * n);
+    dDdy = b.dDdy - 2.0f * (dot(dir, n) * z + ay * n);
+}
+
+// getHitInfo (src/render.cpp:62-79); h.pos must be set by the caller
+__device__ __forceinline__ void hit_info(const SceneView &S, int32_t slot, V3 dir, float t, const Diff &base, V3 &dPdx,
+                                         V3 &dPdy, Surf &h) {
+    const TriShade &F = S.shade[slot];
+    V3 bc = barycentric(ld3(F.v[0]), ld3(F.v[1]), ld3(F.v[2]), h.pos);
+    hit_normals(F, dir, bc, h.shapeN, h.surfN, h.entering);
+    V3 raw = h.surfN;
+    calc_dPdxy(dir, t, h.shapeN, base, dPdx, dPdy);
+    hit_material(S, F, bc, dPdx, dPdy, h);
+    if (dot(h.surfN, dir) >= 0.0f) h.surfN = raw;
+    if (dot(h.surfN, dir) >= 0.0f) h.surfN = h.shapeN;
+}
+
+// SkyBox::get (src/component.cpp:120-140)
+__device__ __forceinline__ V3 sky_get(const SceneView &S, V3 dir) {
+    if (!S.sky) return v3(0.0f);
+    float theta = atan2f(-dir.x, dir.z);
+    float phi = acosf(dir.y);
+    if (theta < 0.0f) theta += 2.0f * PI_F;
+    int u = (int)(theta / (2.0f * PI_F) * float(S.sky_w));
+    int v = (int)(phi / PI_F * float(S.sky_h));
+    if (u < 0) u = 0;
+    if (u >= S.sky_w) u = S.sky_w - 1;
+    if (v < 0) v = 0;
+    if (v >= S.sky_h) v = S.sky_h - 1;
+    phi = PI_F * (float(v) + 0.5f) / float(S.sky_h);
+    float area = sinf(phi) * 2.0f * PI_F / float(S.sky_w * S.sky_h);
+    return ld3(S.sky + ((int64_t)v * S.sky_w + u) * 3) / area;
+}
+
+// ------------------------------------------------------------------ BSDF
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1 - t) + b * t; }
+__device__ __forceinline__ V3 mix3(V3 a, V3 b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float schlick(float u) {
+    float m = 1 - u;
+    m = m < 0 ? 0 : (m > 1 ? 1 : m);
+    float m2 = m * m;
+    return m2 * m2 * m;
+}
+__device__ __forceinline__ float gtr1(float NdotH, float a) {
+    if (a >= 1) return 1 / PI_F;
+    float a2 = a * a;
+    float t = 1 + (a2 - 1) * NdotH * NdotH;
+    return (a2 - 1) / (PI_F * logf(a2) * t);
+}
+__device__ __forceinline__ float gtr2(float NdotH, float a) {
+    float a2 = a * a;
+    float t = 1 + (a2 - 1) * NdotH * NdotH;
+    return a2 / (PI_F * t * t);
+}
+__device__ __forceinline__ float smith_ggx(float NdotV, float alphaG) {
+    float a = alphaG * alphaG;
+    float b = NdotV * NdotV;
+    return 1 / (NdotV + sqrtf(a + b - a * b));
+}
+__device__ __forceinline__ float sqrt_s(float x) {
+    if (x < -T_EPS) return 0.0f;
+    return x <= 0.0f ? 0.0f : sqrtf(x);
+}
+
+// BSDF::getBRDF (src/sampling.cpp:49-157); returns f * N.L
+__device__ __forceinline__ V3 brdf(const Surf &s, V3 V, V3 L) {
+    V3 N = s.surfN;
+    float NdotL = dot(N, L), NdotV = dot(N, V);
+    if (NdotL <= 0.0f || NdotV <= 0.0f) return v3(0.0f);
+    V3 H = normalize(L + V);
+    float NdotH = dot(N, H), LdotH = dot(L, H);
+    V3 Cd = s.base;
+    float Cdlum = dot(Cd, W_RGB());
+    V3 Ctint = Cdlum > 0.0f ? Cd / Cdlum : v3(1.0f);
+    V3 Cspec0 = mix3(s.specular * mix3(v3(1.0f), Ctint, 0.0f), Cd, s.metallic);
+    V3 Csheen = mix3(v3(1.0f), Ctint, 0.0f);
+    float FL = schlick(NdotL), FV = schlick(NdotV);
+    float Fd90 = 0.5f + 2.0f * LdotH * LdotH * s.roughness;
+    float Fd = mixf(1.0f, Fd90, FL) * mixf(1.0f, Fd90, FV);
+    float Fss90 = LdotH * LdotH * s.roughness;
+    float Fss = mixf(1.0f, Fss90, FL) * mixf(1.0f, Fss90, FV);
+    float ss = 1.25f * (Fss * (1.0f / (NdotL + NdotV) - .5f) + .5f);
+    float Ds = gtr2(NdotH, s.roughness);
+    float FH;
+    if (s.eta <= 1.0f) {
+        FH = schlick(LdotH);
+    } else {
+        float ci = fabsf(LdotH);
+        float si = sqrt_s(1.0f - ci * ci);
+        float st = s.eta * si;
+        if (st >= 1.0f) {
+            FH = 1.0f;
+        } else {
+            float ct = sqrt_s(1.0f - st * st);
+            float R0 = (s.eta - 1.0f) / (s.eta + 1.0f);
+            R0 = R0 * R0;
+            FH = mixf(R0, 1.0f, schlick(ct));
+        }
+    }
+    V3 Fs = mix3(Cspec0, v3(1.0f), FH);
+    float Gs = smith_ggx(NdotL, s.roughness);
+    Gs *= smith_ggx(NdotV, s.roughness);
+    V3 Fsheen = FH * 0.0f * Csheen;
+    float Dr = gtr1(NdotH, mixf(.1f, .001f, 0.2f));
+    float Fr = mixf(.04f, 1.0f, FH);
+    float Gr = smith_ggx(NdotL, .25f) * smith_ggx(NdotV, .25f);
+    V3 ret = (1.0f / PI_F) * mixf(Fd, ss, 0.0f) * Cd + Fsheen;
+    ret *= (1.0f - s.metallic);
+    ret += (0.25f * 1.5f * Gr * Fr * Dr) * mix3(v3(1.0f), Ctint, 0.0f);
+    ret *= s.opacity;
+    ret += Fs * Ds * Gs;
+    return ret * NdotL;
+}
+
+// BSDF::getBTDF (src/sampling.cpp:159-192)
+__device__ __forceinline__ V3 btdf(const Surf &s, V3 V, V3 L) {
+    V3 N = s.surfN;
+    float NdotL = dot(N, L);
+    if (NdotL >= 0.0f) return v3(0.0f);
+    V3 H = normalize(L + s.eta * V);
+    float D = gtr2(dot(N, H), s.roughness);
+    float B = D * (-NdotL);
+    float LdotH = dot(L, H), NdotV = dot(N, V), HdotV = dot(H, V);
+    B *= fabsf(LdotH * HdotV) / (fabsf(NdotL * NdotV) + T_EPS);
+    float k = s.eta * HdotV + LdotH;
+    B /= k * k;
+    return v3(B);
+}
+
+// BSDF::getBSDF (src/sampling.cpp:194-199)
+__device__ __forceinline__ V3 bsdf_eval(const Surf &s, V3 V, V3 L) {
+    if (s.opacity > 1.0f - T_EPS || s.entering) return brdf(s, V, L);
+    return btdf(s, V, L);
+}
+
+// clamp (src/sampling.cpp:238-243)
+__device__ __forceinline__ void clamp64(V3 &p) {
+    float c = dot(p, W_RGB());
+    if (c > 64.0f) p /= c / 64.0f;
+}
+
+// BSDF::sampleBRDF (src/sampling.cpp:203-236)
+__device__ __forceinline__ void sample_ggx(const Surf &s, V3 V, Stream &g, V3 &out, V3 &bp, float &pdf, int &fails) {
+    V3 t, b;
+    tangent_space_in(s.surfN, V, t, b);
+    float r2 = s.roughness * s.roughness;
+    for (int T = 1; T <= 16; ++T) {
+        float u = g.next(), phi = g.next() * 2.0f * PI_F;
+        float ct = sqrtf((1.0f - u) / (1.0f + (r2 - 1.0f) * u)), st = sqrtf(1 - ct * ct);
+        V3 H = (st * cosf(phi) * t + st * sinf(phi) * b) + ct * s.surfN;
+        out = 2.0f * dot(V, H) * H - V;
+        float LdotH = dot(out, H), LdotN = dot(out, s.surfN);
+        if (LdotH <= 0.0f || LdotN <= 0.0f)
+            pdf = 0.0f;
+        else
+            pdf = gtr2(ct, s.roughness) / (4.0f * LdotH);
+        if (dot(out, s.shapeN) > 0.0f && pdf > 0.0f) {
+            bp = brdf(s, V, out) / pdf;
+            return;
+        }
+        fails++;
+    }
+    pdf = 0.0f;
+    bp = v3(0.0f);
+    out = vnan();
+}
+
+// BSDF::sampleCos (src/sampling.cpp:245-269)
+__device__ __forceinline__ void sample_cos(const Surf &s, V3 V, Stream &g, V3 &out, V3 &bp, float &pdf, int &fails) {
+    V3 t, b;
+    tangent_space_in(s.surfN, V, t, b);
+    for (int T = 1; T <= 16; ++T) {
+        float u = g.next(), phi = g.next() * 2.0f * PI_F;
+        float d = sqrtf(u), z = sqrtf(1 - d * d);
+        float x = d * cosf(phi), y = d * sinf(phi);
+        out = (x * t + y * b) + z * s.surfN;
+        pdf = dot(out, s.surfN) / PI_F;
+        if (dot(out, s.shapeN) > 0.0f && pdf > 0.0f) {
+            bp = brdf(s, V, out) / pdf;
+            clamp64(bp);
+            return;
+        }
+        fails++;
+    }
+    pdf = 0.0f;
+    bp = v3(0.0f);
+    out = vnan();
+}
+
+// BSDF::preciseRefraction (src/sampling.cpp:271-308)
+__device__ __forceinline__ void precise_refraction(const Surf &s, V3 V, V3 &out, float &F) {
+    V3 N = s.surfN;
+    float eta = s.eta;
+    float NdotV = dot(N, V);
+    if (NdotV < 0.0f) { F = 1.0f; out = vnan(); return; }
+    float delta = 1.0f - eta * eta * (1.0f - NdotV * NdotV);
+    if (delta < 0.0f) { F = 1.0f; out = vnan(); return; }
+    float k = eta * NdotV - sqrtf(delta);
+    out = k * N - eta * V;
+    float LdotN = -dot(out, N);
+    float R0 = (1.0f - eta) / (1.0f + eta);
+    R0 = R0 * R0;
+    F = R0 + (1.0f - R0) * schlick(LdotN);
+}
+
+// BSDF::sampleReflection (src/sampling.cpp:310-340)
+__device__ __forceinline__ void sample_reflection(const Surf &s, V3 V, Stream &g, V3 &dir, V3 &bp, int &fails) {
+    V3 d1, d2, b1, b2;
+    int f1 = 0, f2 = 0;
+    float p1, p2;
+    sample_cos(s, V, g, d1, b1, p1, f1);
+    sample_ggx(s, V, g, d2, b2, p2, f2);
+    float pc = p1 / (p1 + p2);
+    if (g.next() < pc) {
+        dir = d1; bp = b1; fails += f1;
+    } else {
+        dir = d2; bp = b2; fails += f2;
+    }
+}
+
+// refract (src/sampling.cpp:342-349)
+__device__ __forceinline__ V3 refract_dir(V3 V, V3 N, float eta) {
+    float NdotV = dot(N, V);
+    float delta = 1.0f - eta * eta * (1.0f - NdotV * NdotV);
+    if (delta < 0.0f) return vnan();
+    float k = eta * NdotV - sqrtf(delta);
+    return k * N - eta * V;
+}
+
+// BSDF::sampleBTDF (src/sampling.cpp:351-391)
+__device__ __forceinline__ void sample_btdf(const Surf &s, V3 V, Stream &g, V3 &out, V3 &bp, int &fails) {
+    if (fabsf(s.eta - 1.0f) < T_EPS) {
+        out = -V;
+        bp = v3(1.0f);
+        return;
+    }
+    V3 t, b;
+    tangent_space_in(s.surfN, V, t, b);
+    float r2 = s.roughness * s.roughness;
+    for (int T = 1; T <= 16; ++T) {
+        float u = g.next(), phi = g.next() * 2.0f * PI_F;
+        float ct = sqrtf((1.0f - u) / (1.0f + (r2 - 1.0f) * u)), st = sqrtf(1 - ct * ct);
+        V3 H = (st * cosf(phi) * t + st * sinf(phi) * b) + ct * s.surfN;
+        out = refract_dir(V, H, s.eta);
+        float w;
+        if (!any_finite(out)) {
+            w = 0.0f;
+        } else {
+            float VdotH = dot(V, H), VdotN = dot(V, s.surfN), HdotN = dot(H, s.surfN);
+            w = fabsf(VdotH) / fabsf(VdotN * HdotN);
+        }
+        if (w > 0.0f && dot(out, s.shapeN) < 0.0f) {
+            bp = v3(w);
+            return;
+        }
+        fails++;
+    }
+    bp = v3(0.0f);
+    out = vnan();
+}
+
+// ------------------------------------------------------------ light samples
+struct LSample {
+    V3 bsdf, light;
+    float weight;
+};
+
+// RandomDistribution::operator() (src/component.cpp:20-24): lower_bound of
+// total * u in the prefix sums [0, n)
+__device__ __forceinline__ int cdf_pick(const float *cdf, int n, float r) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] < r) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// getLightObjectWeight (src/sampling.cpp:406-417) for one light
+__device__ __forceinline__ float light_weight(const LightRec &L, const Surf &s, V3 V) {
+    V3 c = ld3(L.center);
+    V3 dir = normalize(c - s.pos);
+    float dist = length(c - s.pos);
+    V3 b = bsdf_eval(s, V, dir);
+    float cl = dot(b, W_RGB());
+    return cl * L.power / (dist * dist + 1e-3f);
+}
+
+// One light sample of sampleDirectLight (src/sampling.cpp:467-527), with its
+// shadow ray.  Returns false when the reference `continue`s.
+template <bool COUNT>
+__device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s, V3 V, float total, int cnt, Stream &g,
+                                              LSample &out, TraceCounts &tc) {
+    if (S.sky) {
+        // sampleSkyBox (src/sampling.cpp:450-465)
+        const int n = S.sky_w * S.sky_h;
+        const float tot = S.sky_cdf[n - 1];
+        V3 dir = vnan(), light = v3(0.0f);
+        bool ok = false;
+        for (int T = 1; T <= 16; ++T) {
+            int pi = cdf_pick(S.sky_cdf, n, tot * g.next());
+            int u = pi % S.sky_w, v = pi / S.sky_w;
+            float phi = PI_F * (float(v) + 0.5f) / float(S.sky_h);
+            float th = 2.0f * PI_F * (float(u) + 0.5f) / float(S.sky_w);
+            float sp = sinf(phi);
+            dir = V3{-sp * sinf(th), cosf(phi), sp * cosf(th)};
+            if (dot(dir, s.surfN) > 0.0f) {
+                float now = S.sky_cdf[pi];
+                if (pi > 0) now -= S.sky_cdf[pi - 1];
+                light = ld3(S.sky + (int64_t)pi * 3) / (now / tot);
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) return false;
+        RayPre r = ray_pre(s.pos, dir);
+        float aim = INFINITY, tmax = aim + T_EPS;
+        trace<true, COUNT>(S, r, T_EPS, tmax, aim, tc);
+        if (tmax < aim) return false;
+        V3 b = bsdf_eval(s, V, dir);
+        clamp64(b);
+        out = LSample{b, light, 1.0f / float(cnt)};
+        return true;
+    }
+    // emissive light objects: sample() walks the weights (src/sampling.cpp:396-404)
+    float r = total * g.next();
+    int li = -1;
+    float cum = 0.0f, wsel = 0.0f;
+    for (int i = 0; i < S.n_lights; ++i) {
+        float w = light_weight(S.lights[i], s, V);
+        cum += w;
+        if (r <= cum) { li = i; wsel = w; break; }
+    }
+    if (li < 0) return false;
+    float P = wsel / total;
+    const LightRec &L = S.lights[li];
+    V3 lp = vnan();
+    int fails = 0;
+    // sampleLightFace (src/sampling.cpp:436-448) + generateRandomPointInLightFace (:419-434)
+    bool found = false;
+    for (int T = 1; T <= 16; ++T) {
+        const float *cdf = S.light_cdf + L.first_face;
+        int fi = cdf_pick(cdf, L.n_faces, cdf[L.n_faces - 1] * g.next());
+        const TriShade &F = S.light_faces[L.first_face + fi];
+        float a = g.next(), b = g.next();
+        if (a + b > 1.0f) { a = 1.0f - a; b = 1.0f - b; }
+        float c = 1.0f - a - b;
+        V3 p = (a * ld3(F.v[0]) + b * ld3(F.v[1])) + c * ld3(F.v[2]);
+        V3 ldir = normalize(p - s.pos);
+        V3 sn, un;
+        bool ent;
+        hit_normals(F, ldir, V3{a, b, c}, sn, un, ent);
+        float cphi = dot(un, -ldir);
+        lp = p;
+        if (cphi > 0.0f && g.next() < cphi) { found = true; break; }
+        fails++;
+    }
+    if (!found) return false;
+    V3 dir = normalize(lp - s.pos);
+    float dist = length(lp - s.pos);
+    RayPre ry = ray_pre(s.pos, dir);
+    float aim = dist - T_EPS, tmax = aim + T_EPS;
+    trace<true, COUNT>(S, ry, T_EPS, tmax, aim, tc);
+    if (tmax < aim) return false;
+    V3 b = bsdf_eval(s, V, dir);
+    clamp64(b);
+    V3 light = ((2.0f * PI_F * L.power) * ld3(L.color)) / (dist * dist + 1e-3f) / P;
+    out = LSample{b, light, 1.0f / float(cnt * (fails + 1))};
+    return true;
+}
+
+// Total light-object weight of sampleDirectLight (0 -> no samples); 1 with a sky.
+__device__ __forceinline__ float direct_total(const SceneView &S, const Surf &s, V3 V) {
+    if (S.sky) return 1.0f;
+    float tot = 0.0f;
+    for (int i = 0; i < S.n_lights; ++i) tot += light_weight(S.lights[i], s, V);
+    return tot;
+}
+
+// accumulateInwardRadiance (src/image.cpp:615-659) as increments of the
+// diffuse (d) and specular (s) accumulators (rgb + Var).
+struct Acc4 { float r, g, b, v; };
+__device__ __forceinline__ void acc_basic(Acc4 &a, V3 inr, float w) {
+    if (!any_finite(inr) || !isfinite(w)) return;
+    V3 x = inr * w;
+    a.r += x.x; a.g += x.y; a.b += x.z;
+    a.v += dot(inr, inr) * w;
+}
+__device__ __forceinline__ void accumulate(V3 base, const LSample &smp, Acc4 &d, Acc4 &s) {
+    if (length(smp.light) < T_EPS) return;
+    V3 b0 = normalize(base);
+    if (length(base) < T_EPS) {
+        acc_basic(s, smp.light * smp.bsdf, smp.weight);
+        return;
+    }
+    const V3 white = normalize(v3(1.0f));
+    float xy = dot(b0, white);
+    if (xy > 0.99f) {
+        acc_basic(d, smp.light * smp.bsdf / base, smp.weight);
+        return;
+    }
+    V3 perp = normalize(cross(b0, white));
+    V3 bp = smp.bsdf - perp * dot(perp, smp.bsdf);
+    float d1 = dot(bp, white), d2 = dot(bp, b0);
+    float apb = (d1 + d2) / (1 + xy), amb = (d1 - d2) / (1 - xy);
+    float B = (apb - amb) / 2.0f;
+    V3 bb = B * b0;
+    acc_basic(d, smp.light * B / length(base), smp.weight);
+    acc_basic(s, smp.light * (smp.bsdf - bb), smp.weight);
+}
+
+}  // namespace rm

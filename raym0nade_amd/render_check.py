@@ -1,0 +1,87 @@
+"""Helpers that run the GPU renderer into host arrays laid out like the
+oracle's (G-buffer [h][w][15], radiance [4][h][w][4]) and compare the two.
+Used by the GPU tests and by the smoke check; the oracle itself is passed in
+by the caller, never imported here."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import (RM_BUF_BASE_COLOR, RM_BUF_DD, RM_BUF_DS, RM_BUF_EMISSION, RM_BUF_ID, RM_BUF_IS, RM_BUF_POSITION,
+               RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, Frame)
+
+GB_ORDER = [RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RM_BUF_EMISSION, RM_BUF_BASE_COLOR]
+RAD_ORDER = [RM_BUF_DD, RM_BUF_DS, RM_BUF_ID, RM_BUF_IS]
+
+
+def download_all(fr: Frame):
+    gb = np.concatenate([fr.download(b) for b in GB_ORDER], axis=-1)
+    rad = np.stack([fr.download(b) for b in RAD_ORDER])
+    return gb, rad
+
+
+def gpu_render(scene, args, stream=None):
+    import torch
+    fr = Frame(args.width, args.height)
+    fr.render(scene, args, stream=stream)
+    torch.cuda.synchronize()
+    return download_all(fr)
+
+
+def assert_gbuffer_equal(gb, ogb):
+    a = gb.view(np.uint32)
+    b = ogb.view(np.uint32)
+    nan_a, nan_b = np.isnan(gb), np.isnan(ogb)
+    same = (a == b) | (nan_a & nan_b)
+    if not same.all():
+        idx = np.argwhere(~same)[:5]
+        raise AssertionError(f"G-buffer differs at {len(np.argwhere(~same))} values, e.g. {idx.tolist()}: "
+                             f"{gb[tuple(idx[0])]} vs {ogb[tuple(idx[0])]}")
+
+
+def compare_radiance(rad, orad, rtol=1e-3, atol=1e-5):
+    """Fraction of pixels whose direct (Dd, Ds) and indirect (Id, Is) rgb+Var
+    agree within rtol/atol."""
+    def close(k):
+        x, y = rad[k], orad[k]
+        tol = atol + rtol * np.maximum(np.abs(x), np.abs(y))
+        return (np.abs(x - y) <= tol).all(axis=-1)
+    d = close(0) & close(1)
+    i = close(2) & close(3)
+    return {"direct_close": float(d.mean()), "indirect_close": float(i.mean()),
+            "max_abs_direct": float(np.abs(rad[:2] - orad[:2]).max()),
+            "mean_rel_indirect": float((np.abs(rad[2:, ..., :3] - orad[2:, ..., :3]).sum() /
+                                        max(1e-12, np.abs(orad[2:, ..., :3]).sum())))}
+
+
+def zscores(rad, orad, spp, p_direct):
+    """Per-pixel z-scores of the total demodulated radiance between two
+    independent renders, from the reference's own second moments (Var holds
+    E[x^2] - E[x]^2 per buffer after calcVar)."""
+    n_dir = int(spp * p_direct)
+    n_ind = spp - n_dir
+    def tot(r):
+        return r[..., :3].sum(axis=-1).sum(axis=0)
+    def var(r):
+        v = r[0, ..., 3] / max(1, n_dir) + r[1, ..., 3] / max(1, n_dir) + r[2, ..., 3] / max(1, n_ind) + \
+            r[3, ..., 3] / max(1, n_ind)
+        # Var is the second moment of |L| (summed over channels); the channel sum
+        # has at most 3x that variance (Cauchy-Schwarz)
+        return 3.0 * v
+    s = np.sqrt(var(rad) + var(orad)) + 1e-6
+    return (tot(rad) - tot(orad)) / s
+
+
+def smoke(scene, oracle_scene, gen, dev):
+    """Small render on cuda:0 checked against the oracle."""
+    from . import RmRenderArgs
+    a = RmRenderArgs.from_buffer_copy(gen.args)
+    scale = a.width / 64
+    a.width = a.height = 64
+    a.accuracy = np.float32(a.accuracy * scale)
+    a.spp = 16
+    gb, rad = gpu_render(scene, a)
+    ogb, orad = oracle_scene.render(a)
+    assert_gbuffer_equal(gb, ogb)
+    st = compare_radiance(rad, orad)
+    assert st["direct_close"] >= 0.99 and st["indirect_close"] >= 0.95, st
+    print("smoke: Cornell 64x64 16spp render matches the oracle", st)
