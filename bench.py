@@ -52,7 +52,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-tris", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sample", type=int, default=1000000, help="rays per batch in the CPU sample")
+    ap.add_argument("--cpu-sample", type=int, default=4000000, help="rays per batch in the CPU sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (3x) to scale the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -156,6 +158,15 @@ def main():
                 "kernel": "k_trace_closest", "algorithmic_bytes_per_launch": round(bytes_per_launch),
                 "avg_launch_ms": round(dur_per_launch * 1e3, 4)}
 
+    if args.copy_calibration:
+        # a plain copy of known size: 1 GiB read + 1 GiB written per launch
+        x = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -168,14 +179,18 @@ def main():
         sc = r_coh_h[:: max(1, n_coh // k)][:k]
         si = r_inc_h[:k]
         OS.closest(si[:10000], threads=threads)  # warm-up
-        c0 = time.perf_counter()
-        OS.closest(sc, threads=threads)
-        OS.closest(si, threads=threads)
-        cdt = time.perf_counter() - c0
+        reps = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            OS.closest(sc, threads=threads)
+            OS.closest(si, threads=threads)
+            reps.append(time.perf_counter() - c0)
+        cdt = float(np.median(reps))
         cpu = {"value": round((sc.shape[0] + si.shape[0]) / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                "kind": "port",
                "sample": f"{sc.shape[0]} coherent (every {max(1, n_coh // k)}th) + {si.shape[0]} incoherent rays of "
-                         f"the same batches, closest hit, {cdt:.1f} s; CPU {cpu_info()}; g++ -O2 -ffp-contract=off"}
+                         f"the same batches, closest hit, median of 3 runs of {cdt:.1f} s; CPU {cpu_info()}; "
+                         f"oracle restatement of BVH::rayHit, g++ -O2 -ffp-contract=off"}
         log(f"[bench] CPU: {cpu['value']} Mrays/s")
 
     out = {
