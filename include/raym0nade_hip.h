@@ -225,6 +225,19 @@ int32_t rm_generated_destroy(rm_generated *gen);
  * kind 1 = origins U[0,1]^3, directions uniform on the sphere. */
 int32_t rm_generate_rays(int32_t kind, uint32_t seed, int64_t n, const rm_render_args *cam, float *rays);
 
+/* ------------------------------------------------------------- tuning */
+
+/* Launch parameters (no reference counterpart: the reference's only knob is
+ * its thread count).  All settings give identical results; they trade speed.
+ * Process-wide; set them before launching work. */
+enum rm_tune_key {
+    RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk (default) */
+    RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variant 2 (default 8) */
+    RM_TUNE_FETCH_MIN = 2               /* idle lanes before a wave claims new rays (default 16) */
+};
+int32_t rm_set_tuning(int32_t key, int32_t value);
+int32_t rm_get_tuning(int32_t key, int32_t *value);
+
 /* ------------------------------------------------------------ utilities */
 
 /* Philox-4x32-10 block (the counter RNG shared by host and device code). */

@@ -71,6 +71,19 @@ def philox(ctr, key) -> tuple:
     return tuple(o)
 
 
+TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN = 0, 1, 2
+
+
+def set_tuning(key: int, value: int) -> None:
+    check(lib().rm_set_tuning(key, value))
+
+
+def get_tuning(key: int) -> int:
+    v = C.c_int32()
+    check(lib().rm_get_tuning(key, C.byref(v)))
+    return v.value
+
+
 def _ptr(t) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
 

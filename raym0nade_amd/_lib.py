@@ -125,6 +125,8 @@ SIGNATURES = [
     ("rm_generated_destroy", C.c_int32, [C.c_void_p]),
     ("rm_generate_rays", C.c_int32, [C.c_int32, C.c_uint32, C.c_int64, C.POINTER(RmRenderArgs), f32p]),
     ("rm_philox4x32_10", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("rm_set_tuning", C.c_int32, [C.c_int32, C.c_int32]),
+    ("rm_get_tuning", C.c_int32, [C.c_int32, C.POINTER(C.c_int32)]),
 ]
 
 _lib = None

@@ -61,8 +61,14 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)
+    # the console host (reference CLI grammar) linked against the library
+    tmp = CLI + ".tmp"
+    _run([HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"), "-x", "c++",
+                              os.path.join(CSRC, "cli.cpp"), "-x", "none", "-o", tmp, "-L" + HERE,
+                              "-lraym0nade_hip", "-Wl,-rpath,$ORIGIN", "-lz"])
+    os.replace(tmp, CLI)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, CLI)
     return LIB
 
 

@@ -29,7 +29,55 @@ int32_t ensure_device() {
     return RM_OK;
 }
 
+Tuning &tuning() {
+    static Tuning t;
+    return t;
+}
+
+int device_cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
 }  // namespace rm
+
+extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
+    rm::Tuning &t = rm::tuning();
+    switch (key) {
+        case RM_TUNE_TRACE_VARIANT:
+            if (value < 0 || value > 2) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: trace variant %d", value);
+            t.trace_variant = value;
+            return RM_OK;
+        case RM_TUNE_PERSISTENT_BLOCKS_PER_CU:
+            if (value < 1 || value > 64) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: blocks per CU %d", value);
+            t.persistent_blocks_per_cu = value;
+            return RM_OK;
+        case RM_TUNE_FETCH_MIN:
+            if (value < 1 || value > 64) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: fetch_min %d", value);
+            t.fetch_min = value;
+            return RM_OK;
+        default:
+            return rm::fail(RM_ERR_INVALID, "rm_set_tuning: unknown key %d", key);
+    }
+}
+
+extern "C" int32_t rm_get_tuning(int32_t key, int32_t *value) {
+    if (!value) return rm::fail(RM_ERR_INVALID, "rm_get_tuning: NULL");
+    const rm::Tuning &t = rm::tuning();
+    switch (key) {
+        case RM_TUNE_TRACE_VARIANT: *value = t.trace_variant; return RM_OK;
+        case RM_TUNE_PERSISTENT_BLOCKS_PER_CU: *value = t.persistent_blocks_per_cu; return RM_OK;
+        case RM_TUNE_FETCH_MIN: *value = t.fetch_min; return RM_OK;
+        default: return rm::fail(RM_ERR_INVALID, "rm_get_tuning: unknown key %d", key);
+    }
+}
 
 extern "C" const char *rm_last_error(void) { return rm::g_err; }
 

@@ -81,6 +81,15 @@ struct HostScene {
 
 int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs);
 
+// Launch parameters chosen by the autotuner (rm_set_tuning).
+struct Tuning {
+    int trace_variant = 0;        // 0 stack walk, 1 step walk, 2 persistent step walk
+    int persistent_blocks_per_cu = 8;
+    int fetch_min = 16;           // idle lanes before a wave claims new rays
+};
+Tuning &tuning();
+int device_cu_count();
+
 }  // namespace rm
 
 struct rm_scene {

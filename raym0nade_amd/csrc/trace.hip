@@ -6,6 +6,8 @@
 // the 8 XCDs) works on one contiguous range of rays: neighbouring rays of a
 // coherent batch then share that XCD's L2.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <atomic>
 #include "rm_internal.h"
 #include "traverse.h"
 
@@ -63,6 +65,88 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_any(SceneView S, const fl
     occ[i] = (tmax < aim) ? 1 : 0;
 }
 
+// One ray per lane, step-wise walker (one node per loop iteration).
+template <bool ANY>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                            float tmin, float tmax0, const float *__restrict__ aim_arr,
+                                                            float *__restrict__ t_out, int32_t *__restrict__ id_out,
+                                                            uint8_t *__restrict__ occ) {
+    int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    V3 o, d;
+    load_ray(rays, i, o, d);
+    RayPre r = ray_pre(o, d);
+    float aim = ANY ? (aim_arr ? aim_arr[i] : tmax0) : -INFINITY;
+    Walker<ANY, false> w;
+    w.init(S, r, ANY ? aim + T_EPS : tmax0);
+    TraceCounts c{};
+    while (!w.step(S, r, tmin, aim, c)) {
+    }
+    if (ANY) {
+        occ[i] = (w.tmax < aim) ? 1 : 0;
+    } else {
+        t_out[i] = w.tmax;
+        id_out[i] = w.best >= 0 ? __float_as_int(S.tris[w.best].e1[3]) : -1;
+    }
+}
+
+// Persistent lanes with dynamic ray fetch: a lane whose ray finished takes the
+// next unclaimed ray; a wave claims rays for its idle lanes with one atomic
+// once at least `fetch_min` lanes are idle (or all of its live lanes are).
+// Every wave leaves the loop when the batch is exhausted and its lanes are
+// done, so the grid always drains.
+template <bool ANY>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                                  float tmin, float tmax0,
+                                                                  const float *__restrict__ aim_arr,
+                                                                  float *__restrict__ t_out, int32_t *__restrict__ id_out,
+                                                                  uint8_t *__restrict__ occ,
+                                                                  unsigned long long *__restrict__ next, int fetch_min) {
+    const uint32_t lane = threadIdx.x & 63u;
+    Walker<ANY, false> w;
+    RayPre r;
+    float aim = -INFINITY;
+    int64_t ray = -1;
+    bool exhausted = false;
+    TraceCounts c{};
+    while (true) {
+        bool idle = ray < 0;
+        uint64_t idle_mask = __ballot(idle);
+        uint64_t busy_mask = __ballot(!idle);
+        if (exhausted && busy_mask == 0) break;
+        if (!exhausted && idle_mask && (__popcll(idle_mask) >= fetch_min || busy_mask == 0)) {
+            uint32_t cnt = (uint32_t)__popcll(idle_mask);
+            uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle_mask) - 1u;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(next, (unsigned long long)cnt);
+            base = __shfl(base, (int)leader, 64);
+            if (base + cnt >= (unsigned long long)n) exhausted = true;
+            if (idle) {
+                int64_t k = (int64_t)base + __popcll(idle_mask & ((1ull << lane) - 1ull));
+                if (k < n) {
+                    ray = k;
+                    V3 o, d;
+                    load_ray(rays, k, o, d);
+                    r = ray_pre(o, d);
+                    aim = ANY ? (aim_arr ? aim_arr[k] : tmax0) : -INFINITY;
+                    w.init(S, r, ANY ? aim + T_EPS : tmax0);
+                }
+            }
+        }
+        if (ray >= 0 && w.step(S, r, tmin, aim, c)) {
+            if (ANY) {
+                occ[ray] = (w.tmax < aim) ? 1 : 0;
+            } else {
+                t_out[ray] = w.tmax;
+                id_out[ray] = w.best >= 0 ? __float_as_int(S.tris[w.best].e1[3]) : -1;
+            }
+            ray = -1;
+        }
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_count(SceneView S, const float *__restrict__ rays, int64_t n,
                                                              float tmin, float tmax0, int any_hit,
                                                              unsigned long long *__restrict__ counts) {
@@ -103,17 +187,60 @@ static int32_t check_trace_args(const rm_scene *s, const float *rays, int64_t n,
     return RM_OK;
 }
 
+// Claim counters of the persistent kernels: one slot per call in flight
+// (rotating over 64), cleared by a kernel on the call's stream before use.
+// Calls on one device that overlap in time use different slots unless more
+// than 64 are in flight at once.
+__device__ __align__(256) unsigned long long g_ray_next[64];
+
+static int32_t ray_counter(unsigned long long **out) {
+    static unsigned long long *cached[64] = {nullptr};
+    static std::atomic<uint32_t> slot{0};
+    int dev = 0;
+    RM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(RM_ERR_UNSUPPORTED, "device index %d", dev);
+    if (!cached[dev]) {
+        void *p = nullptr;
+        RM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(g_ray_next)));
+        cached[dev] = static_cast<unsigned long long *>(p);
+    }
+    *out = cached[dev] + (slot.fetch_add(1) & 63u);
+    return RM_OK;
+}
+
+template <bool ANY>
+static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, float tmin, float t0, const float *aim_dev,
+                            float *t_out, int32_t *id_out, uint8_t *occ, hipStream_t st) {
+    const Tuning &T = tuning();
+    unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
+    if (T.trace_variant == 2) {
+        unsigned long long *next = nullptr;
+        int32_t rc = ray_counter(&next);
+        if (rc) return rc;
+        unsigned pgrid = (unsigned)std::min<int64_t>(grid, (int64_t)device_cu_count() * T.persistent_blocks_per_cu);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, next);
+        hipLaunchKernelGGL(k_trace_persistent<ANY>, dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0,
+                           aim_dev, t_out, id_out, occ, next, T.fetch_min);
+    } else if (T.trace_variant == 1) {
+        hipLaunchKernelGGL(k_trace_step<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,
+                           t_out, id_out, occ);
+    } else if (ANY) {
+        hipLaunchKernelGGL(k_trace_any<0>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev, occ);
+    } else {
+        hipLaunchKernelGGL(k_trace_closest<0>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, t_out,
+                           id_out);
+    }
+    RM_HIP(hipGetLastError());
+    return RM_OK;
+}
+
 extern "C" int32_t rm_trace_closest(const rm_scene *s, const float *rays, int64_t n, float tmin, float tmax,
                                     float *t_out, int32_t *id_out, void *stream) {
     int32_t st = check_trace_args(s, rays, n, t_out, "rm_trace_closest");
     if (st) return st;
     if (n > 0 && !id_out) return fail(RM_ERR_INVALID, "rm_trace_closest: id_out is NULL");
     if (n == 0) return RM_OK;
-    unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
-    hipLaunchKernelGGL(k_trace_closest<0>, dim3(grid), dim3(TRACE_BLOCK), 0, (hipStream_t)stream, s->view, rays, n, tmin,
-                       tmax, t_out, id_out);
-    RM_HIP(hipGetLastError());
-    return RM_OK;
+    return launch_trace<false>(s, rays, n, tmin, tmax, nullptr, t_out, id_out, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int32_t rm_trace_any(const rm_scene *s, const float *rays, int64_t n, float tmin, float aim_depth,
@@ -121,11 +248,7 @@ extern "C" int32_t rm_trace_any(const rm_scene *s, const float *rays, int64_t n,
     int32_t st = check_trace_args(s, rays, n, occ, "rm_trace_any");
     if (st) return st;
     if (n == 0) return RM_OK;
-    unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
-    hipLaunchKernelGGL(k_trace_any<0>, dim3(grid), dim3(TRACE_BLOCK), 0, (hipStream_t)stream, s->view, rays, n, tmin,
-                       aim_depth, aim_dev, occ);
-    RM_HIP(hipGetLastError());
-    return RM_OK;
+    return launch_trace<true>(s, rays, n, tmin, aim_depth, aim_dev, nullptr, nullptr, occ, (hipStream_t)stream);
 }
 
 extern "C" int32_t rm_trace_count(const rm_scene *s, const float *rays, int64_t n, float tmin, float tmax,

@@ -183,4 +183,95 @@ __device__ __forceinline__ int32_t trace(const SceneView &S, const RayPre &r, fl
     }
 }
 
+// ---------------------------------------------------------------------------
+// Step-wise form of the same walk: one call of step() processes exactly one
+// node (an interior visit or a whole leaf), so lanes of a wave stay on the
+// same code path, and a lane whose ray is finished can take a new ray.
+//
+// Branch-free slab test: for a non-empty box (lo <= hi) and a non-parallel
+// axis, min((lo-o)*inv, (hi-o)*inv) is exactly the reference's entry distance
+// and max(...) its exit distance (the one the `invD >= 0` test selects), so
+// the three axes are evaluated without branches and the state the reference
+// returns with (after the first axis where tL > tR, or after all three) is
+// selected at the end.  Rays with a parallel axis (|d_i| < 1e-4) take the
+// branching test.
+
+__device__ __forceinline__ void box_test_fast(const RayPre &r, float lx, float ly, float lz, float hx, float hy,
+                                              float hz, float &tL, float &tR) {
+    float ax = (lx - r.o.x) * r.inv[0], bx = (hx - r.o.x) * r.inv[0];
+    float ay = (ly - r.o.y) * r.inv[1], by = (hy - r.o.y) * r.inv[1];
+    float az = (lz - r.o.z) * r.inv[2], bz = (hz - r.o.z) * r.inv[2];
+    float L1 = fmaxf(tL, fminf(ax, bx)), R1 = fminf(tR, fmaxf(ax, bx)) + T_EPS;
+    float L2 = fmaxf(L1, fminf(ay, by)), R2 = fminf(R1, fmaxf(ay, by)) + T_EPS;
+    float L3 = fmaxf(L2, fminf(az, bz)), R3 = fminf(R2, fmaxf(az, bz)) + T_EPS;
+    bool s1 = L1 > R1, s2 = L2 > R2;
+    tL = s1 ? L1 : (s2 ? L2 : L3);
+    tR = s1 ? R1 : (s2 ? R2 : R3);
+}
+
+template <bool ANY, bool COUNT>
+struct Walker {
+    uint32_t stk_ref[MAX_STACK];
+    float stk_tl[MAX_STACK];
+    int sp;
+    uint32_t cur;  // interior node index, leaf ref (bit 31), or 0 when finished
+    int32_t best;
+    float tmax;
+    bool anypar;
+
+    __device__ __forceinline__ void init(const SceneView &S, const RayPre &r, float tmax0) {
+        sp = 0;
+        best = -1;
+        tmax = tmax0;
+        cur = S.root_leaf ? ((uint32_t)S.root_leaf | 0x80000000u) : 1u;
+        anypar = r.par[0] || r.par[1] || r.par[2];
+    }
+
+    // Returns true when the walk has finished (result in best / tmax).
+    __device__ __forceinline__ bool step(const SceneView &S, const RayPre &r, float tmin, float aim, TraceCounts &c) {
+        if (!(cur & 0x80000000u)) {
+            if (COUNT) { c.interior++; c.boxes += 2; }
+            const uint32_t u = cur;
+            const float4 *P = reinterpret_cast<const float4 *>(S.nodes + u);
+            float4 A = P[0], Bq = P[1], C = P[2];
+            uint2 Lf = *reinterpret_cast<const uint2 *>(&P[3]);
+            float tL0 = tmin, tR0 = tmax, tL1 = tmin, tR1 = tmax;
+            if (!anypar) {
+                box_test_fast(r, A.x, A.y, A.z, A.w, Bq.x, Bq.y, tL0, tR0);
+                box_test_fast(r, Bq.z, Bq.w, C.x, C.y, C.z, C.w, tL1, tR1);
+            } else {
+                box_test(r, A.x, A.y, A.z, A.w, Bq.x, Bq.y, tL0, tR0);
+                box_test(r, Bq.z, Bq.w, C.x, C.y, C.z, C.w, tL1, tR1);
+            }
+            uint32_t ref0 = Lf.x ? (Lf.x | 0x80000000u) : (u << 1);
+            uint32_t ref1 = Lf.y ? (Lf.y | 0x80000000u) : (u << 1 | 1u);
+            bool f0 = tL0 < tL1;
+            float tLf = f0 ? tL0 : tL1, tRf = f0 ? tR0 : tR1;
+            float tLs = f0 ? tL1 : tL0, tRs = f0 ? tR1 : tR0;
+            uint32_t rf = f0 ? ref0 : ref1, rs = f0 ? ref1 : ref0;
+            if (tLs < tRs) {
+                stk_ref[sp] = rs;
+                stk_tl[sp] = tLs;
+                ++sp;
+            }
+            cur = (tLf < tRf) ? rf : 0u;
+        } else {
+            leaf_test<COUNT>(S, r, cur & 0x7fffffffu, tmin, tmax, best, c);
+            cur = 0u;
+            if (ANY && tmax < aim) return true;
+        }
+        if (cur == 0u) {
+            while (sp > 0) {
+                --sp;
+                if (stk_tl[sp] < tmax) {
+                    cur = stk_ref[sp];
+                    break;
+                }
+            }
+            if (cur == 0u) return true;
+        }
+        return false;
+    }
+};
+
 }  // namespace rm

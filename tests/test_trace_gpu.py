@@ -29,6 +29,15 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
     return occ.cpu().numpy()
 
 
+@pytest.fixture(params=[0, 1, 2], ids=["stack", "step", "persistent"])
+def variant(request):
+    """Every traversal kernel variant must give the same bits."""
+    old = rm.get_tuning(rm.TUNE_TRACE_VARIANT)
+    rm.set_tuning(rm.TUNE_TRACE_VARIANT, request.param)
+    yield request.param
+    rm.set_tuning(rm.TUNE_TRACE_VARIANT, old)
+
+
 @pytest.fixture(scope="module")
 def soup(dev):
     print("[trace] soup 100k tris", flush=True)
@@ -37,7 +46,7 @@ def soup(dev):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_closest_bitexact_soup(soup, dev, kind):
+def test_closest_bitexact_soup(soup, dev, kind, variant):
     g, S, OS = soup
     if kind == 0:
         cam = g.args
@@ -58,7 +67,7 @@ def test_perm_matches_oracle(soup):
     assert S.info.n_nodes == OS.node_count()
 
 
-def test_any_bitexact_soup(soup, dev):
+def test_any_bitexact_soup(soup, dev, variant):
     g, S, OS = soup
     r = rm.rays(1, 200000, seed=10)
     for aim in [np.inf, 0.3, 0.02]:
@@ -82,7 +91,7 @@ def test_counts_equal_oracle(soup, dev, any_hit):
         assert np.array_equal(c, co)
 
 
-def test_tmin_tmax_window(soup, dev):
+def test_tmin_tmax_window(soup, dev, variant):
     g, S, OS = soup
     r = rm.rays(1, 50000, seed=12)
     for tmin, tmax in [(0.05, 0.4), (1e-4, 0.1), (0.2, np.inf)]:
@@ -91,7 +100,7 @@ def test_tmin_tmax_window(soup, dev):
         assert np.array_equal(ids, io) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
 
 
-def test_axis_parallel_rays(soup, dev):
+def test_axis_parallel_rays(soup, dev, variant):
     # directions with exact zero / tiny components take the |d_i| < 1e-4 branch
     rng = np.random.default_rng(5)
     n = 30000
@@ -108,7 +117,7 @@ def test_axis_parallel_rays(soup, dev):
     assert np.array_equal(ids, io) and np.array_equal(t.view(np.uint32), to.view(np.uint32))
 
 
-def test_cornell_bitexact(dev):
+def test_cornell_bitexact(dev, variant):
     g = rm.generate(0, 1)
     S, OS = rm.Scene(g), O.OracleScene(g.desc)
     r = rm.rays(0, 512 * 512, cam=g.args)
@@ -118,7 +127,7 @@ def test_cornell_bitexact(dev):
 
 
 @pytest.mark.parametrize("n", [1, 10, 11])
-def test_tiny_scenes(dev, n):
+def test_tiny_scenes(dev, n, variant):
     # n <= 10: the whole scene is one leaf (root visited as a leaf)
     g = rm.generate(1, 40 + n, n_tris=n)
     S, OS = rm.Scene(g), O.OracleScene(g.desc)
@@ -137,7 +146,7 @@ def test_zero_rays(soup, dev):
     assert t.numel() == 0
 
 
-def test_full_size_c1(dev):
+def test_full_size_c1(dev, variant):
     """1M-triangle soup, 2828^2 coherent + 8M incoherent rays.  The GPU runs
     every ray; the oracle checks a 100k-ray sample of each batch bit for bit,
     and the GPU's own counters over the sample equal the oracle's."""
