@@ -71,7 +71,7 @@ def philox(ctr, key) -> tuple:
     return tuple(o)
 
 
-TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN = 0, 1, 2
+TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT = 0, 1, 2, 3
 
 
 def set_tuning(key: int, value: int) -> None:

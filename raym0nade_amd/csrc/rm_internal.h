@@ -64,6 +64,7 @@ struct HostScene {
     int32_t depth = 0;
     int64_t n_nodes = 0;      // nodeCount(1,n)+1
     uint32_t root_leaf = 0;
+    float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
     std::vector<int32_t> perm;          // slot -> original id
     std::vector<PairNode> pairs;        // indexed by interior node u
     std::vector<TriRec> tris;           // slot order
@@ -86,6 +87,7 @@ struct Tuning {
     int trace_variant = 0;        // 0 stack walk, 1 step walk, 2 persistent step walk
     int persistent_blocks_per_cu = 8;
     int fetch_min = 16;           // idle lanes before a wave claims new rays
+    int ray_sort = 1;             // 0 never, 1 when consecutive origins are far apart, 2 always
 };
 Tuning &tuning();
 int device_cu_count();

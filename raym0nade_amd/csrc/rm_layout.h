@@ -103,6 +103,7 @@ struct SceneView {
     const float *sky;             // premultiplied by pixel solid angle, [h][w][3]
     const float *sky_cdf;         // prefix sums of luminance weights
     int64_t n_tris;
+    float lo[3], hi[3];           // root box (used to bin ray origins for reordering)
     int32_t root_leaf;            // 0, or leaf code when the whole scene is one leaf
     int32_t depth;
     int32_t n_lights;

@@ -187,6 +187,10 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
         }
     }
     hs.root_leaf = B.kind[1] == 2 ? leaf_code(B.fl[1], B.fr[1] - B.fl[1]) : 0u;
+    for (int k = 0; k < 3; ++k) {
+        hs.root_lo[k] = B.lo[3 + k];
+        hs.root_hi[k] = B.hi[3 + k];
+    }
 
     // ---- triangle records in slot order
     hs.tris.resize(n);
@@ -425,6 +429,10 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     v.sky = h.sky.empty() ? nullptr : s->sky.as<float>();
     v.sky_cdf = s->sky_cdf.as<float>();
     v.n_tris = h.n;
+    for (int k = 0; k < 3; ++k) {
+        v.lo[k] = h.root_lo[k];
+        v.hi[k] = h.root_hi[k];
+    }
     v.root_leaf = (int32_t)h.root_leaf;
     v.depth = h.depth;
     v.n_lights = (int32_t)h.lights.size();

@@ -30,12 +30,20 @@ __device__ __forceinline__ void load_ray(const float *rays, int64_t i, V3 &o, V3
 }
 
 // KIND only distinguishes instantiations by name in profiles (0 = generic).
+// Ray processed by slot j: order[j] when the batch was reordered, else j.
+__device__ __forceinline__ int64_t ray_of(int64_t j, const uint32_t *order, const int *use_order) {
+    return (order && *use_order) ? (int64_t)order[j] : j;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_closest(SceneView S, const float *__restrict__ rays, int64_t n,
                                                                float tmin, float tmax0, float *__restrict__ t_out,
-                                                               int32_t *__restrict__ id_out) {
-    int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
-    if (i >= n) return;
+                                                               int32_t *__restrict__ id_out,
+                                                               const uint32_t *__restrict__ order,
+                                                               const int *__restrict__ use_order) {
+    int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
+    if (j >= n) return;
+    int64_t i = ray_of(j, order, use_order);
     V3 o, d;
     load_ray(rays, i, o, d);
     RayPre r = ray_pre(o, d);
@@ -51,9 +59,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_closest(SceneView S, cons
 template <int KIND>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_any(SceneView S, const float *__restrict__ rays, int64_t n,
                                                            float tmin, float aim0, const float *__restrict__ aim_arr,
-                                                           uint8_t *__restrict__ occ) {
-    int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
-    if (i >= n) return;
+                                                           uint8_t *__restrict__ occ, const uint32_t *__restrict__ order,
+                                                           const int *__restrict__ use_order) {
+    int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
+    if (j >= n) return;
+    int64_t i = ray_of(j, order, use_order);
     V3 o, d;
     load_ray(rays, i, o, d);
     RayPre r = ray_pre(o, d);
@@ -70,9 +80,11 @@ template <bool ANY>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const float *__restrict__ rays, int64_t n,
                                                             float tmin, float tmax0, const float *__restrict__ aim_arr,
                                                             float *__restrict__ t_out, int32_t *__restrict__ id_out,
-                                                            uint8_t *__restrict__ occ) {
-    int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
-    if (i >= n) return;
+                                                            uint8_t *__restrict__ occ, const uint32_t *__restrict__ order,
+                                                            const int *__restrict__ use_order) {
+    int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
+    if (j >= n) return;
+    int64_t i = ray_of(j, order, use_order);
     V3 o, d;
     load_ray(rays, i, o, d);
     RayPre r = ray_pre(o, d);
@@ -101,7 +113,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                                                                   const float *__restrict__ aim_arr,
                                                                   float *__restrict__ t_out, int32_t *__restrict__ id_out,
                                                                   uint8_t *__restrict__ occ,
-                                                                  unsigned long long *__restrict__ next, int fetch_min) {
+                                                                  unsigned long long *__restrict__ next, int fetch_min,
+                                                                  const uint32_t *__restrict__ order,
+                                                                  const int *__restrict__ use_order) {
     const uint32_t lane = threadIdx.x & 63u;
     Walker<ANY, false> w;
     RayPre r;
@@ -122,8 +136,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
             base = __shfl(base, (int)leader, 64);
             if (base + cnt >= (unsigned long long)n) exhausted = true;
             if (idle) {
-                int64_t k = (int64_t)base + __popcll(idle_mask & ((1ull << lane) - 1ull));
-                if (k < n) {
+                int64_t slot = (int64_t)base + __popcll(idle_mask & ((1ull << lane) - 1ull));
+                if (slot < n) {
+                    int64_t k = ray_of(slot, order, use_order);
                     ray = k;
                     V3 o, d;
                     load_ray(rays, k, o, d);
@@ -146,6 +161,114 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
 }
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+// ---------------------------------------------------------------- reordering
+// Each ray's walk depends only on the ray, so the batch may be processed in
+// any order.  Rays whose origins lie close together walk the same subtrees;
+// when consecutive rays of a batch are far apart, the batch is processed in
+// the order of a counting sort by (Morton cell of the origin in the root box,
+// 32^3 cells) x (direction octant).  Results are written back by original
+// index.  The decision is made on the device (k_coherence), so no host
+// round trip is needed.
+constexpr int CELL_BITS = 5;
+constexpr uint32_t NBINS = 1u << (3 * CELL_BITS + 3);
+
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+    uint32_t r = 0;
+    for (int b = 0; b < CELL_BITS; ++b) r |= ((x >> b) & 1u) << (3 * b);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t ray_bin(const SceneView &S, V3 o, V3 d) {
+    const float c[3] = {o.x, o.y, o.z};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float ext = S.hi[k] - S.lo[k];
+        float f = ext > 0.0f ? (c[k] - S.lo[k]) / ext : 0.0f;
+        int v = (int)(f * float(1 << CELL_BITS));
+        v = v < 0 ? 0 : (v > (1 << CELL_BITS) - 1 ? (1 << CELL_BITS) - 1 : v);
+        q[k] = (uint32_t)v;
+    }
+    uint32_t m = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return (m << 3) | oct;
+}
+
+// flag = 1 when the mean distance between consecutive ray origins (4096
+// evenly spaced pairs) exceeds `frac` of the root box diagonal.
+__global__ __launch_bounds__(256) void k_coherence(SceneView S, const float *__restrict__ rays, int64_t n, float frac,
+                                                   int *__restrict__ flag) {
+    __shared__ float part[4];
+    float acc = 0.0f;
+    if (n >= 2) {
+        for (int s = threadIdx.x; s < 4096; s += 256) {
+            int64_t k = (int64_t)s * (n - 2) / 4095;
+            V3 o0, d0, o1, d1;
+            load_ray(rays, k, o0, d0);
+            load_ray(rays, k + 1, o1, d1);
+            acc += length(o1 - o0);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mean = (part[0] + part[1] + part[2] + part[3]) / 4096.0f;
+        V3 ext{S.hi[0] - S.lo[0], S.hi[1] - S.lo[1], S.hi[2] - S.lo[2]};
+        *flag = (n >= 2 && mean > frac * length(ext)) ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero_bins(uint32_t *__restrict__ hist) {
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < NBINS) hist[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_bin_hist(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                  const int *__restrict__ flag, uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ hist) {
+    if (!*flag) return;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    V3 o, d;
+    load_ray(rays, i, o, d);
+    uint32_t b = ray_bin(S, o, d);
+    keys[i] = b;
+    atomicAdd(hist + b, 1u);
+}
+
+// exclusive scan of NBINS counters in place, one block of 1024 lanes
+__global__ __launch_bounds__(1024) void k_bin_scan(const int *__restrict__ flag, uint32_t *__restrict__ hist) {
+    if (!*flag) return;
+    __shared__ uint32_t sh[1024];
+    constexpr uint32_t PER = NBINS / 1024;
+    uint32_t base = threadIdx.x * PER, sum = 0;
+    for (uint32_t k = 0; k < PER; ++k) sum += hist[base + k];
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int s = 1; s < 1024; s <<= 1) {
+        uint32_t add = threadIdx.x >= (unsigned)s ? sh[threadIdx.x - s] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = sh[threadIdx.x] - sum;
+    for (uint32_t k = 0; k < PER; ++k) {
+        uint32_t c = hist[base + k];
+        hist[base + k] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(int64_t n, const int *__restrict__ flag,
+                                                     const uint32_t *__restrict__ keys, uint32_t *__restrict__ hist,
+                                                     uint32_t *__restrict__ order) {
+    if (!*flag) return;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pos = atomicAdd(hist + keys[i], 1u);
+    order[pos] = (uint32_t)i;
+}
 
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_count(SceneView S, const float *__restrict__ rays, int64_t n,
                                                              float tmin, float tmax0, int any_hit,
@@ -213,6 +336,26 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                             float *t_out, int32_t *id_out, uint8_t *occ, hipStream_t st) {
     const Tuning &T = tuning();
     unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
+    // reordering workspace (stream-ordered): keys[n], order[n], bins, flag
+    uint32_t *order = nullptr;
+    int *flag = nullptr;
+    char *ws = nullptr;
+    if (T.ray_sort != 0 && n >= 4096) {
+        size_t bytes = (size_t)n * 8 + (size_t)NBINS * 4 + 256;
+        RM_HIP(hipMallocAsync((void **)&ws, bytes, st));
+        uint32_t *keys = reinterpret_cast<uint32_t *>(ws);
+        order = keys + n;
+        uint32_t *hist = order + n;
+        flag = reinterpret_cast<int *>(hist + NBINS);
+        float frac = T.ray_sort == 2 ? -1.0f : 0.02f;
+        unsigned gn = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_coherence, dim3(1), dim3(256), 0, st, s->view, rays, n, frac, flag);
+        hipLaunchKernelGGL(k_zero_bins, dim3(NBINS / 256), dim3(256), 0, st, hist);
+        hipLaunchKernelGGL(k_bin_hist, dim3(gn), dim3(256), 0, st, s->view, rays, n, (const int *)flag, keys, hist);
+        hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, (const int *)flag, hist);
+        hipLaunchKernelGGL(k_bin_scatter, dim3(gn), dim3(256), 0, st, n, (const int *)flag, (const uint32_t *)keys, hist,
+                           order);
+    }
     if (T.trace_variant == 2) {
         unsigned long long *next = nullptr;
         int32_t rc = ray_counter(&next);
@@ -220,17 +363,19 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
         unsigned pgrid = (unsigned)std::min<int64_t>(grid, (int64_t)device_cu_count() * T.persistent_blocks_per_cu);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, next);
         hipLaunchKernelGGL(k_trace_persistent<ANY>, dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0,
-                           aim_dev, t_out, id_out, occ, next, T.fetch_min);
+                           aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag);
     } else if (T.trace_variant == 1) {
         hipLaunchKernelGGL(k_trace_step<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,
-                           t_out, id_out, occ);
+                           t_out, id_out, occ, (const uint32_t *)order, (const int *)flag);
     } else if (ANY) {
-        hipLaunchKernelGGL(k_trace_any<0>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev, occ);
+        hipLaunchKernelGGL(k_trace_any<0>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev, occ,
+                           (const uint32_t *)order, (const int *)flag);
     } else {
         hipLaunchKernelGGL(k_trace_closest<0>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, t_out,
-                           id_out);
+                           id_out, (const uint32_t *)order, (const int *)flag);
     }
     RM_HIP(hipGetLastError());
+    if (ws) RM_HIP(hipFreeAsync(ws, st));
     return RM_OK;
 }
 

@@ -29,13 +29,17 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
     return occ.cpu().numpy()
 
 
-@pytest.fixture(params=[0, 1, 2], ids=["stack", "step", "persistent"])
+@pytest.fixture(params=[(0, 0), (0, 2), (1, 1), (2, 1), (2, 2)],
+                ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted"])
 def variant(request):
-    """Every traversal kernel variant must give the same bits."""
-    old = rm.get_tuning(rm.TUNE_TRACE_VARIANT)
-    rm.set_tuning(rm.TUNE_TRACE_VARIANT, request.param)
+    """Every traversal kernel variant, with and without ray reordering, must
+    give the same bits."""
+    old = (rm.get_tuning(rm.TUNE_TRACE_VARIANT), rm.get_tuning(rm.TUNE_RAY_SORT))
+    rm.set_tuning(rm.TUNE_TRACE_VARIANT, request.param[0])
+    rm.set_tuning(rm.TUNE_RAY_SORT, request.param[1])
     yield request.param
-    rm.set_tuning(rm.TUNE_TRACE_VARIANT, old)
+    rm.set_tuning(rm.TUNE_TRACE_VARIANT, old[0])
+    rm.set_tuning(rm.TUNE_RAY_SORT, old[1])
 
 
 @pytest.fixture(scope="module")
