@@ -21,7 +21,7 @@ CLI = os.path.join(os.path.dirname(rm.__file__), "raym0nade_cli")
 def read_png(path):
     data = open(path, "rb").read()
     assert data[:8] == b"\x89PNG\r\n\x1a\n"
-    pos, idat, w = 8, b"", h = None
+    pos, idat = 8, b""
     w = h = None
     while pos < len(data):
         n = struct.unpack(">I", data[pos:pos + 4])[0]
