@@ -53,8 +53,13 @@ def main():
     ap.add_argument("--n-tris", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=4000000, help="rays per batch in the CPU sample")
+    ap.add_argument("--no-render", action="store_true", help="skip the 1080p render leg")
+    ap.add_argument("--render-frames", type=int, default=2, help="timed 1080p frames (after one warm-up)")
+    ap.add_argument("--render-spp", type=int, default=0, help="override the 1080p workload's 256 spp")
+    ap.add_argument("--cpu-row-stride", type=int, default=48, help="CPU render sample: every n-th row")
+    ap.add_argument("--cpu-col-stride", type=int, default=32, help="CPU render sample: every n-th column")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 1 GiB device copy (3x) to scale the FETCH_SIZE/WRITE_SIZE counters")
+                    help="accepted for older profile commands; the 1 GiB calibration copy always runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -158,14 +163,23 @@ def main():
                 "kernel": "k_trace_closest", "algorithmic_bytes_per_launch": round(bytes_per_launch),
                 "avg_launch_ms": round(dur_per_launch * 1e3, 4)}
 
-    if args.copy_calibration:
-        # a plain copy of known size: 1 GiB read + 1 GiB written per launch
-        x = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        del x, y
+    # measured streaming bandwidth: a plain 1 GiB device copy (1 GiB read + 1 GiB
+    # written per launch); also the known-size kernel that scales the
+    # FETCH_SIZE / WRITE_SIZE counters
+    x = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(5):
+        y.copy_(x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 2 * (1 << 30) * 5 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del x, y
+    roofline["measured_copy_gbs"] = round(copy_gbs, 1)
+    roofline["frac_of_measured"] = round(achieved / copy_gbs, 4)
 
     cpu = None
     if not args.no_cpu:
@@ -193,6 +207,10 @@ def main():
                          f"oracle restatement of BVH::rayHit, g++ -O2 -ffp-contract=off"}
         log(f"[bench] CPU: {cpu['value']} Mrays/s")
 
+    render = None
+    if not args.no_render:
+        render = render_leg(args, torch, rm, dev, stream)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -213,8 +231,62 @@ def main():
         "algorithmic_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "render_1080p": render,
     }
     print(json.dumps(out), flush=True)
+
+
+def render_leg(args, torch, rm, dev, stream):
+    """Path samples/s of the 1080p workload: 64 displaced geodesic spheres
+    (10,368,002 triangles) under a 2048x1024 fp32 sky, 1920x1080, 256 spp,
+    max depth 16, FXAA post.  Wall time of rm_render + rm_postprocess with the
+    scene resident in HBM."""
+    log("[bench] 1080p workload: generating 10.4M-triangle scene")
+    g = rm.generate(2, 1)
+    S = rm.Scene(g)
+    a = rm.RmRenderArgs.from_buffer_copy(g.args)
+    if args.render_spp:
+        a.spp = args.render_spp
+    log(f"[bench] scene: {S.info.n_tris} tris, build {S.info.build_seconds:.1f}s, "
+        f"{S.info.device_bytes / 2**30:.2f} GiB in HBM")
+    fr = rm.Frame(a.width, a.height)
+    times = []
+    for k in range(args.render_frames + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fr.render(S, a, stream=stream)
+        fr.postprocess(63 | 512, a.exposure, stream=stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"[bench] frame {k}: {dt:.3f} s" + (" (warm-up)" if k == 0 else ""))
+        if k > 0:
+            times.append(dt)
+    wall = float(np.median(times))
+    samples = a.width * a.height * a.spp
+    out = {"path_samples_per_s": round(samples / wall, 1), "frame_s": round(wall, 4), "frames": len(times),
+           "width": a.width, "height": a.height, "spp": a.spp, "max_depth": a.max_depth, "n_tris": S.info.n_tris}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else O.hw_threads()
+        threads = max(1, min(threads, 16))
+        log("[bench] CPU baseline: oracle scene build (single thread, reference builder)")
+        c0 = time.perf_counter()
+        OS = O.OracleScene(g.desc)
+        log(f"[bench] oracle scene built in {time.perf_counter() - c0:.1f} s")
+        rs, cs = args.cpu_row_stride, args.cpu_col_stride
+        pix = [y * a.width + x for y in range(rs // 2, a.height, rs) for x in range(cs // 2, a.width, cs)]
+        c0 = time.perf_counter()
+        O.render_pixels(OS, a, pix, threads=threads)
+        cdt = time.perf_counter() - c0
+        cpu_samples = len(pix) * a.spp
+        out["cpu_baseline"] = {
+            "value": round(cpu_samples / cdt, 1), "unit": "path samples/s", "cores": threads, "kind": "port",
+            "sample": f"{len(pix)} pixels on a regular grid (every {rs}th row x every {cs}th column) of the same "
+                      f"frame at the full {a.spp} spp, {cdt:.1f} s; oracle restatement of renderPixel on a "
+                      f"row-pool of std::threads"}
+        out["gpu_over_cpu"] = round(out["path_samples_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
 
 
 if __name__ == "__main__":

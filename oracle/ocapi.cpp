@@ -325,6 +325,38 @@ int32_t oracle_render(const oracle_scene *S, const rm_render_args *a, int32_t th
     return 0;
 }
 
+// Renders only the listed pixels (index y*width+x); outputs are compact:
+// gbuf [n][15], rad [4][n][4] in list order.
+int32_t oracle_render_pixels(const oracle_scene *S, const rm_render_args *a, int32_t threads, const int32_t *pix,
+                             int32_t n, float *gbuf, float *rad) {
+    if (!S || !a || a->width <= 0 || a->height <= 0 || a->max_depth > 16 || !pix) return 1;
+    RenderArgs args = to_args(a);
+    Photo photo(args.width, args.height);
+    photo.exposure = args.exposure;
+    std::vector<int> list(pix, pix + n);
+    render_pixels(S->model, args, photo, threads, list);
+    for (int k = 0; k < n; ++k) {
+        const HitInfo &G = photo.Gbuffer[list[k]];
+        if (gbuf) {
+            put3(gbuf + (size_t)k * 15 + 0, G.position);
+            put3(gbuf + (size_t)k * 15 + 3, G.shapeNormal);
+            put3(gbuf + (size_t)k * 15 + 6, G.surfaceNormal);
+            put3(gbuf + (size_t)k * 15 + 9, G.emission);
+            put3(gbuf + (size_t)k * 15 + 12, G.baseColor);
+        }
+        if (rad) {
+            const RadianceData *R[4] = {&photo.radiance_Dd[list[k]], &photo.radiance_Ds[list[k]],
+                                        &photo.radiance_Id[list[k]], &photo.radiance_Is[list[k]]};
+            for (int b = 0; b < 4; ++b) {
+                float *p = rad + ((size_t)b * n + k) * 4;
+                put3(p, R[b]->radiance);
+                p[3] = R[b]->Var;
+            }
+        }
+    }
+    return 0;
+}
+
 // Photo::postProcessing on given buffers (same layouts as oracle_render).
 void oracle_postprocess(int32_t w, int32_t h, const float *gbuf, const float *rad, int32_t options, float exposure,
                         float *pixels) {

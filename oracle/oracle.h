@@ -215,5 +215,6 @@ struct Photo {                                          // include/image.h:15-67
 void renderPixel(const Model &model, const RenderArgs &args, Photo &image, int x, int y);   // src/render.cpp:448-551
 void render_multiThread(const Model &model, const RenderArgs &args, Photo &photo, int threads,
                         const std::vector<int> *rows = nullptr);                            // src/render.cpp:593-626
+void render_pixels(const Model &model, const RenderArgs &args, Photo &photo, int threads, const std::vector<int> &pix);
 
 }  // namespace ref

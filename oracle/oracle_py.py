@@ -55,6 +55,7 @@ def lib():
             "oracle_hw_threads": (C.c_int32, []),
             "oracle_render": (C.c_int32, [vp, vp, C.c_int32, i32p, C.c_int32, f32p, f32p]),
             "oracle_postprocess": (None, [C.c_int32, C.c_int32, f32p, f32p, C.c_int32, C.c_float, f32p]),
+            "oracle_render_pixels": (C.c_int32, [vp, vp, C.c_int32, i32p, C.c_int32, f32p, f32p]),
             "oracle_accumulate": (None, [f32p, f32p, f32p, C.c_float, f32p, f32p]),
             "oracle_bsdf_eval": (None, [f32p, f32p, C.c_int32, f32p]),
             "oracle_bsdf_sample": (None, [f32p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, f32p]),
@@ -157,6 +158,19 @@ class OracleScene:
         if st != 0:
             raise ValueError("oracle_render rejected the arguments")
         return g, r
+
+
+def render_pixels(scene, args, pixels, threads=0):
+    """Renders only `pixels` (indices y*width+x); returns gbuf [n][15], rad [4][n][4]."""
+    pix = np.ascontiguousarray(pixels, dtype=np.int32)
+    n = pix.size
+    g = np.empty((n, 15), dtype=np.float32)
+    r = np.empty((4, n, 4), dtype=np.float32)
+    st = lib().oracle_render_pixels(scene.h, C.byref(args), threads or lib().oracle_hw_threads(),
+                                    pix.ctypes.data_as(i32p), n, fp(g), fp(r))
+    if st != 0:
+        raise ValueError("oracle_render_pixels rejected the arguments")
+    return g, r
 
 
 def ray_in_box(ray, box, tL, tR):

@@ -363,6 +363,21 @@ void render_multiThread(const Model &model, const RenderArgs &args, Photo &photo
     for (auto &t : pool) t.join();
 }
 
+// The same worker pool over an explicit pixel list (CPU-baseline sampling).
+void render_pixels(const Model &model, const RenderArgs &args, Photo &photo, int threads, const std::vector<int> &pix) {
+    std::atomic<size_t> next{0};
+    auto worker = [&] {
+        while (true) {
+            size_t k = next.fetch_add(1);
+            if (k >= pix.size()) break;
+            renderPixel(model, args, photo, pix[k] % args.width, pix[k] / args.width);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::max(1, threads); ++t) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+}
+
 // ------------------------------------------------------------------ image
 Photo::Photo(int w, int h)
     : width(w), height(h), Gbuffer((size_t)w * h), radiance_Dd((size_t)w * h), radiance_Ds((size_t)w * h),

@@ -84,7 +84,7 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs);
 
 // Launch parameters chosen by the autotuner (rm_set_tuning).
 struct Tuning {
-    int trace_variant = 0;        // 0 stack walk, 1 step walk, 2 persistent step walk
+    int trace_variant = 2;        // 0 stack walk, 1 step walk, 2 persistent step walk
     int persistent_blocks_per_cu = 8;
     int fetch_min = 16;           // idle lanes before a wave claims new rays
     int ray_sort = 1;             // 0 never, 1 when consecutive origins are far apart, 2 always
