@@ -194,6 +194,21 @@ int32_t rm_frame_download(const rm_frame *frame, int32_t which, float *host);
 int32_t rm_render(const rm_scene *scene, const rm_render_args *args, rm_frame *frame,
                   void *stream);
 
+/* Batched form (no reference counterpart; the reference renders one frame per
+ * render_multiThread call).  All jobs run through one set of kernel launches
+ * over their concatenated pixels -- each launch carries paths of every job --
+ * and with use_graph != 0 the whole sequence after the primary pass is
+ * captured into one hipGraph and launched once.  Each job needs its own frame;
+ * post_options != 0 also runs rm_postprocess(frame, post_options) for it.
+ * Results equal those of rm_render on each job.  Synchronous. */
+typedef struct rm_batch_job {
+    const rm_scene *scene;
+    rm_render_args args;
+    rm_frame *frame;
+    int32_t post_options;
+} rm_batch_job;
+int32_t rm_render_batch(const rm_batch_job *jobs, int32_t n, int32_t use_graph, void *stream);
+
 /* Photo::postProcessing (src/image.cpp:470-479) without bloom / depth of field:
  * shade(options) -> gammaCorrection -> FXAA if options & 512.  Options are
  * bit-compatible with Photo::ShadeOption (include/image.h:17-34).  Writes
@@ -234,8 +249,9 @@ enum rm_tune_key {
     RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk (default) */
     RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variant 2 (default 8) */
     RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 16) */
-    RM_TUNE_RAY_SORT = 3                /* ray reordering for trace calls: 0 never, 1 when consecutive
+    RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive
                                            origins are far apart (default), 2 always */
+    RM_TUNE_RENDER_BLOCKS_PER_CU = 4    /* blocks per CU of the persistent bounce kernel (default 8) */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

@@ -71,7 +71,8 @@ def philox(ctr, key) -> tuple:
     return tuple(o)
 
 
-TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT = 0, 1, 2, 3
+(TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT,
+ TUNE_RENDER_BLOCKS_PER_CU) = 0, 1, 2, 3, 4
 
 
 def set_tuning(key: int, value: int) -> None:
@@ -149,6 +150,22 @@ class Scene:
         if h is not None and h.value:
             lib().rm_scene_destroy(h)
             self._h = None
+
+
+def render_batch(jobs, use_graph: bool = True, stream=None) -> None:
+    """jobs: list of (Scene, RmRenderArgs, Frame, post_options).  All jobs run
+    through one set of launches (one hipGraph with use_graph); synchronous."""
+    from ._lib import RmBatchJob
+    arr = (RmBatchJob * len(jobs))()
+    for i, (scene, args, frame, post) in enumerate(jobs):
+        arr[i].scene = scene.handle.value
+        arr[i].args = args
+        arr[i].frame = frame._h.value
+        arr[i].post_options = post
+    st = None
+    if stream is not None:
+        st = C.c_void_p(stream.cuda_stream)
+    check(lib().rm_render_batch(arr, len(jobs), int(use_graph), st))
 
 
 class Frame:

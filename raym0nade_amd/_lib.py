@@ -91,6 +91,15 @@ class RmRenderArgs(C.Structure):
     ]
 
 
+class RmBatchJob(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("args", RmRenderArgs),
+        ("frame", C.c_void_p),
+        ("post_options", C.c_int32),
+    ]
+
+
 class RmGenOptions(C.Structure):
     _fields_ = [
         ("n_tris", C.c_int64),
@@ -120,6 +129,7 @@ SIGNATURES = [
     ("rm_frame_download", C.c_int32, [C.c_void_p, C.c_int32, f32p]),
     ("rm_render", C.c_int32, [C.c_void_p, C.POINTER(RmRenderArgs), C.c_void_p, C.c_void_p]),
     ("rm_postprocess", C.c_int32, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p]),
+    ("rm_render_batch", C.c_int32, [C.POINTER(RmBatchJob), C.c_int32, C.c_int32, C.c_void_p]),
     ("rm_generate_scene", C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(RmGenOptions), C.POINTER(C.c_void_p)]),
     ("rm_generated_desc", C.c_int32, [C.c_void_p, C.POINTER(RmSceneDesc), C.POINTER(RmRenderArgs)]),
     ("rm_generated_destroy", C.c_int32, [C.c_void_p]),

@@ -128,10 +128,17 @@ __global__ __launch_bounds__(PB) void k_fxaa(const float *__restrict__ d, int w,
 
 using namespace rm;
 
-extern "C" int32_t rm_postprocess(rm_frame *f, int32_t opt, float exposure, void *stream_) {
+// Allocation half of rm_postprocess (kept out of any graph capture).
+int32_t rm::post_prepare(rm_frame *f, int opt) {
     if (!f) return fail(RM_ERR_INVALID, "rm_postprocess: frame is NULL");
     if (opt & (O_Bloom | O_DoF)) return fail(RM_ERR_UNSUPPORTED, "rm_postprocess: bloom and depth of field are not built");
-    hipStream_t st = (hipStream_t)stream_;
+    const size_t need = (size_t)f->w * f->h * 12;
+    if ((opt & O_FXAA) && f->post.bytes < need) return f->post.alloc(need);
+    return RM_OK;
+}
+
+// Launch half: only kernel launches on `st` (capturable).
+int32_t rm::post_launch(rm_frame *f, int opt, float exposure, hipStream_t st) {
     const int64_t n = (int64_t)f->w * f->h;
     PostIn I;
     I.shapeN = f->buf[RM_BUF_SHAPE_NORMAL].as<float>();
@@ -145,10 +152,7 @@ extern "C" int32_t rm_postprocess(rm_frame *f, int32_t opt, float exposure, void
     float *pixels = f->buf[RM_BUF_PIXELS].as<float>();
     unsigned g = (unsigned)((n + PB - 1) / PB);
     if (opt & O_FXAA) {
-        if (f->post.bytes < (size_t)n * 12) {
-            int32_t rc = f->post.alloc((size_t)n * 12);
-            if (rc) return rc;
-        }
+        if (f->post.bytes < (size_t)n * 12) return fail(RM_ERR_INVALID, "rm_postprocess: FXAA buffer not prepared");
         float *tmp = f->post.as<float>();
         hipLaunchKernelGGL(k_shade_gamma, dim3(g), dim3(PB), 0, st, I, n, (int)opt, exposure, tmp);
         hipLaunchKernelGGL(k_fxaa, dim3(g), dim3(PB), 0, st, (const float *)tmp, f->w, f->h, pixels);
@@ -157,4 +161,10 @@ extern "C" int32_t rm_postprocess(rm_frame *f, int32_t opt, float exposure, void
     }
     RM_HIP(hipGetLastError());
     return RM_OK;
+}
+
+extern "C" int32_t rm_postprocess(rm_frame *f, int32_t opt, float exposure, void *stream) {
+    int32_t rc = post_prepare(f, opt);
+    if (rc) return rc;
+    return post_launch(f, opt, exposure, (hipStream_t)stream);
 }

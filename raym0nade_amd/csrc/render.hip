@@ -107,13 +107,13 @@ struct PathQ {
 };
 // f0 o.xyz | pixel      f1 d.xyz | sample     f2 T.xyz | W        f3 p0.xyz | rf
 // f4 dPdx  | rng n      f5 dPdy  | set        f6 dDdx  | depth+exclude<<8
-// f7 dDdy  | medium n   f8-f11 medium entries (id, ior, absorb) x 3
+// f7 dDdy  | medium n + job<<8              f8-f11 medium entries (id, ior, absorb) x 3
 
 struct Path {
     V3 o, d, T, p0;
     float W, rf;
     uint32_t pix, smp, n, set;
-    int depth;
+    int depth, job;
     bool exclude;
     Diff diff;
     Med med;
@@ -130,7 +130,7 @@ __device__ __forceinline__ void store_path(const PathQ &Q, uint32_t q, const Pat
     Q.f[4][q] = make_float4(P.diff.dPdx.x, P.diff.dPdx.y, P.diff.dPdx.z, ub(P.n));
     Q.f[5][q] = make_float4(P.diff.dPdy.x, P.diff.dPdy.y, P.diff.dPdy.z, ub(P.set));
     Q.f[6][q] = make_float4(P.diff.dDdx.x, P.diff.dDdx.y, P.diff.dDdx.z, ub((uint32_t)P.depth | (P.exclude ? 256u : 0u)));
-    Q.f[7][q] = make_float4(P.diff.dDdy.x, P.diff.dDdy.y, P.diff.dDdy.z, ub((uint32_t)P.med.n));
+    Q.f[7][q] = make_float4(P.diff.dDdy.x, P.diff.dDdy.y, P.diff.dDdy.z, ub((uint32_t)P.med.n | ((uint32_t)P.job << 8)));
     const Med &m = P.med;
     Q.f[8][q] = make_float4(ub((uint32_t)m.id[0]), m.ior[0], m.ab[0].x, m.ab[0].y);
     Q.f[9][q] = make_float4(m.ab[0].z, ub((uint32_t)m.id[1]), m.ior[1], m.ab[1].x);
@@ -153,7 +153,8 @@ __device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
     P.depth = (int)(dx & 255u);
     P.exclude = (dx & 256u) != 0;
     P.diff.dDdy = V3{h.x, h.y, h.z};
-    P.med.n = (int)bu(h.w);
+    P.med.n = (int)(bu(h.w) & 255u);
+    P.job = (int)(bu(h.w) >> 8);
     if (P.med.n > 0) {
         float4 m0 = Q.f[8][q], m1 = Q.f[9][q], m2 = Q.f[10][q], m3 = Q.f[11][q];
         P.med.id[0] = (int)bu(m0.x); P.med.ior[0] = m0.y; P.med.ab[0] = V3{m0.z, m0.w, m1.x};
@@ -190,6 +191,40 @@ struct RArgs {
     uint32_t seed;
 };
 
+// One frame of a (possibly batched) launch.  Per-pixel work buffers are
+// indexed by the global pixel index (jobs concatenated); the frame buffers by
+// the job-local pixel index.
+struct Job {
+    SceneView S;
+    RArgs A;
+    FrameView F;
+    int64_t pix_off, np;
+};
+
+// Runs body(j) for the lanes whose job is j, once per distinct job among the
+// active lanes of the wave, with j wave-uniform (readfirstlane), so the job's
+// scene and arguments are loaded as scalars.  A wave usually holds one job.
+template <class F>
+__device__ __forceinline__ void for_each_job(int myjob, bool active, F body) {
+    uint64_t pend = __ballot(active);
+    while (pend) {
+        int lead = (int)__ffsll((unsigned long long)pend) - 1;
+        int j = __builtin_amdgcn_readfirstlane(__shfl(myjob, lead, 64));
+        bool mine = active && myjob == j;
+        if (mine) body(j);
+        pend &= ~__ballot(mine);
+    }
+}
+
+__device__ __forceinline__ int job_of(const Job *J, int nj, int64_t p) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (J[mid].pix_off <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ void st3(float *p, int64_t i, V3 v) {
     p[i * 3] = v.x;
     p[i * 3 + 1] = v.y;
@@ -209,13 +244,14 @@ __device__ __forceinline__ void camera_ray(const RArgs &A, int x, int y, V3 &Dir
     Dir = normalize(d);
 }
 
-__device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F, int64_t p) {
+// p: global pixel (work buffers), lp: job-local pixel (frame buffers)
+__device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F, int64_t p, int64_t lp) {
     Surf s = surf_default();
     float4 a = P.pi0[p], b = P.pi1[p];
-    s.pos = ld3(F.pos + p * 3);
-    s.shapeN = ld3(F.shapeN + p * 3);
-    s.surfN = ld3(F.surfN + p * 3);
-    s.emission = ld3(F.emis + p * 3);
+    s.pos = ld3(F.pos + lp * 3);
+    s.shapeN = ld3(F.shapeN + lp * 3);
+    s.surfN = ld3(F.surfN + lp * 3);
+    s.emission = ld3(F.emis + lp * 3);
     s.base = V3{a.x, a.y, a.z};
     s.roughness = a.w;
     s.metallic = b.x;
@@ -227,17 +263,21 @@ __device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F,
     return s;
 }
 
-__global__ __launch_bounds__(RB) void k_primary(SceneView S, RArgs A, FrameView F, PixBufs P) {
-    int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (p >= (int64_t)A.w * A.h) return;
+__global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P) {
+    const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (gp >= total_px) return;
+    const Job &jb = J[job_of(J, nj, gp)];
+    const SceneView &S = jb.S;
+    const RArgs &A = jb.A;
+    const FrameView &F = jb.F;
+    const int64_t p = gp - jb.pix_off;  // frame-local pixel; work buffers use gp
     int x = (int)(p % A.w), y = (int)(p / A.w);
     V3 Dir;
     Diff base;
     camera_ray(A, x, y, Dir, base);
     RayPre r = ray_pre(A.pos, Dir);
     float t = INFINITY;
-    TraceCounts tc{};
-    int32_t slot = trace<false, false>(S, r, T_EPS, t, -INFINITY, tc);
+    int32_t slot = trace_walk<false>(S, r, T_EPS, t, -INFINITY);
     Surf s = surf_default();
     float4 zero = make_float4(0, 0, 0, 0);
     if (slot < 0) {
@@ -247,10 +287,10 @@ __global__ __launch_bounds__(RB) void k_primary(SceneView S, RArgs A, FrameView 
         st3(F.surfN, p, vnan());
         st3(F.emis, p, sky_get(S, Dir));
         st3(F.base, p, v3(0.0f));
-        P.pi0[p] = zero;
-        P.pi1[p] = zero;
-        P.pi2[p] = zero;
-        P.M[p] = 0;
+        P.pi0[gp] = zero;
+        P.pi1[gp] = zero;
+        P.pi2[gp] = zero;
+        P.M[gp] = 0;
         return;
     }
     s.pos = A.pos + t * Dir;
@@ -269,11 +309,11 @@ __global__ __launch_bounds__(RB) void k_primary(SceneView S, RArgs A, FrameView 
     st3(F.surfN, p, s.surfN);
     st3(F.emis, p, s.emission);
     st3(F.base, p, s.base);  // the nudged colour stays when no indirect sample exists (:529-530)
-    P.pi0[p] = make_float4(s.base.x, s.base.y, s.base.z, s.roughness);
-    P.pi1[p] = make_float4(s.metallic, s.opacity, s.eta, ub((uint32_t)s.id | (s.entering ? (1u << 30) : 0u)));
-    P.pi2[p] = make_float4(sav.x, sav.y, sav.z, ub((uint32_t)n_ind));
+    P.pi0[gp] = make_float4(s.base.x, s.base.y, s.base.z, s.roughness);
+    P.pi1[gp] = make_float4(s.metallic, s.opacity, s.eta, ub((uint32_t)s.id | (s.entering ? (1u << 30) : 0u)));
+    P.pi2[gp] = make_float4(sav.x, sav.y, sav.z, ub((uint32_t)n_ind));
     bool indirect = !(length(s.emission) > 0.0f) && n_ind > 0;
-    P.M[p] = indirect ? (opaque ? A.K : 16 * A.K) : 0;
+    P.M[gp] = indirect ? (opaque ? A.K : 16 * A.K) : 0;
 }
 
 __device__ __forceinline__ void calc_var(Acc4 &a, float e) {
@@ -283,14 +323,19 @@ __device__ __forceinline__ void calc_var(Acc4 &a, float e) {
     if (a.v < 0.0f) a.v = 0.0f;
 }
 
-__global__ __launch_bounds__(RB) void k_direct(SceneView S, RArgs A, FrameView F, PixBufs P) {
-    int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (p >= (int64_t)A.w * A.h) return;
+__global__ __launch_bounds__(RB) void k_direct(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P) {
+    const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (gp >= total_px) return;
+    const Job &jb = J[job_of(J, nj, gp)];
+    const SceneView &S = jb.S;
+    const RArgs &A = jb.A;
+    const FrameView &F = jb.F;
+    const int64_t p = gp - jb.pix_off;  // the RNG streams are keyed by the frame-local pixel
     Acc4 d{0, 0, 0, 0}, sp{0, 0, 0, 0};
     V3 pos = ld3(F.pos + p * 3);
     V3 em = ld3(F.emis + p * 3);
     if (any_finite(pos) && !(length(em) > 0.0f)) {
-        Surf s = pixel_surf(P, F, p);
+        Surf s = pixel_surf(P, F, gp, p);
         V3 in = normalize(s.pos - A.pos);
         V3 V = -in;
         int cnt = (int)(float(A.spp) * A.p_direct);
@@ -324,19 +369,29 @@ __device__ __forceinline__ uint32_t append(uint32_t *counter, bool want) {
 }
 
 // The depth-0 vertex (sampleIndirectLightFromFirstIntersection, :314-397).
-__global__ __launch_bounds__(RB) void k_spawn(SceneView S, RArgs A, FrameView F, PixBufs P, int round, uint32_t total_sets,
-                                              PathQ Q, uint32_t *count) {
+__global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj, PixBufs P, int round,
+                                              uint32_t total_sets, PathQ Q, uint32_t *count) {
     uint32_t g = blockIdx.x * RB + threadIdx.x;
     bool want = false;
     Path path;
+    bool active = false;
+    int64_t gp = 0;
+    uint32_t s_idx = 0;
+    int myjob = -1;
     if (g < total_sets) {
-        int64_t p = P.set_pix[g];
-        int M = P.M[p];
-        int m = (int)(g - (uint32_t)P.set_off[p]);
-        uint32_t n_ind = bu(P.pi2[p].w);
-        uint32_t s_idx = (uint32_t)round * (uint32_t)M + (uint32_t)m;
-        if (s_idx < n_ind) {
-            Surf s = pixel_surf(P, F, p);
+        gp = P.set_pix[g];
+        int M = P.M[gp];
+        int m = (int)(g - (uint32_t)P.set_off[gp]);
+        uint32_t n_ind = bu(P.pi2[gp].w);
+        s_idx = (uint32_t)round * (uint32_t)M + (uint32_t)m;
+        active = s_idx < n_ind;
+        if (active) myjob = job_of(J, nj, gp);
+    }
+    for_each_job(myjob, active, [&](int ji) {
+            const Job &jb = J[ji];
+            const RArgs &A = jb.A;
+            const int64_t p = gp - jb.pix_off;
+            Surf s = pixel_surf(P, jb.F, gp, p);
             int x = (int)(p % A.w), y = (int)(p / A.w);
             V3 Dir;
             Diff cam;
@@ -382,7 +437,8 @@ __global__ __launch_bounds__(RB) void k_spawn(SceneView S, RArgs A, FrameView F,
                 path.p0 = bp;
                 path.W = fails > 0 ? 1.0f / (float)(1 + fails) : 1.0f;
                 path.rf = rf;
-                path.pix = (uint32_t)p;
+                path.pix = (uint32_t)gp;  // global; the RNG key is the frame-local pixel
+                path.job = ji;
                 path.smp = s_idx;
                 path.n = rng.n;
                 path.set = g;
@@ -391,8 +447,7 @@ __global__ __launch_bounds__(RB) void k_spawn(SceneView S, RArgs A, FrameView F,
                 path.diff = next;
                 path.med = med;
             }
-        }
-    }
+    });
     uint32_t q = append(count, want);
     if (want) store_path(Q, q, path);
 }
@@ -440,22 +495,36 @@ __device__ __forceinline__ void store_set(const AccBufs &B, uint32_t g, const Se
 __constant__ int c_sample_count[17] = {0, 1, 2, 2, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 5, 6};  // src/render.cpp:126
 
 // One sampleRay vertex (src/render.cpp:121-312) for every live path.
-__global__ __launch_bounds__(RB) void k_bounce(SceneView S, RArgs A, PixBufs P, AccBufs AB, PathQ Qin,
-                                               const uint32_t *count_in, PathQ Qout, uint32_t *count_out) {
-    uint32_t q = blockIdx.x * RB + threadIdx.x;
-    uint32_t n_in = *count_in;
+// Persistent: each wave claims 64 queue entries per atomic on `work` until the
+// queue (count_in, written by the previous stage) is exhausted, so the grid is
+// sized by the machine, not by the live count, and no host round trip is
+// needed between depths.
+__global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj, PixBufs P, AccBufs AB, PathQ Qin,
+                                               const uint32_t *count_in, uint32_t *work, PathQ Qout,
+                                               uint32_t *count_out) {
+  const uint32_t n_in = *count_in;
+  const uint32_t lane = threadIdx.x & 63u;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n_in) break;
+    uint32_t q = base + lane;
     bool live = q < n_in;
     bool cont = false;
     Path path;
-    if (live) {
-        path = load_path(Qin, q);
-        Stream rng{A.seed, TAG_INDIRECT, path.pix, path.smp, path.n};
+    if (live) path = load_path(Qin, q);
+    for_each_job(live ? path.job : -1, live, [&](int jid) {
+        const Job &jb = J[jid];
+        const SceneView &S = jb.S;
+        const RArgs &A = jb.A;
+        Stream rng{A.seed, TAG_INDIRECT, (uint32_t)(path.pix - jb.pix_off), path.smp, path.n};
         const int depth = path.depth;
         const int maxD = A.max_depth;
         RayPre r = ray_pre(path.o, path.d);
         float t = INFINITY;
         TraceCounts tc{};
-        int32_t slot = trace<false, false>(S, r, T_EPS, t, -INFINITY, tc);
+        int32_t slot = trace_walk<false>(S, r, T_EPS, t, -INFINITY);
         // light samples produced at this vertex (<= 6)
         LSample ls[6];
         int nls = 0;
@@ -571,13 +640,11 @@ __global__ __launch_bounds__(RB) void k_bounce(SceneView S, RArgs A, PixBufs P, 
                 add_sample(a, base, path.p0, L, ls[k].weight * path.W * inv);
             }
             store_set(AB, path.set, a);
-        } else if (!cont) {
-            // a path that ends without light samples still counts toward
-            // `samples.empty()` only through other samples: nothing to do
         }
-    }
+    });
     uint32_t slot_out = append(count_out, cont);
     if (cont) store_path(Qout, slot_out, path);
+  }
 }
 
 __global__ __launch_bounds__(RB) void k_clear_sets(AccBufs AB, uint32_t n) {
@@ -596,13 +663,18 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-__global__ __launch_bounds__(RB) void k_finish(RArgs A, FrameView F, PixBufs P, AccBufs AB) {
-    int64_t p = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (p >= (int64_t)A.w * A.h) return;
-    int M = P.M[p];
+__global__ __launch_bounds__(RB) void k_finish(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P,
+                                               AccBufs AB) {
+    const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (gp >= total_px) return;
+    const Job &jb = J[job_of(J, nj, gp)];
+    const RArgs &A = jb.A;
+    const FrameView &F = jb.F;
+    const int64_t p = gp - jb.pix_off;
+    int M = P.M[gp];
     Acc4 d{0, 0, 0, 0}, s{0, 0, 0, 0};
     if (M > 0) {
-        int off = P.set_off[p];
+        int off = P.set_off[gp];
         float S = 0.0f, cnt = 0.0f, maxC = -1.0f;
         Acc4 md{0, 0, 0, 0}, ms{0, 0, 0, 0};
         for (int k = 0; k < M; ++k) {
@@ -620,7 +692,7 @@ __global__ __launch_bounds__(RB) void k_finish(RArgs A, FrameView F, PixBufs P, 
             }
             calc_var(d, A.exposure);
             calc_var(s, A.exposure);
-            float4 sv = P.pi2[p];
+            float4 sv = P.pi2[gp];
             st3(F.base, p, V3{sv.x, sv.y, sv.z});
         }
     }
@@ -671,27 +743,27 @@ T *carve(char *&p, size_t count) {
     return r;
 }
 
-}  // namespace rm
+// ---------------------------------------------------------------- host side
+struct HostJob {
+    const rm_scene *scene;
+    const rm_render_args *args;
+    rm_frame *frame;
+    int post;  // rm_postprocess options, 0 = none
+};
 
-using namespace rm;
-
-static int32_t ensure_work(rm_frame *f, size_t bytes) {
-    if (f->work.bytes >= bytes) return RM_OK;
-    return f->work.alloc(bytes);
-}
-
-extern "C" int32_t rm_render(const rm_scene *scene, const rm_render_args *a, rm_frame *f, void *stream_) {
-    if (!scene || !a || !f) return fail(RM_ERR_INVALID, "rm_render: NULL argument");
+static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, const char *fn) {
+    const rm_render_args *a = h.args;
+    rm_frame *f = h.frame;
+    if (!h.scene || !a || !f) return fail(RM_ERR_INVALID, "%s: NULL argument", fn);
     if (a->width != f->w || a->height != f->h)
-        return fail(RM_ERR_INVALID, "rm_render: frame is %dx%d, args ask %dx%d", f->w, f->h, a->width, a->height);
-    if (a->spp < 0) return fail(RM_ERR_INVALID, "rm_render: negative spp");
+        return fail(RM_ERR_INVALID, "%s: frame is %dx%d, args ask %dx%d", fn, f->w, f->h, a->width, a->height);
+    if (a->spp < 0) return fail(RM_ERR_INVALID, "%s: negative spp", fn);
     int max_depth = a->max_depth > 0 ? a->max_depth : 16;
-    if (max_depth > 16) return fail(RM_ERR_INVALID, "rm_render: max_depth %d exceeds 16 (sampleCount table)", max_depth);
-    if (!(a->p_direct >= 0.0f && a->p_direct <= 1.0f)) return fail(RM_ERR_INVALID, "rm_render: p_direct outside [0,1]");
-    if (scene->view.depth > MAX_STACK) return fail(RM_ERR_UNSUPPORTED, "rm_render: tree too deep");
-    hipStream_t st = (hipStream_t)stream_;
+    if (max_depth > 16) return fail(RM_ERR_INVALID, "%s: max_depth %d exceeds 16 (sampleCount table)", fn, max_depth);
+    if (!(a->p_direct >= 0.0f && a->p_direct <= 1.0f)) return fail(RM_ERR_INVALID, "%s: p_direct outside [0,1]", fn);
+    if (h.scene->view.depth > MAX_STACK) return fail(RM_ERR_UNSUPPORTED, "%s: tree too deep", fn);
     const int64_t np = (int64_t)f->w * f->h;
-    RArgs A;
+    RArgs &A = J.A;
     A.pos = V3{a->position[0], a->position[1], a->position[2]};
     A.dir = V3{a->direction[0], a->direction[1], a->direction[2]};
     A.up = V3{a->up[0], a->up[1], a->up[2]};
@@ -710,9 +782,10 @@ extern "C" int32_t rm_render(const rm_scene *scene, const rm_render_args *a, rm_
     A.K = K;
     const int spp_direct = (int)(float(a->spp) * a->p_direct);
     const int n_opaque = a->spp - spp_direct;
-    const int rounds = n_opaque > 0 ? (n_opaque + K - 1) / K : 0;
-
-    FrameView F;
+    rounds = n_opaque > 0 ? (n_opaque + K - 1) / K : 0;
+    maxd = max_depth;
+    J.S = h.scene->view;
+    FrameView &F = J.F;
     F.pos = f->buf[RM_BUF_POSITION].as<float>();
     F.shapeN = f->buf[RM_BUF_SHAPE_NORMAL].as<float>();
     F.surfN = f->buf[RM_BUF_SURFACE_NORMAL].as<float>();
@@ -722,34 +795,55 @@ extern "C" int32_t rm_render(const rm_scene *scene, const rm_render_args *a, rm_
     F.Ds = f->buf[RM_BUF_DS].as<float>();
     F.Id = f->buf[RM_BUF_ID].as<float>();
     F.Is = f->buf[RM_BUF_IS].as<float>();
+    J.np = np;
+    return RM_OK;
+}
 
-    // per-pixel part of the work buffer
-    const int64_t nblk = (np + SB - 1) / SB;
-    size_t pix_bytes = 0;
-    {
-        size_t s = 0;
-        auto add = [&](size_t b) { s += (b + 255) & ~(size_t)255; };
-        add(np * 16); add(np * 16); add(np * 16);
-        add(np * 4); add(np * 4); add(nblk * 8); add(NCOUNT * 4);
-        pix_bytes = s;
+// Renders every job in one set of launches over their concatenated pixels.
+// Host round trip: one, after the primary pass (the accumulator-set total
+// sizes the work buffer).  Everything after it is a fixed launch sequence
+// with device-side queue counts; with `graph` it is captured once into a
+// hipGraph and launched as one graph.
+static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStream_t st, bool graph) {
+    const int nj = (int)hj.size();
+    if (nj <= 0) return RM_OK;
+    if (nj > (1 << 23)) return fail(RM_ERR_INVALID, "too many jobs");
+    std::vector<Job> J(nj);
+    int R = 0, D = 1;
+    int64_t total_px = 0;
+    for (int i = 0; i < nj; ++i) {
+        int r = 0, d = 0;
+        int32_t rc = make_job(hj[i], J[i], r, d, graph ? "rm_render_batch" : "rm_render");
+        if (rc) return rc;
+        J[i].pix_off = total_px;
+        total_px += J[i].np;
+        R = std::max(R, r);
+        D = std::max(D, d);
     }
-    int32_t rc = ensure_work(f, pix_bytes);
-    if (rc) return rc;
-    char *cur = f->work.as<char>();
+    const int64_t nblk = (total_px + SB - 1) / SB;
+    auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pix_bytes = pad(nj * sizeof(Job)) + 3 * pad(total_px * 16) + 2 * pad(total_px * 4) + pad(nblk * 8) +
+                             pad(2 * NCOUNT * 4);
+    if (work.bytes < pix_bytes) {
+        int32_t rc = work.alloc(pix_bytes);
+        if (rc) return rc;
+    }
+    char *cur = work.as<char>();
+    Job *dJ = carve<Job>(cur, nj);
     PixBufs P;
-    P.pi0 = carve<float4>(cur, np);
-    P.pi1 = carve<float4>(cur, np);
-    P.pi2 = carve<float4>(cur, np);
-    P.M = carve<int32_t>(cur, np);
-    P.set_off = carve<int32_t>(cur, np);
+    P.pi0 = carve<float4>(cur, total_px);
+    P.pi1 = carve<float4>(cur, total_px);
+    P.pi2 = carve<float4>(cur, total_px);
+    P.M = carve<int32_t>(cur, total_px);
+    P.set_off = carve<int32_t>(cur, total_px);
     int64_t *bsum = carve<int64_t>(cur, nblk);
-    // counters[d] = live paths entering depth d+1 (d = 0..max_depth), max_depth <= 16
-    uint32_t *counters = carve<uint32_t>(cur, NCOUNT);
-
-    unsigned gp = (unsigned)((np + RB - 1) / RB);
-    hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), 0, st, scene->view, A, F, P);
+    // counts[d]: paths entering depth d+1; counts[NCOUNT + d]: claim counter of depth d
+    uint32_t *counts = carve<uint32_t>(cur, 2 * NCOUNT);
+    RM_HIP(hipMemcpyAsync(dJ, J.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st));
+    const unsigned gp = (unsigned)((total_px + RB - 1) / RB);
+    hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P);
+    hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum);
     RM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, np, bsum);
     std::vector<int64_t> hs(nblk);
     RM_HIP(hipMemcpyAsync(hs.data(), bsum, nblk * 8, hipMemcpyDeviceToHost, st));
     RM_HIP(hipStreamSynchronize(st));
@@ -759,64 +853,110 @@ extern "C" int32_t rm_render(const rm_scene *scene, const rm_render_args *a, rm_
         hs[b] = total;
         total += v;
     }
-    if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "rm_render: %lld path slots per round", (long long)total);
-    // second part: set map, accumulators, two path queues
-    size_t set_bytes = 0;
-    {
-        size_t s = 0;
-        auto add = [&](size_t b) { s += (b + 255) & ~(size_t)255; };
-        add(total * 4);
-        for (int k = 0; k < 5; ++k) add(total * 16);
-        for (int k = 0; k < 24; ++k) add(total * 16);
-        set_bytes = s;
-    }
-    if (f->work.bytes < pix_bytes + set_bytes) {
-        // grow and re-run the primary pass (the buffer holds its results)
-        rc = f->work.alloc(pix_bytes + set_bytes);
+    if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "%lld path slots per round", (long long)total);
+    const size_t set_bytes = pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16);
+    if (work.bytes < pix_bytes + set_bytes) {
+        // grow (the buffer holds the primary pass's results) and start over
+        int32_t rc = work.alloc(pix_bytes + set_bytes);
         if (rc) return rc;
-        return rm_render(scene, a, f, stream_);
+        return render_jobs(hj, work, st, graph);
     }
+    for (int i = 0; i < nj; ++i)
+        if (hj[i].post) {
+            int32_t rc = post_prepare(hj[i].frame, hj[i].post);
+            if (rc) return rc;
+        }
     RM_HIP(hipMemcpyAsync(bsum, hs.data(), nblk * 8, hipMemcpyHostToDevice, st));
+    RM_HIP(hipStreamSynchronize(st));  // `hs` is pageable host memory owned by this call
     P.set_pix = carve<int32_t>(cur, total);
     AccBufs AB;
     for (int k = 0; k < 5; ++k) AB.a[k] = carve<float4>(cur, total);
     PathQ Q[2];
     for (int qi = 0; qi < 2; ++qi)
         for (int k = 0; k < 12; ++k) Q[qi].f[k] = carve<float4>(cur, total);
-    hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, np, bsum, P.set_off, P.set_pix);
+    hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum, P.set_off, P.set_pix);
     RM_HIP(hipGetLastError());
 
-    hipLaunchKernelGGL(k_direct, dim3(gp), dim3(RB), 0, st, scene->view, A, F, P);
-    RM_HIP(hipGetLastError());
-
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (graph) RM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_direct, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P);
     const uint32_t nsets = (uint32_t)total;
     if (nsets > 0) {
-        unsigned gs = (nsets + RB - 1) / RB;
+        const unsigned gs = (nsets + RB - 1) / RB;
+        const unsigned gb = (unsigned)(device_cu_count() * tuning().render_blocks_per_cu);
         hipLaunchKernelGGL(k_clear_sets, dim3(gs), dim3(RB), 0, st, AB, nsets);
-        uint32_t *hcount = nullptr;
-        RM_HIP(hipHostMalloc((void **)&hcount, 64, hipHostMallocDefault));
-        for (int j = 0; j < rounds; ++j) {
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counters, NCOUNT);
-            hipLaunchKernelGGL(k_spawn, dim3(gs), dim3(RB), 0, st, scene->view, A, F, P, j, nsets, Q[0], counters);
-            RM_HIP(hipGetLastError());
-            uint32_t live = nsets;
-            int cur_q = 0;
-            for (int depth = 1; depth <= max_depth; ++depth) {
-                RM_HIP(hipMemcpyAsync(hcount, counters + (depth - 1), 4, hipMemcpyDeviceToHost, st));
-                RM_HIP(hipStreamSynchronize(st));
-                live = *hcount;
-                if (live == 0) break;
-                unsigned gb = (live + RB - 1) / RB;
-                hipLaunchKernelGGL(k_bounce, dim3(gb), dim3(RB), 0, st, scene->view, A, P, AB, Q[cur_q],
-                                   counters + (depth - 1), Q[cur_q ^ 1], counters + depth);
-                RM_HIP(hipGetLastError());
-                cur_q ^= 1;
+        for (int j = 0; j < R; ++j) {
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counts, 2 * NCOUNT);
+            hipLaunchKernelGGL(k_spawn, dim3(gs), dim3(RB), 0, st, (const Job *)dJ, nj, P, j, nsets, Q[0], counts);
+            for (int d = 1; d <= D; ++d)
+                hipLaunchKernelGGL(k_bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1],
+                                   (const uint32_t *)(counts + d - 1), counts + NCOUNT + d, Q[d & 1], counts + d);
+        }
+    }
+    hipLaunchKernelGGL(k_finish, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, AB);
+    for (int i = 0; i < nj; ++i)
+        if (hj[i].post) {
+            int32_t rc = post_launch(hj[i].frame, hj[i].post, hj[i].args->exposure, st);
+            if (rc) {
+                if (graph) {
+                    hipGraph_t tmp;
+                    (void)hipStreamEndCapture(st, &tmp);
+                    if (tmp) (void)hipGraphDestroy(tmp);
+                }
+                return rc;
             }
         }
-        RM_HIP(hipStreamSynchronize(st));
-        (void)hipHostFree(hcount);
+    if (graph) {
+        RM_HIP(hipStreamEndCapture(st, &g));
+        hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+        e = hipGraphLaunch(ge, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     }
-    hipLaunchKernelGGL(k_finish, dim3(gp), dim3(RB), 0, st, A, F, P, AB);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }
+
+}  // namespace rm
+
+using namespace rm;
+
+extern "C" int32_t rm_render(const rm_scene *scene, const rm_render_args *a, rm_frame *f, void *stream) {
+    if (!f) return fail(RM_ERR_INVALID, "rm_render: frame is NULL");
+    std::vector<HostJob> hj{HostJob{scene, a, f, 0}};
+    return render_jobs(hj, f->work, (hipStream_t)stream, false);
+}
+
+extern "C" int32_t rm_render_batch(const rm_batch_job *jobs, int32_t n, int32_t use_graph, void *stream) {
+    if (n < 0 || (n > 0 && !jobs)) return fail(RM_ERR_INVALID, "rm_render_batch: bad job list");
+    if (n == 0) return RM_OK;
+    std::vector<HostJob> hj(n);
+    for (int i = 0; i < n; ++i) {
+        if (!jobs[i].frame) return fail(RM_ERR_INVALID, "rm_render_batch: job %d has no frame", i);
+        for (int k = 0; k < i; ++k)
+            if (jobs[k].frame == jobs[i].frame) return fail(RM_ERR_INVALID, "rm_render_batch: frame of job %d reused", i);
+        hj[i] = HostJob{jobs[i].scene, &jobs[i].args, jobs[i].frame, jobs[i].post_options};
+    }
+    // a graph cannot be captured on the legacy default stream
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    if (use_graph && !st) {
+        RM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+        st = own;
+    }
+    DevBuf work;
+    int32_t rc = render_jobs(hj, work, st, use_graph != 0);
+    hipError_t e = hipStreamSynchronize(st);
+    if (own) (void)hipStreamDestroy(own);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "rm_render_batch");
+    return RM_OK;
+}
+

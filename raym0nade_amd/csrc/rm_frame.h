@@ -14,3 +14,9 @@ struct rm_frame {
     size_t work_bytes = 0;
     rm::DevBuf post;  // tone-mapped image read by FXAA
 };
+
+namespace rm {
+// rm_postprocess in two halves so a captured graph contains only launches.
+int32_t post_prepare(rm_frame *f, int opt);
+int32_t post_launch(rm_frame *f, int opt, float exposure, hipStream_t st);
+}  // namespace rm

@@ -77,6 +77,7 @@ struct HostScene {
     std::vector<float> light_cdf;
     std::vector<float> sky;             // premultiplied by solid angle
     std::vector<float> sky_cdf;
+    std::vector<int32_t> sky_guide;
     int32_t sky_w = 0, sky_h = 0;
 };
 
@@ -88,6 +89,7 @@ struct Tuning {
     int persistent_blocks_per_cu = 8;
     int fetch_min = 16;           // idle lanes before a wave claims new rays
     int ray_sort = 1;             // 0 never, 1 when consecutive origins are far apart, 2 always
+    int render_blocks_per_cu = 8; // resident blocks per CU of the persistent bounce kernel
 };
 Tuning &tuning();
 int device_cu_count();
@@ -97,6 +99,6 @@ int device_cu_count();
 struct rm_scene {
     rm::HostScene host;  // perm + sizes kept; big arrays released after upload
     rm::SceneView view{};
-    rm::DevBuf nodes, tris, shade, mats, texs, texels, lights, light_faces, light_cdf, sky, sky_cdf;
+    rm::DevBuf nodes, tris, shade, mats, texs, texels, lights, light_faces, light_cdf, sky, sky_cdf, sky_guide;
     rm_scene_info info{};
 };

@@ -43,6 +43,9 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 48, "TriRec is 48 B");
 
+// Buckets of the sky CDF guide table.
+constexpr int SKY_GUIDE = 65536;
+
 // Leaf-code helpers.
 constexpr uint32_t LEAF_SHIFT = 4;
 constexpr uint32_t LEAF_MASK = 15;
@@ -102,6 +105,7 @@ struct SceneView {
     const float *light_cdf;       // per light object, prefix sums of face weights
     const float *sky;             // premultiplied by pixel solid angle, [h][w][3]
     const float *sky_cdf;         // prefix sums of luminance weights
+    const int32_t *sky_guide;     // [SKY_GUIDE+1] lower_bound(cdf, total*b/SKY_GUIDE)
     int64_t n_tris;
     float lo[3], hi[3];           // root box (used to bin ray origins for reordering)
     int32_t root_leaf;            // 0, or leaf code when the whole scene is one leaf

@@ -381,6 +381,19 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
                 acc = id == 0 ? Clum : acc + Clum;
                 hs.sky_cdf[id] = acc;
             }
+        // guide[b] = lower_bound(cdf, total * b / G): a query r with
+        // total*(b-1)/G <= r <= total*(b+2)/G has its lower_bound inside
+        // [guide[b-1], guide[b+2]] (lower_bound is monotone in r)
+        const int64_t n = (int64_t)w * h;
+        const double tot = hs.sky_cdf[n - 1];
+        hs.sky_guide.resize(SKY_GUIDE + 1);
+        int64_t i = 0;
+        for (int b = 0; b <= SKY_GUIDE; ++b) {
+            double t = tot * (double)b / (double)SKY_GUIDE;
+            while (i < n && (double)hs.sky_cdf[i] < t) ++i;
+            hs.sky_guide[b] = (int32_t)i;
+        }
+        hs.sky_guide[SKY_GUIDE] = (int32_t)n;
     }
     return RM_OK;
 }
@@ -414,6 +427,7 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     UP(light_cdf, light_cdf);
     UP(sky, sky);
     UP(sky_cdf, sky_cdf);
+    UP(sky_guide, sky_guide);
 #undef UP
     auto t2 = std::chrono::steady_clock::now();
     SceneView &v = s->view;
@@ -428,6 +442,7 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     v.light_cdf = s->light_cdf.as<float>();
     v.sky = h.sky.empty() ? nullptr : s->sky.as<float>();
     v.sky_cdf = s->sky_cdf.as<float>();
+    v.sky_guide = s->sky_guide.as<int32_t>();
     v.n_tris = h.n;
     for (int k = 0; k < 3; ++k) {
         v.lo[k] = h.root_lo[k];
@@ -448,7 +463,7 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     I.has_sky = h.sky.empty() ? 0 : 1;
     I.device_bytes = (int64_t)(s->nodes.bytes + s->tris.bytes + s->shade.bytes + s->mats.bytes + s->texs.bytes +
                                s->texels.bytes + s->lights.bytes + s->light_faces.bytes + s->light_cdf.bytes +
-                               s->sky.bytes + s->sky_cdf.bytes);
+                               s->sky.bytes + s->sky_cdf.bytes + s->sky_guide.bytes);
     I.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     I.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
     // keep only what host calls need

@@ -475,6 +475,22 @@ __device__ __forceinline__ int cdf_pick(const float *cdf, int n, float r) {
     return lo;
 }
 
+// The same lower_bound on the sky CDF, restricted to the bracket the guide
+// table gives for r (the bucket of r computed in fp32 is off by at most one,
+// so buckets b-1 .. b+2 are searched).
+__device__ __forceinline__ int sky_pick(const SceneView &S, int n, float tot, float r) {
+    int b = (int)(r * ((float)SKY_GUIDE / tot));
+    b = b < 0 ? 0 : (b > SKY_GUIDE - 1 ? SKY_GUIDE - 1 : b);
+    int lo = S.sky_guide[b > 0 ? b - 1 : 0];
+    int hi = S.sky_guide[b + 2 < SKY_GUIDE ? b + 2 : SKY_GUIDE];
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (S.sky_cdf[mid] < r) lo = mid + 1; else hi = mid;
+    }
+    (void)n;
+    return lo;
+}
+
 // getLightObjectWeight (src/sampling.cpp:406-417) for one light
 __device__ __forceinline__ float light_weight(const LightRec &L, const Surf &s, V3 V) {
     V3 c = ld3(L.center);
@@ -497,7 +513,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
         V3 dir = vnan(), light = v3(0.0f);
         bool ok = false;
         for (int T = 1; T <= 16; ++T) {
-            int pi = cdf_pick(S.sky_cdf, n, tot * g.next());
+            int pi = sky_pick(S, n, tot, tot * g.next());
             int u = pi % S.sky_w, v = pi / S.sky_w;
             float phi = PI_F * (float(v) + 0.5f) / float(S.sky_h);
             float th = 2.0f * PI_F * (float(u) + 0.5f) / float(S.sky_w);
@@ -514,7 +530,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
         if (!ok) return false;
         RayPre r = ray_pre(s.pos, dir);
         float aim = INFINITY, tmax = aim + T_EPS;
-        trace<true, COUNT>(S, r, T_EPS, tmax, aim, tc);
+        trace_walk<true>(S, r, T_EPS, tmax, aim);
         if (tmax < aim) return false;
         V3 b = bsdf_eval(s, V, dir);
         clamp64(b);
@@ -559,7 +575,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
     float dist = length(lp - s.pos);
     RayPre ry = ray_pre(s.pos, dir);
     float aim = dist - T_EPS, tmax = aim + T_EPS;
-    trace<true, COUNT>(S, ry, T_EPS, tmax, aim, tc);
+    trace_walk<true>(S, ry, T_EPS, tmax, aim);
     if (tmax < aim) return false;
     V3 b = bsdf_eval(s, V, dir);
     clamp64(b);

@@ -227,7 +227,7 @@ struct Walker {
         anypar = r.par[0] || r.par[1] || r.par[2];
     }
 
-    // Returns true when the walk has finished (result in best / tmax).
+    // Returns true when the walk has finished (result in best / tmax); see trace_walk.
     __device__ __forceinline__ bool step(const SceneView &S, const RayPre &r, float tmin, float aim, TraceCounts &c) {
         if (!(cur & 0x80000000u)) {
             if (COUNT) { c.interior++; c.boxes += 2; }
@@ -273,5 +273,19 @@ struct Walker {
         return false;
     }
 };
+
+// The whole walk for one ray with the step-wise walker (stack in scratch):
+// the same result as trace<ANY, false>, with far fewer registers, for
+// kernels that do more than traversal.
+template <bool ANY>
+__device__ __forceinline__ int32_t trace_walk(const SceneView &S, const RayPre &r, float tmin, float &tmax, float aim) {
+    Walker<ANY, false> w;
+    w.init(S, r, tmax);
+    TraceCounts c{};
+    while (!w.step(S, r, tmin, aim, c)) {
+    }
+    tmax = w.tmax;
+    return w.best;
+}
 
 }  // namespace rm
