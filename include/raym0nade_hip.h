@@ -211,9 +211,51 @@ int32_t rm_render_batch(const rm_batch_job *jobs, int32_t n, int32_t use_graph, 
 
 /* Photo::postProcessing (src/image.cpp:470-479) without bloom / depth of field:
  * shade(options) -> gammaCorrection -> FXAA if options & 512.  Options are
- * bit-compatible with Photo::ShadeOption (include/image.h:17-34).  Writes
- * RM_BUF_PIXELS. */
+ * bit-compatible with Photo::ShadeOption (include/image.h:17-34); the extra bit
+ * RM_SHADE_LINEAR skips gammaCorrection (the linear image the adjoint
+ * differentiates).  Writes RM_BUF_PIXELS. */
+#define RM_SHADE_FULL 63
+#define RM_SHADE_LINEAR 65536
 int32_t rm_postprocess(rm_frame *frame, int32_t shade_options, float exposure, void *stream);
+
+/* Device-to-device copy of one frame buffer into `dst` (device memory of
+ * width*height*channels floats), stream-ordered. */
+int32_t rm_frame_copy(const rm_frame *frame, int32_t which, float *dst, void *stream);
+
+/* ------------------------------------------------------- differentiation */
+
+/* No reference counterpart (the reference renders only).  Replaces the
+ * values the adjoint differentiates without rebuilding geometry or the BVH:
+ * materials (host array of the scene's n_materials, NULL = keep) and the
+ * level-0 texels of every texture (device memory, the textures of the scene
+ * description concatenated in order, [h][w][3] each; NULL = keep).  Mip levels
+ * and light objects are rebuilt exactly as rm_scene_create builds them.  A
+ * material may change every value except its texture binding and whether it
+ * emits.  Frames rendered before the update no longer match the scene. */
+int32_t rm_scene_update(rm_scene *scene, const rm_material *materials, const float *texels, void *stream);
+
+/* Gradient outputs of rm_adjoint, all device memory, overwritten; any may be NULL. */
+typedef struct rm_grads {
+    float *textures;   /* d loss / d level-0 texel, same layout as rm_scene_update's texels */
+    float *roughness;  /* [n_materials] */
+    float *emission;   /* [n_materials][3] */
+    float *replay;     /* [h][w][3]: the linear image recomputed by the replay (a check of
+                          the replay against the render it differentiates) */
+} rm_grads;
+
+/* Path-replay adjoint of the linear Full image (RM_SHADE_FULL | RM_SHADE_LINEAR)
+ * of rm_render(scene, args, frame): given d_image = d loss / d image (device,
+ * [h][w][3]), writes d loss / d parameters.  `frame` must hold that render.
+ * Every sample path is replayed from its counter-RNG stream, twice (once for
+ * its contribution, once for the derivatives), so memory stays O(pixels).
+ * Albedo textures and emission get the exact derivative of the seeded
+ * estimator; roughness gets the pathwise derivative plus the score of every
+ * roughness-dependent draw (GGX half vectors, the cosine/GGX pick), an unbiased
+ * estimate of the derivative of the expected image.  Light-object and
+ * Russian-roulette probabilities are held fixed (both are unbiased selections).
+ * Scenes with transmissive materials are not supported.  Synchronous. */
+int32_t rm_adjoint(const rm_scene *scene, const rm_render_args *args, rm_frame *frame, const float *d_image,
+                   const rm_grads *grads, void *stream);
 
 /* ------------------------------------------------------ synthetic inputs */
 
@@ -251,7 +293,9 @@ enum rm_tune_key {
     RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 16) */
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive
                                            origins are far apart (default), 2 always */
-    RM_TUNE_RENDER_BLOCKS_PER_CU = 4    /* blocks per CU of the persistent bounce kernel (default 8) */
+    RM_TUNE_RENDER_BLOCKS_PER_CU = 4,   /* blocks per CU of the persistent bounce kernel (default 8) */
+    RM_TUNE_ADJOINT_SPLIT = 5           /* lanes per pixel of the adjoint, each replaying a share of
+                                           the pixel's samples (default 4) */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

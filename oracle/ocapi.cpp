@@ -18,6 +18,8 @@ using namespace ref;
 struct oracle_scene {
     Model model;
     std::vector<int32_t> perm;
+    std::vector<int32_t> mat_tex;                 // texture index per material (-1: none)
+    std::vector<std::pair<int, int>> tex_dims;    // (width, height) per texture of the description
 };
 
 namespace {
@@ -90,9 +92,12 @@ oracle_scene *oracle_scene_create(const rm_scene_desc *d) {
     const int64_t n = d->n_tris;
     const bool has_sky = d->sky_rgb && d->sky_width > 0 && d->sky_height > 0;
     M.materials.resize(d->n_materials);
+    for (int t = 0; t < d->n_textures; ++t) S->tex_dims.emplace_back(d->textures[t].width, d->textures[t].height);
+    S->mat_tex.assign(d->n_materials, -1);
     for (int m = 0; m < d->n_materials; ++m) {
         const rm_material &R = d->materials[m];
         Material &X = M.materials[m];
+        if (R.albedo_tex >= 0 && R.albedo_tex < d->n_textures) S->mat_tex[m] = R.albedo_tex;
         X.id = m;
         X.baseColor = vec3(R.base_color[0], R.base_color[1], R.base_color[2]);
         X.emission = has_sky ? vec3(0.0f) : vec3(R.emission[0], R.emission[1], R.emission[2]);
@@ -435,6 +440,50 @@ void oracle_bsdf_sample(const float *params, int32_t which, uint32_t seed, uint3
     put3(out8 + 3, pdf3);
     out8[6] = pdf;
     out8[7] = (float)fails;
+}
+
+// Forward-mode FP64 adjoint (oadjoint.cpp) along one parameter direction:
+// tan_tex = level-0 texel tangents of every texture of the description,
+// concatenated ([h][w][3] each; NULL = 0), tan_rough[n_materials],
+// tan_emis[n_materials][3].  Writes the linear Full image and its directional
+// derivative, [h][w][3] doubles each.  Returns 2 for a scene with transmission.
+int32_t oracle_adjoint_dir(const oracle_scene *S, const rm_render_args *a, const float *tan_tex,
+                           const double *tan_rough, const double *tan_emis, int32_t threads, double *image,
+                           double *dimage) {
+    if (!S || !a || a->width <= 0 || a->height <= 0 || a->max_depth > 16 || !tan_rough || !tan_emis) return 1;
+    const Model &M = S->model;
+    const size_t nm = M.materials.size();
+    AdjDirection dir;
+    dir.rough.assign(tan_rough, tan_rough + nm);
+    dir.emis.assign(tan_emis, tan_emis + nm * 3);
+    dir.tex.resize(nm);
+    std::vector<size_t> off(S->tex_dims.size() + 1, 0);
+    for (size_t t = 0; t < S->tex_dims.size(); ++t)
+        off[t + 1] = off[t] + (size_t)S->tex_dims[t].first * S->tex_dims[t].second * 3;
+    for (size_t m = 0; m < nm && tan_tex; ++m) {
+        int t = S->mat_tex[m];
+        const ImageData &img = M.materials[m].diffuse;
+        if (t < 0 || img.empty()) continue;
+        // the tangent's mip pyramid, built like ImageData::generateMipmaps
+        auto &lv = dir.tex[m];
+        lv.resize(img.map_depth);
+        lv[0].assign(tan_tex + off[t], tan_tex + off[t + 1]);
+        for (int level = 1; level < img.map_depth; ++level) {
+            int pw = img.width >> (level - 1), ph = img.height >> (level - 1);
+            int cw = img.width >> level, ch = img.height >> level;
+            lv[level].assign((size_t)cw * ch * 3, 0.0);
+            for (int y = 0; y < ch; ++y)
+                for (int x = 0; x < cw; ++x)
+                    for (int c = 0; c < 3; ++c) {
+                        double s = 0.0;
+                        for (int dy = 0; dy < 2; ++dy)
+                            for (int dx = 0; dx < 2; ++dx)
+                                s += lv[level - 1][((size_t)((y * 2 + dy) % ph) * pw + (x * 2 + dx) % pw) * 3 + c];
+                        lv[level][((size_t)y * cw + x) * 3 + c] = s / 4.0;
+                    }
+        }
+    }
+    return adjoint_render(M, to_args(a), dir, threads, image, dimage);
 }
 
 }  // extern "C"

@@ -153,6 +153,11 @@ void getHitMaterial(const Face &face, const vec3 &baryCoords, const vec3 &hit_dP
     if (hitInfo.opacity > 1.0f - eps_zero) hitInfo.entering = true;
     vec2 dUVdx = getDuv(face, hit_dPdx), dUVdy = getDuv(face, hit_dPdy);
     float duv = (isfinite(dUVdx) && isfinite(dUVdx)) ? (length(dUVdx) + length(dUVdy)) / 2.0f : NAN;
+    if (adj_tap) {
+        adj_tap->u = texUV[0];
+        adj_tap->v = texUV[1];
+        adj_tap->duv = duv;
+    }
     if (hitInfo.opacity < eps_zero)
         hitInfo.baseColor = material.transmittingColor;
     else {

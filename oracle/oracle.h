@@ -160,6 +160,21 @@ struct LightSample {                                    // include/sampling.h:28
     LightSample(const vec3 &b, const vec3 &l, float w) : bsdfPdf(b), light(l), weight(w) {}
 };
 const float eps_lightRadius = 1e-3f;                    // include/sampling.h:34
+// Recorder for the forward-mode adjoint (oadjoint.cpp).  While a thread's
+// adj_tap is set, getHitMaterial, the reflection samplers and
+// sampleDirectLight also report the values the derivative needs; they
+// compute nothing differently.
+struct AdjTap {
+    float u = 0, v = 0, duv = 0;           // getHitMaterial: texture coordinate and footprint
+    std::vector<float> ggx_ct;             // sampleBRDF: cos(theta_h) of every draw
+    float ct2 = 0, lh2 = 1;                // sampleBRDF: the accepted draw
+    vec3 raw1{0.0f};                       // sampleCos: brdfPdf before clampPdf
+    float pdf1 = 0, pdf2 = 0;              // sampleReflection: the two pdfs and the pick
+    bool first = false;
+    struct Light { vec3 dir; float dist, P; int index; };
+    std::vector<Light> lights;             // sampleDirectLight: accepted samples in order
+};
+extern thread_local AdjTap *adj_tap;
 void clampPdf(vec3 &pdf);                               // src/sampling.cpp:238-243 (`clamp`)
 // EXT: `genFor(T)` returns the generator used by light sample T.  The
 // reference passes one generator; the primary-vertex call gives every sample
@@ -212,7 +227,24 @@ struct Photo {                                          // include/image.h:15-67
     void postProcessing(int shadeOptions);              // src/image.cpp:470-479 (no bloom / DoF)
 };
 
+void calc_dPdxy(const Ray &ray, float hit_t, const vec3 &normal, const RayDifferential &base_diff, vec3 &hit_dPdx,
+                vec3 &hit_dPdy);                                                                // src/render.cpp:44-50
+void calc_dDdxy(const Ray &ray, const vec3 &normal, const RayDifferential &base_diff, vec3 &hit_dDdx,
+                vec3 &hit_dDdy);                                                                // src/render.cpp:52-60
+void getHitInfo(const HitRecord &hit, const Ray &ray, const RayDifferential &base_diff, vec3 &hit_dPdx,
+                vec3 &hit_dPdy, HitInfo &hitInfo);                                              // src/render.cpp:62-79
+void initRayDiff(const vec3 &d, const RenderArgs &args, RayDifferential &base_diff);           // src/render.cpp:439-446
 void renderPixel(const Model &model, const RenderArgs &args, Photo &image, int x, int y);   // src/render.cpp:448-551
+// Forward-mode FP64 adjoint (oadjoint.cpp): per-pixel linear Full image and
+// its derivative along one parameter direction (texel tangents per material
+// as mip pyramids, roughness and emission tangents per material).
+struct AdjDirection {
+    std::vector<std::vector<std::vector<double>>> tex;   // [material][level] texel tangents, empty = none
+    std::vector<double> rough;                            // [material]
+    std::vector<double> emis;                             // [material * 3]
+};
+int adjoint_render(const Model &model, const RenderArgs &args, const AdjDirection &dir, int threads, double *image,
+                   double *dimage);
 void render_multiThread(const Model &model, const RenderArgs &args, Photo &photo, int threads,
                         const std::vector<int> *rows = nullptr);                            // src/render.cpp:593-626
 void render_pixels(const Model &model, const RenderArgs &args, Photo &photo, int threads, const std::vector<int> &pix);

@@ -19,6 +19,7 @@ f32p = C.POINTER(C.c_float)
 i32p = C.POINTER(C.c_int32)
 u8p = C.POINTER(C.c_uint8)
 u64p = C.POINTER(C.c_uint64)
+f64p = C.POINTER(C.c_double)
 
 _lib = None
 
@@ -59,6 +60,7 @@ def lib():
             "oracle_accumulate": (None, [f32p, f32p, f32p, C.c_float, f32p, f32p]),
             "oracle_bsdf_eval": (None, [f32p, f32p, C.c_int32, f32p]),
             "oracle_bsdf_sample": (None, [f32p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, f32p]),
+            "oracle_adjoint_dir": (C.c_int32, [vp, vp, f32p, f64p, f64p, C.c_int32, f64p, f64p]),
         }
         for k, (res, args) in sig.items():
             fn = getattr(L, k)
@@ -171,6 +173,25 @@ def render_pixels(scene, args, pixels, threads=0):
     if st != 0:
         raise ValueError("oracle_render_pixels rejected the arguments")
     return g, r
+
+
+def adjoint_dir(scene, args, tan_tex, tan_rough, tan_emis, threads=0):
+    """FP64 forward-mode reference adjoint along one parameter direction.
+    tan_tex: level-0 texel tangents of all textures concatenated (or None);
+    tan_rough [n_materials]; tan_emis [n_materials, 3].  Returns the linear
+    Full image and its directional derivative, both float64 [h, w, 3]."""
+    h, w = args.height, args.width
+    img = np.empty((h, w, 3), dtype=np.float64)
+    dimg = np.empty((h, w, 3), dtype=np.float64)
+    tt = None if tan_tex is None else np.ascontiguousarray(tan_tex, dtype=np.float32)
+    tr = np.ascontiguousarray(tan_rough, dtype=np.float64)
+    te = np.ascontiguousarray(tan_emis, dtype=np.float64)
+    st = lib().oracle_adjoint_dir(scene.h, C.byref(args), fp(tt) if tt is not None else None,
+                                  tr.ctypes.data_as(f64p), te.ctypes.data_as(f64p), threads or lib().oracle_hw_threads(),
+                                  img.ctypes.data_as(f64p), dimg.ctypes.data_as(f64p))
+    if st != 0:
+        raise ValueError(f"oracle_adjoint_dir rejected the scene or arguments ({st})")
+    return img, dimg
 
 
 def ray_in_box(ray, box, tL, tR):

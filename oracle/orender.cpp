@@ -49,6 +49,8 @@ struct PathCtx {
     Medium mediums;
 };
 
+}  // namespace
+
 // src/render.cpp:44-50
 void calc_dPdxy(const Ray &ray, float hit_t, const vec3 &normal, const RayDifferential &base_diff, vec3 &hit_dPdx,
                 vec3 &hit_dPdy) {
@@ -79,6 +81,8 @@ void getHitInfo(const HitRecord &hit, const Ray &ray, const RayDifferential &bas
     if (dot(hitInfo.surfaceNormal, ray.direction) >= 0.0f) hitInfo.surfaceNormal = surfaceNormal_raw;
     if (dot(hitInfo.surfaceNormal, ray.direction) >= 0.0f) hitInfo.surfaceNormal = hitInfo.shapeNormal;
 }
+
+namespace {
 
 // src/render.cpp:83-87
 vec3 getAbsorb(const vec3 &absorb, float dis) {
@@ -253,6 +257,8 @@ void sampleIndirectLightFromFirstIntersection(const HitInfo &hitInfo, const vec3
     for (const auto &s : samplesNew) samples.push_back(s);
 }
 
+}  // namespace
+
 // src/render.cpp:439-446
 void initRayDiff(const vec3 &d, const RenderArgs &args, RayDifferential &base_diff) {
     base_diff.dPdx = vec3(0.0f), base_diff.dPdy = vec3(0.0f);
@@ -261,8 +267,6 @@ void initRayDiff(const vec3 &d, const RenderArgs &args, RayDifferential &base_di
     base_diff.dDdx = (d_dot_d * dddx - d * d_dot_dddx) / (std::sqrt(d_dot_d) * d_dot_d);
     base_diff.dDdy = (d_dot_d * dddy - d * d_dot_dddy) / (std::sqrt(d_dot_d) * d_dot_d);
 }
-
-}  // namespace
 
 // src/render.cpp:448-551
 void renderPixel(const Model &model, const RenderArgs &args, Photo &image, int x, int y) {

@@ -118,14 +118,18 @@ static const int MaxTrys = 16;   // src/sampling.cpp:201
 void BSDF::sampleBRDF(Generator &gen, vec3 &outDir, vec3 &brdfPdf, float &pdf, int &fails) const {
     vec3 tangent, bitangent;
     getTangentSpaceWithInDir(surface.surfaceNormal, inDir, tangent, bitangent);
+    float lastCt = 0.0f, lastLH = 1.0f;
     auto sampleGTR2 = [&](vec3 &od, float &p) {
         float u = gen(), phi = gen() * 2.0f * PI;
         float cosTheta = std::sqrt((1.0f - u) / (1.0f + (sqr(surface.roughness) - 1.0f) * u)),
               sinTheta = std::sqrt(1 - cosTheta * cosTheta);
+        if (adj_tap) adj_tap->ggx_ct.push_back(cosTheta);
         vec3 H = sinTheta * std::cos(phi) * tangent + sinTheta * std::sin(phi) * bitangent + cosTheta * surface.surfaceNormal;
         od = 2.0f * dot(inDir, H) * H - inDir;
         float LDotH = dot(od, H);
         float LdotN = dot(od, surface.surfaceNormal);
+        lastCt = cosTheta;
+        lastLH = LDotH;
         if (LDotH <= 0.0f || LdotN <= 0.0f) {
             p = 0.0f;
             return;
@@ -135,6 +139,10 @@ void BSDF::sampleBRDF(Generator &gen, vec3 &outDir, vec3 &brdfPdf, float &pdf, i
     for (int T = 1; T <= MaxTrys; T++) {
         sampleGTR2(outDir, pdf);
         if (dot(outDir, surface.shapeNormal) > 0.0f && pdf > 0.0f) {
+            if (adj_tap) {
+                adj_tap->ct2 = lastCt;
+                adj_tap->lh2 = lastLH;
+            }
             brdfPdf = getBRDF(outDir) / pdf;
             return;
         }
@@ -168,6 +176,7 @@ void BSDF::sampleCos(Generator &gen, vec3 &outDir, vec3 &brdfPdf, float &pdf, in
         pdf = dot(outDir, surface.surfaceNormal) / PI;
         if (dot(outDir, surface.shapeNormal) > 0.0f && pdf > 0.0f) {
             brdfPdf = getBRDF(outDir) / pdf;
+            if (adj_tap) adj_tap->raw1 = brdfPdf;
             clampPdf(brdfPdf);
             return;
         }
@@ -211,7 +220,13 @@ void BSDF::sampleReflection(Generator &gen, vec3 &Dir, vec3 &brdfPdf, int &fails
     sampleCos(gen, Dir1, brdfPdf1, pdf1, fail1);
     sampleBRDF(gen, Dir2, brdfPdf2, pdf2, fail2);
     float p1 = pdf1 / (pdf1 + pdf2);
-    if (gen() < p1) {
+    bool first = gen() < p1;
+    if (adj_tap) {
+        adj_tap->pdf1 = pdf1;
+        adj_tap->pdf2 = pdf2;
+        adj_tap->first = first;
+    }
+    if (first) {
         Dir = Dir1;
         brdfPdf = brdfPdf1;
         fails += fail1;
@@ -368,6 +383,7 @@ std::vector<LightSample> sampleDirectLight(const BSDF &bsdf, const Model &model,
             vec3 light = 2.0f * PI * lightObject.power * lightObject.color / (distance * distance + eps_lightRadius) /
                          P_lightObject;
             samples.emplace_back(bsdfPdf, light, 1.0f / static_cast<float>(sampleCnt * (fails + 1)));
+            if (adj_tap) adj_tap->lights.push_back({lightDir, distance, P_lightObject, lightIndex});
         }
     } else {
         for (int T = 0; T < sampleCnt; T++) {
@@ -379,6 +395,7 @@ std::vector<LightSample> sampleDirectLight(const BSDF &bsdf, const Model &model,
             vec3 bsdfPdf = bsdf.getBSDF(Dir);
             clampPdf(bsdfPdf);
             samples.emplace_back(bsdfPdf, light, 1.0f / static_cast<float>(sampleCnt));
+            if (adj_tap) adj_tap->lights.push_back({Dir, INFINITY, 1.0f, -1});
         }
     }
     return samples;

@@ -100,6 +100,19 @@ class RmBatchJob(C.Structure):
     ]
 
 
+class RmGrads(C.Structure):
+    _fields_ = [
+        ("textures", C.c_void_p),
+        ("roughness", C.c_void_p),
+        ("emission", C.c_void_p),
+        ("replay", C.c_void_p),
+    ]
+
+
+RM_SHADE_FULL = 63
+RM_SHADE_LINEAR = 65536
+
+
 class RmGenOptions(C.Structure):
     _fields_ = [
         ("n_tris", C.c_int64),
@@ -130,6 +143,10 @@ SIGNATURES = [
     ("rm_render", C.c_int32, [C.c_void_p, C.POINTER(RmRenderArgs), C.c_void_p, C.c_void_p]),
     ("rm_postprocess", C.c_int32, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p]),
     ("rm_render_batch", C.c_int32, [C.POINTER(RmBatchJob), C.c_int32, C.c_int32, C.c_void_p]),
+    ("rm_frame_copy", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("rm_scene_update", C.c_int32, [C.c_void_p, C.POINTER(RmMaterial), C.c_void_p, C.c_void_p]),
+    ("rm_adjoint", C.c_int32, [C.c_void_p, C.POINTER(RmRenderArgs), C.c_void_p, C.c_void_p, C.POINTER(RmGrads),
+                               C.c_void_p]),
     ("rm_generate_scene", C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(RmGenOptions), C.POINTER(C.c_void_p)]),
     ("rm_generated_desc", C.c_int32, [C.c_void_p, C.POINTER(RmSceneDesc), C.POINTER(RmRenderArgs)]),
     ("rm_generated_destroy", C.c_int32, [C.c_void_p]),

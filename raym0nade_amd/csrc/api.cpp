@@ -71,6 +71,10 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
             if (value < 1 || value > 64) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: render blocks per CU %d", value);
             t.render_blocks_per_cu = value;
             return RM_OK;
+        case RM_TUNE_ADJOINT_SPLIT:
+            if (value < 1 || value > 64) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: adjoint split %d", value);
+            t.adjoint_split = value;
+            return RM_OK;
         default:
             return rm::fail(RM_ERR_INVALID, "rm_set_tuning: unknown key %d", key);
     }
@@ -85,6 +89,7 @@ extern "C" int32_t rm_get_tuning(int32_t key, int32_t *value) {
         case RM_TUNE_FETCH_MIN: *value = t.fetch_min; return RM_OK;
         case RM_TUNE_RAY_SORT: *value = t.ray_sort; return RM_OK;
         case RM_TUNE_RENDER_BLOCKS_PER_CU: *value = t.render_blocks_per_cu; return RM_OK;
+        case RM_TUNE_ADJOINT_SPLIT: *value = t.adjoint_split; return RM_OK;
         default: return rm::fail(RM_ERR_INVALID, "rm_get_tuning: unknown key %d", key);
     }
 }

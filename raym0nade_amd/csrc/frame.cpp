@@ -37,6 +37,13 @@ extern "C" int32_t rm_frame_buffer(rm_frame *f, int32_t which, float **dev_ptr) 
     return RM_OK;
 }
 
+extern "C" int32_t rm_frame_copy(const rm_frame *f, int32_t which, float *dst, void *stream) {
+    if (!f || !dst || which < 0 || which >= RM_NUM_BUFFERS) return fail(RM_ERR_INVALID, "rm_frame_copy: bad argument");
+    RM_HIP(hipMemcpyAsync(dst, f->buf[which].p, (size_t)f->w * f->h * frame_channels(which) * sizeof(float),
+                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return RM_OK;
+}
+
 extern "C" int32_t rm_frame_download(const rm_frame *f, int32_t which, float *host) {
     if (!f || !host || which < 0 || which >= RM_NUM_BUFFERS) return fail(RM_ERR_INVALID, "rm_frame_download: bad argument");
     RM_HIP(hipDeviceSynchronize());

@@ -12,7 +12,8 @@ namespace rm {
 constexpr int PB = 256;
 enum : int {
     O_BaseColor = 1, O_Emission = 2, O_Direct = 4, O_Indirect = 8, O_Diffuse = 16, O_Specular = 32,
-    O_ShapeNormal = 64, O_SurfaceNormal = 128, O_Bloom = 256, O_FXAA = 512, O_DoF = 1024
+    O_ShapeNormal = 64, O_SurfaceNormal = 128, O_Bloom = 256, O_FXAA = 512, O_DoF = 1024,
+    O_Linear = RM_SHADE_LINEAR  // no gammaCorrection (not a Photo::ShadeOption)
 };
 
 __device__ __forceinline__ V3 ld3p(const float *p, int64_t i) { return V3{p[i * 3], p[i * 3 + 1], p[i * 3 + 2]}; }
@@ -45,13 +46,15 @@ __global__ __launch_bounds__(PB) void k_shade_gamma(PostIn I, int64_t n, int opt
         px = dc * rd + rs;
         if (opt & O_Emission) px += ld3p(I.emis, i) * exposure;
     }
-    // gammaCorrection: max(0), tanh luminance soft clip above 0.75, min(1), ^(1/2.2)
-    px = vmax(px, v3(0.0f));
-    float c = dot(px, V3{0.3f, 0.6f, 0.1f});
-    if (c > 0.75f) px = px / c * (tanhf(3.0f * (c - 0.75f)) / 3.0f + 0.75f);
-    px = vmin(px, v3(1.0f));
-    const float e = 1.0f / 2.2f;
-    px = V3{powf(px.x, e), powf(px.y, e), powf(px.z, e)};
+    if (!(opt & O_Linear)) {
+        // gammaCorrection: max(0), tanh luminance soft clip above 0.75, min(1), ^(1/2.2)
+        px = vmax(px, v3(0.0f));
+        float c = dot(px, V3{0.3f, 0.6f, 0.1f});
+        if (c > 0.75f) px = px / c * (tanhf(3.0f * (c - 0.75f)) / 3.0f + 0.75f);
+        px = vmin(px, v3(1.0f));
+        const float e = 1.0f / 2.2f;
+        px = V3{powf(px.x, e), powf(px.y, e), powf(px.z, e)};
+    }
     out[i * 3] = px.x;
     out[i * 3 + 1] = px.y;
     out[i * 3 + 2] = px.z;

@@ -31,7 +31,7 @@
 #include <algorithm>
 #include <vector>
 #include "rm_frame.h"
-#include "shade.h"
+#include "integrator.h"
 
 namespace rm {
 
@@ -180,17 +180,6 @@ struct AccBufs {
     float4 *a[5];
 };
 
-struct FrameView {
-    float *pos, *shapeN, *surfN, *emis, *base, *Dd, *Ds, *Id, *Is;
-};
-
-struct RArgs {
-    V3 pos, dir, up, right;
-    float acc, exposure, p_direct;
-    int w, h, spp, max_depth, rr_min, K;
-    uint32_t seed;
-};
-
 // One frame of a (possibly batched) launch.  Per-pixel work buffers are
 // indexed by the global pixel index (jobs concatenated); the frame buffers by
 // the job-local pixel index.
@@ -223,25 +212,6 @@ __device__ __forceinline__ int job_of(const Job *J, int nj, int64_t p) {
         if (J[mid].pix_off <= p) lo = mid; else hi = mid - 1;
     }
     return lo;
-}
-
-__device__ __forceinline__ void st3(float *p, int64_t i, V3 v) {
-    p[i * 3] = v.x;
-    p[i * 3 + 1] = v.y;
-    p[i * 3 + 2] = v.z;
-}
-
-// renderPixel's camera ray and initRayDiff (src/render.cpp:439-477)
-__device__ __forceinline__ void camera_ray(const RArgs &A, int x, int y, V3 &Dir, Diff &diff) {
-    float rx = float(x) - float(A.w) / 2.0f, ry = float(y) - float(A.h) / 2.0f;
-    V3 d = A.dir + A.acc * (rx * A.right + ry * A.up);
-    diff.dPdx = v3(0.0f);
-    diff.dPdy = v3(0.0f);
-    V3 ddx = A.acc * A.right, ddy = A.acc * A.up;
-    float dd = dot(d, d), dx = dot(d, ddx), dy = dot(d, ddy);
-    diff.dDdx = (dd * ddx - d * dx) / (sqrtf(dd) * dd);
-    diff.dDdy = (dd * ddy - d * dy) / (sqrtf(dd) * dd);
-    Dir = normalize(d);
 }
 
 // p: global pixel (work buffers), lp: job-local pixel (frame buffers)
@@ -298,10 +268,7 @@ __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int n
     hit_info(S, slot, Dir, t, base, dPdx, dPdy, s);
     bool opaque = s.opacity > 1.0f - T_EPS;
     V3 sav = s.base;
-    float c0 = dot(s.base, W_RGB());
-    // the reference compares against double literals and adds 4e-2 in double
-    if ((double)c0 < 2e-2 || (c0 < 0.8f && (double)length(s.base / c0 - v3(1.0f)) < 2e-2))
-        s.base.x = (float)((double)s.base.x + 4e-2);
+    nudge_base(s.base);
     int spp_direct = (int)(float(A.spp) * A.p_direct);
     int n_ind = (A.spp - spp_direct) * (opaque ? 1 : 16);
     st3(F.pos, p, s.pos);
@@ -453,13 +420,16 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
 }
 
 // Light sample of a finished path -> its accumulator set.
+// Light sample k of indirect sample s is named s * 8 + k (k < 6): the name
+// k_finish records for the dropped outlier, so the adjoint skips the same one.
 struct SetAcc {
     Acc4 d, s;
     float S, maxC, cnt;
+    uint32_t maxid;
     Acc4 md, ms;
 };
 
-__device__ __forceinline__ void add_sample(SetAcc &a, V3 base, V3 p0, V3 light, float w) {
+__device__ __forceinline__ void add_sample(SetAcc &a, V3 base, V3 p0, V3 light, float w, uint32_t id) {
     float c = dot(p0 * light, W_RGB()) * w;
     Acc4 dd{0, 0, 0, 0}, ds{0, 0, 0, 0};
     accumulate(base, LSample{p0, light, w}, dd, ds);
@@ -469,6 +439,7 @@ __device__ __forceinline__ void add_sample(SetAcc &a, V3 base, V3 p0, V3 light, 
     a.cnt += 1.0f;
     if (c > a.maxC) {
         a.maxC = c;
+        a.maxid = id;
         a.md = dd;
         a.ms = ds;
     }
@@ -479,7 +450,7 @@ __device__ __forceinline__ SetAcc load_set(const AccBufs &B, uint32_t g) {
     float4 x0 = B.a[0][g], x1 = B.a[1][g], x2 = B.a[2][g], x3 = B.a[3][g], x4 = B.a[4][g];
     a.d = Acc4{x0.x, x0.y, x0.z, x0.w};
     a.s = Acc4{x1.x, x1.y, x1.z, x1.w};
-    a.S = x2.x; a.maxC = x2.y; a.cnt = x2.z;
+    a.S = x2.x; a.maxC = x2.y; a.cnt = x2.z; a.maxid = bu(x2.w);
     a.md = Acc4{x3.x, x3.y, x3.z, x3.w};
     a.ms = Acc4{x4.x, x4.y, x4.z, x4.w};
     return a;
@@ -487,12 +458,10 @@ __device__ __forceinline__ SetAcc load_set(const AccBufs &B, uint32_t g) {
 __device__ __forceinline__ void store_set(const AccBufs &B, uint32_t g, const SetAcc &a) {
     B.a[0][g] = make_float4(a.d.r, a.d.g, a.d.b, a.d.v);
     B.a[1][g] = make_float4(a.s.r, a.s.g, a.s.b, a.s.v);
-    B.a[2][g] = make_float4(a.S, a.maxC, a.cnt, 0.0f);
+    B.a[2][g] = make_float4(a.S, a.maxC, a.cnt, ub(a.maxid));
     B.a[3][g] = make_float4(a.md.r, a.md.g, a.md.b, a.md.v);
     B.a[4][g] = make_float4(a.ms.r, a.ms.g, a.ms.b, a.ms.v);
 }
-
-__constant__ int c_sample_count[17] = {0, 1, 2, 2, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 5, 6};  // src/render.cpp:126
 
 // One sampleRay vertex (src/render.cpp:121-312) for every live path.
 // Persistent: each wave claims 64 queue entries per atomic on `work` until the
@@ -600,7 +569,7 @@ __global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj
                     }
                 }
                 if (nee) {
-                    int cnt = c_sample_count[depth];
+                    int cnt = sample_count(depth);
                     float total = direct_total(S, s, V);
                     if (total != 0.0f) {
                         for (int T = 0; T < cnt; ++T) {
@@ -637,7 +606,7 @@ __global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj
                 V3 L = ls[k].light * ls[k].bsdf;
                 if (extra_on) L = L * extra;
                 L = L * path.T;
-                add_sample(a, base, path.p0, L, ls[k].weight * path.W * inv);
+                add_sample(a, base, path.p0, L, ls[k].weight * path.W * inv, path.smp * 8u + (uint32_t)k);
             }
             store_set(AB, path.set, a);
         }
@@ -673,9 +642,11 @@ __global__ __launch_bounds__(RB) void k_finish(const Job *__restrict__ J, int nj
     const int64_t p = gp - jb.pix_off;
     int M = P.M[gp];
     Acc4 d{0, 0, 0, 0}, s{0, 0, 0, 0};
+    int32_t dropped = -1;
     if (M > 0) {
         int off = P.set_off[gp];
         float S = 0.0f, cnt = 0.0f, maxC = -1.0f;
+        uint32_t maxid = 0;
         Acc4 md{0, 0, 0, 0}, ms{0, 0, 0, 0};
         for (int k = 0; k < M; ++k) {
             SetAcc a = load_set(AB, (uint32_t)(off + k));
@@ -683,12 +654,13 @@ __global__ __launch_bounds__(RB) void k_finish(const Job *__restrict__ J, int nj
             s.r += a.s.r; s.g += a.s.g; s.b += a.s.b; s.v += a.s.v;
             S += a.S;
             cnt += a.cnt;
-            if (a.maxC > maxC) { maxC = a.maxC; md = a.md; ms = a.ms; }
+            if (a.maxC > maxC) { maxC = a.maxC; maxid = a.maxid; md = a.md; ms = a.ms; }
         }
         if (cnt > 0.0f) {
             if (maxC >= 0.0f && maxC / (S - maxC + T_EPS) > 16.0f) {
                 d.r -= md.r; d.g -= md.g; d.b -= md.b; d.v -= md.v;
                 s.r -= ms.r; s.g -= ms.g; s.b -= ms.b; s.v -= ms.v;
+                dropped = (int32_t)maxid;
             }
             calc_var(d, A.exposure);
             calc_var(s, A.exposure);
@@ -698,6 +670,7 @@ __global__ __launch_bounds__(RB) void k_finish(const Job *__restrict__ J, int nj
     }
     reinterpret_cast<float4 *>(F.Id)[p] = make_float4(d.r, d.g, d.b, d.v);
     reinterpret_cast<float4 *>(F.Is)[p] = make_float4(s.r, s.g, s.b, s.v);
+    F.drop[p] = dropped;
 }
 
 // Exclusive prefix of M over pixels, two passes with the block totals scanned
@@ -763,20 +736,12 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, const 
     if (!(a->p_direct >= 0.0f && a->p_direct <= 1.0f)) return fail(RM_ERR_INVALID, "%s: p_direct outside [0,1]", fn);
     if (h.scene->view.depth > MAX_STACK) return fail(RM_ERR_UNSUPPORTED, "%s: tree too deep", fn);
     const int64_t np = (int64_t)f->w * f->h;
+    if (f->drop.bytes < (size_t)np * 4) {
+        int32_t rc = f->drop.alloc((size_t)np * 4);
+        if (rc) return rc;
+    }
     RArgs &A = J.A;
-    A.pos = V3{a->position[0], a->position[1], a->position[2]};
-    A.dir = V3{a->direction[0], a->direction[1], a->direction[2]};
-    A.up = V3{a->up[0], a->up[1], a->up[2]};
-    A.right = V3{a->right[0], a->right[1], a->right[2]};
-    A.acc = a->accuracy;
-    A.exposure = a->exposure;
-    A.p_direct = a->p_direct;
-    A.w = a->width;
-    A.h = a->height;
-    A.spp = a->spp;
-    A.max_depth = max_depth;
-    A.rr_min = a->rr_min_depth > 0 ? a->rr_min_depth : 1;
-    A.seed = a->seed;
+    A = rargs_from(a);
     int K = a->paths_per_round;
     if (K <= 0) K = (int)std::min<int64_t>(16, std::max<int64_t>(1, (int64_t)(4 << 20) / std::max<int64_t>(1, np)));
     A.K = K;
@@ -795,6 +760,7 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, const 
     F.Ds = f->buf[RM_BUF_DS].as<float>();
     F.Id = f->buf[RM_BUF_ID].as<float>();
     F.Is = f->buf[RM_BUF_IS].as<float>();
+    F.drop = f->drop.as<int32_t>();
     J.np = np;
     return RM_OK;
 }
@@ -921,6 +887,14 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     }
     RM_HIP(hipGetLastError());
+    // what each frame now holds, for rm_adjoint
+    for (int i = 0; i < nj; ++i) {
+        rm_frame *f = hj[i].frame;
+        f->stamp_scene = hj[i].scene;
+        f->stamp_version = hj[i].scene->version;
+        f->stamp_args = *hj[i].args;
+        f->stamped = true;
+    }
     return RM_OK;
 }
 

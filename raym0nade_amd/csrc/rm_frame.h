@@ -13,6 +13,13 @@ struct rm_frame {
     rm::DevBuf work;
     size_t work_bytes = 0;
     rm::DevBuf post;  // tone-mapped image read by FXAA
+    rm::DevBuf drop;  // per pixel: the dropped outlier light sample (render -> adjoint)
+    rm::DevBuf adj;   // adjoint scratch: texel gradients of every mip level, material gradients
+    // the render the buffers hold (the adjoint differentiates exactly that one)
+    const rm_scene *stamp_scene = nullptr;
+    uint64_t stamp_version = 0;
+    rm_render_args stamp_args{};
+    bool stamped = false;
 };
 
 namespace rm {

@@ -90,6 +90,7 @@ struct Tuning {
     int fetch_min = 16;           // idle lanes before a wave claims new rays
     int ray_sort = 1;             // 0 never, 1 when consecutive origins are far apart, 2 always
     int render_blocks_per_cu = 8; // resident blocks per CU of the persistent bounce kernel
+    int adjoint_split = 4;        // lanes per pixel of the adjoint kernel
 };
 Tuning &tuning();
 int device_cu_count();
@@ -101,4 +102,11 @@ struct rm_scene {
     rm::SceneView view{};
     rm::DevBuf nodes, tris, shade, mats, texs, texels, lights, light_faces, light_cdf, sky, sky_cdf, sky_guide;
     rm_scene_info info{};
+    uint64_t version = 0;  // bumped by rm_scene_update
 };
+
+namespace rm {
+// Box-filtered mip levels 1.. of texture `t` from its level 0 in the device
+// texel pool, the same arithmetic as the host build (stream-ordered).
+int32_t launch_mip_build(const rm_scene *s, int t, hipStream_t st);
+}  // namespace rm

@@ -119,6 +119,74 @@ float reject_threshold(float L) {
 
 uint32_t leaf_code(int32_t first, int32_t count) { return ((uint32_t)first << LEAF_SHIFT) | (uint32_t)count; }
 
+MatRec mat_rec(const rm_material &M, bool has_sky) {
+    MatRec R;
+    memset(&R, 0, sizeof(R));
+    for (int k = 0; k < 3; ++k) {
+        R.base[k] = M.base_color[k];
+        // with a sky the reference never loads emissive textures (src/model.cpp:140)
+        R.emission[k] = has_sky ? 0.0f : M.emission[k];
+        R.trans[k] = M.transmitting_color[k];
+    }
+    R.roughness = M.roughness;
+    R.metallic = M.metallic;
+    R.opacity = M.opacity;
+    R.ior = M.ior;
+    R.tex = M.albedo_tex;
+    return R;
+}
+
+// getAverageEmissiveColor (src/model.cpp:23-42): the mean of an 8x8
+// barycentric grid of emission lookups, constant emission here.
+V3 emissive_avg(V3 em) {
+    V3 avg = v3(0.0f);
+    for (int q = 0; q < 64; ++q) avg += em;
+    return avg / 64.0f;
+}
+float emissive_lum(V3 em) { return dot(emissive_avg(em), v3(0.3f, 0.6f, 0.1f)); }
+
+// The rest of checkLightObject (src/model.cpp:44-82) for every light object
+// whose faces are chosen: colour = area-weighted mean emission normalised to
+// unit luminance, power = sum of area * luminance, centre = power-weighted
+// centroid, and the face CDF (RandomDistribution::Init, src/component.cpp:12-19).
+void light_params(HostScene &hs) {
+    for (LightRec &L : hs.lights) {
+        V3 color = v3(0.0f);
+        std::vector<float> w(L.n_faces);
+        for (int k = 0; k < L.n_faces; ++k) {
+            const TriShade &F = hs.light_faces[L.first_face + k];
+            const MatRec &M = hs.mats[F.material];
+            V3 avg = emissive_avg(V3{M.emission[0], M.emission[1], M.emission[2]});
+            float Clum = dot(avg, v3(0.3f, 0.6f, 0.1f));
+            V3 v0{F.v[0][0], F.v[0][1], F.v[0][2]}, v1{F.v[1][0], F.v[1][1], F.v[1][2]},
+                v2{F.v[2][0], F.v[2][1], F.v[2][2]};
+            float area = length(cross(v1 - v0, v2 - v0)) / 2.0f;
+            color += avg * area;
+            w[k] = area * Clum;
+        }
+        color = color / dot(color, v3(0.3f, 0.6f, 0.1f));
+        float power = 0.0f;
+        V3 center = v3(0.0f);
+        for (int k = 0; k < L.n_faces; ++k) {
+            power += w[k];
+            const TriShade &F = hs.light_faces[L.first_face + k];
+            V3 v0{F.v[0][0], F.v[0][1], F.v[0][2]}, v1{F.v[1][0], F.v[1][1], F.v[1][2]},
+                v2{F.v[2][0], F.v[2][1], F.v[2][2]};
+            V3 c = (v0 + v1 + v2) / 3.0f;
+            center += c * w[k];
+        }
+        center /= power;
+        L.center[0] = center.x; L.center[1] = center.y; L.center[2] = center.z;
+        L.power = power;
+        L.color[0] = color.x; L.color[1] = color.y; L.color[2] = color.z;
+        float acc = 0.0f;
+        for (int k = 0; k < L.n_faces; ++k) {
+            acc = k == 0 ? w[0] : acc + w[k];
+            hs.light_cdf[L.first_face + k] = acc;
+        }
+    }
+}
+
 }  // namespace
 
 int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
@@ -228,22 +296,7 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
 
     // ---- materials and textures
     hs.mats.resize(d->n_materials);
-    for (int32_t m = 0; m < d->n_materials; ++m) {
-        const rm_material &M = d->materials[m];
-        MatRec &R = hs.mats[m];
-        memset(&R, 0, sizeof(R));
-        for (int k = 0; k < 3; ++k) {
-            R.base[k] = M.base_color[k];
-            // with a sky the reference never loads emissive textures (src/model.cpp:140)
-            R.emission[k] = has_sky ? 0.0f : M.emission[k];
-            R.trans[k] = M.transmitting_color[k];
-        }
-        R.roughness = M.roughness;
-        R.metallic = M.metallic;
-        R.opacity = M.opacity;
-        R.ior = M.ior;
-        R.tex = M.albedo_tex;
-    }
+    for (int32_t m = 0; m < d->n_materials; ++m) hs.mats[m] = mat_rec(d->materials[m], has_sky);
     for (int32_t t = 0; t < d->n_textures; ++t) {
         // ImageData::generateMipmaps (src/material.cpp:113-148) on fp32 texels:
         // up to 8 levels, 2x2 box average with wrap, stop when a side reaches 0.
@@ -290,44 +343,21 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
             bool emissive = M.emission[0] != 0.0f || M.emission[1] != 0.0f || M.emission[2] != 0.0f;
             if (emissive) {
                 LightRec L{};
-                V3 color = v3(0.0f);
-                std::vector<float> w;
                 std::vector<int64_t> faces;
                 V3 em{M.emission[0], M.emission[1], M.emission[2]};
                 for (int64_t f = i; f < j; ++f) {
-                    // getAverageEmissiveColor: 8x8 barycentric grid average
-                    V3 avg = v3(0.0f);
-                    for (int q = 0; q < 64; ++q) avg += em;
-                    avg = avg / 64.0f;
-                    float Clum = dot(avg, v3(0.3f, 0.6f, 0.1f));
+                    float Clum = emissive_lum(em);
                     V3 v0 = B.vert((int32_t)f, 0), v1 = B.vert((int32_t)f, 1), v2 = B.vert((int32_t)f, 2);
                     float area = length(cross(v1 - v0, v2 - v0)) / 2.0f;
                     if (Clum < EPS || area < EPS) continue;
-                    color += avg * area;
-                    w.push_back(area * Clum);
                     faces.push_back(f);
                 }
                 if (!faces.empty()) {
-                    color = color / dot(color, v3(0.3f, 0.6f, 0.1f));
-                    float power = 0.0f;
-                    V3 center = v3(0.0f);
-                    for (size_t k = 0; k < faces.size(); ++k) {
-                        power += w[k];
-                        int32_t f = (int32_t)faces[k];
-                        V3 c = (B.vert(f, 0) + B.vert(f, 1) + B.vert(f, 2)) / 3.0f;
-                        center += c * w[k];
-                    }
-                    center /= power;
-                    L.center[0] = center.x; L.center[1] = center.y; L.center[2] = center.z;
-                    L.power = power;
-                    L.color[0] = color.x; L.color[1] = color.y; L.color[2] = color.z;
+                    // the faces only; power, colour, centre and face CDF come from light_params
                     L.first_face = (int32_t)hs.light_faces.size();
                     L.n_faces = (int32_t)faces.size();
-                    float acc = 0.0f;
                     for (size_t k = 0; k < faces.size(); ++k) {
-                        acc = k == 0 ? w[0] : acc + w[k];  // RandomDistribution::Init
-                        hs.light_cdf.push_back(acc);
-                        // find the slot of this original face to copy its shading record
+                        hs.light_cdf.push_back(0.0f);
                         hs.light_faces.push_back(TriShade{});
                     }
                     // fill copies from original-order data
@@ -359,6 +389,7 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
             }
             i = j;
         }
+        light_params(hs);
     }
 
     // ---- sky (SkyBox::Init, src/component.cpp:54-67)
@@ -513,5 +544,49 @@ extern "C" int32_t rm_scene_get_info(const rm_scene *scene, rm_scene_info *info)
 extern "C" int32_t rm_scene_get_perm(const rm_scene *scene, int32_t *perm) {
     if (!scene || !perm) return fail(RM_ERR_INVALID, "rm_scene_get_perm: NULL argument");
     memcpy(perm, scene->host.perm.data(), scene->host.perm.size() * sizeof(int32_t));
+    return RM_OK;
+}
+
+extern "C" int32_t rm_scene_update(rm_scene *scene, const rm_material *materials, const float *texels, void *stream) {
+    if (!scene) return fail(RM_ERR_INVALID, "rm_scene_update: scene is NULL");
+    HostScene &h = scene->host;
+    hipStream_t st = (hipStream_t)stream;
+    const bool has_sky = !h.sky.empty();
+    if (materials) {
+        std::vector<MatRec> next(h.mats.size());
+        for (size_t m = 0; m < h.mats.size(); ++m) {
+            next[m] = mat_rec(materials[m], has_sky);
+            const MatRec &o = h.mats[m], &n = next[m];
+            if (n.tex != o.tex) return fail(RM_ERR_INVALID, "rm_scene_update: material %zu changes its texture", m);
+            bool e0 = o.emission[0] != 0.0f || o.emission[1] != 0.0f || o.emission[2] != 0.0f;
+            bool e1 = n.emission[0] != 0.0f || n.emission[1] != 0.0f || n.emission[2] != 0.0f;
+            if (e0 != e1) return fail(RM_ERR_INVALID, "rm_scene_update: material %zu changes whether it emits", m);
+            if (e1 && emissive_lum(V3{n.emission[0], n.emission[1], n.emission[2]}) < EPS)
+                return fail(RM_ERR_INVALID, "rm_scene_update: emission of material %zu is below the light threshold", m);
+        }
+        h.mats.swap(next);
+        light_params(h);
+        RM_HIP(hipMemcpy(scene->mats.p, h.mats.data(), h.mats.size() * sizeof(MatRec), hipMemcpyHostToDevice));
+        if (!h.lights.empty()) {
+            RM_HIP(hipMemcpy(scene->lights.p, h.lights.data(), h.lights.size() * sizeof(LightRec),
+                             hipMemcpyHostToDevice));
+            RM_HIP(hipMemcpy(scene->light_cdf.p, h.light_cdf.data(), h.light_cdf.size() * sizeof(float),
+                             hipMemcpyHostToDevice));
+        }
+    }
+    if (texels) {
+        int64_t off = 0;
+        for (size_t t = 0; t < h.texs.size(); ++t) {
+            const TexRec &T = h.texs[t];
+            const int64_t n = (int64_t)T.width * T.height * 3;
+            RM_HIP(hipMemcpyAsync(scene->texels.as<float>() + T.offset[0], texels + off, n * sizeof(float),
+                                  hipMemcpyDeviceToDevice, st));
+            int32_t rc = launch_mip_build(scene, (int)t, st);
+            if (rc) return rc;
+            off += n;
+        }
+        RM_HIP(hipStreamSynchronize(st));
+    }
+    ++scene->version;
     return RM_OK;
 }

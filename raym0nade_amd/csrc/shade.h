@@ -158,8 +158,17 @@ __device__ __forceinline__ void calc_surface_normal(const TriShade &F, V3 shapeN
     if (!any_finite(surfN)) surfN = sav;
 }
 
-// getHitMaterial (src/model.cpp:300-328)
-__device__ __forceinline__ void hit_material(const SceneView &S, const TriShade &F, V3 bc, V3 dPdx, V3 dPdy, Surf &h) {
+// Where a vertex's albedo came from: texture `tex` at (u, v) and mip depth
+// `depth` as tex_get read it, or tex = -1 for a constant colour.
+struct TexFoot {
+    int tex;
+    float u, v, depth;
+};
+
+// getHitMaterial (src/model.cpp:300-328).  `tf`, when given, receives the
+// albedo footprint (the adjoint scatters texel gradients through it).
+__device__ __forceinline__ void hit_material(const SceneView &S, const TriShade &F, V3 bc, V3 dPdx, V3 dPdy, Surf &h,
+                                             TexFoot *tf = nullptr) {
     V2 u0{F.uv[0][0], F.uv[0][1]}, u1{F.uv[1][0], F.uv[1][1]}, u2{F.uv[2][0], F.uv[2][1]};
     V2 uv = (bc.x * u0 + bc.y * u1) + bc.z * u2;
     const MatRec &M = S.mats[F.material];
@@ -171,12 +180,14 @@ __device__ __forceinline__ void hit_material(const SceneView &S, const TriShade 
     if (h.opacity > 1.0f - T_EPS) h.entering = true;
     V2 ddx = get_duv(F, dPdx), ddy = get_duv(F, dPdy);
     float duv = any_finite2(ddx) ? (len2d(ddx) + len2d(ddy)) / 2.0f : NAN;
+    if (tf) tf->tex = -1;
     if (h.opacity < T_EPS) {
         h.base = ld3(M.trans);
     } else if (M.tex >= 0) {
         const TexRec &T = S.texs[M.tex];
         float depth = isnan(duv) ? 0.0f : log2f(duv * float(T.width));
         h.base = tex_get(S, T, uv.x, uv.y, depth);
+        if (tf) *tf = TexFoot{M.tex, uv.x, uv.y, depth};
     } else {
         h.base = ld3(M.base);
     }
@@ -203,13 +214,13 @@ __device__ __forceinline__ void calc_dDdxy(V3 dir, V3 n, const Diff &b, V3 &dDdx
 
 // getHitInfo (src/render.cpp:62-79); h.pos must be set by the caller
 __device__ __forceinline__ void hit_info(const SceneView &S, int32_t slot, V3 dir, float t, const Diff &base, V3 &dPdx,
-                                         V3 &dPdy, Surf &h) {
+                                         V3 &dPdy, Surf &h, TexFoot *tf = nullptr) {
     const TriShade &F = S.shade[slot];
     V3 bc = barycentric(ld3(F.v[0]), ld3(F.v[1]), ld3(F.v[2]), h.pos);
     hit_normals(F, dir, bc, h.shapeN, h.surfN, h.entering);
     V3 raw = h.surfN;
     calc_dPdxy(dir, t, h.shapeN, base, dPdx, dPdy);
-    hit_material(S, F, bc, dPdx, dPdy, h);
+    hit_material(S, F, bc, dPdx, dPdy, h, tf);
     if (dot(h.surfN, dir) >= 0.0f) h.surfN = raw;
     if (dot(h.surfN, dir) >= 0.0f) h.surfN = h.shapeN;
 }
@@ -311,6 +322,68 @@ __device__ __forceinline__ V3 brdf(const Surf &s, V3 V, V3 L) {
     return ret * NdotL;
 }
 
+// d gtr2 / d a and d smith_ggx / d alphaG (closed forms of the two functions above)
+__device__ __forceinline__ float gtr2_da(float NdotH, float a) {
+    float a2 = a * a, c2 = NdotH * NdotH;
+    float t = 1 + (a2 - 1) * c2;
+    return 2.0f * a / (PI_F * t * t) * (1.0f - 2.0f * a2 * c2 / t);
+}
+__device__ __forceinline__ float smith_ggx_da(float NdotV, float alphaG) {
+    float a = alphaG * alphaG, b = NdotV * NdotV;
+    float r = sqrtf(a + b - a * b);
+    float den = NdotV + r;
+    return -(alphaG * (1.0f - b) / r) / (den * den);
+}
+// d log D(H) / d a for the GGX distribution gtr2 at cos(theta_h) = c: the
+// score of one half-vector draw of sample_ggx.
+__device__ __forceinline__ float gtr2_dlog(float c, float a) {
+    float c2 = c * c;
+    float t = 1 + (a * a - 1) * c2;
+    return 2.0f / a - 4.0f * a * c2 / t;
+}
+
+// Partial derivatives of brdf() at fixed directions.  The albedo enters
+// linearly and identically in each channel (diffuse lobe and metallic
+// specular tint), so d brdf / d base = kd * I; d brdf / d roughness = dr.
+__device__ __forceinline__ void brdf_grad(const Surf &s, V3 V, V3 L, float &kd, V3 &dr) {
+    V3 N = s.surfN;
+    float NdotL = dot(N, L), NdotV = dot(N, V);
+    kd = 0.0f;
+    dr = v3(0.0f);
+    if (NdotL <= 0.0f || NdotV <= 0.0f) return;
+    V3 H = normalize(L + V);
+    float NdotH = dot(N, H), LdotH = dot(L, H);
+    const float r = s.roughness, m = s.metallic;
+    float FL = schlick(NdotL), FV = schlick(NdotV);
+    float Fd90 = 0.5f + 2.0f * LdotH * LdotH * r;
+    float mL = mixf(1.0f, Fd90, FL), mV = mixf(1.0f, Fd90, FV);
+    float Fd = mL * mV;
+    float dFd = 2.0f * LdotH * LdotH * (FL * mV + mL * FV);
+    float Ds = gtr2(NdotH, r), dDs = gtr2_da(NdotH, r);
+    float FH;
+    if (s.eta <= 1.0f) {
+        FH = schlick(LdotH);
+    } else {
+        float ci = fabsf(LdotH);
+        float si = sqrt_s(1.0f - ci * ci);
+        float st = s.eta * si;
+        if (st >= 1.0f) {
+            FH = 1.0f;
+        } else {
+            float ct = sqrt_s(1.0f - st * st);
+            float R0 = (s.eta - 1.0f) / (s.eta + 1.0f);
+            R0 = R0 * R0;
+            FH = mixf(R0, 1.0f, schlick(ct));
+        }
+    }
+    V3 Cspec0 = mix3(v3(s.specular), s.base, m);
+    V3 Fs = mix3(Cspec0, v3(1.0f), FH);
+    float gl = smith_ggx(NdotL, r), gv = smith_ggx(NdotV, r);
+    float Gs = gl * gv, dGs = smith_ggx_da(NdotL, r) * gv + gl * smith_ggx_da(NdotV, r);
+    kd = ((1.0f / PI_F) * Fd * (1.0f - m) * s.opacity + m * (1.0f - FH) * Ds * Gs) * NdotL;
+    dr = (((1.0f / PI_F) * dFd * (1.0f - m) * s.opacity) * s.base + Fs * (dDs * Gs + Ds * dGs)) * NdotL;
+}
+
 // BSDF::getBTDF (src/sampling.cpp:159-192)
 __device__ __forceinline__ V3 btdf(const Surf &s, V3 V, V3 L) {
     V3 N = s.surfN;
@@ -338,14 +411,28 @@ __device__ __forceinline__ void clamp64(V3 &p) {
     if (c > 64.0f) p /= c / 64.0f;
 }
 
+// Observer of the reflection samplers.  The renderer passes NoTap (every call
+// compiles away); the adjoint passes a recorder of the values its derivative
+// needs: every GGX half-vector cosine drawn, the accepted GGX sample, the
+// cosine sample's value before the clamp, and the pick between the two.
+struct NoTap {
+    __device__ __forceinline__ void ggx_try(float) {}
+    __device__ __forceinline__ void ggx_ok(float, float) {}
+    __device__ __forceinline__ void cos_raw(V3) {}
+    __device__ __forceinline__ void pick(float, float, bool) {}
+};
+
 // BSDF::sampleBRDF (src/sampling.cpp:203-236)
-__device__ __forceinline__ void sample_ggx(const Surf &s, V3 V, Stream &g, V3 &out, V3 &bp, float &pdf, int &fails) {
+template <class Tap>
+__device__ __forceinline__ void sample_ggx(const Surf &s, V3 V, Stream &g, V3 &out, V3 &bp, float &pdf, int &fails,
+                                           Tap &tap) {
     V3 t, b;
     tangent_space_in(s.surfN, V, t, b);
     float r2 = s.roughness * s.roughness;
     for (int T = 1; T <= 16; ++T) {
         float u = g.next(), phi = g.next() * 2.0f * PI_F;
         float ct = sqrtf((1.0f - u) / (1.0f + (r2 - 1.0f) * u)), st = sqrtf(1 - ct * ct);
+        tap.ggx_try(ct);
         V3 H = (st * cosf(phi) * t + st * sinf(phi) * b) + ct * s.surfN;
         out = 2.0f * dot(V, H) * H - V;
         float LdotH = dot(out, H), LdotN = dot(out, s.surfN);
@@ -354,6 +441,7 @@ __device__ __forceinline__ void sample_ggx(const Surf &s, V3 V, Stream &g, V3 &o
         else
             pdf = gtr2(ct, s.roughness) / (4.0f * LdotH);
         if (dot(out, s.shapeN) > 0.0f && pdf > 0.0f) {
+            tap.ggx_ok(ct, LdotH);
             bp = brdf(s, V, out) / pdf;
             return;
         }
@@ -365,7 +453,9 @@ __device__ __forceinline__ void sample_ggx(const Surf &s, V3 V, Stream &g, V3 &o
 }
 
 // BSDF::sampleCos (src/sampling.cpp:245-269)
-__device__ __forceinline__ void sample_cos(const Surf &s, V3 V, Stream &g, V3 &out, V3 &bp, float &pdf, int &fails) {
+template <class Tap>
+__device__ __forceinline__ void sample_cos(const Surf &s, V3 V, Stream &g, V3 &out, V3 &bp, float &pdf, int &fails,
+                                           Tap &tap) {
     V3 t, b;
     tangent_space_in(s.surfN, V, t, b);
     for (int T = 1; T <= 16; ++T) {
@@ -376,6 +466,7 @@ __device__ __forceinline__ void sample_cos(const Surf &s, V3 V, Stream &g, V3 &o
         pdf = dot(out, s.surfN) / PI_F;
         if (dot(out, s.shapeN) > 0.0f && pdf > 0.0f) {
             bp = brdf(s, V, out) / pdf;
+            tap.cos_raw(bp);
             clamp64(bp);
             return;
         }
@@ -403,18 +494,26 @@ __device__ __forceinline__ void precise_refraction(const Surf &s, V3 V, V3 &out,
 }
 
 // BSDF::sampleReflection (src/sampling.cpp:310-340)
-__device__ __forceinline__ void sample_reflection(const Surf &s, V3 V, Stream &g, V3 &dir, V3 &bp, int &fails) {
+template <class Tap>
+__device__ __forceinline__ void sample_reflection(const Surf &s, V3 V, Stream &g, V3 &dir, V3 &bp, int &fails,
+                                                  Tap &tap) {
     V3 d1, d2, b1, b2;
     int f1 = 0, f2 = 0;
     float p1, p2;
-    sample_cos(s, V, g, d1, b1, p1, f1);
-    sample_ggx(s, V, g, d2, b2, p2, f2);
+    sample_cos(s, V, g, d1, b1, p1, f1, tap);
+    sample_ggx(s, V, g, d2, b2, p2, f2, tap);
     float pc = p1 / (p1 + p2);
-    if (g.next() < pc) {
+    bool first = g.next() < pc;
+    tap.pick(p1, p2, first);
+    if (first) {
         dir = d1; bp = b1; fails += f1;
     } else {
         dir = d2; bp = b2; fails += f2;
     }
+}
+__device__ __forceinline__ void sample_reflection(const Surf &s, V3 V, Stream &g, V3 &dir, V3 &bp, int &fails) {
+    NoTap tap;
+    sample_reflection(s, V, g, dir, bp, fails, tap);
 }
 
 // refract (src/sampling.cpp:342-349)
@@ -501,11 +600,19 @@ __device__ __forceinline__ float light_weight(const LightRec &L, const Surf &s, 
     return cl * L.power / (dist * dist + 1e-3f);
 }
 
+// What an accepted light sample was made of: direction, distance to the light
+// point, the light object's pick probability and index (-1 for the sky).
+struct DirectInfo {
+    V3 dir;
+    float dist, P;
+    int li;
+};
+
 // One light sample of sampleDirectLight (src/sampling.cpp:467-527), with its
 // shadow ray.  Returns false when the reference `continue`s.
 template <bool COUNT>
 __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s, V3 V, float total, int cnt, Stream &g,
-                                              LSample &out, TraceCounts &tc) {
+                                              LSample &out, TraceCounts &tc, DirectInfo *info = nullptr) {
     if (S.sky) {
         // sampleSkyBox (src/sampling.cpp:450-465)
         const int n = S.sky_w * S.sky_h;
@@ -535,6 +642,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
         V3 b = bsdf_eval(s, V, dir);
         clamp64(b);
         out = LSample{b, light, 1.0f / float(cnt)};
+        if (info) *info = DirectInfo{dir, INFINITY, 1.0f, -1};
         return true;
     }
     // emissive light objects: sample() walks the weights (src/sampling.cpp:396-404)
@@ -581,6 +689,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
     clamp64(b);
     V3 light = ((2.0f * PI_F * L.power) * ld3(L.color)) / (dist * dist + 1e-3f) / P;
     out = LSample{b, light, 1.0f / float(cnt * (fails + 1))};
+    if (info) *info = DirectInfo{dir, dist, P, li};
     return true;
 }
 
