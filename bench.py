@@ -56,10 +56,15 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the 1080p render leg")
     ap.add_argument("--render-frames", type=int, default=2, help="timed 1080p frames (after one warm-up)")
     ap.add_argument("--render-spp", type=int, default=0, help="override the 1080p workload's 256 spp")
-    ap.add_argument("--cpu-row-stride", type=int, default=48, help="CPU render sample: every n-th row")
-    ap.add_argument("--cpu-col-stride", type=int, default=32, help="CPU render sample: every n-th column")
+    ap.add_argument("--cpu-row-stride", type=int, default=6, help="CPU render sample: every n-th row")
+    ap.add_argument("--cpu-col-stride", type=int, default=8, help="CPU render sample: every n-th column")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="accepted for older profile commands; the 1 GiB calibration copy always runs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
+    ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
+    ap.add_argument("--adam-steps", type=int, default=0, help="also run an inverse-rendering Adam loop of n steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -101,6 +106,11 @@ def main():
             algo[(name, any_hit)] = n * (24 + out_b) + 24 * boxes + 8 * (interior + leaves) + 36 * tris
     log("[bench] algorithmic bytes/ray:", {f"{k[0]}{'_any' if k[1] else ''}": round(v / (n_coh if k[0] == 'coh' else n_inc), 1)
                                            for k, v in algo.items()})
+    tuned = None
+    if not args.no_autotune:
+        from raym0nade_amd import autotune
+        tuned = autotune.tune_trace(S, [r_coh, r_inc], reps=2)
+        log(f"[bench] autotuned traversal: {tuned}")
 
     def step(times=None):
         for name, r in (("coh", r_coh), ("inc", r_inc)):
@@ -147,12 +157,26 @@ def main():
     mrays.update({f"any_{k}": (n_coh if k == "coh" else n_inc) / (a_ms[k].mean() * 1e-3) / 1e6 for k in a_ms})
     gbs = {f"closest_{k}": algo[(k, False)] / (k_ms[k].mean() * 1e-3) / 1e9 for k in k_ms}
     gbs.update({f"any_{k}": algo[(k, True)] / (a_ms[k].mean() * 1e-3) / 1e9 for k in a_ms})
-    # dominant kernel: the closest-hit traversal kernel over both batches
+    # dominant kernel: the closest-hit traversal kernel over both batches, timed
+    # alone by the library's own HIP events on the trace stream (a call also
+    # runs the ray-reordering kernels when the batch is incoherent)
+    rm.set_tuning(rm.TUNE_KERNEL_TIMING, 1)
+    kt = {"coh": [], "inc": []}
+    rt = {"coh": [], "inc": []}
+    for _ in range(max(3, args.steps // 2)):
+        for name, r in (("coh", r_coh), ("inc", r_inc)):
+            t_out, id_out, _ = bufs[name]
+            S.trace_closest(r, t_out=t_out, id_out=id_out, stream=stream)
+            reo, trav = rm.last_trace_ms()
+            kt[name].append(trav)
+            rt[name].append(reo)
+    rm.set_tuning(rm.TUNE_KERNEL_TIMING, 0)
     bytes_per_launch = (algo[("coh", False)] + algo[("inc", False)]) / 2
-    dur_per_launch = (k_ms["coh"].mean() + k_ms["inc"].mean()) / 2 * 1e-3
+    dur_per_launch = (np.mean(kt["coh"]) + np.mean(kt["inc"])) / 2 * 1e-3
     achieved = bytes_per_launch / dur_per_launch / 1e9
+    kernel_name = {0: "k_trace_closest", 1: "k_trace_step", 2: "k_trace_persistent"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_trace_closest.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}.json")
     if os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
@@ -160,8 +184,10 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_trace_closest", "algorithmic_bytes_per_launch": round(bytes_per_launch),
-                "avg_launch_ms": round(dur_per_launch * 1e3, 4)}
+                "kernel": kernel_name, "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                "avg_launch_ms": round(dur_per_launch * 1e3, 4),
+                "launch_ms": {k: round(float(np.mean(v)), 4) for k, v in kt.items()},
+                "reorder_ms": {k: round(float(np.mean(v)), 4) for k, v in rt.items()}}
 
     # measured streaming bandwidth: a plain 1 GiB device copy (1 GiB read + 1 GiB
     # written per launch); also the known-size kernel that scales the
@@ -210,6 +236,12 @@ def main():
     render = None
     if not args.no_render:
         render = render_leg(args, torch, rm, dev, stream)
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(args, torch, rm)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(args, torch, rm)
 
     out = {
         "metric": METRIC,
@@ -231,9 +263,92 @@ def main():
         "algorithmic_gbs": {k: round(v, 1) for k, v in gbs.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "autotune": tuned,
         "render_1080p": render,
+        "batch": batch,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
+
+
+def batch_leg(args, torch, rm):
+    """Batched serving: the job set of raym0nade_amd.workloads (seeded Cornell
+    and displaced-sphere variants, 128^2..1024^2, spp {16, 64, 256}, depth
+    8..16) rendered (a) as one rm_render_batch hipGraph launch and (b) job by
+    job with rm_render; both with FXAA post.  Both produce the same samples
+    per job (equal RMSE by construction), so samples/s compare directly."""
+    from raym0nade_amd import autotune, workloads
+    specs = workloads.batch_specs(args.batch_jobs, seed=1)
+    log(f"[bench] batch: building {len(specs)} job scenes")
+    built = [workloads.build_job(s) for s in specs]
+    frames = [rm.Frame(a.width, a.height) for _, _, a in built]
+    jobs = [(S, a, fr, 63 | 512) for (_, S, a), fr in zip(built, frames)]
+    total = sum(s.samples for s in specs)
+    tris = [S.info.n_tris for _, S, _ in built]
+    log(f"[bench] batch: {total / 1e6:.0f}M samples, {min(tris)}..{max(tris)} triangles per scene")
+    tuned = None
+    if not args.no_autotune:
+        tuned = autotune.tune_render(jobs, log=log)
+        for _, a, _, _ in jobs:
+            a.paths_per_round = tuned["paths_per_round"]
+        log(f"[bench] batch autotune: {tuned}")
+    rm.render_batch(jobs[:2], use_graph=True)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rm.render_batch(jobs, use_graph=True)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    log(f"[bench] batch: one graph launch {tb:.2f} s")
+    t0 = time.perf_counter()
+    for S, a, fr, post in jobs:
+        fr.render(S, a)
+        fr.postprocess(post, a.exposure)
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    log(f"[bench] batch: sequential rm_render {ts:.2f} s")
+    return {"jobs": len(jobs), "samples": total, "batched_samples_per_s": round(total / tb, 1),
+            "sequential_samples_per_s": round(total / ts, 1), "batched_over_sequential": round(ts / tb, 3),
+            "batched_s": round(tb, 3), "sequential_s": round(ts, 3), "autotune": tuned,
+            "tris_min": min(tris), "tris_max": max(tris)}
+
+
+def adjoint_leg(args, torch, rm):
+    """Differentiable rendering at 512^2, 128 spp on the textured Cornell box
+    (five 256^2 fp32 albedo textures, per-wall GGX roughness): time of the
+    forward render (rm_render + linear shade) and of the path-replay adjoint
+    (rm_adjoint, gradients for every texel, roughness and emission)."""
+    from raym0nade_amd.diff import DiffRenderer
+    g = rm.generate(4, 1)
+    R = DiffRenderer(g)
+    tex, rough, emis = R.initial_params()
+    img = R.forward_image(tex, rough, emis)
+    d = torch.randn_like(img)
+    R.adjoint(d, tex.numel())  # warm-up
+    fw, rd, ad = [], [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        R.frame.render(R.scene, R.args)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        R.forward_image(tex, rough, emis)  # parameter upload + render + linear shade
+        t2 = time.perf_counter()
+        R.adjoint(d, tex.numel())
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        rd.append(t1 - t0)
+        fw.append(t2 - t1)
+        ad.append(t3 - t2)
+    r, f, a = float(np.median(rd)), float(np.median(fw)), float(np.median(ad))
+    out = {"render_s": round(r, 4), "forward_s": round(f, 4), "adjoint_s": round(a, 4),
+           "adjoint_over_forward": round(a / r, 3), "adjoint_over_forward_with_upload": round(a / f, 3),
+           "width": R.args.width, "height": R.args.height, "spp": R.args.spp, "texels": int(tex.numel()),
+           "split": rm.get_tuning(rm.TUNE_ADJOINT_SPLIT)}
+    log(f"[bench] adjoint: render {r:.3f} s, forward {f:.3f} s, adjoint {a:.3f} s, ratio {a / r:.2f}")
+    if args.adam_steps > 0:
+        from raym0nade_amd.inverse import inverse_render
+        out["adam"] = inverse_render(g, steps=args.adam_steps, log=log)
+    return out
 
 
 def render_leg(args, torch, rm, dev, stream):

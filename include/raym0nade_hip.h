@@ -294,11 +294,20 @@ enum rm_tune_key {
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive
                                            origins are far apart (default), 2 always */
     RM_TUNE_RENDER_BLOCKS_PER_CU = 4,   /* blocks per CU of the persistent bounce kernel (default 8) */
-    RM_TUNE_ADJOINT_SPLIT = 5           /* lanes per pixel of the adjoint, each replaying a share of
+    RM_TUNE_ADJOINT_SPLIT = 5,          /* lanes per pixel of the adjoint, each replaying a share of
                                            the pixel's samples (default 4) */
+    RM_TUNE_KERNEL_TIMING = 6,          /* 1: trace calls record HIP events around their reordering
+                                           and traversal kernels (default 0) */
+    RM_TUNE_RENDER_MIN_WAVES = 7        /* 1 (default) or 4: resident waves per SIMD the render's
+                                           shading and NEE kernels are compiled for (4 caps them
+                                           at 128 VGPRs, spilling the rest) */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);
+/* Durations of the last rm_trace_closest / rm_trace_any call that ran with
+ * kernel timing on: its ray-reordering kernels and its traversal kernel (HIP
+ * events on the call's stream; waits for them). */
+int32_t rm_last_trace_ms(float *reorder_ms, float *traverse_ms);
 
 /* ------------------------------------------------------------ utilities */
 

@@ -72,7 +72,15 @@ def philox(ctr, key) -> tuple:
 
 
 (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT,
- TUNE_RENDER_BLOCKS_PER_CU) = 0, 1, 2, 3, 4
+ TUNE_RENDER_BLOCKS_PER_CU, TUNE_ADJOINT_SPLIT, TUNE_KERNEL_TIMING, TUNE_RENDER_MIN_WAVES) = range(8)
+
+
+def last_trace_ms() -> tuple:
+    """(reordering ms, traversal ms) of the last trace call made with
+    TUNE_KERNEL_TIMING on (waits for that call's kernels)."""
+    a, b = C.c_float(), C.c_float()
+    check(lib().rm_last_trace_ms(C.byref(a), C.byref(b)))
+    return a.value, b.value
 
 
 def set_tuning(key: int, value: int) -> None:

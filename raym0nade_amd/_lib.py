@@ -154,6 +154,7 @@ SIGNATURES = [
     ("rm_philox4x32_10", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("rm_set_tuning", C.c_int32, [C.c_int32, C.c_int32]),
     ("rm_get_tuning", C.c_int32, [C.c_int32, C.POINTER(C.c_int32)]),
+    ("rm_last_trace_ms", C.c_int32, [C.POINTER(C.c_float), C.POINTER(C.c_float)]),
 ]
 
 _lib = None

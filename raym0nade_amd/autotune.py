@@ -1,0 +1,104 @@
+"""Autotuner for the launch parameters behind rm_set_tuning.
+
+Every setting gives the same results (traversal bit for bit; renders up to
+the summation order of accumulator sets), so the tuner only times candidates
+on the caller's own workload and keeps the fastest:
+
+  tune_trace   traversal variant (stack walk / step walk / persistent step
+               walk), resident blocks per CU, the idle-lane threshold at which
+               a persistent wave fetches new rays, and ray reordering, on the
+               caller's ray batches;
+  tune_render  resident blocks per CU of the persistent bounce kernel (the
+               ray-queue consumers) and paths per pixel per round (the depth of
+               the path queues), on a copy of the caller's batch with the spp
+               scaled down.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+from . import (TUNE_FETCH_MIN, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_RAY_SORT, TUNE_RENDER_BLOCKS_PER_CU,
+               TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, RmRenderArgs, get_tuning, render_batch, set_tuning)
+
+TRACE_KEYS = (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT)
+
+
+def _time(fn, reps):
+    import torch
+    best = float("inf")
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
+    """batches: list of device ray tensors [n, 6].  Sets and returns the
+    fastest {key: value} for closest-hit traversal of all batches."""
+    if candidates is None:
+        candidates = [(0, 8, 16, 1)]
+        candidates += [(1, 8, 16, s) for s in (0, 1)]
+        candidates += [(2, b, f, s) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
+
+    def run():
+        for r in batches:
+            scene.trace_closest(r)
+
+    results = []
+    for c in candidates:
+        for k, v in zip(TRACE_KEYS, c):
+            set_tuning(k, v)
+        run()  # first use of a setting (workspace growth)
+        t = _time(run, reps)
+        results.append((t, c))
+        if log:
+            log(f"[autotune] trace {c}: {t * 1e3:.2f} ms")
+    t, best = min(results)
+    for k, v in zip(TRACE_KEYS, best):
+        set_tuning(k, v)
+    return {"variant": best[0], "blocks_per_cu": best[1], "fetch_min": best[2], "ray_sort": best[3],
+            "seconds": t, "tried": len(results)}
+
+
+def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2), log=None) -> dict:
+    """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
+    render_batch.  Times the batch (hipGraph launch) with every job's spp
+    scaled by spp_scale: first every (blocks per CU, register cap) pair with
+    automatic paths per round, then the other paths-per-round values at the
+    best pair.  Sets the launch parameters and returns the choice; the caller
+    applies paths_per_round (0 = automatic) to its jobs' arguments."""
+    small = []
+    for S, a, fr, post in jobs:
+        b = RmRenderArgs.from_buffer_copy(a)
+        b.spp = max(4, int(round(a.spp * spp_scale)))
+        small.append([S, b, fr, post])
+
+    def trial(bpc, w, k):
+        for j in small:
+            j[1].paths_per_round = k
+        set_tuning(TUNE_RENDER_BLOCKS_PER_CU, bpc)
+        set_tuning(TUNE_RENDER_MIN_WAVES, w)
+        t = _time(lambda: render_batch([tuple(j) for j in small], use_graph=True), 1)
+        if log:
+            log(f"[autotune] render blocks/CU {bpc} min waves {w} paths/round {k or 'auto'}: {t:.3f} s")
+        return (t, bpc, w, k)
+
+    results = [trial(b, w, paths_per_round[0]) for b in blocks for w in waves]
+    _, bpc, w, _ = min(results)
+    results += [trial(bpc, w, k) for k in paths_per_round[1:]]
+    t, bpc, w, k = min(results)
+    set_tuning(TUNE_RENDER_BLOCKS_PER_CU, bpc)
+    set_tuning(TUNE_RENDER_MIN_WAVES, w)
+    return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "seconds_scaled": t,
+            "spp_scale": spp_scale, "tried": len(results)}
+
+
+def current() -> dict:
+    keys = {"trace_variant": TUNE_TRACE_VARIANT, "persistent_blocks_per_cu": TUNE_PERSISTENT_BLOCKS_PER_CU,
+            "fetch_min": TUNE_FETCH_MIN, "ray_sort": TUNE_RAY_SORT, "render_blocks_per_cu": TUNE_RENDER_BLOCKS_PER_CU,
+            "render_min_waves": TUNE_RENDER_MIN_WAVES}
+    return {n: get_tuning(k) for n, k in keys.items()}

@@ -75,6 +75,14 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
             if (value < 1 || value > 64) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: adjoint split %d", value);
             t.adjoint_split = value;
             return RM_OK;
+        case RM_TUNE_KERNEL_TIMING:
+            if (value < 0 || value > 1) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: kernel timing %d", value);
+            t.kernel_timing = value;
+            return RM_OK;
+        case RM_TUNE_RENDER_MIN_WAVES:
+            if (value != 1 && value != 4) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: render min waves %d (1 or 4)", value);
+            t.render_min_waves = value;
+            return RM_OK;
         default:
             return rm::fail(RM_ERR_INVALID, "rm_set_tuning: unknown key %d", key);
     }
@@ -90,6 +98,8 @@ extern "C" int32_t rm_get_tuning(int32_t key, int32_t *value) {
         case RM_TUNE_RAY_SORT: *value = t.ray_sort; return RM_OK;
         case RM_TUNE_RENDER_BLOCKS_PER_CU: *value = t.render_blocks_per_cu; return RM_OK;
         case RM_TUNE_ADJOINT_SPLIT: *value = t.adjoint_split; return RM_OK;
+        case RM_TUNE_KERNEL_TIMING: *value = t.kernel_timing; return RM_OK;
+        case RM_TUNE_RENDER_MIN_WAVES: *value = t.render_min_waves; return RM_OK;
         default: return rm::fail(RM_ERR_INVALID, "rm_get_tuning: unknown key %d", key);
     }
 }

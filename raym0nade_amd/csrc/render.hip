@@ -468,9 +468,113 @@ __device__ __forceinline__ void store_set(const AccBufs &B, uint32_t g, const Se
 // queue (count_in, written by the previous stage) is exhausted, so the grid is
 // sized by the machine, not by the live count, and no host round trip is
 // needed between depths.
-__global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj, PixBufs P, AccBufs AB, PathQ Qin,
-                                               const uint32_t *count_in, uint32_t *work, PathQ Qout,
-                                               uint32_t *count_out) {
+// A path that ends in sampleRay's NEE-and-stop branch (:147-160,229-240)
+// leaves the shading state of its last vertex here; k_nee draws that vertex's
+// light samples (sampleDirectLight with the path's own stream) and adds them
+// to the path's accumulator set.  9 float4 per record:
+//   f0 pos | pixel   f1 shapeN | set   f2 surfN | sample   f3 V | rng counter
+//   f4 base | roughness   f5 metallic, opacity, eta, flags   f6 T | W
+//   f7 p0 | factor   f8 extra | specular
+//   flags = entering | extra_on << 1 | light-sample count << 2 | job << 8
+struct NeeQ {
+    float4 *f[9];
+};
+struct NeeRec {
+    Surf s;
+    V3 V, extra;
+    uint32_t n;
+    int cnt;
+    float factor;
+    bool extra_on;
+};
+__device__ __forceinline__ void store_nee(const NeeQ &Q, uint32_t q, const Path &P, const NeeRec &R) {
+    const Surf &s = R.s;
+    Q.f[0][q] = make_float4(s.pos.x, s.pos.y, s.pos.z, ub(P.pix));
+    Q.f[1][q] = make_float4(s.shapeN.x, s.shapeN.y, s.shapeN.z, ub(P.set));
+    Q.f[2][q] = make_float4(s.surfN.x, s.surfN.y, s.surfN.z, ub(P.smp));
+    Q.f[3][q] = make_float4(R.V.x, R.V.y, R.V.z, ub(R.n));
+    Q.f[4][q] = make_float4(s.base.x, s.base.y, s.base.z, s.roughness);
+    uint32_t flags = (s.entering ? 1u : 0u) | (R.extra_on ? 2u : 0u) | ((uint32_t)R.cnt << 2) | ((uint32_t)P.job << 8);
+    Q.f[5][q] = make_float4(s.metallic, s.opacity, s.eta, ub(flags));
+    Q.f[6][q] = make_float4(P.T.x, P.T.y, P.T.z, P.W);
+    Q.f[7][q] = make_float4(P.p0.x, P.p0.y, P.p0.z, R.factor);
+    Q.f[8][q] = make_float4(R.extra.x, R.extra.y, R.extra.z, s.specular);
+}
+
+// Adds light sample x of a finished path (sample name smp * 8 + k) to its
+// accumulator set, passBsdf'd up to the first vertex: light *= terminal bsdf
+// (and, on the refraction side, the medium's absorb) and every deeper bsdfPdf.
+// A read-modify-write: the set belongs to this path for the round.
+__device__ __forceinline__ void add_light_sample(const PixBufs &P, const AccBufs &AB, uint32_t pix, uint32_t set,
+                                                 uint32_t smp, uint32_t k, V3 T, float W, V3 p0, const LSample &x,
+                                                 bool extra_on, V3 extra) {
+    float4 b0 = P.pi0[pix], b1 = P.pi1[pix];
+    V3 base = b1.y < T_EPS ? v3(0.0f) : V3{b0.x, b0.y, b0.z};
+    float inv = 1.0f / (float)bu(P.pi2[pix].w);
+    V3 L = x.light * x.bsdf;
+    if (extra_on) L = L * extra;
+    L = L * T;
+    SetAcc acc = load_set(AB, set);
+    add_sample(acc, base, p0, L, x.weight * W * inv, smp * 8u + k);
+    store_set(AB, set, acc);
+}
+
+// Closest hit of every queued path (sampleRay's model.rayHit, :127), ahead of
+// the shading kernel: a lean persistent traversal.  A wave hands a new queue
+// entry to each of its idle lanes once at least `fetch_min` lanes are idle
+// (one atomic per refill), so lanes do not wait for the wave's longest walk.
+// Writes (t, triangle slot or -1) per entry.  The traversal is the same walk
+// as everywhere else, so the hits are bit-identical.
+__global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ Q, const uint32_t *count_in,
+                                               uint32_t *work, uint2 *__restrict__ hit, int fetch_min) {
+    const uint32_t n = *count_in;
+    const uint32_t lane = threadIdx.x & 63u;
+    Walker<false, false> w;
+    RayPre r;
+    SceneView S;  // per lane: the nodes, triangles and root of its path's scene
+    int64_t q = -1;
+    bool exhausted = false;
+    TraceCounts c{};
+    while (true) {
+        bool idle = q < 0;
+        uint64_t idle_mask = __ballot(idle);
+        uint64_t busy_mask = __ballot(!idle);
+        if (exhausted && busy_mask == 0) break;
+        if (!exhausted && idle_mask && (__popcll(idle_mask) >= fetch_min || busy_mask == 0)) {
+            uint32_t cnt = (uint32_t)__popcll(idle_mask);
+            uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle_mask) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(work, cnt);
+            base = __shfl(base, (int)leader, 64);
+            if (base + cnt >= n) exhausted = true;
+            if (idle) {
+                uint32_t slot = base + (uint32_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
+                if (slot < n) {
+                    q = slot;
+                    float4 a = Q.f[0][slot], b = Q.f[1][slot];
+                    int job = (int)(bu(Q.f[7][slot].w) >> 8);
+                    const SceneView &SJ = J[job].S;
+                    S.nodes = SJ.nodes;
+                    S.tris = SJ.tris;
+                    S.root_leaf = SJ.root_leaf;
+                    r = ray_pre(V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z});
+                    w.init(S, r, INFINITY);
+                }
+            }
+        }
+        if (q >= 0 && w.step(S, r, T_EPS, -INFINITY, c)) {
+            hit[q] = make_uint2(bu(w.tmax), (uint32_t)w.best);
+            q = -1;
+        }
+    }
+}
+
+// MINW: minimum resident waves per SIMD the register allocation must allow
+// (1: no cap; 4: at most 128 VGPRs, the rest spilled).  Chosen by tuning.
+template <int MINW>
+__global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, int nj, PixBufs P, AccBufs AB, PathQ Qin,
+                                               const uint32_t *count_in, uint32_t *work, const uint2 *__restrict__ hit,
+                                               PathQ Qout, uint32_t *count_out, NeeQ NQ, uint32_t *nee_count) {
   const uint32_t n_in = *count_in;
   const uint32_t lane = threadIdx.x & 63u;
   while (true) {
@@ -482,7 +586,13 @@ __global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj
     bool live = q < n_in;
     bool cont = false;
     Path path;
-    if (live) path = load_path(Qin, q);
+    uint2 hq = make_uint2(0u, 0u);
+    if (live) {
+        path = load_path(Qin, q);
+        hq = hit[q];
+    }
+    bool nee_out = false;
+    NeeRec nee_rec;
     for_each_job(live ? path.job : -1, live, [&](int jid) {
         const Job &jb = J[jid];
         const SceneView &S = jb.S;
@@ -490,17 +600,17 @@ __global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj
         Stream rng{A.seed, TAG_INDIRECT, (uint32_t)(path.pix - jb.pix_off), path.smp, path.n};
         const int depth = path.depth;
         const int maxD = A.max_depth;
-        RayPre r = ray_pre(path.o, path.d);
-        float t = INFINITY;
+        float t = ub(hq.x);
         TraceCounts tc{};
-        int32_t slot = trace_walk<false>(S, r, T_EPS, t, -INFINITY);
-        // light samples produced at this vertex (<= 6)
-        LSample ls[6];
-        int nls = 0;
+        const int32_t slot = (int32_t)hq.y;
         V3 extra = v3(1.0f);  // refract-side NEE multiplies by absorb after the samples (passBsdf)
         bool extra_on = false;
+        // an emitter or sky hit ends the path with one light sample
+        auto emit = [&](const LSample &x) {
+            add_light_sample(P, AB, path.pix, path.set, path.smp, 0u, path.T, path.W, path.p0, x, false, extra);
+        };
         if (slot < 0) {
-            if (!(path.exclude || !S.sky)) ls[nls++] = LSample{v3(1.0f), sky_get(S, path.d), 1.0f};
+            if (!(path.exclude || !S.sky)) emit(LSample{v3(1.0f), sky_get(S, path.d), 1.0f});
         } else {
             Surf s = surf_default();
             s.pos = path.o + path.d * t;
@@ -514,7 +624,7 @@ __global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj
             bool doNEE = P_RR < 0.9f && depth >= A.rr_min;
             V3 absorb = get_absorb(med_absorb(path.med), t);
             if (length(s.emission) > T_EPS) {
-                if (!path.exclude) ls[nls++] = LSample{absorb, s.emission, 1.0f};
+                if (!path.exclude) emit(LSample{absorb, s.emission, 1.0f});
             } else {
                 float P_reflect = 1.0f, Fr = 0.0f;
                 if (s.opacity < T_EPS) {
@@ -569,17 +679,16 @@ __global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj
                     }
                 }
                 if (nee) {
-                    int cnt = sample_count(depth);
-                    float total = direct_total(S, s, V);
-                    if (total != 0.0f) {
-                        for (int T = 0; T < cnt; ++T) {
-                            LSample x;
-                            if (direct_sample<false>(S, s, V, total, cnt, rng, x, tc)) {
-                                x.bsdf *= factor;  // scaling(samples, factor)
-                                ls[nls++] = x;
-                            }
-                        }
-                    }
+                    // the light samples of this vertex are drawn by k_nee,
+                    // from the same stream position
+                    nee_out = true;
+                    nee_rec.s = s;
+                    nee_rec.V = V;
+                    nee_rec.n = rng.n;
+                    nee_rec.cnt = sample_count(depth);
+                    nee_rec.factor = factor;
+                    nee_rec.extra = extra;
+                    nee_rec.extra_on = extra_on;
                 } else if (any_finite(nd) && depth != maxD) {
                     bp *= absorb;
                     cont = true;
@@ -594,25 +703,71 @@ __global__ __launch_bounds__(RB) void k_bounce(const Job *__restrict__ J, int nj
                 }
             }
         }
-        if (nls > 0) {
-            // passBsdf up to the first vertex: light *= terminal bsdf (and, on the
-            // refraction side, the medium's absorb) and every deeper bsdfPdf
-            uint32_t p = path.pix;
-            float4 b0 = P.pi0[p], b1 = P.pi1[p];
-            V3 base = b1.y < T_EPS ? v3(0.0f) : V3{b0.x, b0.y, b0.z};
-            float inv = 1.0f / (float)bu(P.pi2[p].w);
-            SetAcc a = load_set(AB, path.set);
-            for (int k = 0; k < nls; ++k) {
-                V3 L = ls[k].light * ls[k].bsdf;
-                if (extra_on) L = L * extra;
-                L = L * path.T;
-                add_sample(a, base, path.p0, L, ls[k].weight * path.W * inv, path.smp * 8u + (uint32_t)k);
-            }
-            store_set(AB, path.set, a);
-        }
     });
     uint32_t slot_out = append(count_out, cont);
     if (cont) store_path(Qout, slot_out, path);
+    uint32_t slot_nee = append(nee_count, nee_out);
+    if (nee_out) store_nee(NQ, slot_nee, path, nee_rec);
+  }
+}
+
+// sampleDirectLight at the last vertex of every path that ended with
+// NEE-and-stop this round (src/render.cpp:147-160,229-240 with
+// src/sampling.cpp:467-527): up to sampleCount[depth] light samples, each
+// with its shadow ray, scaled by the branch factor, into the path's set.
+template <int MINW>
+__global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, PixBufs P, AccBufs AB, NeeQ NQ,
+                                            const uint32_t *count, uint32_t *work) {
+  const uint32_t n = *count;
+  const uint32_t lane = threadIdx.x & 63u;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const uint32_t q = base + lane;
+    const bool live = q < n;
+    float4 f5 = make_float4(0, 0, 0, 0);
+    if (live) f5 = NQ.f[5][q];
+    const uint32_t flags = bu(f5.w);
+    for_each_job(live ? (int)(flags >> 8) : -1, live, [&](int jid) {
+        const Job &jb = J[jid];
+        const SceneView &S = jb.S;
+        const RArgs &A = jb.A;
+        float4 f0 = NQ.f[0][q], f1 = NQ.f[1][q], f2 = NQ.f[2][q], f3 = NQ.f[3][q], f4 = NQ.f[4][q];
+        float4 f6 = NQ.f[6][q], f7 = NQ.f[7][q], f8 = NQ.f[8][q];
+        Surf s = surf_default();
+        s.pos = V3{f0.x, f0.y, f0.z};
+        s.shapeN = V3{f1.x, f1.y, f1.z};
+        s.surfN = V3{f2.x, f2.y, f2.z};
+        s.base = V3{f4.x, f4.y, f4.z};
+        s.roughness = f4.w;
+        s.metallic = f5.x;
+        s.opacity = f5.y;
+        s.eta = f5.z;
+        s.entering = (flags & 1u) != 0;
+        s.specular = f8.w;
+        const V3 V{f3.x, f3.y, f3.z};
+        const uint32_t pix = bu(f0.w), set = bu(f1.w), smp = bu(f2.w);
+        const bool extra_on = (flags & 2u) != 0;
+        const int cnt = (int)((flags >> 2) & 63u);
+        const V3 T{f6.x, f6.y, f6.z}, p0{f7.x, f7.y, f7.z}, extra{f8.x, f8.y, f8.z};
+        const float W = f6.w, factor = f7.w;
+        Stream rng{A.seed, TAG_INDIRECT, (uint32_t)(pix - jb.pix_off), smp, bu(f3.w)};
+        float total = direct_total(S, s, V);
+        if (total != 0.0f) {
+            uint32_t k = 0;
+            TraceCounts tc{};
+            for (int Ti = 0; Ti < cnt; ++Ti) {
+                LSample x;
+                if (direct_sample<false>(S, s, V, total, cnt, rng, x, tc)) {
+                    x.bsdf *= factor;  // scaling(samples, factor)
+                    add_light_sample(P, AB, pix, set, smp, k, T, W, p0, x, extra_on, extra);
+                    ++k;
+                }
+            }
+        }
+    });
   }
 }
 
@@ -724,7 +879,9 @@ struct HostJob {
     int post;  // rm_postprocess options, 0 = none
 };
 
-static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, const char *fn) {
+// all_px: pixels of the whole launch; the automatic paths-per-round keeps
+// the live accumulator sets of a launch near 4M (468 B each).
+static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, int64_t all_px, const char *fn) {
     const rm_render_args *a = h.args;
     rm_frame *f = h.frame;
     if (!h.scene || !a || !f) return fail(RM_ERR_INVALID, "%s: NULL argument", fn);
@@ -743,7 +900,7 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, const 
     RArgs &A = J.A;
     A = rargs_from(a);
     int K = a->paths_per_round;
-    if (K <= 0) K = (int)std::min<int64_t>(16, std::max<int64_t>(1, (int64_t)(4 << 20) / std::max<int64_t>(1, np)));
+    if (K <= 0) K = (int)std::min<int64_t>(16, std::max<int64_t>(1, (int64_t)(4 << 20) / std::max<int64_t>(1, all_px)));
     A.K = K;
     const int spp_direct = (int)(float(a->spp) * a->p_direct);
     const int n_opaque = a->spp - spp_direct;
@@ -776,10 +933,13 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     if (nj > (1 << 23)) return fail(RM_ERR_INVALID, "too many jobs");
     std::vector<Job> J(nj);
     int R = 0, D = 1;
+    int64_t all_px = 0;
+    for (int i = 0; i < nj; ++i)
+        if (hj[i].args) all_px += (int64_t)std::max(0, hj[i].args->width) * std::max(0, hj[i].args->height);
     int64_t total_px = 0;
     for (int i = 0; i < nj; ++i) {
         int r = 0, d = 0;
-        int32_t rc = make_job(hj[i], J[i], r, d, graph ? "rm_render_batch" : "rm_render");
+        int32_t rc = make_job(hj[i], J[i], r, d, all_px, nj > 1 ? "rm_render_batch" : "rm_render");
         if (rc) return rc;
         J[i].pix_off = total_px;
         total_px += J[i].np;
@@ -789,7 +949,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     const int64_t nblk = (total_px + SB - 1) / SB;
     auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t pix_bytes = pad(nj * sizeof(Job)) + 3 * pad(total_px * 16) + 2 * pad(total_px * 4) + pad(nblk * 8) +
-                             pad(2 * NCOUNT * 4);
+                             pad(4 * NCOUNT * 4);
     if (work.bytes < pix_bytes) {
         int32_t rc = work.alloc(pix_bytes);
         if (rc) return rc;
@@ -803,8 +963,10 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     P.M = carve<int32_t>(cur, total_px);
     P.set_off = carve<int32_t>(cur, total_px);
     int64_t *bsum = carve<int64_t>(cur, nblk);
-    // counts[d]: paths entering depth d+1; counts[NCOUNT + d]: claim counter of depth d
-    uint32_t *counts = carve<uint32_t>(cur, 2 * NCOUNT);
+    // counts[d]: paths entering depth d+1; counts[NCOUNT + d]: shading claims of
+    // depth d; counts[2 NCOUNT + d]: traversal claims of depth d;
+    // counts[3 NCOUNT]: NEE records of the round, counts[3 NCOUNT + 1]: their claims
+    uint32_t *counts = carve<uint32_t>(cur, 4 * NCOUNT);
     RM_HIP(hipMemcpyAsync(dJ, J.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st));
     const unsigned gp = (unsigned)((total_px + RB - 1) / RB);
     hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P);
@@ -820,7 +982,8 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         total += v;
     }
     if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "%lld path slots per round", (long long)total);
-    const size_t set_bytes = pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16);
+    const size_t set_bytes =
+        pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16) + pad(total * 8) + 9 * pad(total * 16);
     if (work.bytes < pix_bytes + set_bytes) {
         // grow (the buffer holds the primary pass's results) and start over
         int32_t rc = work.alloc(pix_bytes + set_bytes);
@@ -840,6 +1003,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     PathQ Q[2];
     for (int qi = 0; qi < 2; ++qi)
         for (int k = 0; k < 12; ++k) Q[qi].f[k] = carve<float4>(cur, total);
+    uint2 *hits = carve<uint2>(cur, total);
+    NeeQ NQ;
+    for (int k = 0; k < 9; ++k) NQ.f[k] = carve<float4>(cur, total);
     hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum, P.set_off, P.set_pix);
     RM_HIP(hipGetLastError());
 
@@ -852,12 +1018,24 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         const unsigned gs = (nsets + RB - 1) / RB;
         const unsigned gb = (unsigned)(device_cu_count() * tuning().render_blocks_per_cu);
         hipLaunchKernelGGL(k_clear_sets, dim3(gs), dim3(RB), 0, st, AB, nsets);
+        const unsigned ge = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
+        const int fetch_min = tuning().fetch_min;
+        const bool capped = tuning().render_min_waves == 4;
+        auto bounce = capped ? k_bounce<4> : k_bounce<1>;
+        auto nee = capped ? k_nee<4> : k_nee<1>;
         for (int j = 0; j < R; ++j) {
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counts, 2 * NCOUNT);
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, counts, 4 * NCOUNT);
             hipLaunchKernelGGL(k_spawn, dim3(gs), dim3(RB), 0, st, (const Job *)dJ, nj, P, j, nsets, Q[0], counts);
-            for (int d = 1; d <= D; ++d)
-                hipLaunchKernelGGL(k_bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1],
-                                   (const uint32_t *)(counts + d - 1), counts + NCOUNT + d, Q[d & 1], counts + d);
+            for (int d = 1; d <= D; ++d) {
+                const uint32_t *n_in = counts + d - 1;
+                hipLaunchKernelGGL(k_extend, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1], n_in,
+                                   counts + 2 * NCOUNT + d, hits, fetch_min);
+                hipLaunchKernelGGL(bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1], n_in,
+                                   counts + NCOUNT + d, (const uint2 *)hits, Q[d & 1], counts + d, NQ,
+                                   counts + 3 * NCOUNT);
+            }
+            hipLaunchKernelGGL(nee, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, P, AB, NQ,
+                               (const uint32_t *)(counts + 3 * NCOUNT), counts + 3 * NCOUNT + 1);
         }
     }
     hipLaunchKernelGGL(k_finish, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, AB);

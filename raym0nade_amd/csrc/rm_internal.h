@@ -91,6 +91,8 @@ struct Tuning {
     int ray_sort = 1;             // 0 never, 1 when consecutive origins are far apart, 2 always
     int render_blocks_per_cu = 8; // resident blocks per CU of the persistent bounce kernel
     int adjoint_split = 4;        // lanes per pixel of the adjoint kernel
+    int kernel_timing = 0;        // record HIP events around the trace kernels (rm_last_trace_ms)
+    int render_min_waves = 1;     // 1 or 4: register cap of the shading and NEE kernels
 };
 Tuning &tuning();
 int device_cu_count();

@@ -331,11 +331,31 @@ static int32_t ray_counter(unsigned long long **out) {
     return RM_OK;
 }
 
+// HIP events bracketing the reordering kernels and the traversal kernel of
+// the last trace call (recorded when kernel timing is on; rm_last_trace_ms).
+struct TraceEvents {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    bool recorded = false;
+};
+static TraceEvents &trace_events() {
+    static TraceEvents ev;
+    return ev;
+}
+static int32_t mark(int k, hipStream_t st) {
+    TraceEvents &ev = trace_events();
+    if (!ev.e[k]) RM_HIP(hipEventCreate(&ev.e[k]));
+    RM_HIP(hipEventRecord(ev.e[k], st));
+    return RM_OK;
+}
+
 template <bool ANY>
 static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, float tmin, float t0, const float *aim_dev,
                             float *t_out, int32_t *id_out, uint8_t *occ, hipStream_t st) {
     const Tuning &T = tuning();
     unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
+    const bool timing = T.kernel_timing != 0;
+    int32_t mrc;
+    if (timing && (mrc = mark(0, st))) return mrc;
     // reordering workspace (stream-ordered): keys[n], order[n], bins, flag
     uint32_t *order = nullptr;
     int *flag = nullptr;
@@ -356,6 +376,7 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
         hipLaunchKernelGGL(k_bin_scatter, dim3(gn), dim3(256), 0, st, n, (const int *)flag, (const uint32_t *)keys, hist,
                            order);
     }
+    if (timing && (mrc = mark(1, st))) return mrc;
     if (T.trace_variant == 2) {
         unsigned long long *next = nullptr;
         int32_t rc = ray_counter(&next);
@@ -375,7 +396,23 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                            id_out, (const uint32_t *)order, (const int *)flag);
     }
     RM_HIP(hipGetLastError());
+    if (timing) {
+        if ((mrc = mark(2, st))) return mrc;
+        trace_events().recorded = true;
+    }
     if (ws) RM_HIP(hipFreeAsync(ws, st));
+    return RM_OK;
+}
+
+extern "C" int32_t rm_last_trace_ms(float *reorder_ms, float *traverse_ms) {
+    TraceEvents &ev = trace_events();
+    if (!ev.recorded) return fail(RM_ERR_INVALID, "rm_last_trace_ms: no trace call ran with kernel timing on");
+    RM_HIP(hipEventSynchronize(ev.e[2]));
+    float a = 0.0f, b = 0.0f;
+    RM_HIP(hipEventElapsedTime(&a, ev.e[0], ev.e[1]));
+    RM_HIP(hipEventElapsedTime(&b, ev.e[1], ev.e[2]));
+    if (reorder_ms) *reorder_ms = a;
+    if (traverse_ms) *traverse_ms = b;
     return RM_OK;
 }
 
