@@ -293,6 +293,12 @@ def batch_leg(args, torch, rm):
             a.paths_per_round = tuned["paths_per_round"]
         log(f"[bench] batch autotune: {tuned}")
     rm.render_batch(jobs[:2], use_graph=True)  # warm-up
+    # every frame's own work buffers exist before the sequential timing (a
+    # low-spp render sizes them: the sizes do not depend on spp)
+    for S, a, fr, post in jobs:
+        b = rm.RmRenderArgs.from_buffer_copy(a)
+        b.spp = 4
+        fr.render(S, b)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     rm.render_batch(jobs, use_graph=True)
@@ -324,6 +330,10 @@ def adjoint_leg(args, torch, rm):
     img = R.forward_image(tex, rough, emis)
     d = torch.randn_like(img)
     R.adjoint(d, tex.numel())  # warm-up
+    tuned = None
+    if not args.no_autotune:
+        from raym0nade_amd import autotune
+        tuned = autotune.tune_adjoint(R, (tex, rough, emis), d, log=log)
     fw, rd, ad = [], [], []
     for _ in range(3):
         torch.cuda.synchronize()
@@ -343,7 +353,8 @@ def adjoint_leg(args, torch, rm):
     out = {"render_s": round(r, 4), "forward_s": round(f, 4), "adjoint_s": round(a, 4),
            "adjoint_over_forward": round(a / r, 3), "adjoint_over_forward_with_upload": round(a / f, 3),
            "width": R.args.width, "height": R.args.height, "spp": R.args.spp, "texels": int(tex.numel()),
-           "split": rm.get_tuning(rm.TUNE_ADJOINT_SPLIT)}
+           "split": rm.get_tuning(rm.TUNE_ADJOINT_SPLIT), "autotune": tuned,
+           "note": "render with per-sample recording on (the adjoint replays each sample once)"}
     log(f"[bench] adjoint: render {r:.3f} s, forward {f:.3f} s, adjoint {a:.3f} s, ratio {a / r:.2f}")
     if args.adam_steps > 0:
         from raym0nade_amd.inverse import inverse_render

@@ -234,6 +234,12 @@ int32_t rm_frame_copy(const rm_frame *frame, int32_t which, float *dst, void *st
  * emits.  Frames rendered before the update no longer match the scene. */
 int32_t rm_scene_update(rm_scene *scene, const rm_material *materials, const float *texels, void *stream);
 
+/* With recording on, rm_render also keeps per indirect sample of every pixel
+ * the sums rm_adjoint would otherwise recompute by a first replay of the
+ * sample (48 B per sample, scenes without transmission only), so the adjoint
+ * replays each sample once.  Off by default; off frees the record. */
+int32_t rm_frame_set_recording(rm_frame *frame, int32_t on);
+
 /* Gradient outputs of rm_adjoint, all device memory, overwritten; any may be NULL. */
 typedef struct rm_grads {
     float *textures;   /* d loss / d level-0 texel, same layout as rm_scene_update's texels */

@@ -191,6 +191,10 @@ class Frame:
     def render(self, scene: Scene, args: RmRenderArgs, stream=None):
         check(lib().rm_render(scene.handle, C.byref(args), self._h, _stream(stream)))
 
+    def set_recording(self, on: bool = True):
+        """Keep per-sample sums for rm_adjoint (one replay per sample instead of two)."""
+        check(lib().rm_frame_set_recording(self._h, int(on)))
+
     def postprocess(self, options: int, exposure: float = 1.0, stream=None):
         check(lib().rm_postprocess(self._h, options, exposure, _stream(stream)))
 

@@ -144,6 +144,7 @@ SIGNATURES = [
     ("rm_postprocess", C.c_int32, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p]),
     ("rm_render_batch", C.c_int32, [C.POINTER(RmBatchJob), C.c_int32, C.c_int32, C.c_void_p]),
     ("rm_frame_copy", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("rm_frame_set_recording", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rm_scene_update", C.c_int32, [C.c_void_p, C.POINTER(RmMaterial), C.c_void_p, C.c_void_p]),
     ("rm_adjoint", C.c_int32, [C.c_void_p, C.POINTER(RmRenderArgs), C.c_void_p, C.c_void_p, C.POINTER(RmGrads),
                                C.c_void_p]),

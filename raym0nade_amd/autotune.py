@@ -64,7 +64,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
             "seconds": t, "tried": len(results)}
 
 
-def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2), log=None) -> dict:
+def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4), log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp
     scaled by spp_scale: first every (blocks per CU, register cap) pair with
@@ -95,6 +95,24 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
     set_tuning(TUNE_RENDER_MIN_WAVES, w)
     return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "seconds_scaled": t,
             "spp_scale": spp_scale, "tried": len(results)}
+
+
+def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> dict:
+    """Lanes per pixel of the adjoint kernel (each replays a share of the
+    pixel's samples): times rm_adjoint of the renderer's last forward for each
+    split and keeps the fastest."""
+    from . import TUNE_ADJOINT_SPLIT
+    renderer.forward_image(*params)
+    results = []
+    for s in splits:
+        set_tuning(TUNE_ADJOINT_SPLIT, s)
+        t = _time(lambda: renderer.adjoint(d_image, params[0].numel()), 2)
+        results.append((t, s))
+        if log:
+            log(f"[autotune] adjoint split {s}: {t * 1e3:.1f} ms")
+    t, s = min(results)
+    set_tuning(TUNE_ADJOINT_SPLIT, s)
+    return {"split": s, "seconds": t, "tried": len(results)}
 
 
 def current() -> dict:

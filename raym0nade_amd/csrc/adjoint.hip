@@ -108,6 +108,8 @@ struct AdjArgs {
     float *gtex;           // texel gradients, laid out like S.texels (every level), or null
     float *gmat;           // [n_mats][4]: d roughness, d emission rgb
     float *replay;         // [np][3] or null
+    const float4 *rec;     // the render's per-sample record (sum dd | sum ds | sum w L), or null
+    int32_t rec_stride;
     int64_t np;
     int split;
 };
@@ -562,7 +564,16 @@ __device__ __forceinline__ void adjoint_pixel(const AdjArgs &G, int64_t p, int c
     const float hit_t = length(s.pos - A.pos);
     for (int si = chunk; si < n_ind; si += G.split) {
         V3 Lam, Cv;
-        replay_path<false>(G, s0, in, hit_t, cam, iw, p, (uint32_t)si, inv, drop, Lam, Cv, dl, gb, gr0, sh);
+        if (G.rec && !(drop >= 0 && (drop >> 3) == si)) {
+            // the render recorded this sample's sums (its dropped outlier, if
+            // any, is not excluded there, so such a sample is replayed instead)
+            const float4 *r = G.rec + ((int64_t)p * G.rec_stride + si) * 3;
+            float4 r0 = r[0], r1 = r[1], r2 = r[2];
+            Cv = Fb * V3{r0.x, r0.y, r0.z} + V3{r1.x, r1.y, r1.z};
+            Lam = V3{r2.x, r2.y, r2.z};
+        } else {
+            replay_path<false>(G, s0, in, hit_t, cam, iw, p, (uint32_t)si, inv, drop, Lam, Cv, dl, gb, gr0, sh);
+        }
         rep += Cv * ex;
         if (want_grad && nonzero(Cv))
             replay_path<true>(G, s0, in, hit_t, cam, iw, p, (uint32_t)si, inv, drop, Lam, Cv, dl, gb, gr0, sh);
@@ -639,6 +650,8 @@ extern "C" int32_t rm_adjoint(const rm_scene *scene, const rm_render_args *a, rm
     G.gtex = g->textures ? gtex : nullptr;
     G.gmat = gmat;
     G.replay = g->replay;
+    G.rec = f->rec_valid ? f->rec.as<float4>() : nullptr;
+    G.rec_stride = f->rec_stride;
     G.np = np;
     G.split = tuning().adjoint_split;
     const int64_t lanes = np * G.split;

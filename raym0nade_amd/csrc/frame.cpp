@@ -37,6 +37,16 @@ extern "C" int32_t rm_frame_buffer(rm_frame *f, int32_t which, float **dev_ptr) 
     return RM_OK;
 }
 
+extern "C" int32_t rm_frame_set_recording(rm_frame *f, int32_t on) {
+    if (!f) return fail(RM_ERR_INVALID, "rm_frame_set_recording: frame is NULL");
+    f->record = on != 0;
+    if (!f->record) {
+        f->rec.release();
+        f->rec_valid = false;
+    }
+    return RM_OK;
+}
+
 extern "C" int32_t rm_frame_copy(const rm_frame *f, int32_t which, float *dst, void *stream) {
     if (!f || !dst || which < 0 || which >= RM_NUM_BUFFERS) return fail(RM_ERR_INVALID, "rm_frame_copy: bad argument");
     RM_HIP(hipMemcpyAsync(dst, f->buf[which].p, (size_t)f->w * f->h * frame_channels(which) * sizeof(float),

@@ -429,9 +429,11 @@ struct SetAcc {
     Acc4 md, ms;
 };
 
-__device__ __forceinline__ void add_sample(SetAcc &a, V3 base, V3 p0, V3 light, float w, uint32_t id) {
+__device__ __forceinline__ void add_sample(SetAcc &a, V3 base, V3 p0, V3 light, float w, uint32_t id, Acc4 &dd,
+                                           Acc4 &ds) {
     float c = dot(p0 * light, W_RGB()) * w;
-    Acc4 dd{0, 0, 0, 0}, ds{0, 0, 0, 0};
+    dd = Acc4{0, 0, 0, 0};
+    ds = Acc4{0, 0, 0, 0};
     accumulate(base, LSample{p0, light, w}, dd, ds);
     a.d.r += dd.r; a.d.g += dd.g; a.d.b += dd.b; a.d.v += dd.v;
     a.s.r += ds.r; a.s.g += ds.g; a.s.b += ds.b; a.s.v += ds.v;
@@ -505,18 +507,35 @@ __device__ __forceinline__ void store_nee(const NeeQ &Q, uint32_t q, const Path 
 // accumulator set, passBsdf'd up to the first vertex: light *= terminal bsdf
 // (and, on the refraction side, the medium's absorb) and every deeper bsdfPdf.
 // A read-modify-write: the set belongs to this path for the round.
+// With `rec` (the sample's record, rm_frame_set_recording) the sample's
+// increments and w L are also summed into the record.
 __device__ __forceinline__ void add_light_sample(const PixBufs &P, const AccBufs &AB, uint32_t pix, uint32_t set,
                                                  uint32_t smp, uint32_t k, V3 T, float W, V3 p0, const LSample &x,
-                                                 bool extra_on, V3 extra) {
+                                                 bool extra_on, V3 extra, float4 *rec) {
     float4 b0 = P.pi0[pix], b1 = P.pi1[pix];
     V3 base = b1.y < T_EPS ? v3(0.0f) : V3{b0.x, b0.y, b0.z};
     float inv = 1.0f / (float)bu(P.pi2[pix].w);
     V3 L = x.light * x.bsdf;
     if (extra_on) L = L * extra;
     L = L * T;
+    const float w = x.weight * W * inv;
     SetAcc acc = load_set(AB, set);
-    add_sample(acc, base, p0, L, x.weight * W * inv, smp * 8u + k);
+    Acc4 dd, ds;
+    add_sample(acc, base, p0, L, w, smp * 8u + k, dd, ds);
     store_set(AB, set, acc);
+    if (rec && !(length(L) < T_EPS)) {
+        float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        rec[0] = make_float4(r0.x + dd.r, r0.y + dd.g, r0.z + dd.b, 1.0f);
+        rec[1] = make_float4(r1.x + ds.r, r1.y + ds.g, r1.z + ds.b, 0.0f);
+        V3 wl = w * L;
+        rec[2] = make_float4(r2.x + wl.x, r2.y + wl.y, r2.z + wl.z, 0.0f);
+    }
+}
+
+// The record slot of indirect sample `smp` of global pixel `gp` in job jb.
+__device__ __forceinline__ float4 *rec_slot(const Job &jb, uint32_t gp, uint32_t smp) {
+    if (!jb.F.rec) return nullptr;
+    return jb.F.rec + ((int64_t)(gp - jb.pix_off) * jb.F.rec_stride + smp) * 3;
 }
 
 // Closest hit of every queued path (sampleRay's model.rayHit, :127), ahead of
@@ -607,7 +626,8 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
         bool extra_on = false;
         // an emitter or sky hit ends the path with one light sample
         auto emit = [&](const LSample &x) {
-            add_light_sample(P, AB, path.pix, path.set, path.smp, 0u, path.T, path.W, path.p0, x, false, extra);
+            add_light_sample(P, AB, path.pix, path.set, path.smp, 0u, path.T, path.W, path.p0, x, false, extra,
+                             rec_slot(jb, path.pix, path.smp));
         };
         if (slot < 0) {
             if (!(path.exclude || !S.sky)) emit(LSample{v3(1.0f), sky_get(S, path.d), 1.0f});
@@ -762,7 +782,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
                 LSample x;
                 if (direct_sample<false>(S, s, V, total, cnt, rng, x, tc)) {
                     x.bsdf *= factor;  // scaling(samples, factor)
-                    add_light_sample(P, AB, pix, set, smp, k, T, W, p0, x, extra_on, extra);
+                    add_light_sample(P, AB, pix, set, smp, k, T, W, p0, x, extra_on, extra, rec_slot(jb, pix, smp));
                     ++k;
                 }
             }
@@ -785,6 +805,11 @@ __global__ __launch_bounds__(RB) void k_clear_sets(AccBufs AB, uint32_t n) {
 __global__ void k_zero_u32(uint32_t *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+__global__ void k_zero_f4(float4 *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(RB) void k_finish(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P,
@@ -918,6 +943,26 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, int64_
     F.Id = f->buf[RM_BUF_ID].as<float>();
     F.Is = f->buf[RM_BUF_IS].as<float>();
     F.drop = f->drop.as<int32_t>();
+    F.rec = nullptr;
+    F.rec_stride = 0;
+    f->rec_valid = false;
+    if (f->record) {
+        // one record per (pixel, indirect sample); only for scenes without
+        // transmission (16x sample budgets on transparent pixels)
+        bool opaque = true;
+        for (const MatRec &m : h.scene->host.mats) opaque = opaque && m.opacity > 1.0f - T_EPS;
+        const int stride = a->spp - (int)(float(a->spp) * a->p_direct);
+        if (opaque && stride > 0) {
+            const size_t need = (size_t)np * stride * 48;
+            if (f->rec.bytes < need) {
+                int32_t rc = f->rec.alloc(need);
+                if (rc) return rc;
+            }
+            F.rec = f->rec.as<float4>();
+            F.rec_stride = stride;
+            f->rec_stride = stride;
+        }
+    }
     J.np = np;
     return RM_OK;
 }
@@ -968,6 +1013,11 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     // counts[3 NCOUNT]: NEE records of the round, counts[3 NCOUNT + 1]: their claims
     uint32_t *counts = carve<uint32_t>(cur, 4 * NCOUNT);
     RM_HIP(hipMemcpyAsync(dJ, J.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st));
+    for (int i = 0; i < nj; ++i)
+        if (J[i].F.rec) {
+            const int64_t n4 = J[i].np * J[i].F.rec_stride * 3;
+            hipLaunchKernelGGL(k_zero_f4, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, J[i].F.rec, n4);
+        }
     const unsigned gp = (unsigned)((total_px + RB - 1) / RB);
     hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P);
     hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum);
@@ -1072,6 +1122,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         f->stamp_version = hj[i].scene->version;
         f->stamp_args = *hj[i].args;
         f->stamped = true;
+        f->rec_valid = J[i].F.rec != nullptr;
     }
     return RM_OK;
 }

@@ -15,6 +15,12 @@ struct rm_frame {
     rm::DevBuf post;  // tone-mapped image read by FXAA
     rm::DevBuf drop;  // per pixel: the dropped outlier light sample (render -> adjoint)
     rm::DevBuf adj;   // adjoint scratch: texel gradients of every mip level, material gradients
+    // per indirect sample (rm_frame_set_recording): sums over its light samples
+    // of the diffuse and specular increments and of w L, [pixel][sample] x 3 float4
+    rm::DevBuf rec;
+    bool record = false;
+    bool rec_valid = false;  // the last render recorded (all pixels opaque)
+    int32_t rec_stride = 0;  // indirect samples per pixel
     // the render the buffers hold (the adjoint differentiates exactly that one)
     const rm_scene *stamp_scene = nullptr;
     uint64_t stamp_version = 0;

@@ -28,11 +28,12 @@ def _p(t: torch.Tensor | None):
 
 
 class DiffRenderer:
-    def __init__(self, gen: GeneratedScene, args: RmRenderArgs | None = None):
+    def __init__(self, gen: GeneratedScene, args: RmRenderArgs | None = None, record: bool = True):
         self.gen = gen
         self.scene = Scene(gen)
         self.args = RmRenderArgs.from_buffer_copy(args if args is not None else gen.args)
         self.frame = Frame(self.args.width, self.args.height)
+        self.frame.set_recording(record)
         d = gen.desc
         self.n_materials = d.n_materials
         self.materials = [RmMaterial.from_buffer_copy(d.materials[i]) for i in range(d.n_materials)]

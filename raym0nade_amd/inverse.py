@@ -16,7 +16,7 @@ import torch
 from .diff import DiffRenderer
 
 
-def inverse_render(gen, steps: int = 200, args=None, lr=(0.02, 0.01, 0.3), log=None) -> dict:
+def inverse_render(gen, steps: int = 200, args=None, lr=(0.005, 0.01, 0.3), log=None) -> dict:
     R = DiffRenderer(gen, args)
     tex0, rough0, emis0 = R.initial_params()
     emitting = emis0.abs().sum(1) > 0

@@ -50,6 +50,47 @@ def test_batch_equals_single_renders(dev, use_graph):
         assert np.isclose(px, px1, rtol=1e-4, atol=1e-5).all(axis=-1).mean() >= 0.999
 
 
+def test_serving_batch_with_autotune_equals_single_renders(dev):
+    from raym0nade_amd import autotune, workloads
+    specs = workloads.batch_specs(6, seed=3, min_log2=5.0, max_log2=6.5)
+    built = [workloads.build_job(s, sky=(64, 32)) for s in specs]
+    jobs = [(S, a, rm.Frame(a.width, a.height), 63 | 512) for _, S, a in built]
+    saved = autotune.current()
+    try:
+        choice = autotune.tune_render(jobs, spp_scale=0.25)
+        assert choice["blocks_per_cu"] in (4, 8, 16) and choice["min_waves"] in (1, 4)
+        rm.render_batch(jobs, use_graph=True)
+        torch.cuda.synchronize()
+        for S, a, fr, post in jobs:
+            fr1 = rm.Frame(a.width, a.height)
+            fr1.render(S, a)
+            fr1.postprocess(post, a.exposure)
+            torch.cuda.synchronize()
+            px, px1 = fr.download(rm.RM_BUF_PIXELS), fr1.download(rm.RM_BUF_PIXELS)
+            assert np.isclose(px, px1, rtol=1e-4, atol=1e-5).all(axis=-1).mean() >= 0.999
+    finally:
+        rm.set_tuning(rm.TUNE_RENDER_BLOCKS_PER_CU, saved["render_blocks_per_cu"])
+        rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, saved["render_min_waves"])
+
+
+def test_trace_autotune_keeps_results(dev):
+    from raym0nade_amd import autotune
+    g = rm.generate(1, 2, n_tris=20000)
+    S = rm.Scene(g)
+    r = torch.from_numpy(rm.rays(1, 200000, seed=4)).to(dev)
+    t0, i0 = S.trace_closest(r)
+    saved = autotune.current()
+    try:
+        choice = autotune.tune_trace(S, [r], reps=1)
+        assert choice["tried"] >= 3
+        t1, i1 = S.trace_closest(r)
+        assert torch.equal(i0, i1) and torch.equal(t0.view(torch.int32), t1.view(torch.int32))
+    finally:
+        for k, name in ((rm.TUNE_TRACE_VARIANT, "trace_variant"), (rm.TUNE_PERSISTENT_BLOCKS_PER_CU,
+                        "persistent_blocks_per_cu"), (rm.TUNE_FETCH_MIN, "fetch_min"), (rm.TUNE_RAY_SORT, "ray_sort")):
+            rm.set_tuning(k, saved[name])
+
+
 def test_batch_argument_errors(dev):
     g = rm.generate(0, 1)
     S = rm.Scene(g)
