@@ -305,17 +305,35 @@ def batch_leg(args, torch, rm):
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     log(f"[bench] batch: one graph launch {tb:.2f} s")
-    t0 = time.perf_counter()
-    for S, a, fr, post in jobs:
-        fr.render(S, a)
-        fr.postprocess(post, a.exposure)
-    torch.cuda.synchronize()
-    ts = time.perf_counter() - t0
-    log(f"[bench] batch: sequential rm_render {ts:.2f} s")
+    def sequential():
+        t0 = time.perf_counter()
+        for S, a, fr, post in jobs:
+            fr.render(S, a)
+            fr.postprocess(post, a.exposure)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    ts_tuned = sequential()
+    log(f"[bench] batch: sequential rm_render with the batch's launch parameters {ts_tuned:.2f} s")
+    # plain launches: rm_render per job with the library's default parameters
+    saved = autotune.current()
+    rm.set_tuning(rm.TUNE_RENDER_BLOCKS_PER_CU, 8)
+    rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, 1)
+    for _, a, _, _ in jobs:
+        a.paths_per_round = 0
+    ts = sequential()
+    rm.set_tuning(rm.TUNE_RENDER_BLOCKS_PER_CU, saved["render_blocks_per_cu"])
+    rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, saved["render_min_waves"])
+    log(f"[bench] batch: sequential plain rm_render {ts:.2f} s")
     return {"jobs": len(jobs), "samples": total, "batched_samples_per_s": round(total / tb, 1),
             "sequential_samples_per_s": round(total / ts, 1), "batched_over_sequential": round(ts / tb, 3),
-            "batched_s": round(tb, 3), "sequential_s": round(ts, 3), "autotune": tuned,
-            "tris_min": min(tris), "tris_max": max(tris)}
+            "sequential_tuned_samples_per_s": round(total / ts_tuned, 1),
+            "batched_over_sequential_tuned": round(ts_tuned / tb, 3),
+            "batched_s": round(tb, 3), "sequential_s": round(ts, 3), "sequential_tuned_s": round(ts_tuned, 3),
+            "autotune": tuned, "tris_min": min(tris), "tris_max": max(tris),
+            "note": "sequential = one rm_render + rm_postprocess per job with the library's default launch "
+                    "parameters (blocks/CU 8, no register cap, automatic paths per round); sequential_tuned = "
+                    "the same with the batch's autotuned parameters; every frame's buffers exist beforehand"}
 
 
 def adjoint_leg(args, torch, rm):
@@ -376,6 +394,12 @@ def render_leg(args, torch, rm, dev, stream):
     log(f"[bench] scene: {S.info.n_tris} tris, build {S.info.build_seconds:.1f}s, "
         f"{S.info.device_bytes / 2**30:.2f} GiB in HBM")
     fr = rm.Frame(a.width, a.height)
+    tuned = None
+    if not args.no_autotune:
+        from raym0nade_amd import autotune
+        tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 16, blocks=(4, 8), waves=(1, 4),
+                                     paths_per_round=(0,), log=log)
+        a.paths_per_round = tuned["paths_per_round"]
     times = []
     for k in range(args.render_frames + 1):
         torch.cuda.synchronize()
@@ -390,7 +414,8 @@ def render_leg(args, torch, rm, dev, stream):
     wall = float(np.median(times))
     samples = a.width * a.height * a.spp
     out = {"path_samples_per_s": round(samples / wall, 1), "frame_s": round(wall, 4), "frames": len(times),
-           "width": a.width, "height": a.height, "spp": a.spp, "max_depth": a.max_depth, "n_tris": S.info.n_tris}
+           "width": a.width, "height": a.height, "spp": a.spp, "max_depth": a.max_depth, "n_tris": S.info.n_tris,
+           "autotune": tuned}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O

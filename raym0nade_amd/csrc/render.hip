@@ -29,6 +29,7 @@
 // increments when it is dropped.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include "rm_frame.h"
 #include "integrator.h"
@@ -1154,8 +1155,16 @@ extern "C" int32_t rm_render_batch(const rm_batch_job *jobs, int32_t n, int32_t 
         RM_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
         st = own;
     }
-    DevBuf work;
-    int32_t rc = render_jobs(hj, work, st, use_graph != 0);
+    // one workspace per device, kept between batches (a serving loop would
+    // otherwise allocate gigabytes per call); batches on one device take turns
+    // (never destroyed: freeing at exit could run after the HIP runtime's teardown)
+    static std::mutex mu;
+    static DevBuf *work = new DevBuf[64];
+    std::lock_guard<std::mutex> lock(mu);
+    int dev = 0;
+    RM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(RM_ERR_UNSUPPORTED, "rm_render_batch: device %d", dev);
+    int32_t rc = render_jobs(hj, work[dev], st, use_graph != 0);
     hipError_t e = hipStreamSynchronize(st);
     if (own) (void)hipStreamDestroy(own);
     if (rc) return rc;
