@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="accepted for older profile commands; the 1 GiB calibration copy always runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort'")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
@@ -107,9 +108,15 @@ def main():
     log("[bench] algorithmic bytes/ray:", {f"{k[0]}{'_any' if k[1] else ''}": round(v / (n_coh if k[0] == 'coh' else n_inc), 1)
                                            for k, v in algo.items()})
     tuned = None
-    if not args.no_autotune:
+    if args.trace:
+        v, b, f, s = (int(x) for x in args.trace.split(","))
+        for k, val in ((rm.TUNE_TRACE_VARIANT, v), (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, b), (rm.TUNE_FETCH_MIN, f),
+                       (rm.TUNE_RAY_SORT, s)):
+            rm.set_tuning(k, val)
+        tuned = {"variant": v, "blocks_per_cu": b, "fetch_min": f, "ray_sort": s, "fixed": True}
+    elif not args.no_autotune:
         from raym0nade_amd import autotune
-        tuned = autotune.tune_trace(S, [r_coh, r_inc], reps=2)
+        tuned = autotune.tune_trace(S, [r_coh, r_inc], reps=2, log=log)
         log(f"[bench] autotuned traversal: {tuned}")
 
     def step(times=None):
@@ -174,7 +181,8 @@ def main():
     bytes_per_launch = (algo[("coh", False)] + algo[("inc", False)]) / 2
     dur_per_launch = (np.mean(kt["coh"]) + np.mean(kt["inc"])) / 2 * 1e-3
     achieved = bytes_per_launch / dur_per_launch / 1e9
-    kernel_name = {0: "k_trace_closest", 1: "k_trace_step", 2: "k_trace_persistent"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
+    kernel_name = {0: "k_trace_closest", 1: "k_trace_step", 2: "k_trace_persistent",
+                   3: "k_trace_persistent_lds"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}.json")
     if os.path.exists(tf):

@@ -294,7 +294,9 @@ int32_t rm_generate_rays(int32_t kind, uint32_t seed, int64_t n, const rm_render
  * its thread count).  All settings give identical results; they trade speed.
  * Process-wide; set them before launching work. */
 enum rm_tune_key {
-    RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk (default) */
+    RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk (default),
+                                           3 persistent step walk with the stack in LDS (also used by
+                                           the render's extension kernel) */
     RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variant 2 (default 8) */
     RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 16) */
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive

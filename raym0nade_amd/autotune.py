@@ -43,6 +43,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
         candidates = [(0, 8, 16, 1)]
         candidates += [(1, 8, 16, s) for s in (0, 1)]
         candidates += [(2, b, f, s) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
+        candidates += [(3, b, f, s) for b, f, s in itertools.product((2, 4, 8), (8, 16, 32), (0, 1))]
 
     def run():
         for r in batches:

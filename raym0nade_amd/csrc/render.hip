@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 #include "rm_frame.h"
 #include "integrator.h"
@@ -545,11 +546,20 @@ __device__ __forceinline__ float4 *rec_slot(const Job &jb, uint32_t gp, uint32_t
 // (one atomic per refill), so lanes do not wait for the wave's longest walk.
 // Writes (t, triangle slot or -1) per entry.  The traversal is the same walk
 // as everywhere else, so the hits are bit-identical.
+// LDS: the walk's pending entries in the block's LDS (stack_depth per lane).
+template <bool LDS>
 __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ Q, const uint32_t *count_in,
-                                               uint32_t *work, uint2 *__restrict__ hit, int fetch_min) {
+                                               uint32_t *work, uint2 *__restrict__ hit, int fetch_min,
+                                               int stack_depth) {
     const uint32_t n = *count_in;
     const uint32_t lane = threadIdx.x & 63u;
-    Walker<false, false> w;
+    Walker<false, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
+    if constexpr (LDS) {
+        extern __shared__ uint32_t lds_stack[];
+        w.stk.ref = lds_stack + threadIdx.x;
+        w.stk.tl = reinterpret_cast<float *>(lds_stack + stack_depth * RB) + threadIdx.x;
+        w.stk.stride = RB;
+    }
     RayPre r;
     SceneView S;  // per lane: the nodes, triangles and root of its path's scene
     int64_t q = -1;
@@ -1072,6 +1082,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         const unsigned ge = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
         const int fetch_min = tuning().fetch_min;
         const bool capped = tuning().render_min_waves == 4;
+        const bool lds_walk = tuning().trace_variant == 3;
+        int max_depth = 1;  // deepest tree of the launch's scenes: LDS entries per lane
+        for (int i = 0; i < nj; ++i) max_depth = std::max(max_depth, (int)J[i].S.depth);
         auto bounce = capped ? k_bounce<4> : k_bounce<1>;
         auto nee = capped ? k_nee<4> : k_nee<1>;
         for (int j = 0; j < R; ++j) {
@@ -1079,8 +1092,13 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
             hipLaunchKernelGGL(k_spawn, dim3(gs), dim3(RB), 0, st, (const Job *)dJ, nj, P, j, nsets, Q[0], counts);
             for (int d = 1; d <= D; ++d) {
                 const uint32_t *n_in = counts + d - 1;
-                hipLaunchKernelGGL(k_extend, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1], n_in,
-                                   counts + 2 * NCOUNT + d, hits, fetch_min);
+                if (lds_walk)
+                    hipLaunchKernelGGL(k_extend<true>, dim3(ge), dim3(RB), (size_t)max_depth * RB * 8, st,
+                                       (const Job *)dJ, Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min,
+                                       max_depth);
+                else
+                    hipLaunchKernelGGL(k_extend<false>, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1],
+                                       n_in, counts + 2 * NCOUNT + d, hits, fetch_min, max_depth);
                 hipLaunchKernelGGL(bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1], n_in,
                                    counts + NCOUNT + d, (const uint2 *)hits, Q[d & 1], counts + d, NQ,
                                    counts + 3 * NCOUNT);

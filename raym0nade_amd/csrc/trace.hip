@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include "rm_internal.h"
 #include "traverse.h"
 
@@ -106,8 +107,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const f
 // next unclaimed ray; a wave claims rays for its idle lanes with one atomic
 // once at least `fetch_min` lanes are idle (or all of its live lanes are).
 // Every wave leaves the loop when the batch is exhausted and its lanes are
-// done, so the grid always drains.
-template <bool ANY>
+// done, so the grid always drains.  LDS: the walker's pending entries live in
+// the block's LDS (stack_depth entries per lane, dynamic shared memory)
+// instead of per-lane scratch.
+template <bool ANY, bool LDS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, const float *__restrict__ rays, int64_t n,
                                                                   float tmin, float tmax0,
                                                                   const float *__restrict__ aim_arr,
@@ -115,9 +118,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                                                                   uint8_t *__restrict__ occ,
                                                                   unsigned long long *__restrict__ next, int fetch_min,
                                                                   const uint32_t *__restrict__ order,
-                                                                  const int *__restrict__ use_order) {
+                                                                  const int *__restrict__ use_order, int stack_depth) {
     const uint32_t lane = threadIdx.x & 63u;
-    Walker<ANY, false> w;
+    Walker<ANY, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
+    if constexpr (LDS) {
+        extern __shared__ uint32_t lds_stack[];
+        w.stk.ref = lds_stack + threadIdx.x;
+        w.stk.tl = reinterpret_cast<float *>(lds_stack + stack_depth * TRACE_BLOCK) + threadIdx.x;
+        w.stk.stride = TRACE_BLOCK;
+    }
     RayPre r;
     float aim = -INFINITY;
     int64_t ray = -1;
@@ -377,14 +386,21 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                            order);
     }
     if (timing && (mrc = mark(1, st))) return mrc;
-    if (T.trace_variant == 2) {
+    if (T.trace_variant == 2 || T.trace_variant == 3) {
         unsigned long long *next = nullptr;
         int32_t rc = ray_counter(&next);
         if (rc) return rc;
         unsigned pgrid = (unsigned)std::min<int64_t>(grid, (int64_t)device_cu_count() * T.persistent_blocks_per_cu);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, next);
-        hipLaunchKernelGGL(k_trace_persistent<ANY>, dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0,
-                           aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag);
+        const int depth = std::max(1, (int)s->view.depth);
+        if (T.trace_variant == 3)
+            hipLaunchKernelGGL((k_trace_persistent<ANY, true>), dim3(pgrid), dim3(TRACE_BLOCK),
+                               (size_t)depth * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
+                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, depth);
+        else
+            hipLaunchKernelGGL((k_trace_persistent<ANY, false>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n,
+                               tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
+                               (const int *)flag, depth);
     } else if (T.trace_variant == 1) {
         hipLaunchKernelGGL(k_trace_step<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,
                            t_out, id_out, occ, (const uint32_t *)order, (const int *)flag);

@@ -209,10 +209,29 @@ __device__ __forceinline__ void box_test_fast(const RayPre &r, float lx, float l
     tR = s1 ? R1 : (s2 ? R2 : R3);
 }
 
-template <bool ANY, bool COUNT>
+// Storage of a walker's pending entries.  PrivateStack: per-lane arrays, which
+// the compiler places in scratch (backed by HBM through the caches).
+// LdsStack: a block's LDS, entry i of lane t at [i * stride + t], so a wave's
+// accesses to one level fall in distinct banks; sized by the tree depth.
+struct PrivateStack {
+    uint32_t ref[MAX_STACK];
+    float tl[MAX_STACK];
+    __device__ __forceinline__ void put(int i, uint32_t r, float t) { ref[i] = r; tl[i] = t; }
+    __device__ __forceinline__ uint32_t ref_at(int i) const { return ref[i]; }
+    __device__ __forceinline__ float tl_at(int i) const { return tl[i]; }
+};
+struct LdsStack {
+    uint32_t *ref;  // this lane's column
+    float *tl;
+    int stride;
+    __device__ __forceinline__ void put(int i, uint32_t r, float t) { ref[i * stride] = r; tl[i * stride] = t; }
+    __device__ __forceinline__ uint32_t ref_at(int i) const { return ref[i * stride]; }
+    __device__ __forceinline__ float tl_at(int i) const { return tl[i * stride]; }
+};
+
+template <bool ANY, bool COUNT, class Stack = PrivateStack>
 struct Walker {
-    uint32_t stk_ref[MAX_STACK];
-    float stk_tl[MAX_STACK];
+    Stack stk;
     int sp;
     uint32_t cur;  // interior node index, leaf ref (bit 31), or 0 when finished
     int32_t best;
@@ -250,8 +269,7 @@ struct Walker {
             float tLs = f0 ? tL1 : tL0, tRs = f0 ? tR1 : tR0;
             uint32_t rf = f0 ? ref0 : ref1, rs = f0 ? ref1 : ref0;
             if (tLs < tRs) {
-                stk_ref[sp] = rs;
-                stk_tl[sp] = tLs;
+                stk.put(sp, rs, tLs);
                 ++sp;
             }
             cur = (tLf < tRf) ? rf : 0u;
@@ -263,8 +281,8 @@ struct Walker {
         if (cur == 0u) {
             while (sp > 0) {
                 --sp;
-                if (stk_tl[sp] < tmax) {
-                    cur = stk_ref[sp];
+                if (stk.tl_at(sp) < tmax) {
+                    cur = stk.ref_at(sp);
                     break;
                 }
             }

@@ -151,3 +151,22 @@ def test_full_size_cornell_properties(cornell):
     st = RC.compare_radiance(rad[:, rows], orad[:, rows])
     print("full C0 sampled rows:", st)
     assert st["direct_close"] >= 0.995 and st["indirect_close"] >= 0.97
+
+
+@pytest.mark.parametrize("scene", ["cornell", "spheres"])
+def test_lds_stack_extension_is_bit_identical(dev, scene):
+    # the extension kernel with its traversal stack in LDS (trace variant 3)
+    # walks the same nodes in the same order as the scratch-stack walker
+    g = rm.generate(0, 1) if scene == "cornell" else rm.generate(2, 3, grid=2, freq=12, sky_width=256, sky_height=128)
+    S = rm.Scene(g)
+    a = small_args(g, 80, 48, 12, max_depth=16)
+    old = rm.get_tuning(rm.TUNE_TRACE_VARIANT)
+    try:
+        rm.set_tuning(rm.TUNE_TRACE_VARIANT, 2)
+        gb2, r2 = RC.gpu_render(S, a)
+        rm.set_tuning(rm.TUNE_TRACE_VARIANT, 3)
+        gb3, r3 = RC.gpu_render(S, a)
+    finally:
+        rm.set_tuning(rm.TUNE_TRACE_VARIANT, old)
+    assert np.array_equal(gb2.view(np.uint32), gb3.view(np.uint32))
+    assert np.array_equal(r2.view(np.uint32), r3.view(np.uint32))

@@ -29,8 +29,9 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
     return occ.cpu().numpy()
 
 
-@pytest.fixture(params=[(0, 0), (0, 2), (1, 1), (2, 1), (2, 2)],
-                ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted"])
+@pytest.fixture(params=[(0, 0), (0, 2), (1, 1), (2, 1), (2, 2), (3, 1), (3, 0)],
+                ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted",
+                     "persistent-lds-adaptive", "persistent-lds"])
 def variant(request):
     """Every traversal kernel variant, with and without ray reordering, must
     give the same bits."""
