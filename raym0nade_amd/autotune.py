@@ -65,36 +65,43 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
             "seconds": t, "tried": len(results)}
 
 
-def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4), log=None) -> dict:
+def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4), walks=(3, 2),
+                log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp
     scaled by spp_scale: first every (blocks per CU, register cap) pair with
-    automatic paths per round, then the other paths-per-round values at the
-    best pair.  Sets the launch parameters and returns the choice; the caller
-    applies paths_per_round (0 = automatic) to its jobs' arguments."""
+    automatic paths per round and the walk stacks of trace variant walks[0],
+    then the other paths-per-round values at the best pair, then the other
+    walk variants (3: stacks in LDS, 2: in scratch).  Sets the launch
+    parameters and returns the choice; the caller applies paths_per_round
+    (0 = automatic) to its jobs' arguments."""
     small = []
     for S, a, fr, post in jobs:
         b = RmRenderArgs.from_buffer_copy(a)
         b.spp = max(4, int(round(a.spp * spp_scale)))
         small.append([S, b, fr, post])
 
-    def trial(bpc, w, k):
+    def trial(bpc, w, k, v):
         for j in small:
             j[1].paths_per_round = k
         set_tuning(TUNE_RENDER_BLOCKS_PER_CU, bpc)
         set_tuning(TUNE_RENDER_MIN_WAVES, w)
+        set_tuning(TUNE_TRACE_VARIANT, v)
         t = _time(lambda: render_batch([tuple(j) for j in small], use_graph=True), 1)
         if log:
-            log(f"[autotune] render blocks/CU {bpc} min waves {w} paths/round {k or 'auto'}: {t:.3f} s")
-        return (t, bpc, w, k)
+            log(f"[autotune] render blocks/CU {bpc} min waves {w} paths/round {k or 'auto'} walk {v}: {t:.3f} s")
+        return (t, bpc, w, k, v)
 
-    results = [trial(b, w, paths_per_round[0]) for b in blocks for w in waves]
-    _, bpc, w, _ = min(results)
-    results += [trial(bpc, w, k) for k in paths_per_round[1:]]
-    t, bpc, w, k = min(results)
+    results = [trial(b, w, paths_per_round[0], walks[0]) for b in blocks for w in waves]
+    _, bpc, w, _, _ = min(results)
+    results += [trial(bpc, w, k, walks[0]) for k in paths_per_round[1:]]
+    _, bpc, w, k, _ = min(results)
+    results += [trial(bpc, w, k, v) for v in walks[1:]]
+    t, bpc, w, k, v = min(results)
     set_tuning(TUNE_RENDER_BLOCKS_PER_CU, bpc)
     set_tuning(TUNE_RENDER_MIN_WAVES, w)
-    return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "seconds_scaled": t,
+    set_tuning(TUNE_TRACE_VARIANT, v)
+    return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "walk_variant": v, "seconds_scaled": t,
             "spp_scale": spp_scale, "tried": len(results)}
 
 
@@ -116,8 +123,16 @@ def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> di
     return {"split": s, "seconds": t, "tried": len(results)}
 
 
+_KEYS = {"trace_variant": TUNE_TRACE_VARIANT, "persistent_blocks_per_cu": TUNE_PERSISTENT_BLOCKS_PER_CU,
+         "fetch_min": TUNE_FETCH_MIN, "ray_sort": TUNE_RAY_SORT, "render_blocks_per_cu": TUNE_RENDER_BLOCKS_PER_CU,
+         "render_min_waves": TUNE_RENDER_MIN_WAVES}
+
+
 def current() -> dict:
-    keys = {"trace_variant": TUNE_TRACE_VARIANT, "persistent_blocks_per_cu": TUNE_PERSISTENT_BLOCKS_PER_CU,
-            "fetch_min": TUNE_FETCH_MIN, "ray_sort": TUNE_RAY_SORT, "render_blocks_per_cu": TUNE_RENDER_BLOCKS_PER_CU,
-            "render_min_waves": TUNE_RENDER_MIN_WAVES}
-    return {n: get_tuning(k) for n, k in keys.items()}
+    return {n: get_tuning(k) for n, k in _KEYS.items()}
+
+
+def restore(saved: dict) -> None:
+    """Sets the launch parameters back to a current() snapshot."""
+    for n, v in saved.items():
+        set_tuning(_KEYS[n], v)

@@ -330,7 +330,7 @@ __device__ __forceinline__ void replay_path(const AdjArgs &G, const Surf &s0, V3
     for (;;) {
         RayPre r = ray_pre(o, d);
         float t = INFINITY;
-        int32_t slot = trace_walk<false>(S, r, T_EPS, t, -INFINITY);
+        int32_t slot = trace_walk<false, false>(S, r, T_EPS, t, -INFINITY);
         if (slot < 0) {
             if (!exclude && S.sky) {
                 V3 L = sky_get(S, d) * v3(1.0f);
@@ -393,7 +393,7 @@ __device__ __forceinline__ void replay_path(const AdjArgs &G, const Surf &s0, V3
                     LSample x;
                     DirectInfo di;
                     TraceCounts tc{};
-                    if (!direct_sample<false>(S, s, V, total, cnt, rng, x, tc, &di)) continue;
+                    if (!direct_sample<false, false>(S, s, V, total, cnt, rng, x, tc, &di)) continue;
                     x.bsdf *= factor;
                     const uint32_t id = si * 8u + k;
                     ++k;
@@ -472,7 +472,7 @@ __device__ __forceinline__ void adjoint_pixel(const AdjArgs &G, int64_t p, int c
     camera_ray(A, x, y, Dir, cam);
     RayPre r = ray_pre(A.pos, Dir);
     float t = INFINITY;
-    int32_t slot = trace_walk<false>(S, r, T_EPS, t, -INFINITY);
+    int32_t slot = trace_walk<false, false>(S, r, T_EPS, t, -INFINITY);
     const V3 dimg = ld3(G.dimg + p * 3);
     const float ex = A.exposure;
     V3 rep = v3(0.0f);
@@ -529,7 +529,7 @@ __device__ __forceinline__ void adjoint_pixel(const AdjArgs &G, int64_t p, int c
                 LSample ls;
                 DirectInfo di;
                 TraceCounts tc{};
-                if (!direct_sample<false>(S, s, V, total, cnt, g, ls, tc, &di)) continue;
+                if (!direct_sample<false, false>(S, s, V, total, cnt, g, ls, tc, &di)) continue;
                 if (length(ls.light) < T_EPS) continue;
                 if (G.replay) {
                     Acc4 dd{0, 0, 0, 0}, ds{0, 0, 0, 0};

@@ -38,6 +38,7 @@
 namespace rm {
 
 constexpr int RB = 256;      // block size of the render kernels
+static_assert(RB == WALK_BLOCK, "LDS walk stacks assume blocks of WALK_BLOCK lanes");
 constexpr int NCOUNT = 32;   // queue counters per round (>= max_depth + 1)
 
 // ---------------------------------------------------------------- medium
@@ -546,19 +547,16 @@ __device__ __forceinline__ float4 *rec_slot(const Job &jb, uint32_t gp, uint32_t
 // (one atomic per refill), so lanes do not wait for the wave's longest walk.
 // Writes (t, triangle slot or -1) per entry.  The traversal is the same walk
 // as everywhere else, so the hits are bit-identical.
-// LDS: the walk's pending entries in the block's LDS (stack_depth per lane).
+// LDS: the walk's pending entries in the block's dynamic LDS (the deepest
+// tree's depth * RB * 8 bytes).
 template <bool LDS>
 __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ Q, const uint32_t *count_in,
-                                               uint32_t *work, uint2 *__restrict__ hit, int fetch_min,
-                                               int stack_depth) {
+                                               uint32_t *work, uint2 *__restrict__ hit, int fetch_min) {
     const uint32_t n = *count_in;
     const uint32_t lane = threadIdx.x & 63u;
     Walker<false, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
     if constexpr (LDS) {
-        extern __shared__ uint32_t lds_stack[];
-        w.stk.ref = lds_stack + threadIdx.x;
-        w.stk.tl = reinterpret_cast<float *>(lds_stack + stack_depth * RB) + threadIdx.x;
-        w.stk.stride = RB;
+        w.stk = lds_stack_of_lane();
     }
     RayPre r;
     SceneView S;  // per lane: the nodes, triangles and root of its path's scene
@@ -1023,6 +1021,14 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     // depth d; counts[2 NCOUNT + d]: traversal claims of depth d;
     // counts[3 NCOUNT]: NEE records of the round, counts[3 NCOUNT + 1]: their claims
     uint32_t *counts = carve<uint32_t>(cur, 4 * NCOUNT);
+    // Trace variant 3 keeps every walk's pending entries in LDS: each kernel of
+    // the launch that traverses gets the deepest tree's stack, and the job
+    // table's views say so.
+    const bool lds_walk = tuning().trace_variant == 3;
+    int max_depth = 1;
+    for (int i = 0; i < nj; ++i) max_depth = std::max(max_depth, (int)J[i].S.depth);
+    const size_t walk_lds = lds_walk ? (size_t)max_depth * RB * 8 : 0;
+    for (int i = 0; i < nj; ++i) J[i].S.walk_lds = lds_walk ? 1 : 0;
     RM_HIP(hipMemcpyAsync(dJ, J.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st));
     for (int i = 0; i < nj; ++i)
         if (J[i].F.rec) {
@@ -1030,7 +1036,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
             hipLaunchKernelGGL(k_zero_f4, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, J[i].F.rec, n4);
         }
     const unsigned gp = (unsigned)((total_px + RB - 1) / RB);
-    hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P);
+    hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
     hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum);
     RM_HIP(hipGetLastError());
     std::vector<int64_t> hs(nblk);
@@ -1073,7 +1079,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     if (graph) RM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_direct, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P);
+    hipLaunchKernelGGL(k_direct, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
     const uint32_t nsets = (uint32_t)total;
     if (nsets > 0) {
         const unsigned gs = (nsets + RB - 1) / RB;
@@ -1082,9 +1088,6 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         const unsigned ge = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
         const int fetch_min = tuning().fetch_min;
         const bool capped = tuning().render_min_waves == 4;
-        const bool lds_walk = tuning().trace_variant == 3;
-        int max_depth = 1;  // deepest tree of the launch's scenes: LDS entries per lane
-        for (int i = 0; i < nj; ++i) max_depth = std::max(max_depth, (int)J[i].S.depth);
         auto bounce = capped ? k_bounce<4> : k_bounce<1>;
         auto nee = capped ? k_nee<4> : k_nee<1>;
         for (int j = 0; j < R; ++j) {
@@ -1093,17 +1096,16 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
             for (int d = 1; d <= D; ++d) {
                 const uint32_t *n_in = counts + d - 1;
                 if (lds_walk)
-                    hipLaunchKernelGGL(k_extend<true>, dim3(ge), dim3(RB), (size_t)max_depth * RB * 8, st,
-                                       (const Job *)dJ, Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min,
-                                       max_depth);
+                    hipLaunchKernelGGL(k_extend<true>, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ,
+                                       Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min);
                 else
                     hipLaunchKernelGGL(k_extend<false>, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1],
-                                       n_in, counts + 2 * NCOUNT + d, hits, fetch_min, max_depth);
+                                       n_in, counts + 2 * NCOUNT + d, hits, fetch_min);
                 hipLaunchKernelGGL(bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1], n_in,
                                    counts + NCOUNT + d, (const uint2 *)hits, Q[d & 1], counts + d, NQ,
                                    counts + 3 * NCOUNT);
             }
-            hipLaunchKernelGGL(nee, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, P, AB, NQ,
+            hipLaunchKernelGGL(nee, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, P, AB, NQ,
                                (const uint32_t *)(counts + 3 * NCOUNT), counts + 3 * NCOUNT + 1);
         }
     }

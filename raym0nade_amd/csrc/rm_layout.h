@@ -113,6 +113,10 @@ struct SceneView {
     int32_t n_lights;
     int32_t sky_w, sky_h;
     int32_t n_mats;
+    // 1: trace_walk keeps its pending entries in the launching kernel's dynamic
+    // LDS (depth * blockDim.x * 8 bytes); set only on the job table's copies of
+    // a view for launches that allocate that LDS, 0 in the scene's own view.
+    int32_t walk_lds;
 };
 
 }  // namespace rm

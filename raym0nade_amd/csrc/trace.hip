@@ -15,6 +15,7 @@
 namespace rm {
 
 constexpr int TRACE_BLOCK = 256;
+static_assert(TRACE_BLOCK == WALK_BLOCK, "LDS walk stacks assume blocks of WALK_BLOCK lanes");
 
 // Bijection blockIdx -> logical block with contiguous ranges per XCD.
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
@@ -108,8 +109,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const f
 // once at least `fetch_min` lanes are idle (or all of its live lanes are).
 // Every wave leaves the loop when the batch is exhausted and its lanes are
 // done, so the grid always drains.  LDS: the walker's pending entries live in
-// the block's LDS (stack_depth entries per lane, dynamic shared memory)
-// instead of per-lane scratch.
+// the block's dynamic LDS (depth * TRACE_BLOCK * 8 bytes) instead of per-lane
+// scratch.
 template <bool ANY, bool LDS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, const float *__restrict__ rays, int64_t n,
                                                                   float tmin, float tmax0,
@@ -118,14 +119,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                                                                   uint8_t *__restrict__ occ,
                                                                   unsigned long long *__restrict__ next, int fetch_min,
                                                                   const uint32_t *__restrict__ order,
-                                                                  const int *__restrict__ use_order, int stack_depth) {
+                                                                  const int *__restrict__ use_order) {
     const uint32_t lane = threadIdx.x & 63u;
     Walker<ANY, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
     if constexpr (LDS) {
-        extern __shared__ uint32_t lds_stack[];
-        w.stk.ref = lds_stack + threadIdx.x;
-        w.stk.tl = reinterpret_cast<float *>(lds_stack + stack_depth * TRACE_BLOCK) + threadIdx.x;
-        w.stk.stride = TRACE_BLOCK;
+        w.stk = lds_stack_of_lane();
     }
     RayPre r;
     float aim = -INFINITY;
@@ -396,11 +394,11 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
         if (T.trace_variant == 3)
             hipLaunchKernelGGL((k_trace_persistent<ANY, true>), dim3(pgrid), dim3(TRACE_BLOCK),
                                (size_t)depth * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
-                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, depth);
+                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag);
         else
             hipLaunchKernelGGL((k_trace_persistent<ANY, false>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n,
                                tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
-                               (const int *)flag, depth);
+                               (const int *)flag);
     } else if (T.trace_variant == 1) {
         hipLaunchKernelGGL(k_trace_step<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,
                            t_out, id_out, occ, (const uint32_t *)order, (const int *)flag);

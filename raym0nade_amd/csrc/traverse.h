@@ -211,8 +211,12 @@ __device__ __forceinline__ void box_test_fast(const RayPre &r, float lx, float l
 
 // Storage of a walker's pending entries.  PrivateStack: per-lane arrays, which
 // the compiler places in scratch (backed by HBM through the caches).
-// LdsStack: a block's LDS, entry i of lane t at [i * stride + t], so a wave's
-// accesses to one level fall in distinct banks; sized by the tree depth.
+// LdsStack: the block's dynamic LDS (blocks of WALK_BLOCK lanes), entry i of
+// lane t holding its node at word [2 i WALK_BLOCK + t] and its entry distance at
+// [(2 i + 1) WALK_BLOCK + t], so a wave's accesses to one entry fall in
+// distinct banks; depth * WALK_BLOCK * 8 bytes hold a tree of that depth.
+constexpr int WALK_BLOCK = 256;
+
 struct PrivateStack {
     uint32_t ref[MAX_STACK];
     float tl[MAX_STACK];
@@ -221,13 +225,20 @@ struct PrivateStack {
     __device__ __forceinline__ float tl_at(int i) const { return tl[i]; }
 };
 struct LdsStack {
-    uint32_t *ref;  // this lane's column
-    float *tl;
-    int stride;
-    __device__ __forceinline__ void put(int i, uint32_t r, float t) { ref[i * stride] = r; tl[i * stride] = t; }
-    __device__ __forceinline__ uint32_t ref_at(int i) const { return ref[i * stride]; }
-    __device__ __forceinline__ float tl_at(int i) const { return tl[i * stride]; }
+    uint32_t *col;  // this lane's column
+    __device__ __forceinline__ void put(int i, uint32_t r, float t) {
+        col[i * (2 * WALK_BLOCK)] = r;
+        col[i * (2 * WALK_BLOCK) + WALK_BLOCK] = __float_as_uint(t);
+    }
+    __device__ __forceinline__ uint32_t ref_at(int i) const { return col[i * (2 * WALK_BLOCK)]; }
+    __device__ __forceinline__ float tl_at(int i) const { return __uint_as_float(col[i * (2 * WALK_BLOCK) + WALK_BLOCK]); }
 };
+
+// The launching kernel's dynamic LDS, as the walk stack of this lane.
+__device__ __forceinline__ LdsStack lds_stack_of_lane() {
+    extern __shared__ uint32_t walk_lds[];
+    return LdsStack{walk_lds + threadIdx.x};
+}
 
 template <bool ANY, bool COUNT, class Stack = PrivateStack>
 struct Walker {
@@ -292,18 +303,34 @@ struct Walker {
     }
 };
 
-// The whole walk for one ray with the step-wise walker (stack in scratch):
-// the same result as trace<ANY, false>, with far fewer registers, for
-// kernels that do more than traversal.
-template <bool ANY>
-__device__ __forceinline__ int32_t trace_walk(const SceneView &S, const RayPre &r, float tmin, float &tmax, float aim) {
-    Walker<ANY, false> w;
+// The whole walk for one ray with the step-wise walker: the same result as
+// trace<ANY, false>, with far fewer registers, for kernels that do more than
+// traversal.  The pending entries live in scratch, or in the kernel's dynamic
+// LDS when the view says so (S.walk_lds; the launch then allocates
+// S.depth * WALK_BLOCK * 8 bytes and runs blocks of WALK_BLOCK lanes).
+template <bool ANY, class Stack>
+__device__ __forceinline__ int32_t walk_with(Walker<ANY, false, Stack> &w, const SceneView &S, const RayPre &r,
+                                             float tmin, float &tmax, float aim) {
     w.init(S, r, tmax);
     TraceCounts c{};
     while (!w.step(S, r, tmin, aim, c)) {
     }
     tmax = w.tmax;
     return w.best;
+}
+
+// LDSW = false compiles the scratch walk only (kernels that never get LDS stacks).
+template <bool ANY, bool LDSW = true>
+__device__ __forceinline__ int32_t trace_walk(const SceneView &S, const RayPre &r, float tmin, float &tmax, float aim) {
+    if constexpr (LDSW) {
+        if (S.walk_lds) {
+            Walker<ANY, false, LdsStack> w;
+            w.stk = lds_stack_of_lane();
+            return walk_with(w, S, r, tmin, tmax, aim);
+        }
+    }
+    Walker<ANY, false> w;
+    return walk_with(w, S, r, tmin, tmax, aim);
 }
 
 }  // namespace rm

@@ -59,6 +59,7 @@ def test_serving_batch_with_autotune_equals_single_renders(dev):
     try:
         choice = autotune.tune_render(jobs, spp_scale=0.25)
         assert choice["blocks_per_cu"] in (4, 8, 16) and choice["min_waves"] in (1, 4)
+        assert choice["walk_variant"] in (2, 3)
         rm.render_batch(jobs, use_graph=True)
         torch.cuda.synchronize()
         for S, a, fr, post in jobs:
@@ -69,8 +70,7 @@ def test_serving_batch_with_autotune_equals_single_renders(dev):
             px, px1 = fr.download(rm.RM_BUF_PIXELS), fr1.download(rm.RM_BUF_PIXELS)
             assert np.isclose(px, px1, rtol=1e-4, atol=1e-5).all(axis=-1).mean() >= 0.999
     finally:
-        rm.set_tuning(rm.TUNE_RENDER_BLOCKS_PER_CU, saved["render_blocks_per_cu"])
-        rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, saved["render_min_waves"])
+        autotune.restore(saved)
 
 
 def test_trace_autotune_keeps_results(dev):
@@ -86,9 +86,7 @@ def test_trace_autotune_keeps_results(dev):
         t1, i1 = S.trace_closest(r)
         assert torch.equal(i0, i1) and torch.equal(t0.view(torch.int32), t1.view(torch.int32))
     finally:
-        for k, name in ((rm.TUNE_TRACE_VARIANT, "trace_variant"), (rm.TUNE_PERSISTENT_BLOCKS_PER_CU,
-                        "persistent_blocks_per_cu"), (rm.TUNE_FETCH_MIN, "fetch_min"), (rm.TUNE_RAY_SORT, "ray_sort")):
-            rm.set_tuning(k, saved[name])
+        autotune.restore(saved)
 
 
 def test_batch_argument_errors(dev):
