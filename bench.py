@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "closest-hit Mrays/s + path samples/s at 1080p; adjoint:forward time ratio"
 HBM_PEAK_GBS = 8000.0
+LIBRARY_DEFAULTS = {}
 
 
 def log(*a):
@@ -61,7 +62,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="accepted for older profile commands; the 1 GiB calibration copy always runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
-    ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort'")
+    ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort[,lds_stack_entries]'")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
@@ -77,6 +78,9 @@ def main():
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+    from raym0nade_amd import autotune as _at
+    global LIBRARY_DEFAULTS
+    LIBRARY_DEFAULTS = _at.current()  # before any tuning: the parameters a plain launch uses
     log(f"[bench] generating soup ({args.n_tris} tris) and rays")
     g = rm.generate(1, 1, n_tris=args.n_tris)
     t0 = time.time()
@@ -109,11 +113,12 @@ def main():
                                            for k, v in algo.items()})
     tuned = None
     if args.trace:
-        v, b, f, s = (int(x) for x in args.trace.split(","))
+        v, b, f, s, e = (int(x) for x in (args.trace + ",32").split(",")[:5])
         for k, val in ((rm.TUNE_TRACE_VARIANT, v), (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, b), (rm.TUNE_FETCH_MIN, f),
-                       (rm.TUNE_RAY_SORT, s)):
+                       (rm.TUNE_RAY_SORT, s), (rm.TUNE_LDS_STACK_ENTRIES, e)):
             rm.set_tuning(k, val)
-        tuned = {"variant": v, "blocks_per_cu": b, "fetch_min": f, "ray_sort": s, "fixed": True}
+        tuned = {"variant": v, "blocks_per_cu": b, "fetch_min": f, "ray_sort": s, "lds_stack_entries": e,
+                 "fixed": True}
     elif not args.no_autotune:
         from raym0nade_amd import autotune
         tuned = autotune.tune_trace(S, [r_coh, r_inc], reps=2, log=log)
@@ -325,13 +330,11 @@ def batch_leg(args, torch, rm):
     log(f"[bench] batch: sequential rm_render with the batch's launch parameters {ts_tuned:.2f} s")
     # plain launches: rm_render per job with the library's default parameters
     saved = autotune.current()
-    rm.set_tuning(rm.TUNE_RENDER_BLOCKS_PER_CU, 8)
-    rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, 1)
+    autotune.restore(LIBRARY_DEFAULTS)
     for _, a, _, _ in jobs:
         a.paths_per_round = 0
     ts = sequential()
-    rm.set_tuning(rm.TUNE_RENDER_BLOCKS_PER_CU, saved["render_blocks_per_cu"])
-    rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, saved["render_min_waves"])
+    autotune.restore(saved)
     log(f"[bench] batch: sequential plain rm_render {ts:.2f} s")
     return {"jobs": len(jobs), "samples": total, "batched_samples_per_s": round(total / tb, 1),
             "sequential_samples_per_s": round(total / ts, 1), "batched_over_sequential": round(ts / tb, 3),
@@ -339,9 +342,10 @@ def batch_leg(args, torch, rm):
             "batched_over_sequential_tuned": round(ts_tuned / tb, 3),
             "batched_s": round(tb, 3), "sequential_s": round(ts, 3), "sequential_tuned_s": round(ts_tuned, 3),
             "autotune": tuned, "tris_min": min(tris), "tris_max": max(tris),
+            "library_defaults": LIBRARY_DEFAULTS,
             "note": "sequential = one rm_render + rm_postprocess per job with the library's default launch "
-                    "parameters (blocks/CU 8, no register cap, automatic paths per round); sequential_tuned = "
-                    "the same with the batch's autotuned parameters; every frame's buffers exist beforehand"}
+                    "parameters (library_defaults, automatic paths per round); sequential_tuned = the same with "
+                    "the batch's autotuned parameters; every frame's buffers exist beforehand"}
 
 
 def adjoint_leg(args, torch, rm):

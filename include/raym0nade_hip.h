@@ -294,21 +294,24 @@ int32_t rm_generate_rays(int32_t kind, uint32_t seed, int64_t n, const rm_render
  * its thread count).  All settings give identical results; they trade speed.
  * Process-wide; set them before launching work. */
 enum rm_tune_key {
-    RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk (default),
-                                           3 persistent step walk with the stack in LDS (also used by
-                                           the render's extension kernel) */
-    RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variant 2 (default 8) */
+    RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk,
+                                           3 persistent step walk with the first stack entries in LDS
+                                           (default; in a render launch: every walk of its kernels) */
+    RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variants 2, 3 (default 8) */
     RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 16) */
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive
                                            origins are far apart (default), 2 always */
-    RM_TUNE_RENDER_BLOCKS_PER_CU = 4,   /* blocks per CU of the persistent bounce kernel (default 8) */
+    RM_TUNE_RENDER_BLOCKS_PER_CU = 4,   /* blocks per CU of the persistent bounce kernel (default 4) */
     RM_TUNE_ADJOINT_SPLIT = 5,          /* lanes per pixel of the adjoint, each replaying a share of
                                            the pixel's samples (default 4) */
     RM_TUNE_KERNEL_TIMING = 6,          /* 1: trace calls record HIP events around their reordering
                                            and traversal kernels (default 0) */
-    RM_TUNE_RENDER_MIN_WAVES = 7        /* 1 (default) or 4: resident waves per SIMD the render's
+    RM_TUNE_RENDER_MIN_WAVES = 7,       /* 1 or 4 (default): resident waves per SIMD the render's
                                            shading and NEE kernels are compiled for (4 caps them
                                            at 128 VGPRs, spilling the rest) */
+    RM_TUNE_LDS_STACK_ENTRIES = 8       /* trace variant 3: walk-stack entries per lane kept in LDS
+                                           (1..32, default 8; deeper entries go to scratch; 32 holds
+                                           the whole stack of any tree) */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

@@ -18,10 +18,11 @@ from __future__ import annotations
 import itertools
 import time
 
-from . import (TUNE_FETCH_MIN, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_RAY_SORT, TUNE_RENDER_BLOCKS_PER_CU,
-               TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, RmRenderArgs, get_tuning, render_batch, set_tuning)
+from . import (TUNE_FETCH_MIN, TUNE_LDS_STACK_ENTRIES, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_RAY_SORT,
+               TUNE_RENDER_BLOCKS_PER_CU, TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, RmRenderArgs, get_tuning,
+               render_batch, set_tuning)
 
-TRACE_KEYS = (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT)
+TRACE_KEYS = (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT, TUNE_LDS_STACK_ENTRIES)
 
 
 def _time(fn, reps):
@@ -40,10 +41,10 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     """batches: list of device ray tensors [n, 6].  Sets and returns the
     fastest {key: value} for closest-hit traversal of all batches."""
     if candidates is None:
-        candidates = [(0, 8, 16, 1)]
-        candidates += [(1, 8, 16, s) for s in (0, 1)]
-        candidates += [(2, b, f, s) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
-        candidates += [(3, b, f, s) for b, f, s in itertools.product((2, 4, 8), (8, 16, 32), (0, 1))]
+        candidates = [(0, 8, 16, 1, 32)]
+        candidates += [(1, 8, 16, s, 32) for s in (0, 1)]
+        candidates += [(2, b, f, s, 32) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
+        candidates += [(3, b, f, 1, e) for b, f, e in itertools.product((4, 8, 16), (16, 32), (4, 6, 8, 12, 32))]
 
     def run():
         for r in batches:
@@ -62,47 +63,50 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     for k, v in zip(TRACE_KEYS, best):
         set_tuning(k, v)
     return {"variant": best[0], "blocks_per_cu": best[1], "fetch_min": best[2], "ray_sort": best[3],
-            "seconds": t, "tried": len(results)}
+            "lds_stack_entries": best[4], "seconds": t, "tried": len(results)}
 
 
-def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4), walks=(3, 2),
-                log=None) -> dict:
+def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4),
+                walks=((3, 8), (3, 4), (3, 6), (3, 12), (2, 32)), log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp
     scaled by spp_scale: first every (blocks per CU, register cap) pair with
-    automatic paths per round and the walk stacks of trace variant walks[0],
-    then the other paths-per-round values at the best pair, then the other
-    walk variants (3: stacks in LDS, 2: in scratch).  Sets the launch
-    parameters and returns the choice; the caller applies paths_per_round
-    (0 = automatic) to its jobs' arguments."""
+    automatic paths per round and the walk stacks of walks[0], then the other
+    paths-per-round values at the best pair, then the other walk stacks
+    (trace variant, LDS entries per lane: variant 2 keeps them in scratch,
+    3 its first entries in LDS).  Sets the launch parameters and returns the
+    choice; the caller applies paths_per_round (0 = automatic) to its jobs'
+    arguments."""
     small = []
     for S, a, fr, post in jobs:
         b = RmRenderArgs.from_buffer_copy(a)
         b.spp = max(4, int(round(a.spp * spp_scale)))
         small.append([S, b, fr, post])
 
-    def trial(bpc, w, k, v):
+    def trial(bpc, w, k, walk):
         for j in small:
             j[1].paths_per_round = k
         set_tuning(TUNE_RENDER_BLOCKS_PER_CU, bpc)
         set_tuning(TUNE_RENDER_MIN_WAVES, w)
-        set_tuning(TUNE_TRACE_VARIANT, v)
+        set_tuning(TUNE_TRACE_VARIANT, walk[0])
+        set_tuning(TUNE_LDS_STACK_ENTRIES, walk[1])
         t = _time(lambda: render_batch([tuple(j) for j in small], use_graph=True), 1)
         if log:
-            log(f"[autotune] render blocks/CU {bpc} min waves {w} paths/round {k or 'auto'} walk {v}: {t:.3f} s")
-        return (t, bpc, w, k, v)
+            log(f"[autotune] render blocks/CU {bpc} min waves {w} paths/round {k or 'auto'} walk {walk}: {t:.3f} s")
+        return (t, bpc, w, k, walk)
 
     results = [trial(b, w, paths_per_round[0], walks[0]) for b in blocks for w in waves]
     _, bpc, w, _, _ = min(results)
     results += [trial(bpc, w, k, walks[0]) for k in paths_per_round[1:]]
     _, bpc, w, k, _ = min(results)
-    results += [trial(bpc, w, k, v) for v in walks[1:]]
-    t, bpc, w, k, v = min(results)
+    results += [trial(bpc, w, k, walk) for walk in walks[1:]]
+    t, bpc, w, k, walk = min(results)
     set_tuning(TUNE_RENDER_BLOCKS_PER_CU, bpc)
     set_tuning(TUNE_RENDER_MIN_WAVES, w)
-    set_tuning(TUNE_TRACE_VARIANT, v)
-    return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "walk_variant": v, "seconds_scaled": t,
-            "spp_scale": spp_scale, "tried": len(results)}
+    set_tuning(TUNE_TRACE_VARIANT, walk[0])
+    set_tuning(TUNE_LDS_STACK_ENTRIES, walk[1])
+    return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "walk_variant": walk[0],
+            "lds_stack_entries": walk[1], "seconds_scaled": t, "spp_scale": spp_scale, "tried": len(results)}
 
 
 def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> dict:
@@ -125,7 +129,7 @@ def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> di
 
 _KEYS = {"trace_variant": TUNE_TRACE_VARIANT, "persistent_blocks_per_cu": TUNE_PERSISTENT_BLOCKS_PER_CU,
          "fetch_min": TUNE_FETCH_MIN, "ray_sort": TUNE_RAY_SORT, "render_blocks_per_cu": TUNE_RENDER_BLOCKS_PER_CU,
-         "render_min_waves": TUNE_RENDER_MIN_WAVES}
+         "render_min_waves": TUNE_RENDER_MIN_WAVES, "lds_stack_entries": TUNE_LDS_STACK_ENTRIES}
 
 
 def current() -> dict:

@@ -547,17 +547,20 @@ __device__ __forceinline__ float4 *rec_slot(const Job &jb, uint32_t gp, uint32_t
 // (one atomic per refill), so lanes do not wait for the wave's longest walk.
 // Writes (t, triangle slot or -1) per entry.  The traversal is the same walk
 // as everywhere else, so the hits are bit-identical.
-// LDS: the walk's pending entries in the block's dynamic LDS (the deepest
-// tree's depth * RB * 8 bytes).
+// LDS: the walk's first lds_entries pending entries in the block's dynamic
+// LDS (lds_entries * RB * 8 bytes), deeper ones in per-lane scratch.
 template <bool LDS>
 __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ Q, const uint32_t *count_in,
-                                               uint32_t *work, uint2 *__restrict__ hit, int fetch_min) {
+                                               uint32_t *work, uint2 *__restrict__ hit, int fetch_min,
+                                               int lds_entries) {
     const uint32_t n = *count_in;
     const uint32_t lane = threadIdx.x & 63u;
+    ScratchEntries ent;
     Walker<false, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
-    if constexpr (LDS) {
-        w.stk = lds_stack_of_lane();
-    }
+    if constexpr (LDS)
+        w.stk.bind(&ent, lds_entries);
+    else
+        w.stk.bind(&ent);
     RayPre r;
     SceneView S;  // per lane: the nodes, triangles and root of its path's scene
     int64_t q = -1;
@@ -1021,14 +1024,15 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     // depth d; counts[2 NCOUNT + d]: traversal claims of depth d;
     // counts[3 NCOUNT]: NEE records of the round, counts[3 NCOUNT + 1]: their claims
     uint32_t *counts = carve<uint32_t>(cur, 4 * NCOUNT);
-    // Trace variant 3 keeps every walk's pending entries in LDS: each kernel of
-    // the launch that traverses gets the deepest tree's stack, and the job
-    // table's views say so.
+    // Trace variant 3 keeps the first pending entries of every walk in LDS:
+    // each kernel of the launch that traverses gets nl entries per lane (at
+    // most the deepest tree's depth), and the job table's views say so.
     const bool lds_walk = tuning().trace_variant == 3;
     int max_depth = 1;
     for (int i = 0; i < nj; ++i) max_depth = std::max(max_depth, (int)J[i].S.depth);
-    const size_t walk_lds = lds_walk ? (size_t)max_depth * RB * 8 : 0;
-    for (int i = 0; i < nj; ++i) J[i].S.walk_lds = lds_walk ? 1 : 0;
+    const int nl = lds_walk ? std::min(max_depth, tuning().lds_stack_entries) : 0;
+    const size_t walk_lds = (size_t)nl * RB * 8;
+    for (int i = 0; i < nj; ++i) J[i].S.walk_lds = nl;
     RM_HIP(hipMemcpyAsync(dJ, J.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st));
     for (int i = 0; i < nj; ++i)
         if (J[i].F.rec) {
@@ -1097,10 +1101,10 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
                 const uint32_t *n_in = counts + d - 1;
                 if (lds_walk)
                     hipLaunchKernelGGL(k_extend<true>, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ,
-                                       Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min);
+                                       Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min, nl);
                 else
                     hipLaunchKernelGGL(k_extend<false>, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1],
-                                       n_in, counts + 2 * NCOUNT + d, hits, fetch_min);
+                                       n_in, counts + 2 * NCOUNT + d, hits, fetch_min, 0);
                 hipLaunchKernelGGL(bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1], n_in,
                                    counts + NCOUNT + d, (const uint2 *)hits, Q[d & 1], counts + d, NQ,
                                    counts + 3 * NCOUNT);

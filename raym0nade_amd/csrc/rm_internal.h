@@ -85,14 +85,15 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs);
 
 // Launch parameters chosen by the autotuner (rm_set_tuning).
 struct Tuning {
-    int trace_variant = 2;        // 0 stack walk, 1 step walk, 2 persistent step walk
+    int trace_variant = 3;        // 0 stack walk, 1 step walk, 2 persistent step walk, 3 same with LDS stacks
     int persistent_blocks_per_cu = 8;
     int fetch_min = 16;           // idle lanes before a wave claims new rays
     int ray_sort = 1;             // 0 never, 1 when consecutive origins are far apart, 2 always
-    int render_blocks_per_cu = 8; // resident blocks per CU of the persistent bounce kernel
+    int render_blocks_per_cu = 4; // resident blocks per CU of the persistent bounce kernel
     int adjoint_split = 4;        // lanes per pixel of the adjoint kernel
     int kernel_timing = 0;        // record HIP events around the trace kernels (rm_last_trace_ms)
-    int render_min_waves = 1;     // 1 or 4: register cap of the shading and NEE kernels
+    int render_min_waves = 4;     // 1 or 4: register cap of the shading and NEE kernels
+    int lds_stack_entries = 8;    // trace variant 3: walk-stack entries per lane kept in LDS
 };
 Tuning &tuning();
 int device_cu_count();

@@ -43,6 +43,9 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 48, "TriRec is 48 B");
 
+// Walk-stack capacity: pending entries <= tree depth - 1 (2^27 faces -> 25).
+constexpr int MAX_STACK = 32;
+
 // Buckets of the sky CDF guide table.
 constexpr int SKY_GUIDE = 65536;
 
@@ -113,9 +116,10 @@ struct SceneView {
     int32_t n_lights;
     int32_t sky_w, sky_h;
     int32_t n_mats;
-    // 1: trace_walk keeps its pending entries in the launching kernel's dynamic
-    // LDS (depth * blockDim.x * 8 bytes); set only on the job table's copies of
-    // a view for launches that allocate that LDS, 0 in the scene's own view.
+    // n > 0: trace_walk keeps its first n pending entries in the launching
+    // kernel's dynamic LDS (n * blockDim.x * 8 bytes); set only on the job
+    // table's copies of a view for launches that allocate that LDS, 0 in the
+    // scene's own view.
     int32_t walk_lds;
 };
 

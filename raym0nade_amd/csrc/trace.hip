@@ -91,7 +91,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const f
     load_ray(rays, i, o, d);
     RayPre r = ray_pre(o, d);
     float aim = ANY ? (aim_arr ? aim_arr[i] : tmax0) : -INFINITY;
+    ScratchEntries ent;
     Walker<ANY, false> w;
+    w.stk.bind(&ent);
     w.init(S, r, ANY ? aim + T_EPS : tmax0);
     TraceCounts c{};
     while (!w.step(S, r, tmin, aim, c)) {
@@ -108,9 +110,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const f
 // next unclaimed ray; a wave claims rays for its idle lanes with one atomic
 // once at least `fetch_min` lanes are idle (or all of its live lanes are).
 // Every wave leaves the loop when the batch is exhausted and its lanes are
-// done, so the grid always drains.  LDS: the walker's pending entries live in
-// the block's dynamic LDS (depth * TRACE_BLOCK * 8 bytes) instead of per-lane
-// scratch.
+// done, so the grid always drains.  LDS: the walker's first lds_entries
+// pending entries live in the block's dynamic LDS (lds_entries * TRACE_BLOCK *
+// 8 bytes), deeper ones in per-lane scratch.
 template <bool ANY, bool LDS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, const float *__restrict__ rays, int64_t n,
                                                                   float tmin, float tmax0,
@@ -119,12 +121,14 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                                                                   uint8_t *__restrict__ occ,
                                                                   unsigned long long *__restrict__ next, int fetch_min,
                                                                   const uint32_t *__restrict__ order,
-                                                                  const int *__restrict__ use_order) {
+                                                                  const int *__restrict__ use_order, int lds_entries) {
     const uint32_t lane = threadIdx.x & 63u;
+    ScratchEntries ent;
     Walker<ANY, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
-    if constexpr (LDS) {
-        w.stk = lds_stack_of_lane();
-    }
+    if constexpr (LDS)
+        w.stk.bind(&ent, lds_entries);
+    else
+        w.stk.bind(&ent);
     RayPre r;
     float aim = -INFINITY;
     int64_t ray = -1;
@@ -390,15 +394,15 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
         if (rc) return rc;
         unsigned pgrid = (unsigned)std::min<int64_t>(grid, (int64_t)device_cu_count() * T.persistent_blocks_per_cu);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, next);
-        const int depth = std::max(1, (int)s->view.depth);
+        const int nl = std::min(std::max(1, (int)s->view.depth), T.lds_stack_entries);
         if (T.trace_variant == 3)
             hipLaunchKernelGGL((k_trace_persistent<ANY, true>), dim3(pgrid), dim3(TRACE_BLOCK),
-                               (size_t)depth * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
-                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag);
+                               (size_t)nl * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
+                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, nl);
         else
             hipLaunchKernelGGL((k_trace_persistent<ANY, false>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n,
                                tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
-                               (const int *)flag);
+                               (const int *)flag, 0);
     } else if (T.trace_variant == 1) {
         hipLaunchKernelGGL(k_trace_step<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,
                            t_out, id_out, occ, (const uint32_t *)order, (const int *)flag);

@@ -24,7 +24,6 @@
 namespace rm {
 
 constexpr float T_EPS = 1e-4f;   // eps_zero
-constexpr int MAX_STACK = 32;    // pending entries <= tree depth - 1 (2^27 faces -> 25)
 
 struct RayPre {
     V3 o, d;
@@ -211,34 +210,54 @@ __device__ __forceinline__ void box_test_fast(const RayPre &r, float lx, float l
 
 // Storage of a walker's pending entries.  PrivateStack: per-lane arrays, which
 // the compiler places in scratch (backed by HBM through the caches).
-// LdsStack: the block's dynamic LDS (blocks of WALK_BLOCK lanes), entry i of
-// lane t holding its node at word [2 i WALK_BLOCK + t] and its entry distance at
-// [(2 i + 1) WALK_BLOCK + t], so a wave's accesses to one entry fall in
-// distinct banks; depth * WALK_BLOCK * 8 bytes hold a tree of that depth.
+// LdsStack: entries [0, nl) in the block's dynamic LDS (blocks of WALK_BLOCK
+// lanes), entry i of lane t holding its node at word [2 i WALK_BLOCK + t] and
+// its entry distance at [(2 i + 1) WALK_BLOCK + t], so a wave's accesses to one
+// entry fall in distinct banks; entries from nl on (deeper than the LDS share)
+// in per-lane scratch.  nl * WALK_BLOCK * 8 bytes of LDS; nl >= the tree depth
+// keeps every entry in LDS.
 constexpr int WALK_BLOCK = 256;
 
-struct PrivateStack {
+// Per-lane scratch entries of a walk.  Kept apart from the walker's scalar
+// state: a struct holding a dynamically indexed array stays in scratch as a
+// whole, so the walker only points at this one.
+struct ScratchEntries {
     uint32_t ref[MAX_STACK];
     float tl[MAX_STACK];
-    __device__ __forceinline__ void put(int i, uint32_t r, float t) { ref[i] = r; tl[i] = t; }
-    __device__ __forceinline__ uint32_t ref_at(int i) const { return ref[i]; }
-    __device__ __forceinline__ float tl_at(int i) const { return tl[i]; }
-};
-struct LdsStack {
-    uint32_t *col;  // this lane's column
-    __device__ __forceinline__ void put(int i, uint32_t r, float t) {
-        col[i * (2 * WALK_BLOCK)] = r;
-        col[i * (2 * WALK_BLOCK) + WALK_BLOCK] = __float_as_uint(t);
-    }
-    __device__ __forceinline__ uint32_t ref_at(int i) const { return col[i * (2 * WALK_BLOCK)]; }
-    __device__ __forceinline__ float tl_at(int i) const { return __uint_as_float(col[i * (2 * WALK_BLOCK) + WALK_BLOCK]); }
 };
 
-// The launching kernel's dynamic LDS, as the walk stack of this lane.
-__device__ __forceinline__ LdsStack lds_stack_of_lane() {
-    extern __shared__ uint32_t walk_lds[];
-    return LdsStack{walk_lds + threadIdx.x};
-}
+struct PrivateStack {
+    ScratchEntries *e;
+    __device__ __forceinline__ void bind(ScratchEntries *m) { e = m; }
+    __device__ __forceinline__ void put(int i, uint32_t r, float t) { e->ref[i] = r; e->tl[i] = t; }
+    __device__ __forceinline__ uint32_t ref_at(int i) const { return e->ref[i]; }
+    __device__ __forceinline__ float tl_at(int i) const { return e->tl[i]; }
+};
+struct LdsStack {
+    uint32_t *col;     // this lane's column
+    int nl;            // entries held in LDS
+    ScratchEntries *e; // the deeper ones
+    // Binds the launching kernel's dynamic LDS as this lane's first n entries.
+    __device__ __forceinline__ void bind(ScratchEntries *m, int n) {
+        extern __shared__ uint32_t walk_lds[];
+        col = walk_lds + threadIdx.x;
+        nl = n;
+        e = m;
+    }
+    __device__ __forceinline__ void put(int i, uint32_t r, float t) {
+        if (i < nl) {
+            col[i * (2 * WALK_BLOCK)] = r;
+            col[i * (2 * WALK_BLOCK) + WALK_BLOCK] = __float_as_uint(t);
+        } else {
+            e->ref[i] = r;
+            e->tl[i] = t;
+        }
+    }
+    __device__ __forceinline__ uint32_t ref_at(int i) const { return i < nl ? col[i * (2 * WALK_BLOCK)] : e->ref[i]; }
+    __device__ __forceinline__ float tl_at(int i) const {
+        return i < nl ? __uint_as_float(col[i * (2 * WALK_BLOCK) + WALK_BLOCK]) : e->tl[i];
+    }
+};
 
 template <bool ANY, bool COUNT, class Stack = PrivateStack>
 struct Walker {
@@ -305,9 +324,9 @@ struct Walker {
 
 // The whole walk for one ray with the step-wise walker: the same result as
 // trace<ANY, false>, with far fewer registers, for kernels that do more than
-// traversal.  The pending entries live in scratch, or in the kernel's dynamic
-// LDS when the view says so (S.walk_lds; the launch then allocates
-// S.depth * WALK_BLOCK * 8 bytes and runs blocks of WALK_BLOCK lanes).
+// traversal.  The pending entries live in scratch, or their first S.walk_lds
+// in the kernel's dynamic LDS (the launch then allocates S.walk_lds *
+// WALK_BLOCK * 8 bytes and runs blocks of WALK_BLOCK lanes).
 template <bool ANY, class Stack>
 __device__ __forceinline__ int32_t walk_with(Walker<ANY, false, Stack> &w, const SceneView &S, const RayPre &r,
                                              float tmin, float &tmax, float aim) {
@@ -324,12 +343,15 @@ template <bool ANY, bool LDSW = true>
 __device__ __forceinline__ int32_t trace_walk(const SceneView &S, const RayPre &r, float tmin, float &tmax, float aim) {
     if constexpr (LDSW) {
         if (S.walk_lds) {
+            ScratchEntries deep;
             Walker<ANY, false, LdsStack> w;
-            w.stk = lds_stack_of_lane();
+            w.stk.bind(&deep, S.walk_lds);
             return walk_with(w, S, r, tmin, tmax, aim);
         }
     }
+    ScratchEntries ent;
     Walker<ANY, false> w;
+    w.stk.bind(&ent);
     return walk_with(w, S, r, tmin, tmax, aim);
 }
 

@@ -154,19 +154,24 @@ def test_full_size_cornell_properties(cornell):
 
 
 @pytest.mark.parametrize("scene", ["cornell", "spheres"])
-def test_lds_stack_extension_is_bit_identical(dev, scene):
-    # the extension kernel with its traversal stack in LDS (trace variant 3)
-    # walks the same nodes in the same order as the scratch-stack walker
+@pytest.mark.parametrize("entries", [32, 2])
+def test_lds_stack_walks_are_bit_identical(dev, scene, entries):
+    # every walk of a render launch (primary, direct, extension, NEE) with its
+    # stack in LDS (trace variant 3), wholly or split with scratch, visits the
+    # same nodes in the same order as the scratch-stack walker
     g = rm.generate(0, 1) if scene == "cornell" else rm.generate(2, 3, grid=2, freq=12, sky_width=256, sky_height=128)
     S = rm.Scene(g)
     a = small_args(g, 80, 48, 12, max_depth=16)
-    old = rm.get_tuning(rm.TUNE_TRACE_VARIANT)
+    keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_LDS_STACK_ENTRIES)
+    old = tuple(rm.get_tuning(k) for k in keys)
     try:
         rm.set_tuning(rm.TUNE_TRACE_VARIANT, 2)
         gb2, r2 = RC.gpu_render(S, a)
         rm.set_tuning(rm.TUNE_TRACE_VARIANT, 3)
+        rm.set_tuning(rm.TUNE_LDS_STACK_ENTRIES, entries)
         gb3, r3 = RC.gpu_render(S, a)
     finally:
-        rm.set_tuning(rm.TUNE_TRACE_VARIANT, old)
+        for k, v in zip(keys, old):
+            rm.set_tuning(k, v)
     assert np.array_equal(gb2.view(np.uint32), gb3.view(np.uint32))
     assert np.array_equal(r2.view(np.uint32), r3.view(np.uint32))

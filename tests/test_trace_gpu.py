@@ -29,18 +29,21 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
     return occ.cpu().numpy()
 
 
-@pytest.fixture(params=[(0, 0), (0, 2), (1, 1), (2, 1), (2, 2), (3, 1), (3, 0)],
+@pytest.fixture(params=[(0, 0, 32), (0, 2, 32), (1, 1, 32), (2, 1, 32), (2, 2, 32), (3, 1, 32), (3, 0, 32),
+                        (3, 1, 2)],
                 ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted",
-                     "persistent-lds-adaptive", "persistent-lds"])
+                     "persistent-lds-adaptive", "persistent-lds", "persistent-lds2-spill"])
 def variant(request):
-    """Every traversal kernel variant, with and without ray reordering, must
-    give the same bits."""
-    old = (rm.get_tuning(rm.TUNE_TRACE_VARIANT), rm.get_tuning(rm.TUNE_RAY_SORT))
-    rm.set_tuning(rm.TUNE_TRACE_VARIANT, request.param[0])
-    rm.set_tuning(rm.TUNE_RAY_SORT, request.param[1])
+    """Every traversal kernel variant, with and without ray reordering, and
+    with the walk stack in scratch, in LDS, or split between them (2 LDS
+    entries per lane, the rest in scratch), must give the same bits."""
+    keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_RAY_SORT, rm.TUNE_LDS_STACK_ENTRIES)
+    old = tuple(rm.get_tuning(k) for k in keys)
+    for k, v in zip(keys, request.param):
+        rm.set_tuning(k, v)
     yield request.param
-    rm.set_tuning(rm.TUNE_TRACE_VARIANT, old[0])
-    rm.set_tuning(rm.TUNE_RAY_SORT, old[1])
+    for k, v in zip(keys, old):
+        rm.set_tuning(k, v)
 
 
 @pytest.fixture(scope="module")
