@@ -188,15 +188,19 @@ def main():
     achieved = bytes_per_launch / dur_per_launch / 1e9
     kernel_name = {0: "k_trace_closest", 1: "k_trace_step", 2: "k_trace_persistent",
                    3: "k_trace_persistent_lds"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
-    traffic = None
+    # HBM bytes per launch from the committed PMC passes of this kernel (the
+    # launch parameters they were taken at are reported beside them)
+    traffic, traffic_launch = None, None
     tf = os.path.join(ROOT, "profiles", f"traffic_{kernel_name}.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
+            tj = json.load(open(tf))
+            traffic, traffic_launch = tj.get("hbm_bytes_per_launch"), tj.get("launch")
         except (OSError, ValueError):
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_launch": traffic_launch,
                 "kernel": kernel_name, "algorithmic_bytes_per_launch": round(bytes_per_launch),
                 "avg_launch_ms": round(dur_per_launch * 1e3, 4),
                 "launch_ms": {k: round(float(np.mean(v)), 4) for k, v in kt.items()},
