@@ -187,7 +187,7 @@ def main():
     dur_per_launch = (np.mean(kt["coh"]) + np.mean(kt["inc"])) / 2 * 1e-3
     achieved = bytes_per_launch / dur_per_launch / 1e9
     kernel_name = {0: "k_trace_closest", 1: "k_trace_step", 2: "k_trace_persistent",
-                   3: "k_trace_persistent_lds"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
+                   3: "k_trace_persistent_lds", 4: "k_trace_persistent2"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
     # HBM bytes per launch from the committed PMC passes of this kernel (the
     # launch parameters they were taken at are reported beside them)
     traffic, traffic_launch = None, None
@@ -413,8 +413,8 @@ def render_leg(args, torch, rm, dev, stream):
     tuned = None
     if not args.no_autotune:
         from raym0nade_amd import autotune
-        tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 16, blocks=(4, 8), waves=(1, 4),
-                                     paths_per_round=(0,), log=log)
+        tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 4, blocks=(4, 8), waves=(1, 4),
+                                     paths_per_round=(0, 4, 8), log=log)
         a.paths_per_round = tuned["paths_per_round"]
     times = []
     for k in range(args.render_frames + 1):

@@ -296,7 +296,8 @@ int32_t rm_generate_rays(int32_t kind, uint32_t seed, int64_t n, const rm_render
 enum rm_tune_key {
     RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk,
                                            3 persistent step walk with the first stack entries in LDS
-                                           (default; in a render launch: every walk of its kernels) */
+                                           (default; in a render launch: every walk of its kernels),
+                                           4 as 3 with two walks per lane (traces; renders as 3) */
     RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variants 2, 3 (default 8) */
     RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 16) */
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive

@@ -18,6 +18,7 @@ from __future__ import annotations
 import itertools
 import time
 
+from ._lib import RM_ERR_OOM, RmError
 from . import (TUNE_FETCH_MIN, TUNE_LDS_STACK_ENTRIES, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_RAY_SORT,
                TUNE_RENDER_BLOCKS_PER_CU, TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, RmRenderArgs, get_tuning,
                render_batch, set_tuning)
@@ -45,6 +46,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
         candidates += [(1, 8, 16, s, 32) for s in (0, 1)]
         candidates += [(2, b, f, s, 32) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
         candidates += [(3, b, f, 1, e) for b, f, e in itertools.product((4, 8, 16), (16, 32), (4, 6, 8, 12, 32))]
+        candidates += [(4, b, 32, 1, 6) for b in (6, 8)]
 
     def run():
         for r in batches:
@@ -66,7 +68,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
             "lds_stack_entries": best[4], "seconds": t, "tried": len(results)}
 
 
-def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4),
+def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4, 8),
                 walks=((3, 8), (3, 4), (3, 6), (3, 12), (2, 32)), log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp
@@ -90,7 +92,12 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
         set_tuning(TUNE_RENDER_MIN_WAVES, w)
         set_tuning(TUNE_TRACE_VARIANT, walk[0])
         set_tuning(TUNE_LDS_STACK_ENTRIES, walk[1])
-        t = _time(lambda: render_batch([tuple(j) for j in small], use_graph=True), 1)
+        try:
+            t = _time(lambda: render_batch([tuple(j) for j in small], use_graph=True), 1)
+        except RmError as e:
+            if e.code != RM_ERR_OOM:
+                raise
+            t = float("inf")  # the candidate's work buffers do not fit
         if log:
             log(f"[autotune] render blocks/CU {bpc} min waves {w} paths/round {k or 'auto'} walk {walk}: {t:.3f} s")
         return (t, bpc, w, k, walk)

@@ -52,7 +52,7 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
     rm::Tuning &t = rm::tuning();
     switch (key) {
         case RM_TUNE_TRACE_VARIANT:
-            if (value < 0 || value > 3) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: trace variant %d", value);
+            if (value < 0 || value > 4) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: trace variant %d", value);
             t.trace_variant = value;
             return RM_OK;
         case RM_TUNE_PERSISTENT_BLOCKS_PER_CU:

@@ -1027,7 +1027,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     // Trace variant 3 keeps the first pending entries of every walk in LDS:
     // each kernel of the launch that traverses gets nl entries per lane (at
     // most the deepest tree's depth), and the job table's views say so.
-    const bool lds_walk = tuning().trace_variant == 3;
+    const bool lds_walk = tuning().trace_variant >= 3;
     int max_depth = 1;
     for (int i = 0; i < nj; ++i) max_depth = std::max(max_depth, (int)J[i].S.depth);
     const int nl = lds_walk ? std::min(max_depth, tuning().lds_stack_entries) : 0;
@@ -1055,9 +1055,16 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "%lld path slots per round", (long long)total);
     const size_t set_bytes =
         pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16) + pad(total * 8) + 9 * pad(total * 16);
-    if (work.bytes < pix_bytes + set_bytes) {
-        // grow (the buffer holds the primary pass's results) and start over
-        int32_t rc = work.alloc(pix_bytes + set_bytes);
+    const size_t need = pix_bytes + set_bytes;
+    const bool oversized = work.bytes > 2 * need + ((size_t)1 << 30);
+    work.slack_calls = oversized ? work.slack_calls + 1 : 0;
+    if (work.bytes < need || work.slack_calls >= 8) {
+        // grow, or give back a workspace far larger than the last 8 launches
+        // needed (a kept workspace would otherwise hold its largest launch's
+        // size for good, and one small call between large ones must not cost
+        // a reallocation); the buffer holds the primary pass's results, so
+        // start over
+        int32_t rc = work.alloc(need);
         if (rc) return rc;
         return render_jobs(hj, work, st, graph);
     }

@@ -28,6 +28,7 @@ int32_t ensure_device();
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    int slack_calls = 0;  // consecutive uses that needed far less (see render_jobs)
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -39,6 +40,7 @@ struct DevBuf {
     }
     int32_t alloc(size_t n) {
         release();
+        slack_calls = 0;
         if (n == 0) n = 16;
         hipError_t e = hipMalloc(&p, n);
         if (e != hipSuccess) { p = nullptr; return hip_fail(e, "hipMalloc"); }

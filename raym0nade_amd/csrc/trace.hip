@@ -171,6 +171,95 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
     }
 }
 
+// Two walks per lane: the same persistent scheme with two ray slots per lane,
+// whose node loads are issued together before either is used, so each lane
+// keeps two loads in flight.  A wave claims rays for all its idle slots with
+// one atomic (first the idle A slots in lane order, then the idle B slots).
+// LDS: each slot's first lds_entries entries in its own rows of the block's
+// dynamic LDS (2 * lds_entries * TRACE_BLOCK * 8 bytes).
+template <bool ANY>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent2(SceneView S, const float *__restrict__ rays,
+                                                                   int64_t n, float tmin, float tmax0,
+                                                                   const float *__restrict__ aim_arr,
+                                                                   float *__restrict__ t_out,
+                                                                   int32_t *__restrict__ id_out,
+                                                                   uint8_t *__restrict__ occ,
+                                                                   unsigned long long *__restrict__ next,
+                                                                   int fetch_min, const uint32_t *__restrict__ order,
+                                                                   const int *__restrict__ use_order,
+                                                                   int lds_entries) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    ScratchEntries ea, eb;
+    Walker<ANY, false, LdsStack> wa, wb;
+    wa.stk.bind(&ea, lds_entries, 0);
+    wb.stk.bind(&eb, lds_entries, lds_entries);
+    RayPre ra, rb;
+    float aa = -INFINITY, ab = -INFINITY;
+    int64_t ya = -1, yb = -1;
+    bool exhausted = false;
+    TraceCounts c{};
+    while (true) {
+        const bool ia = ya < 0, ib = yb < 0;
+        const uint64_t ma = __ballot(ia), mb = __ballot(ib);
+        const uint64_t busy = __ballot(!ia || !ib);
+        if (exhausted && busy == 0) break;
+        const uint32_t na = (uint32_t)__popcll(ma), nidle = na + (uint32_t)__popcll(mb);
+        if (!exhausted && nidle && (nidle >= (uint32_t)fetch_min || busy == 0)) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)(ma | mb)) - 1u;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(next, (unsigned long long)nidle);
+            base = __shfl(base, (int)leader, 64);
+            if (base + nidle >= (unsigned long long)n) exhausted = true;
+            if (ia) {
+                const int64_t slot = (int64_t)base + __popcll(ma & below);
+                if (slot < n) {
+                    const int64_t k = ray_of(slot, order, use_order);
+                    ya = k;
+                    V3 o, d;
+                    load_ray(rays, k, o, d);
+                    ra = ray_pre(o, d);
+                    aa = ANY ? (aim_arr ? aim_arr[k] : tmax0) : -INFINITY;
+                    wa.init(S, ra, ANY ? aa + T_EPS : tmax0);
+                }
+            }
+            if (ib) {
+                const int64_t slot = (int64_t)base + na + __popcll(mb & below);
+                if (slot < n) {
+                    const int64_t k = ray_of(slot, order, use_order);
+                    yb = k;
+                    V3 o, d;
+                    load_ray(rays, k, o, d);
+                    rb = ray_pre(o, d);
+                    ab = ANY ? (aim_arr ? aim_arr[k] : tmax0) : -INFINITY;
+                    wb.init(S, rb, ANY ? ab + T_EPS : tmax0);
+                }
+            }
+        }
+        typename Walker<ANY, false, LdsStack>::Node nda{}, ndb{};
+        if (ya >= 0) nda = wa.fetch(S);
+        if (yb >= 0) ndb = wb.fetch(S);
+        if (ya >= 0 && wa.step_with(S, ra, tmin, aa, c, nda)) {
+            if (ANY) {
+                occ[ya] = (wa.tmax < aa) ? 1 : 0;
+            } else {
+                t_out[ya] = wa.tmax;
+                id_out[ya] = wa.best >= 0 ? __float_as_int(S.tris[wa.best].e1[3]) : -1;
+            }
+            ya = -1;
+        }
+        if (yb >= 0 && wb.step_with(S, rb, tmin, ab, c, ndb)) {
+            if (ANY) {
+                occ[yb] = (wb.tmax < ab) ? 1 : 0;
+            } else {
+                t_out[yb] = wb.tmax;
+                id_out[yb] = wb.best >= 0 ? __float_as_int(S.tris[wb.best].e1[3]) : -1;
+            }
+            yb = -1;
+        }
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 // ---------------------------------------------------------------- reordering
@@ -388,14 +477,18 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                            order);
     }
     if (timing && (mrc = mark(1, st))) return mrc;
-    if (T.trace_variant == 2 || T.trace_variant == 3) {
+    if (T.trace_variant >= 2) {
         unsigned long long *next = nullptr;
         int32_t rc = ray_counter(&next);
         if (rc) return rc;
         unsigned pgrid = (unsigned)std::min<int64_t>(grid, (int64_t)device_cu_count() * T.persistent_blocks_per_cu);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, next);
         const int nl = std::min(std::max(1, (int)s->view.depth), T.lds_stack_entries);
-        if (T.trace_variant == 3)
+        if (T.trace_variant == 4)
+            hipLaunchKernelGGL((k_trace_persistent2<ANY>), dim3(pgrid), dim3(TRACE_BLOCK),
+                               (size_t)2 * nl * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
+                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, nl);
+        else if (T.trace_variant == 3)
             hipLaunchKernelGGL((k_trace_persistent<ANY, true>), dim3(pgrid), dim3(TRACE_BLOCK),
                                (size_t)nl * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
                                occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, nl);

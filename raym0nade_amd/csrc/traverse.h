@@ -237,10 +237,12 @@ struct LdsStack {
     uint32_t *col;     // this lane's column
     int nl;            // entries held in LDS
     ScratchEntries *e; // the deeper ones
-    // Binds the launching kernel's dynamic LDS as this lane's first n entries.
-    __device__ __forceinline__ void bind(ScratchEntries *m, int n) {
+    // Binds the launching kernel's dynamic LDS as this lane's first n entries,
+    // starting at entry row `first` of the block's LDS (a second walker of the
+    // same lane takes the rows after the first one's).
+    __device__ __forceinline__ void bind(ScratchEntries *m, int n, int first = 0) {
         extern __shared__ uint32_t walk_lds[];
-        col = walk_lds + threadIdx.x;
+        col = walk_lds + first * (2 * WALK_BLOCK) + threadIdx.x;
         nl = n;
         e = m;
     }
@@ -253,9 +255,14 @@ struct LdsStack {
             e->tl[i] = t;
         }
     }
-    __device__ __forceinline__ uint32_t ref_at(int i) const { return i < nl ? col[i * (2 * WALK_BLOCK)] : e->ref[i]; }
+    // The LDS reads are relaxed atomic loads only so that the compiler keeps
+    // them apart from the scratch reads (merged, both become flat loads).
+    __device__ __forceinline__ uint32_t lds_at(int w) const {
+        return __hip_atomic_load(col + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ __forceinline__ uint32_t ref_at(int i) const { return i < nl ? lds_at(i * (2 * WALK_BLOCK)) : e->ref[i]; }
     __device__ __forceinline__ float tl_at(int i) const {
-        return i < nl ? __uint_as_float(col[i * (2 * WALK_BLOCK) + WALK_BLOCK]) : e->tl[i];
+        return i < nl ? __uint_as_float(lds_at(i * (2 * WALK_BLOCK) + WALK_BLOCK)) : e->tl[i];
     }
 };
 
@@ -276,14 +283,38 @@ struct Walker {
         anypar = r.par[0] || r.par[1] || r.par[2];
     }
 
+    // The record of the interior node about to be visited (nothing for a leaf):
+    // a walker that loads it ahead of step_with lets a lane keep two walks'
+    // loads in flight.
+    struct Node {
+        float4 a, b, c;
+        uint2 leaf;
+    };
+    __device__ __forceinline__ Node fetch(const SceneView &S) const {
+        Node n;
+        if (!(cur & 0x80000000u)) {
+            const float4 *P = reinterpret_cast<const float4 *>(S.nodes + cur);
+            n.a = P[0];
+            n.b = P[1];
+            n.c = P[2];
+            n.leaf = *reinterpret_cast<const uint2 *>(&P[3]);
+        }
+        return n;
+    }
+
     // Returns true when the walk has finished (result in best / tmax); see trace_walk.
     __device__ __forceinline__ bool step(const SceneView &S, const RayPre &r, float tmin, float aim, TraceCounts &c) {
+        return step_with(S, r, tmin, aim, c, fetch(S));
+    }
+
+    // step() with the current interior node's record already loaded.
+    __device__ __forceinline__ bool step_with(const SceneView &S, const RayPre &r, float tmin, float aim, TraceCounts &c,
+                                              const Node &nd) {
         if (!(cur & 0x80000000u)) {
             if (COUNT) { c.interior++; c.boxes += 2; }
             const uint32_t u = cur;
-            const float4 *P = reinterpret_cast<const float4 *>(S.nodes + u);
-            float4 A = P[0], Bq = P[1], C = P[2];
-            uint2 Lf = *reinterpret_cast<const uint2 *>(&P[3]);
+            const float4 A = nd.a, Bq = nd.b, C = nd.c;
+            const uint2 Lf = nd.leaf;
             float tL0 = tmin, tR0 = tmax, tL1 = tmin, tR1 = tmax;
             if (!anypar) {
                 box_test_fast(r, A.x, A.y, A.z, A.w, Bq.x, Bq.y, tL0, tR0);

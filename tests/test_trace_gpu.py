@@ -30,13 +30,15 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
 
 
 @pytest.fixture(params=[(0, 0, 32), (0, 2, 32), (1, 1, 32), (2, 1, 32), (2, 2, 32), (3, 1, 32), (3, 0, 32),
-                        (3, 1, 2)],
+                        (3, 1, 2), (4, 1, 8), (4, 0, 2)],
                 ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted",
-                     "persistent-lds-adaptive", "persistent-lds", "persistent-lds2-spill"])
+                     "persistent-lds-adaptive", "persistent-lds", "persistent-lds2-spill", "two-walks-lds8",
+                     "two-walks-lds2-spill"])
 def variant(request):
-    """Every traversal kernel variant, with and without ray reordering, and
-    with the walk stack in scratch, in LDS, or split between them (2 LDS
-    entries per lane, the rest in scratch), must give the same bits."""
+    """Every traversal kernel variant (one or two walks per lane), with and
+    without ray reordering, and with the walk stack in scratch, in LDS, or
+    split between them (2 LDS entries per lane, the rest in scratch), must
+    give the same bits."""
     keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_RAY_SORT, rm.TUNE_LDS_STACK_ENTRIES)
     old = tuple(rm.get_tuning(k) for k in keys)
     for k, v in zip(keys, request.param):
