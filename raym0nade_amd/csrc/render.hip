@@ -293,7 +293,9 @@ __device__ __forceinline__ void calc_var(Acc4 &a, float e) {
     if (a.v < 0.0f) a.v = 0.0f;
 }
 
-__global__ __launch_bounds__(RB) void k_direct(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P) {
+// MINW as for k_bounce / k_nee (register cap chosen by tuning).
+template <int MINW>
+__global__ __launch_bounds__(RB, MINW) void k_direct(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P) {
     const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
     if (gp >= total_px) return;
     const Job &jb = J[job_of(J, nj, gp)];
@@ -1090,7 +1092,8 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     if (graph) RM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_direct, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
+    hipLaunchKernelGGL(tuning().render_min_waves == 4 ? k_direct<4> : k_direct<1>, dim3(gp), dim3(RB), walk_lds, st,
+                       (const Job *)dJ, nj, total_px, P);
     const uint32_t nsets = (uint32_t)total;
     if (nsets > 0) {
         const unsigned gs = (nsets + RB - 1) / RB;
