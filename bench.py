@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="accepted for older profile commands; the 1 GiB calibration copy always runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
-    ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort[,lds_stack_entries]'")
+    ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort[,lds_stack_entries[,tris_per_step]]'")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
@@ -113,12 +113,12 @@ def main():
                                            for k, v in algo.items()})
     tuned = None
     if args.trace:
-        v, b, f, s, e = (int(x) for x in (args.trace + ",32").split(",")[:5])
+        v, b, f, s, e, p = (int(x) for x in (args.trace + ",32,0").split(",")[:6])
         for k, val in ((rm.TUNE_TRACE_VARIANT, v), (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, b), (rm.TUNE_FETCH_MIN, f),
-                       (rm.TUNE_RAY_SORT, s), (rm.TUNE_LDS_STACK_ENTRIES, e)):
+                       (rm.TUNE_RAY_SORT, s), (rm.TUNE_LDS_STACK_ENTRIES, e), (rm.TUNE_TRIS_PER_STEP, p)):
             rm.set_tuning(k, val)
         tuned = {"variant": v, "blocks_per_cu": b, "fetch_min": f, "ray_sort": s, "lds_stack_entries": e,
-                 "fixed": True}
+                 "tris_per_step": p, "fixed": True}
     elif not args.no_autotune:
         from raym0nade_amd import autotune
         tuned = autotune.tune_trace(S, [r_coh, r_inc], reps=2, log=log)

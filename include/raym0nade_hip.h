@@ -310,9 +310,12 @@ enum rm_tune_key {
     RM_TUNE_RENDER_MIN_WAVES = 7,       /* 1 or 4 (default): resident waves per SIMD the render's
                                            shading and NEE kernels are compiled for (4 caps them
                                            at 128 VGPRs, spilling the rest) */
-    RM_TUNE_LDS_STACK_ENTRIES = 8       /* trace variant 3: walk-stack entries per lane kept in LDS
+    RM_TUNE_LDS_STACK_ENTRIES = 8,      /* trace variant 3: walk-stack entries per lane kept in LDS
                                            (1..32, default 8; deeper entries go to scratch; 32 holds
                                            the whole stack of any tree) */
+    RM_TUNE_TRIS_PER_STEP = 9           /* trace variant 3 and render walks: triangles one walk step
+                                           tests (0 = a whole leaf; 1, 2 (default) or 4: a leaf over
+                                           several steps; the render's shading kernels use 0 or 2) */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

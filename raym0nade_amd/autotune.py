@@ -20,10 +20,11 @@ import time
 
 from ._lib import RM_ERR_OOM, RmError
 from . import (TUNE_FETCH_MIN, TUNE_LDS_STACK_ENTRIES, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_RAY_SORT,
-               TUNE_RENDER_BLOCKS_PER_CU, TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, RmRenderArgs, get_tuning,
-               render_batch, set_tuning)
+               TUNE_RENDER_BLOCKS_PER_CU, TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, TUNE_TRIS_PER_STEP, RmRenderArgs,
+               get_tuning, render_batch, set_tuning)
 
-TRACE_KEYS = (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT, TUNE_LDS_STACK_ENTRIES)
+TRACE_KEYS = (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT, TUNE_LDS_STACK_ENTRIES,
+              TUNE_TRIS_PER_STEP)
 
 
 def _time(fn, reps):
@@ -42,11 +43,12 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     """batches: list of device ray tensors [n, 6].  Sets and returns the
     fastest {key: value} for closest-hit traversal of all batches."""
     if candidates is None:
-        candidates = [(0, 8, 16, 1, 32)]
-        candidates += [(1, 8, 16, s, 32) for s in (0, 1)]
-        candidates += [(2, b, f, s, 32) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
-        candidates += [(3, b, f, 1, e) for b, f, e in itertools.product((4, 8, 16), (16, 32), (4, 6, 8, 12, 32))]
-        candidates += [(4, b, 32, 1, 6) for b in (6, 8)]
+        candidates = [(0, 8, 16, 1, 32, 0)]
+        candidates += [(1, 8, 16, s, 32, 0) for s in (0, 1)]
+        candidates += [(2, b, f, s, 32, 0) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
+        candidates += [(3, b, f, 1, e, 0) for b, f, e in itertools.product((4, 8, 16), (16, 32), (4, 6, 8, 12, 32))]
+        candidates += [(3, b, f, 1, e, p) for b, f, e, p in itertools.product((8, 16), (16, 32), (6, 8), (1, 2, 4))]
+        candidates += [(4, b, 32, 1, 6, 0) for b in (6, 8)]
 
     def run():
         for r in batches:
@@ -65,20 +67,21 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     for k, v in zip(TRACE_KEYS, best):
         set_tuning(k, v)
     return {"variant": best[0], "blocks_per_cu": best[1], "fetch_min": best[2], "ray_sort": best[3],
-            "lds_stack_entries": best[4], "seconds": t, "tried": len(results)}
+            "lds_stack_entries": best[4], "tris_per_step": best[5], "seconds": t, "tried": len(results)}
 
 
 def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4, 8),
-                walks=((3, 8), (3, 4), (3, 6), (3, 12), (2, 32)), log=None) -> dict:
+                walks=((3, 8, 2), (3, 8, 0), (3, 8, 1), (3, 8, 4), (3, 6, 2), (3, 12, 2), (2, 32, 2)),
+                log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp
     scaled by spp_scale: first every (blocks per CU, register cap) pair with
-    automatic paths per round and the walk stacks of walks[0], then the other
-    paths-per-round values at the best pair, then the other walk stacks
-    (trace variant, LDS entries per lane: variant 2 keeps them in scratch,
-    3 its first entries in LDS).  Sets the launch parameters and returns the
-    choice; the caller applies paths_per_round (0 = automatic) to its jobs'
-    arguments."""
+    automatic paths per round and the walk settings of walks[0], then the
+    other paths-per-round values at the best pair, then the other walk
+    settings (trace variant, LDS entries per lane, triangles per step: variant
+    2 keeps the walk stacks in scratch, 3 their first entries in LDS).  Sets
+    the launch parameters and returns the choice; the caller applies
+    paths_per_round (0 = automatic) to its jobs' arguments."""
     small = []
     for S, a, fr, post in jobs:
         b = RmRenderArgs.from_buffer_copy(a)
@@ -92,6 +95,7 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
         set_tuning(TUNE_RENDER_MIN_WAVES, w)
         set_tuning(TUNE_TRACE_VARIANT, walk[0])
         set_tuning(TUNE_LDS_STACK_ENTRIES, walk[1])
+        set_tuning(TUNE_TRIS_PER_STEP, walk[2])
         try:
             t = _time(lambda: render_batch([tuple(j) for j in small], use_graph=True), 1)
         except RmError as e:
@@ -112,8 +116,10 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
     set_tuning(TUNE_RENDER_MIN_WAVES, w)
     set_tuning(TUNE_TRACE_VARIANT, walk[0])
     set_tuning(TUNE_LDS_STACK_ENTRIES, walk[1])
+    set_tuning(TUNE_TRIS_PER_STEP, walk[2])
     return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "walk_variant": walk[0],
-            "lds_stack_entries": walk[1], "seconds_scaled": t, "spp_scale": spp_scale, "tried": len(results)}
+            "lds_stack_entries": walk[1], "tris_per_step": walk[2], "seconds_scaled": t, "spp_scale": spp_scale,
+            "tried": len(results)}
 
 
 def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> dict:
@@ -136,7 +142,8 @@ def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> di
 
 _KEYS = {"trace_variant": TUNE_TRACE_VARIANT, "persistent_blocks_per_cu": TUNE_PERSISTENT_BLOCKS_PER_CU,
          "fetch_min": TUNE_FETCH_MIN, "ray_sort": TUNE_RAY_SORT, "render_blocks_per_cu": TUNE_RENDER_BLOCKS_PER_CU,
-         "render_min_waves": TUNE_RENDER_MIN_WAVES, "lds_stack_entries": TUNE_LDS_STACK_ENTRIES}
+         "render_min_waves": TUNE_RENDER_MIN_WAVES, "lds_stack_entries": TUNE_LDS_STACK_ENTRIES,
+         "tris_per_step": TUNE_TRIS_PER_STEP}
 
 
 def current() -> dict:

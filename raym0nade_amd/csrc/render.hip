@@ -236,6 +236,8 @@ __device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F,
     return s;
 }
 
+// TPS (here and in k_direct, k_extend, k_nee): triangles per walk step.
+template <int TPS>
 __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P) {
     const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
     if (gp >= total_px) return;
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int n
     camera_ray(A, x, y, Dir, base);
     RayPre r = ray_pre(A.pos, Dir);
     float t = INFINITY;
-    int32_t slot = trace_walk<false>(S, r, T_EPS, t, -INFINITY);
+    int32_t slot = trace_walk<false, true, TPS>(S, r, T_EPS, t, -INFINITY);
     Surf s = surf_default();
     float4 zero = make_float4(0, 0, 0, 0);
     if (slot < 0) {
@@ -294,7 +296,7 @@ __device__ __forceinline__ void calc_var(Acc4 &a, float e) {
 }
 
 // MINW as for k_bounce / k_nee (register cap chosen by tuning).
-template <int MINW>
+template <int MINW, int TPS>
 __global__ __launch_bounds__(RB, MINW) void k_direct(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P) {
     const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
     if (gp >= total_px) return;
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(RB, MINW) void k_direct(const Job *__restrict__ J, 
             for (int T = 0; T < cnt; ++T) {
                 Stream g{A.seed, TAG_DIRECT, (uint32_t)p, (uint32_t)T, 0u};
                 LSample ls;
-                if (direct_sample<false>(S, s, V, total, cnt, g, ls, tc)) accumulate(s.base, ls, d, sp);
+                if (direct_sample<false, true, TPS>(S, s, V, total, cnt, g, ls, tc)) accumulate(s.base, ls, d, sp);
             }
         }
     }
@@ -551,14 +553,14 @@ __device__ __forceinline__ float4 *rec_slot(const Job &jb, uint32_t gp, uint32_t
 // as everywhere else, so the hits are bit-identical.
 // LDS: the walk's first lds_entries pending entries in the block's dynamic
 // LDS (lds_entries * RB * 8 bytes), deeper ones in per-lane scratch.
-template <bool LDS>
+template <bool LDS, int TPS>
 __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ Q, const uint32_t *count_in,
                                                uint32_t *work, uint2 *__restrict__ hit, int fetch_min,
                                                int lds_entries) {
     const uint32_t n = *count_in;
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
-    Walker<false, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
+    Walker<false, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
     if constexpr (LDS)
         w.stk.bind(&ent, lds_entries);
     else
@@ -749,7 +751,7 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
 // NEE-and-stop this round (src/render.cpp:147-160,229-240 with
 // src/sampling.cpp:467-527): up to sampleCount[depth] light samples, each
 // with its shadow ray, scaled by the branch factor, into the path's set.
-template <int MINW>
+template <int MINW, int TPS>
 __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, PixBufs P, AccBufs AB, NeeQ NQ,
                                             const uint32_t *count, uint32_t *work) {
   const uint32_t n = *count;
@@ -794,7 +796,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
             TraceCounts tc{};
             for (int Ti = 0; Ti < cnt; ++Ti) {
                 LSample x;
-                if (direct_sample<false>(S, s, V, total, cnt, rng, x, tc)) {
+                if (direct_sample<false, true, TPS>(S, s, V, total, cnt, rng, x, tc)) {
                     x.bsdf *= factor;  // scaling(samples, factor)
                     add_light_sample(P, AB, pix, set, smp, k, T, W, p0, x, extra_on, extra, rec_slot(jb, pix, smp));
                     ++k;
@@ -1042,7 +1044,11 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
             hipLaunchKernelGGL(k_zero_f4, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, J[i].F.rec, n4);
         }
     const unsigned gp = (unsigned)((total_px + RB - 1) / RB);
-    hipLaunchKernelGGL(k_primary, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
+    // triangles per walk step: any of 0, 1, 2, 4 for the extension kernel;
+    // the shading kernels' walks take a whole leaf (0) or two triangles
+    const int tps = tuning().tris_per_step;
+    hipLaunchKernelGGL(tps ? k_primary<2> : k_primary<0>, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj,
+                       total_px, P);
     hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum);
     RM_HIP(hipGetLastError());
     std::vector<int64_t> hs(nblk);
@@ -1092,8 +1098,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     if (graph) RM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(tuning().render_min_waves == 4 ? k_direct<4> : k_direct<1>, dim3(gp), dim3(RB), walk_lds, st,
-                       (const Job *)dJ, nj, total_px, P);
+    const bool capped = tuning().render_min_waves == 4;
+    auto direct = capped ? (tps ? k_direct<4, 2> : k_direct<4, 0>) : (tps ? k_direct<1, 2> : k_direct<1, 0>);
+    hipLaunchKernelGGL(direct, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
     const uint32_t nsets = (uint32_t)total;
     if (nsets > 0) {
         const unsigned gs = (nsets + RB - 1) / RB;
@@ -1101,19 +1108,22 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         hipLaunchKernelGGL(k_clear_sets, dim3(gs), dim3(RB), 0, st, AB, nsets);
         const unsigned ge = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
         const int fetch_min = tuning().fetch_min;
-        const bool capped = tuning().render_min_waves == 4;
         auto bounce = capped ? k_bounce<4> : k_bounce<1>;
-        auto nee = capped ? k_nee<4> : k_nee<1>;
+        auto nee = capped ? (tps ? k_nee<4, 2> : k_nee<4, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
+        auto extend_lds = tps == 1 ? k_extend<true, 1> : tps == 2 ? k_extend<true, 2>
+                        : tps == 4 ? k_extend<true, 4> : k_extend<true, 0>;
+        auto extend_scratch = tps == 1 ? k_extend<false, 1> : tps == 2 ? k_extend<false, 2>
+                            : tps == 4 ? k_extend<false, 4> : k_extend<false, 0>;
         for (int j = 0; j < R; ++j) {
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, counts, 4 * NCOUNT);
             hipLaunchKernelGGL(k_spawn, dim3(gs), dim3(RB), 0, st, (const Job *)dJ, nj, P, j, nsets, Q[0], counts);
             for (int d = 1; d <= D; ++d) {
                 const uint32_t *n_in = counts + d - 1;
                 if (lds_walk)
-                    hipLaunchKernelGGL(k_extend<true>, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ,
+                    hipLaunchKernelGGL(extend_lds, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ,
                                        Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min, nl);
                 else
-                    hipLaunchKernelGGL(k_extend<false>, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1],
+                    hipLaunchKernelGGL(extend_scratch, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1],
                                        n_in, counts + 2 * NCOUNT + d, hits, fetch_min, 0);
                 hipLaunchKernelGGL(bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1], n_in,
                                    counts + NCOUNT + d, (const uint2 *)hits, Q[d & 1], counts + d, NQ,

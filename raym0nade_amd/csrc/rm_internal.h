@@ -96,6 +96,7 @@ struct Tuning {
     int kernel_timing = 0;        // record HIP events around the trace kernels (rm_last_trace_ms)
     int render_min_waves = 4;     // 1 or 4: register cap of the shading and NEE kernels
     int lds_stack_entries = 8;    // trace variant 3: walk-stack entries per lane kept in LDS
+    int tris_per_step = 2;        // trace variant 3: triangles tested per step (0: a whole leaf)
 };
 Tuning &tuning();
 int device_cu_count();

@@ -610,7 +610,7 @@ struct DirectInfo {
 
 // One light sample of sampleDirectLight (src/sampling.cpp:467-527), with its
 // shadow ray.  Returns false when the reference `continue`s.
-template <bool COUNT, bool LDSW = true>
+template <bool COUNT, bool LDSW = true, int TPS = 0>
 __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s, V3 V, float total, int cnt, Stream &g,
                                               LSample &out, TraceCounts &tc, DirectInfo *info = nullptr) {
     if (S.sky) {
@@ -637,7 +637,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
         if (!ok) return false;
         RayPre r = ray_pre(s.pos, dir);
         float aim = INFINITY, tmax = aim + T_EPS;
-        trace_walk<true, LDSW>(S, r, T_EPS, tmax, aim);
+        trace_walk<true, LDSW, TPS>(S, r, T_EPS, tmax, aim);
         if (tmax < aim) return false;
         V3 b = bsdf_eval(s, V, dir);
         clamp64(b);
@@ -683,7 +683,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
     float dist = length(lp - s.pos);
     RayPre ry = ray_pre(s.pos, dir);
     float aim = dist - T_EPS, tmax = aim + T_EPS;
-    trace_walk<true, LDSW>(S, ry, T_EPS, tmax, aim);
+    trace_walk<true, LDSW, TPS>(S, ry, T_EPS, tmax, aim);
     if (tmax < aim) return false;
     V3 b = bsdf_eval(s, V, dir);
     clamp64(b);

@@ -112,8 +112,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const f
 // Every wave leaves the loop when the batch is exhausted and its lanes are
 // done, so the grid always drains.  LDS: the walker's first lds_entries
 // pending entries live in the block's dynamic LDS (lds_entries * TRACE_BLOCK *
-// 8 bytes), deeper ones in per-lane scratch.
-template <bool ANY, bool LDS>
+// 8 bytes), deeper ones in per-lane scratch.  TPS: triangles per step (0: a
+// whole leaf per step).
+template <bool ANY, bool LDS, int TPS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, const float *__restrict__ rays, int64_t n,
                                                                   float tmin, float tmax0,
                                                                   const float *__restrict__ aim_arr,
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                                                                   const int *__restrict__ use_order, int lds_entries) {
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
-    Walker<ANY, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type> w;
+    Walker<ANY, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
     if constexpr (LDS)
         w.stk.bind(&ent, lds_entries);
     else
@@ -488,13 +489,17 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
             hipLaunchKernelGGL((k_trace_persistent2<ANY>), dim3(pgrid), dim3(TRACE_BLOCK),
                                (size_t)2 * nl * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
                                occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, nl);
-        else if (T.trace_variant == 3)
-            hipLaunchKernelGGL((k_trace_persistent<ANY, true>), dim3(pgrid), dim3(TRACE_BLOCK),
-                               (size_t)nl * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
-                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, nl);
-        else
-            hipLaunchKernelGGL((k_trace_persistent<ANY, false>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n,
+        else if (T.trace_variant == 3) {
+            auto kern = T.tris_per_step == 1   ? k_trace_persistent<ANY, true, 1>
+                        : T.tris_per_step == 2 ? k_trace_persistent<ANY, true, 2>
+                        : T.tris_per_step == 4 ? k_trace_persistent<ANY, true, 4>
+                                               : k_trace_persistent<ANY, true, 0>;
+            hipLaunchKernelGGL(kern, dim3(pgrid), dim3(TRACE_BLOCK), (size_t)nl * TRACE_BLOCK * 8, st, s->view, rays, n,
                                tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
+                               (const int *)flag, nl);
+        } else
+            hipLaunchKernelGGL((k_trace_persistent<ANY, false, 0>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays,
+                               n, tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
                                (const int *)flag, 0);
     } else if (T.trace_variant == 1) {
         hipLaunchKernelGGL(k_trace_step<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,

@@ -266,7 +266,10 @@ struct LdsStack {
     }
 };
 
-template <bool ANY, bool COUNT, class Stack = PrivateStack>
+// TPS: triangles a step tests (0: a whole leaf).  Splitting a leaf's tests
+// over steps keeps their order, so the result is the same; lanes of a wave
+// that stand at interior nodes then wait one triangle, not a whole leaf.
+template <bool ANY, bool COUNT, class Stack = PrivateStack, int TPS = 0>
 struct Walker {
     Stack stk;
     int sp;
@@ -274,6 +277,7 @@ struct Walker {
     int32_t best;
     float tmax;
     bool anypar;
+    uint32_t li;   // TPS > 0: next triangle of the current leaf
 
     __device__ __forceinline__ void init(const SceneView &S, const RayPre &r, float tmax0) {
         sp = 0;
@@ -281,6 +285,7 @@ struct Walker {
         tmax = tmax0;
         cur = S.root_leaf ? ((uint32_t)S.root_leaf | 0x80000000u) : 1u;
         anypar = r.par[0] || r.par[1] || r.par[2];
+        li = 0;
     }
 
     // The record of the interior node about to be visited (nothing for a leaf):
@@ -334,10 +339,28 @@ struct Walker {
                 ++sp;
             }
             cur = (tLf < tRf) ? rf : 0u;
-        } else {
+        } else if (TPS == 0) {
             leaf_test<COUNT>(S, r, cur & 0x7fffffffu, tmin, tmax, best, c);
             cur = 0u;
             if (ANY && tmax < aim) return true;
+        } else {
+            const uint32_t code = cur & 0x7fffffffu;
+            const uint32_t first = code >> LEAF_SHIFT, cnt = code & LEAF_MASK;
+            if (COUNT && li == 0) { c.leaves++; c.tris += cnt; }
+            const uint32_t end = li + TPS < cnt ? li + TPS : cnt;
+            for (uint32_t i = li; i < end; ++i) {
+                const TriRec T = load_tri(S.tris, first + i);
+                const float t = tri_test(r, T);
+                if (tmin < t && t < tmax) {
+                    tmax = t;
+                    best = (int32_t)(first + i);
+                }
+            }
+            li = end;
+            if (ANY && tmax < aim) return true;
+            if (li < cnt) return false;
+            li = 0;
+            cur = 0u;
         }
         if (cur == 0u) {
             while (sp > 0) {
@@ -358,8 +381,8 @@ struct Walker {
 // traversal.  The pending entries live in scratch, or their first S.walk_lds
 // in the kernel's dynamic LDS (the launch then allocates S.walk_lds *
 // WALK_BLOCK * 8 bytes and runs blocks of WALK_BLOCK lanes).
-template <bool ANY, class Stack>
-__device__ __forceinline__ int32_t walk_with(Walker<ANY, false, Stack> &w, const SceneView &S, const RayPre &r,
+template <bool ANY, class Stack, int TPS>
+__device__ __forceinline__ int32_t walk_with(Walker<ANY, false, Stack, TPS> &w, const SceneView &S, const RayPre &r,
                                              float tmin, float &tmax, float aim) {
     w.init(S, r, tmax);
     TraceCounts c{};
@@ -369,19 +392,20 @@ __device__ __forceinline__ int32_t walk_with(Walker<ANY, false, Stack> &w, const
     return w.best;
 }
 
-// LDSW = false compiles the scratch walk only (kernels that never get LDS stacks).
-template <bool ANY, bool LDSW = true>
+// LDSW = false compiles the scratch walk only (kernels that never get LDS
+// stacks); TPS as for Walker.
+template <bool ANY, bool LDSW = true, int TPS = 0>
 __device__ __forceinline__ int32_t trace_walk(const SceneView &S, const RayPre &r, float tmin, float &tmax, float aim) {
     if constexpr (LDSW) {
         if (S.walk_lds) {
             ScratchEntries deep;
-            Walker<ANY, false, LdsStack> w;
+            Walker<ANY, false, LdsStack, TPS> w;
             w.stk.bind(&deep, S.walk_lds);
             return walk_with(w, S, r, tmin, tmax, aim);
         }
     }
     ScratchEntries ent;
-    Walker<ANY, false> w;
+    Walker<ANY, false, PrivateStack, TPS> w;
     w.stk.bind(&ent);
     return walk_with(w, S, r, tmin, tmax, aim);
 }

@@ -14,7 +14,8 @@ def test_tuning_keys_round_trip_and_reject_out_of_range():
         for key, good, bad in [(rm.TUNE_TRACE_VARIANT, (0, 4), (-1, 5)),
                                (rm.TUNE_LDS_STACK_ENTRIES, (1, 32), (0, 33)),
                                (rm.TUNE_RENDER_MIN_WAVES, (1, 4), (2, 8)),
-                               (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, (1, 64), (0, 65))]:
+                               (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, (1, 64), (0, 65)),
+                               (rm.TUNE_TRIS_PER_STEP, (0, 1, 2, 4), (3, 8))]:
             for v in good:
                 rm.set_tuning(key, v)
                 assert rm.get_tuning(key) == v
@@ -59,7 +60,7 @@ def test_tune_render_skips_candidates_that_run_out_of_memory(monkeypatch):
         a = rm.RmRenderArgs()
         a.spp = 64
         choice = autotune.tune_render([(None, a, None, 0)], blocks=(4, 8), waves=(1,), paths_per_round=(0, 4, 8),
-                                      walks=((3, 8),))
+                                      walks=((3, 8, 0),))
         assert choice["paths_per_round"] == 4
         assert any(k == 8 for k, _ in calls)
     finally:
@@ -72,6 +73,6 @@ def test_tune_render_skips_candidates_that_run_out_of_memory(monkeypatch):
     try:
         with pytest.raises(RmError):
             autotune.tune_render([(None, rm.RmRenderArgs(), None, 0)], blocks=(4,), waves=(1,),
-                                 paths_per_round=(0,), walks=((3, 8),))
+                                 paths_per_round=(0,), walks=((3, 8, 0),))
     finally:
         autotune.restore(saved)

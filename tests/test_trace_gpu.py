@@ -29,17 +29,18 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
     return occ.cpu().numpy()
 
 
-@pytest.fixture(params=[(0, 0, 32), (0, 2, 32), (1, 1, 32), (2, 1, 32), (2, 2, 32), (3, 1, 32), (3, 0, 32),
-                        (3, 1, 2), (4, 1, 8), (4, 0, 2)],
+@pytest.fixture(params=[(0, 0, 32, 0), (0, 2, 32, 0), (1, 1, 32, 0), (2, 1, 32, 0), (2, 2, 32, 0), (3, 1, 32, 0),
+                        (3, 0, 32, 0), (3, 1, 2, 0), (3, 1, 8, 1), (3, 1, 2, 2), (3, 0, 8, 4), (4, 1, 8, 0),
+                        (4, 0, 2, 0)],
                 ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted",
-                     "persistent-lds-adaptive", "persistent-lds", "persistent-lds2-spill", "two-walks-lds8",
-                     "two-walks-lds2-spill"])
+                     "persistent-lds-adaptive", "persistent-lds", "persistent-lds2-spill", "lds8-1tri",
+                     "lds2-spill-2tri", "lds8-4tri", "two-walks-lds8", "two-walks-lds2-spill"])
 def variant(request):
     """Every traversal kernel variant (one or two walks per lane), with and
-    without ray reordering, and with the walk stack in scratch, in LDS, or
-    split between them (2 LDS entries per lane, the rest in scratch), must
-    give the same bits."""
-    keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_RAY_SORT, rm.TUNE_LDS_STACK_ENTRIES)
+    without ray reordering, with the walk stack in scratch, in LDS, or split
+    between them (2 LDS entries per lane, the rest in scratch), and testing a
+    whole leaf or 1, 2, 4 triangles per step, must give the same bits."""
+    keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_RAY_SORT, rm.TUNE_LDS_STACK_ENTRIES, rm.TUNE_TRIS_PER_STEP)
     old = tuple(rm.get_tuning(k) for k in keys)
     for k, v in zip(keys, request.param):
         rm.set_tuning(k, v)
