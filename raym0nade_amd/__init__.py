@@ -73,7 +73,7 @@ def philox(ctr, key) -> tuple:
 
 (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT,
  TUNE_RENDER_BLOCKS_PER_CU, TUNE_ADJOINT_SPLIT, TUNE_KERNEL_TIMING, TUNE_RENDER_MIN_WAVES,
- TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP) = range(10)
+ TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP, TUNE_NEE_SPLIT) = range(11)
 
 
 def last_trace_ms() -> tuple:

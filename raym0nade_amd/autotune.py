@@ -20,8 +20,8 @@ import time
 
 from ._lib import RM_ERR_OOM, RmError
 from . import (TUNE_FETCH_MIN, TUNE_LDS_STACK_ENTRIES, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_RAY_SORT,
-               TUNE_RENDER_BLOCKS_PER_CU, TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, TUNE_TRIS_PER_STEP, RmRenderArgs,
-               get_tuning, render_batch, set_tuning)
+               TUNE_NEE_SPLIT, TUNE_RENDER_BLOCKS_PER_CU, TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, TUNE_TRIS_PER_STEP,
+               RmRenderArgs, get_tuning, render_batch, set_tuning)
 
 TRACE_KEYS = (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT, TUNE_LDS_STACK_ENTRIES,
               TUNE_TRIS_PER_STEP)
@@ -143,7 +143,7 @@ def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> di
 _KEYS = {"trace_variant": TUNE_TRACE_VARIANT, "persistent_blocks_per_cu": TUNE_PERSISTENT_BLOCKS_PER_CU,
          "fetch_min": TUNE_FETCH_MIN, "ray_sort": TUNE_RAY_SORT, "render_blocks_per_cu": TUNE_RENDER_BLOCKS_PER_CU,
          "render_min_waves": TUNE_RENDER_MIN_WAVES, "lds_stack_entries": TUNE_LDS_STACK_ENTRIES,
-         "tris_per_step": TUNE_TRIS_PER_STEP}
+         "tris_per_step": TUNE_TRIS_PER_STEP, "nee_split": TUNE_NEE_SPLIT}
 
 
 def current() -> dict:

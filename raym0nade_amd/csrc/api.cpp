@@ -88,6 +88,10 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
                 return rm::fail(RM_ERR_INVALID, "rm_set_tuning: triangles per step %d (0, 1, 2 or 4)", value);
             t.tris_per_step = value;
             return RM_OK;
+        case RM_TUNE_NEE_SPLIT:
+            if (value < 0 || value > 1) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: NEE split %d", value);
+            t.nee_split = value;
+            return RM_OK;
         case RM_TUNE_LDS_STACK_ENTRIES:
             if (value < 1 || value > rm::MAX_STACK)
                 return rm::fail(RM_ERR_INVALID, "rm_set_tuning: LDS stack entries %d (1..%d)", value, rm::MAX_STACK);
@@ -112,6 +116,7 @@ extern "C" int32_t rm_get_tuning(int32_t key, int32_t *value) {
         case RM_TUNE_RENDER_MIN_WAVES: *value = t.render_min_waves; return RM_OK;
         case RM_TUNE_LDS_STACK_ENTRIES: *value = t.lds_stack_entries; return RM_OK;
         case RM_TUNE_TRIS_PER_STEP: *value = t.tris_per_step; return RM_OK;
+        case RM_TUNE_NEE_SPLIT: *value = t.nee_split; return RM_OK;
         default: return rm::fail(RM_ERR_INVALID, "rm_get_tuning: unknown key %d", key);
     }
 }

@@ -807,6 +807,173 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
   }
 }
 
+// The NEE pass split in three, with the same result: k_nee_gen draws every
+// light sample of each NEE record (sampleCount[depth] <= NEE_MAX of them) up
+// to its shadow ray, k_shadow traces all shadow rays in a lean persistent
+// any-hit walk, and k_nee_add adds the clear samples to the path's set in
+// the reference order.  Drawing ahead is exact because a shadow ray draws
+// no random numbers.  Entry e = record * NEE_MAX + i of the shadow queue:
+// o = (origin, aim), d = (direction, job + 1 or 0 when there is no sample),
+// x = (light * bsdf * factor, weight), vis = 1 when the ray is clear.
+constexpr int NEE_MAX = 6;  // sample_count(16)
+struct ShadowQ {
+    float4 *o, *d, *x;
+    uint8_t *vis;
+};
+
+template <int MINW>
+__global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J, NeeQ NQ, const uint32_t *count,
+                                                uint32_t *work, ShadowQ SQ) {
+  const uint32_t n = *count;
+  const uint32_t lane = threadIdx.x & 63u;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const uint32_t q = base + lane;
+    const bool live = q < n;
+    float4 f5 = make_float4(0, 0, 0, 0);
+    if (live) f5 = NQ.f[5][q];
+    const uint32_t flags = bu(f5.w);
+    for_each_job(live ? (int)(flags >> 8) : -1, live, [&](int jid) {
+        const Job &jb = J[jid];
+        const SceneView &S = jb.S;
+        const RArgs &A = jb.A;
+        float4 f0 = NQ.f[0][q], f1 = NQ.f[1][q], f2 = NQ.f[2][q], f3 = NQ.f[3][q], f4 = NQ.f[4][q];
+        float4 f8 = NQ.f[8][q];
+        Surf s = surf_default();
+        s.pos = V3{f0.x, f0.y, f0.z};
+        s.shapeN = V3{f1.x, f1.y, f1.z};
+        s.surfN = V3{f2.x, f2.y, f2.z};
+        s.base = V3{f4.x, f4.y, f4.z};
+        s.roughness = f4.w;
+        s.metallic = f5.x;
+        s.opacity = f5.y;
+        s.eta = f5.z;
+        s.entering = (flags & 1u) != 0;
+        s.specular = f8.w;
+        const V3 V{f3.x, f3.y, f3.z};
+        const uint32_t pix = bu(f0.w), smp = bu(f2.w);
+        const int cnt = (int)((flags >> 2) & 63u);
+        const float factor = NQ.f[7][q].w;
+        Stream rng{A.seed, TAG_INDIRECT, (uint32_t)(pix - jb.pix_off), smp, bu(f3.w)};
+        const float total = direct_total(S, s, V);
+        const float4 none = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int Ti = 0; Ti < NEE_MAX; ++Ti) {
+            const int64_t e = (int64_t)q * NEE_MAX + Ti;
+            ShadowRay sr;
+            LSample x;
+            if (total != 0.0f && Ti < cnt && direct_ray(S, s, V, total, cnt, rng, sr, x)) {
+                x.bsdf *= factor;  // scaling(samples, factor)
+                const V3 L = x.light * x.bsdf;
+                SQ.o[e] = make_float4(s.pos.x, s.pos.y, s.pos.z, sr.aim);
+                SQ.d[e] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, ub((uint32_t)jid + 1u));
+                SQ.x[e] = make_float4(L.x, L.y, L.z, x.weight);
+            } else {
+                SQ.d[e] = none;
+            }
+        }
+    });
+  }
+}
+
+// Any-hit walk of every shadow ray of the round (the k_extend scheme with
+// ANY: it stops once the running closest t is below aim).
+template <bool LDS, int TPS>
+__global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, ShadowQ SQ, const uint32_t *count_rec,
+                                               uint32_t *work, int fetch_min, int lds_entries) {
+    const uint32_t n = *count_rec * (uint32_t)NEE_MAX;
+    const uint32_t lane = threadIdx.x & 63u;
+    ScratchEntries ent;
+    Walker<true, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
+    if constexpr (LDS)
+        w.stk.bind(&ent, lds_entries);
+    else
+        w.stk.bind(&ent);
+    RayPre r;
+    SceneView S;  // per lane: the nodes, triangles and root of its ray's scene
+    float aim = 0.0f;
+    int64_t q = -1;
+    bool exhausted = false;
+    TraceCounts c{};
+    while (true) {
+        bool idle = q < 0;
+        uint64_t idle_mask = __ballot(idle);
+        uint64_t busy_mask = __ballot(!idle);
+        if (exhausted && busy_mask == 0) break;
+        if (!exhausted && idle_mask && (__popcll(idle_mask) >= fetch_min || busy_mask == 0)) {
+            uint32_t cnt = (uint32_t)__popcll(idle_mask);
+            uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle_mask) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(work, cnt);
+            base = __shfl(base, (int)leader, 64);
+            if (base + cnt >= n) exhausted = true;
+            if (idle) {
+                uint32_t slot = base + (uint32_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
+                if (slot < n) {
+                    const float4 dd = SQ.d[slot];
+                    const uint32_t tag = bu(dd.w);
+                    if (tag == 0u) {
+                        SQ.vis[slot] = 0;  // no sample: nothing to trace
+                    } else {
+                        q = slot;
+                        const float4 oo = SQ.o[slot];
+                        const SceneView &SJ = J[tag - 1u].S;
+                        S.nodes = SJ.nodes;
+                        S.tris = SJ.tris;
+                        S.root_leaf = SJ.root_leaf;
+                        aim = oo.w;
+                        r = ray_pre(V3{oo.x, oo.y, oo.z}, V3{dd.x, dd.y, dd.z});
+                        w.init(S, r, aim + T_EPS);
+                    }
+                }
+            }
+        }
+        if (q >= 0 && w.step(S, r, T_EPS, aim, c)) {
+            SQ.vis[q] = (w.tmax < aim) ? 0 : 1;
+            q = -1;
+        }
+    }
+}
+
+template <int MINW>
+__global__ __launch_bounds__(RB, MINW) void k_nee_add(const Job *__restrict__ J, PixBufs P, AccBufs AB, NeeQ NQ,
+                                                const uint32_t *count, uint32_t *work, ShadowQ SQ) {
+  const uint32_t n = *count;
+  const uint32_t lane = threadIdx.x & 63u;
+  while (true) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(work, 64u);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const uint32_t q = base + lane;
+    const bool live = q < n;
+    float4 f5 = make_float4(0, 0, 0, 0);
+    if (live) f5 = NQ.f[5][q];
+    const uint32_t flags = bu(f5.w);
+    for_each_job(live ? (int)(flags >> 8) : -1, live, [&](int jid) {
+        const Job &jb = J[jid];
+        const uint32_t pix = bu(NQ.f[0][q].w), set = bu(NQ.f[1][q].w), smp = bu(NQ.f[2][q].w);
+        const bool extra_on = (flags & 2u) != 0;
+        const int cnt = (int)((flags >> 2) & 63u);
+        const float4 f6 = NQ.f[6][q], f7 = NQ.f[7][q], f8 = NQ.f[8][q];
+        const V3 T{f6.x, f6.y, f6.z}, p0{f7.x, f7.y, f7.z}, extra{f8.x, f8.y, f8.z};
+        const float W = f6.w;
+        uint32_t k = 0;
+        for (int Ti = 0; Ti < cnt; ++Ti) {
+            const int64_t e = (int64_t)q * NEE_MAX + Ti;
+            if (bu(SQ.d[e].w) == 0u || !SQ.vis[e]) continue;
+            const float4 xx = SQ.x[e];
+            // light * bsdf was formed when the sample was drawn; times 1 is exact
+            const LSample x{V3{xx.x, xx.y, xx.z}, v3(1.0f), xx.w};
+            add_light_sample(P, AB, pix, set, smp, k, T, W, p0, x, extra_on, extra, rec_slot(jb, pix, smp));
+            ++k;
+        }
+    });
+  }
+}
+
 __global__ __launch_bounds__(RB) void k_clear_sets(AccBufs AB, uint32_t n) {
     uint32_t g = blockIdx.x * RB + threadIdx.x;
     if (g >= n) return;
@@ -1026,7 +1193,8 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     int64_t *bsum = carve<int64_t>(cur, nblk);
     // counts[d]: paths entering depth d+1; counts[NCOUNT + d]: shading claims of
     // depth d; counts[2 NCOUNT + d]: traversal claims of depth d;
-    // counts[3 NCOUNT]: NEE records of the round, counts[3 NCOUNT + 1]: their claims
+    // counts[3 NCOUNT]: NEE records of the round, counts[3 NCOUNT + 1..3]: claims of
+    // the NEE kernels (one pass, or generate / shadow / add)
     uint32_t *counts = carve<uint32_t>(cur, 4 * NCOUNT);
     // Trace variant 3 keeps the first pending entries of every walk in LDS:
     // each kernel of the launch that traverses gets nl entries per lane (at
@@ -1061,8 +1229,11 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         total += v;
     }
     if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "%lld path slots per round", (long long)total);
-    const size_t set_bytes =
-        pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16) + pad(total * 8) + 9 * pad(total * 16);
+    // the split NEE pass's shadow queue (NEE_MAX entries per path slot)
+    const bool nee_split = tuning().nee_split != 0 && total * NEE_MAX <= 0xffffffffLL;  // k_shadow counts in 32 bits
+    const int64_t nshadow = nee_split ? total * NEE_MAX : 0;
+    const size_t set_bytes = pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16) + pad(total * 8) +
+                             9 * pad(total * 16) + 3 * pad(nshadow * 16) + pad(nshadow);
     const size_t need = pix_bytes + set_bytes;
     const bool oversized = work.bytes > 2 * need + ((size_t)1 << 30);
     work.slack_calls = oversized ? work.slack_calls + 1 : 0;
@@ -1092,6 +1263,11 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     uint2 *hits = carve<uint2>(cur, total);
     NeeQ NQ;
     for (int k = 0; k < 9; ++k) NQ.f[k] = carve<float4>(cur, total);
+    ShadowQ SQ;
+    SQ.o = carve<float4>(cur, nshadow);
+    SQ.d = carve<float4>(cur, nshadow);
+    SQ.x = carve<float4>(cur, nshadow);
+    SQ.vis = carve<uint8_t>(cur, nshadow);
     hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum, P.set_off, P.set_pix);
     RM_HIP(hipGetLastError());
 
@@ -1110,6 +1286,12 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         const int fetch_min = tuning().fetch_min;
         auto bounce = capped ? k_bounce<4> : k_bounce<1>;
         auto nee = capped ? (tps ? k_nee<4, 2> : k_nee<4, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
+        auto nee_gen = capped ? k_nee_gen<4> : k_nee_gen<1>;
+        auto nee_add = capped ? k_nee_add<4> : k_nee_add<1>;
+        auto shadow_lds = tps == 1 ? k_shadow<true, 1> : tps == 2 ? k_shadow<true, 2>
+                        : tps == 4 ? k_shadow<true, 4> : k_shadow<true, 0>;
+        auto shadow_scratch = tps == 1 ? k_shadow<false, 1> : tps == 2 ? k_shadow<false, 2>
+                            : tps == 4 ? k_shadow<false, 4> : k_shadow<false, 0>;
         auto extend_lds = tps == 1 ? k_extend<true, 1> : tps == 2 ? k_extend<true, 2>
                         : tps == 4 ? k_extend<true, 4> : k_extend<true, 0>;
         auto extend_scratch = tps == 1 ? k_extend<false, 1> : tps == 2 ? k_extend<false, 2>
@@ -1129,8 +1311,22 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
                                    counts + NCOUNT + d, (const uint2 *)hits, Q[d & 1], counts + d, NQ,
                                    counts + 3 * NCOUNT);
             }
-            hipLaunchKernelGGL(nee, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, P, AB, NQ,
-                               (const uint32_t *)(counts + 3 * NCOUNT), counts + 3 * NCOUNT + 1);
+            const uint32_t *n_nee = counts + 3 * NCOUNT;
+            if (nee_split) {
+                hipLaunchKernelGGL(nee_gen, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, NQ, n_nee,
+                                   counts + 3 * NCOUNT + 1, SQ);
+                if (lds_walk)
+                    hipLaunchKernelGGL(shadow_lds, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ, SQ, n_nee,
+                                       counts + 3 * NCOUNT + 2, fetch_min, nl);
+                else
+                    hipLaunchKernelGGL(shadow_scratch, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, SQ, n_nee,
+                                       counts + 3 * NCOUNT + 2, fetch_min, 0);
+                hipLaunchKernelGGL(nee_add, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, P, AB, NQ, n_nee,
+                                   counts + 3 * NCOUNT + 3, SQ);
+            } else {
+                hipLaunchKernelGGL(nee, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, P, AB, NQ, n_nee,
+                                   counts + 3 * NCOUNT + 1);
+            }
         }
     }
     hipLaunchKernelGGL(k_finish, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, AB);

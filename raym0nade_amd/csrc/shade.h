@@ -608,11 +608,21 @@ struct DirectInfo {
     int li;
 };
 
-// One light sample of sampleDirectLight (src/sampling.cpp:467-527), with its
-// shadow ray.  Returns false when the reference `continue`s.
-template <bool COUNT, bool LDSW = true, int TPS = 0>
-__device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s, V3 V, float total, int cnt, Stream &g,
-                                              LSample &out, TraceCounts &tc, DirectInfo *info = nullptr) {
+// The shadow ray of a light sample: from the surface point along dir; the
+// sample counts when the closest hit on (T_EPS, aim + T_EPS) is not below aim
+// (aim = INFINITY toward the sky).
+struct ShadowRay {
+    V3 dir;
+    float aim;
+};
+
+// One light sample of sampleDirectLight (src/sampling.cpp:467-527) up to its
+// shadow ray: the ray, the sample as it counts when the ray is clear, and
+// what it was made of.  Returns false when the reference `continue`s before
+// tracing.  The shadow ray draws no random numbers, so a caller may trace it
+// later.
+__device__ __forceinline__ bool direct_ray(const SceneView &S, const Surf &s, V3 V, float total, int cnt, Stream &g,
+                                           ShadowRay &sr, LSample &out, DirectInfo *info = nullptr) {
     if (S.sky) {
         // sampleSkyBox (src/sampling.cpp:450-465)
         const int n = S.sky_w * S.sky_h;
@@ -635,10 +645,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
             }
         }
         if (!ok) return false;
-        RayPre r = ray_pre(s.pos, dir);
-        float aim = INFINITY, tmax = aim + T_EPS;
-        trace_walk<true, LDSW, TPS>(S, r, T_EPS, tmax, aim);
-        if (tmax < aim) return false;
+        sr = ShadowRay{dir, INFINITY};
         V3 b = bsdf_eval(s, V, dir);
         clamp64(b);
         out = LSample{b, light, 1.0f / float(cnt)};
@@ -681,16 +688,26 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
     if (!found) return false;
     V3 dir = normalize(lp - s.pos);
     float dist = length(lp - s.pos);
-    RayPre ry = ray_pre(s.pos, dir);
-    float aim = dist - T_EPS, tmax = aim + T_EPS;
-    trace_walk<true, LDSW, TPS>(S, ry, T_EPS, tmax, aim);
-    if (tmax < aim) return false;
+    sr = ShadowRay{dir, dist - T_EPS};
     V3 b = bsdf_eval(s, V, dir);
     clamp64(b);
     V3 light = ((2.0f * PI_F * L.power) * ld3(L.color)) / (dist * dist + 1e-3f) / P;
     out = LSample{b, light, 1.0f / float(cnt * (fails + 1))};
     if (info) *info = DirectInfo{dir, dist, P, li};
     return true;
+}
+
+// One light sample with its shadow ray traced here.  Returns false when the
+// reference `continue`s (no sample, or the shadow ray is blocked).
+template <bool COUNT, bool LDSW = true, int TPS = 0>
+__device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s, V3 V, float total, int cnt, Stream &g,
+                                              LSample &out, TraceCounts &tc, DirectInfo *info = nullptr) {
+    ShadowRay sr;
+    if (!direct_ray(S, s, V, total, cnt, g, sr, out, info)) return false;
+    RayPre r = ray_pre(s.pos, sr.dir);
+    float tmax = sr.aim + T_EPS;
+    trace_walk<true, LDSW, TPS>(S, r, T_EPS, tmax, sr.aim);
+    return !(tmax < sr.aim);
 }
 
 // Total light-object weight of sampleDirectLight (0 -> no samples); 1 with a sky.

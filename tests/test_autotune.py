@@ -15,7 +15,8 @@ def test_tuning_keys_round_trip_and_reject_out_of_range():
                                (rm.TUNE_LDS_STACK_ENTRIES, (1, 32), (0, 33)),
                                (rm.TUNE_RENDER_MIN_WAVES, (1, 4), (2, 8)),
                                (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, (1, 64), (0, 65)),
-                               (rm.TUNE_TRIS_PER_STEP, (0, 1, 2, 4), (3, 8))]:
+                               (rm.TUNE_TRIS_PER_STEP, (0, 1, 2, 4), (3, 8)),
+                               (rm.TUNE_NEE_SPLIT, (0, 1), (-1, 2))]:
             for v in good:
                 rm.set_tuning(key, v)
                 assert rm.get_tuning(key) == v

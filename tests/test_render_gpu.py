@@ -159,19 +159,21 @@ def test_lds_stack_walks_are_bit_identical(dev, scene, entries, tps):
     # every walk of a render launch (primary, direct, extension, NEE) with its
     # stack in LDS (trace variant 3), wholly or split with scratch, and with a
     # leaf's triangles tested over several steps, visits the same nodes and
-    # triangles in the same order as the scratch-stack, whole-leaf walker
+    # triangles in the same order as the scratch-stack, whole-leaf walker; the
+    # NEE pass split into generate / shadow / add gives the same sets as the
+    # single NEE kernel
     g = rm.generate(0, 1) if scene == "cornell" else rm.generate(2, 3, grid=2, freq=12, sky_width=256, sky_height=128)
     S = rm.Scene(g)
     a = small_args(g, 80, 48, 12, max_depth=16)
-    keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_LDS_STACK_ENTRIES, rm.TUNE_TRIS_PER_STEP)
+    keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_LDS_STACK_ENTRIES, rm.TUNE_TRIS_PER_STEP, rm.TUNE_NEE_SPLIT)
     old = tuple(rm.get_tuning(k) for k in keys)
     try:
-        rm.set_tuning(rm.TUNE_TRACE_VARIANT, 2)
-        rm.set_tuning(rm.TUNE_TRIS_PER_STEP, 0)
+        # reference: scratch stacks, whole leaves, NEE shadow rays traced inside the NEE kernel
+        for k, v in zip(keys, (2, 32, 0, 0)):
+            rm.set_tuning(k, v)
         gb2, r2 = RC.gpu_render(S, a)
-        rm.set_tuning(rm.TUNE_TRACE_VARIANT, 3)
-        rm.set_tuning(rm.TUNE_LDS_STACK_ENTRIES, entries)
-        rm.set_tuning(rm.TUNE_TRIS_PER_STEP, tps)
+        for k, v in zip(keys, (3, entries, tps, 1)):
+            rm.set_tuning(k, v)
         gb3, r3 = RC.gpu_render(S, a)
     finally:
         for k, v in zip(keys, old):
