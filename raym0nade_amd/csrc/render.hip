@@ -329,6 +329,111 @@ __global__ __launch_bounds__(RB, MINW) void k_direct(const Job *__restrict__ J, 
     reinterpret_cast<float4 *>(F.Ds)[p] = make_float4(sp.r, sp.g, sp.b, sp.v);
 }
 
+// The NEE pass split in three, with the same result: k_nee_gen draws every
+// light sample of each NEE record (sampleCount[depth] <= NEE_MAX of them) up
+// to its shadow ray, k_shadow traces all shadow rays in a lean persistent
+// any-hit walk, and k_nee_add adds the clear samples to the path's set in
+// the reference order.  Drawing ahead is exact because a shadow ray draws
+// no random numbers.  Entry e = record * NEE_MAX + i of the shadow queue:
+// o = (origin, aim), d = (direction, job + 1 or 0 when there is no sample),
+// x = (light * bsdf * factor, weight), vis = 1 when the ray is clear.
+
+constexpr int NEE_MAX = 6;  // sample_count(16)
+struct ShadowQ {
+    float4 *o, *d, *x;
+    float4 *x2;  // the primary-vertex pass: (light, 0); x then holds (bsdf, weight)
+    uint8_t *vis;
+};
+
+// The primary-vertex light samples (k_direct) split like the NEE pass, in
+// chunks of C samples per pixel: k_direct_gen draws sample chunk * C + t of
+// pixel gp in lane gp * C + t (each sample has its own stream, so they are
+// independent) up to its shadow ray; k_shadow traces them; k_direct_add
+// continues each pixel's sums in sample order in the frame's Dd / Ds; after
+// the last chunk k_direct_final applies calcVar.  Same result as k_direct.
+template <int MINW>
+__global__ __launch_bounds__(RB, MINW) void k_direct_gen(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P,
+                                                   int C, int chunk, ShadowQ SQ) {
+    const int64_t g = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (g >= total_px * C) return;
+    const int64_t gp = g / C;
+    const int T = chunk * C + (int)(g - gp * C);
+    const Job &jb = J[job_of(J, nj, gp)];
+    const SceneView &S = jb.S;
+    const RArgs &A = jb.A;
+    const FrameView &F = jb.F;
+    const int64_t p = gp - jb.pix_off;
+    const int cnt = (int)(float(A.spp) * A.p_direct);
+    bool ok = false;
+    if (T < cnt) {
+        V3 pos = ld3(F.pos + p * 3);
+        V3 em = ld3(F.emis + p * 3);
+        if (any_finite(pos) && !(length(em) > 0.0f)) {
+            Surf s = pixel_surf(P, F, gp, p);
+            V3 V = -normalize(s.pos - A.pos);
+            float total = direct_total(S, s, V);
+            if (total != 0.0f) {
+                Stream rng{A.seed, TAG_DIRECT, (uint32_t)p, (uint32_t)T, 0u};
+                ShadowRay sr;
+                LSample ls;
+                if (direct_ray(S, s, V, total, cnt, rng, sr, ls)) {
+                    SQ.o[g] = make_float4(s.pos.x, s.pos.y, s.pos.z, sr.aim);
+                    SQ.d[g] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, ub((uint32_t)(&jb - J) + 1u));
+                    SQ.x[g] = make_float4(ls.bsdf.x, ls.bsdf.y, ls.bsdf.z, ls.weight);
+                    SQ.x2[g] = make_float4(ls.light.x, ls.light.y, ls.light.z, 0.0f);
+                    ok = true;
+                }
+            }
+        }
+    }
+    if (!ok) SQ.d[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(RB) void k_direct_add(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P,
+                                                   int C, int chunk, ShadowQ SQ) {
+    const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (gp >= total_px) return;
+    const Job &jb = J[job_of(J, nj, gp)];
+    const FrameView &F = jb.F;
+    const int64_t p = gp - jb.pix_off;
+    float4 *Dd = reinterpret_cast<float4 *>(F.Dd), *Ds = reinterpret_cast<float4 *>(F.Ds);
+    Acc4 d{0, 0, 0, 0}, sp{0, 0, 0, 0};
+    if (chunk > 0) {
+        const float4 a = Dd[p], b = Ds[p];
+        d = Acc4{a.x, a.y, a.z, a.w};
+        sp = Acc4{b.x, b.y, b.z, b.w};
+    }
+    const float4 b0 = P.pi0[gp];
+    const V3 base{b0.x, b0.y, b0.z};
+    for (int t = 0; t < C; ++t) {
+        const int64_t e = gp * C + t;
+        if (bu(SQ.d[e].w) == 0u || !SQ.vis[e]) continue;
+        const float4 x = SQ.x[e], l = SQ.x2[e];
+        accumulate(base, LSample{V3{x.x, x.y, x.z}, V3{l.x, l.y, l.z}, x.w}, d, sp);
+    }
+    Dd[p] = make_float4(d.r, d.g, d.b, d.v);
+    Ds[p] = make_float4(sp.r, sp.g, sp.b, sp.v);
+}
+
+__global__ __launch_bounds__(RB) void k_direct_final(const Job *__restrict__ J, int nj, int64_t total_px, int chunks) {
+    const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
+    if (gp >= total_px) return;
+    const Job &jb = J[job_of(J, nj, gp)];
+    const FrameView &F = jb.F;
+    const int64_t p = gp - jb.pix_off;
+    float4 *Dd = reinterpret_cast<float4 *>(F.Dd), *Ds = reinterpret_cast<float4 *>(F.Ds);
+    Acc4 d{0, 0, 0, 0}, sp{0, 0, 0, 0};
+    if (chunks > 0) {
+        const float4 a = Dd[p], b = Ds[p];
+        d = Acc4{a.x, a.y, a.z, a.w};
+        sp = Acc4{b.x, b.y, b.z, b.w};
+    }
+    calc_var(d, jb.A.exposure);
+    calc_var(sp, jb.A.exposure);
+    Dd[p] = make_float4(d.r, d.g, d.b, d.v);
+    Ds[p] = make_float4(sp.r, sp.g, sp.b, sp.v);
+}
+
 // Wave-aggregated append: one atomic per wave.
 __device__ __forceinline__ uint32_t append(uint32_t *counter, bool want) {
     uint64_t mask = __ballot(want);
@@ -807,20 +912,6 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
   }
 }
 
-// The NEE pass split in three, with the same result: k_nee_gen draws every
-// light sample of each NEE record (sampleCount[depth] <= NEE_MAX of them) up
-// to its shadow ray, k_shadow traces all shadow rays in a lean persistent
-// any-hit walk, and k_nee_add adds the clear samples to the path's set in
-// the reference order.  Drawing ahead is exact because a shadow ray draws
-// no random numbers.  Entry e = record * NEE_MAX + i of the shadow queue:
-// o = (origin, aim), d = (direction, job + 1 or 0 when there is no sample),
-// x = (light * bsdf * factor, weight), vis = 1 when the ray is clear.
-constexpr int NEE_MAX = 6;  // sample_count(16)
-struct ShadowQ {
-    float4 *o, *d, *x;
-    uint8_t *vis;
-};
-
 template <int MINW>
 __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J, NeeQ NQ, const uint32_t *count,
                                                 uint32_t *work, ShadowQ SQ) {
@@ -880,10 +971,11 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
 
 // Any-hit walk of every shadow ray of the round (the k_extend scheme with
 // ANY: it stops once the running closest t is below aim).
+// Entries: *count * per when count is given (NEE records), else n_fixed.
 template <bool LDS, int TPS>
-__global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, ShadowQ SQ, const uint32_t *count_rec,
-                                               uint32_t *work, int fetch_min, int lds_entries) {
-    const uint32_t n = *count_rec * (uint32_t)NEE_MAX;
+__global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, ShadowQ SQ, const uint32_t *count, uint32_t per,
+                                               uint32_t n_fixed, uint32_t *work, int fetch_min, int lds_entries) {
+    const uint32_t n = count ? *count * per : n_fixed;
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
     Walker<true, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
@@ -1229,11 +1321,18 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         total += v;
     }
     if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "%lld path slots per round", (long long)total);
-    // the split NEE pass's shadow queue (NEE_MAX entries per path slot)
-    const bool nee_split = tuning().nee_split != 0 && total * NEE_MAX <= 0xffffffffLL;  // k_shadow counts in 32 bits
-    const int64_t nshadow = nee_split ? total * NEE_MAX : 0;
+    // The split shadow passes' queue: NEE_MAX entries per path slot for the
+    // NEE pass, C per pixel for each chunk of the primary-vertex pass (C such
+    // that a chunk holds about 16M entries).  k_shadow counts in 32 bits.
+    int max_direct = 0;
+    for (int i = 0; i < nj; ++i) max_direct = std::max(max_direct, (int)(float(J[i].A.spp) * J[i].A.p_direct));
+    const int dC = (int)std::max<int64_t>(1, std::min<int64_t>(32, ((int64_t)16 << 20) / std::max<int64_t>(1, total_px)));
+    const int dchunks = (max_direct + dC - 1) / dC;
+    const bool split = tuning().nee_split != 0 && total * NEE_MAX <= 0xffffffffLL && total_px * dC <= 0xffffffffLL;
+    const int64_t ndirect = split ? total_px * dC : 0;
+    const int64_t nshadow = split ? std::max<int64_t>(total * NEE_MAX, ndirect) : 0;
     const size_t set_bytes = pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16) + pad(total * 8) +
-                             9 * pad(total * 16) + 3 * pad(nshadow * 16) + pad(nshadow);
+                             9 * pad(total * 16) + 3 * pad(nshadow * 16) + pad(nshadow) + pad(ndirect * 16);
     const size_t need = pix_bytes + set_bytes;
     const bool oversized = work.bytes > 2 * need + ((size_t)1 << 30);
     work.slack_calls = oversized ? work.slack_calls + 1 : 0;
@@ -1268,6 +1367,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     SQ.d = carve<float4>(cur, nshadow);
     SQ.x = carve<float4>(cur, nshadow);
     SQ.vis = carve<uint8_t>(cur, nshadow);
+    SQ.x2 = carve<float4>(cur, ndirect);
     hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum, P.set_off, P.set_pix);
     RM_HIP(hipGetLastError());
 
@@ -1275,23 +1375,38 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     hipGraphExec_t ge = nullptr;
     if (graph) RM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const bool capped = tuning().render_min_waves == 4;
-    auto direct = capped ? (tps ? k_direct<4, 2> : k_direct<4, 0>) : (tps ? k_direct<1, 2> : k_direct<1, 0>);
-    hipLaunchKernelGGL(direct, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
+    const unsigned gpers = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
+    const int fetch_min = tuning().fetch_min;
+    auto shadow = lds_walk ? (tps == 1 ? k_shadow<true, 1> : tps == 2 ? k_shadow<true, 2>
+                              : tps == 4 ? k_shadow<true, 4> : k_shadow<true, 0>)
+                           : (tps == 1 ? k_shadow<false, 1> : tps == 2 ? k_shadow<false, 2>
+                              : tps == 4 ? k_shadow<false, 4> : k_shadow<false, 0>);
+    if (split) {
+        uint32_t *claims = counts + 3 * NCOUNT + 8;  // zeroed before each chunk's shadow pass
+        const unsigned gd = (unsigned)((ndirect + RB - 1) / RB);
+        auto direct_gen = capped ? k_direct_gen<4> : k_direct_gen<1>;
+        for (int c = 0; c < dchunks; ++c) {
+            hipLaunchKernelGGL(direct_gen, dim3(gd), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, dC, c, SQ);
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, claims, 1);
+            hipLaunchKernelGGL(shadow, dim3(gpers), dim3(RB), walk_lds, st, (const Job *)dJ, SQ, (const uint32_t *)nullptr,
+                               0u, (uint32_t)ndirect, claims, fetch_min, nl);
+            hipLaunchKernelGGL(k_direct_add, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, dC, c, SQ);
+        }
+        hipLaunchKernelGGL(k_direct_final, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, dchunks);
+    } else {
+        auto direct = capped ? (tps ? k_direct<4, 2> : k_direct<4, 0>) : (tps ? k_direct<1, 2> : k_direct<1, 0>);
+        hipLaunchKernelGGL(direct, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
+    }
     const uint32_t nsets = (uint32_t)total;
     if (nsets > 0) {
         const unsigned gs = (nsets + RB - 1) / RB;
         const unsigned gb = (unsigned)(device_cu_count() * tuning().render_blocks_per_cu);
         hipLaunchKernelGGL(k_clear_sets, dim3(gs), dim3(RB), 0, st, AB, nsets);
-        const unsigned ge = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
-        const int fetch_min = tuning().fetch_min;
+        const unsigned ge = gpers;
         auto bounce = capped ? k_bounce<4> : k_bounce<1>;
         auto nee = capped ? (tps ? k_nee<4, 2> : k_nee<4, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
         auto nee_gen = capped ? k_nee_gen<4> : k_nee_gen<1>;
         auto nee_add = capped ? k_nee_add<4> : k_nee_add<1>;
-        auto shadow_lds = tps == 1 ? k_shadow<true, 1> : tps == 2 ? k_shadow<true, 2>
-                        : tps == 4 ? k_shadow<true, 4> : k_shadow<true, 0>;
-        auto shadow_scratch = tps == 1 ? k_shadow<false, 1> : tps == 2 ? k_shadow<false, 2>
-                            : tps == 4 ? k_shadow<false, 4> : k_shadow<false, 0>;
         auto extend_lds = tps == 1 ? k_extend<true, 1> : tps == 2 ? k_extend<true, 2>
                         : tps == 4 ? k_extend<true, 4> : k_extend<true, 0>;
         auto extend_scratch = tps == 1 ? k_extend<false, 1> : tps == 2 ? k_extend<false, 2>
@@ -1312,15 +1427,11 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
                                    counts + 3 * NCOUNT);
             }
             const uint32_t *n_nee = counts + 3 * NCOUNT;
-            if (nee_split) {
+            if (split) {
                 hipLaunchKernelGGL(nee_gen, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, NQ, n_nee,
                                    counts + 3 * NCOUNT + 1, SQ);
-                if (lds_walk)
-                    hipLaunchKernelGGL(shadow_lds, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ, SQ, n_nee,
-                                       counts + 3 * NCOUNT + 2, fetch_min, nl);
-                else
-                    hipLaunchKernelGGL(shadow_scratch, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, SQ, n_nee,
-                                       counts + 3 * NCOUNT + 2, fetch_min, 0);
+                hipLaunchKernelGGL(shadow, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ, SQ, n_nee,
+                                   (uint32_t)NEE_MAX, 0u, counts + 3 * NCOUNT + 2, fetch_min, nl);
                 hipLaunchKernelGGL(nee_add, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, P, AB, NQ, n_nee,
                                    counts + 3 * NCOUNT + 3, SQ);
             } else {

@@ -97,7 +97,7 @@ struct Tuning {
     int render_min_waves = 4;     // 1 or 4: register cap of the shading and NEE kernels
     int lds_stack_entries = 8;    // trace variant 3: walk-stack entries per lane kept in LDS
     int tris_per_step = 2;        // trace variant 3: triangles tested per step (0: a whole leaf)
-    int nee_split = 1;            // render: NEE shadow rays traced in their own pass (0: inside the NEE kernel)
+    int nee_split = 1;            // render: shadow rays of light samples traced in their own passes
 };
 Tuning &tuning();
 int device_cu_count();
