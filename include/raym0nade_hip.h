@@ -316,9 +316,10 @@ enum rm_tune_key {
     RM_TUNE_TRIS_PER_STEP = 9           /* trace variant 3 and render walks: triangles one walk step
                                            tests (0 = a whole leaf; 1, 2 (default) or 4: a leaf over
                                            several steps; the render's shading kernels use 0 or 2) */,
-    RM_TUNE_NEE_SPLIT = 10              /* render: 1 (default) traces the shadow rays of light samples
-                                           (the primary vertex's and the NEE records') in their own
-                                           persistent passes; 0 inside the sampling kernels */
+    RM_TUNE_NEE_SPLIT = 10              /* render: 1 traces the shadow rays of light samples (the
+                                           primary vertex's and the NEE records') in their own
+                                           persistent passes, 0 inside the sampling kernels, 2 (default)
+                                           splits when the launch's deepest tree has >= 12 levels */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

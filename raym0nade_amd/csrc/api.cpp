@@ -89,7 +89,7 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
             t.tris_per_step = value;
             return RM_OK;
         case RM_TUNE_NEE_SPLIT:
-            if (value < 0 || value > 1) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: NEE split %d", value);
+            if (value < 0 || value > 2) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: shadow split %d", value);
             t.nee_split = value;
             return RM_OK;
         case RM_TUNE_LDS_STACK_ENTRIES:

@@ -29,6 +29,7 @@
 // increments when it is dropped.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -339,25 +340,71 @@ __global__ __launch_bounds__(RB, MINW) void k_direct(const Job *__restrict__ J, 
 // x = (light * bsdf * factor, weight), vis = 1 when the ray is clear.
 
 constexpr int NEE_MAX = 6;  // sample_count(16)
+// The queue: rays appended by the generating kernels (one atomic per wave)
+// at o = (origin, aim), d = (direction, job index), with vis written by
+// k_shadow; and per light-sample entry e, ref[e] = index of its queued ray,
+// or SH_NONE (no sample), SH_CLEAR / SH_BLOCKED (a job whose tree is
+// shallower than the launch's queue depth: there a walk costs less than
+// queueing it, so the generating kernel traced it itself), and its payload x
+// (and x2).
+constexpr uint32_t SH_NONE = 0xffffffffu, SH_CLEAR = 0xfffffffeu, SH_BLOCKED = 0xfffffffdu;
 struct ShadowQ {
-    float4 *o, *d, *x;
-    float4 *x2;  // the primary-vertex pass: (light, 0); x then holds (bsdf, weight)
+    float4 *o, *d;
     uint8_t *vis;
+    uint32_t *ref;
+    float4 *x;
+    float4 *x2;  // the primary-vertex pass: (light, 0); x then holds (bsdf, weight)
 };
+__device__ __forceinline__ bool shadow_clear(const ShadowQ &Q, uint32_t r) {
+    return r == SH_CLEAR || (r < SH_BLOCKED && Q.vis[r]);
+}
+
+// Wave-aggregated append: one atomic per wave.
+__device__ __forceinline__ uint32_t append(uint32_t *counter, bool want) {
+    uint64_t mask = __ballot(want);
+    if (!want) return 0xffffffffu;
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader, 64);
+    return base + rank;
+}
+
+// Queues the shadow ray of light-sample entry e, or traces it here for a
+// shallow tree; sets ref[e].
+template <int TPS>
+__device__ __forceinline__ void shadow_entry(const ShadowQ &Q, int64_t e, const SceneView &S, int job, V3 o,
+                                             const ShadowRay &sr, int queue_depth, uint32_t *queued) {
+    const bool q = S.depth >= queue_depth;
+    const uint32_t slot = append(queued, q);
+    if (q) {
+        Q.o[slot] = make_float4(o.x, o.y, o.z, sr.aim);
+        Q.d[slot] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, ub((uint32_t)job));
+        Q.ref[e] = slot;
+    } else {
+        RayPre r = ray_pre(o, sr.dir);
+        float tmax = sr.aim + T_EPS;
+        trace_walk<true, true, TPS>(S, r, T_EPS, tmax, sr.aim);
+        Q.ref[e] = (tmax < sr.aim) ? SH_BLOCKED : SH_CLEAR;
+    }
+}
 
 // The primary-vertex light samples (k_direct) split like the NEE pass, in
-// chunks of C samples per pixel: k_direct_gen draws sample chunk * C + t of
-// pixel gp in lane gp * C + t (each sample has its own stream, so they are
+// chunks of C samples per pixel: k_direct_gen draws sample t0 + t of pixel gp
+// in lane gp * C + t (each sample has its own stream, so they are
 // independent) up to its shadow ray; k_shadow traces them; k_direct_add
 // continues each pixel's sums in sample order in the frame's Dd / Ds; after
 // the last chunk k_direct_final applies calcVar.  Same result as k_direct.
-template <int MINW>
-__global__ __launch_bounds__(RB, MINW) void k_direct_gen(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P,
-                                                   int C, int chunk, ShadowQ SQ) {
+// A chunk covers the pixels [0, npx) of the jobs with more than t0 samples.
+template <int MINW, int TPS>
+__global__ __launch_bounds__(RB, MINW) void k_direct_gen(const Job *__restrict__ J, int nj, int64_t npx, PixBufs P,
+                                                   int C, int t0, ShadowQ SQ, int queue_depth, uint32_t *queued) {
     const int64_t g = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (g >= total_px * C) return;
+    if (g >= npx * C) return;
     const int64_t gp = g / C;
-    const int T = chunk * C + (int)(g - gp * C);
+    const int T = t0 + (int)(g - gp * C);
     const Job &jb = J[job_of(J, nj, gp)];
     const SceneView &S = jb.S;
     const RArgs &A = jb.A;
@@ -377,28 +424,27 @@ __global__ __launch_bounds__(RB, MINW) void k_direct_gen(const Job *__restrict__
                 ShadowRay sr;
                 LSample ls;
                 if (direct_ray(S, s, V, total, cnt, rng, sr, ls)) {
-                    SQ.o[g] = make_float4(s.pos.x, s.pos.y, s.pos.z, sr.aim);
-                    SQ.d[g] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, ub((uint32_t)(&jb - J) + 1u));
                     SQ.x[g] = make_float4(ls.bsdf.x, ls.bsdf.y, ls.bsdf.z, ls.weight);
                     SQ.x2[g] = make_float4(ls.light.x, ls.light.y, ls.light.z, 0.0f);
+                    shadow_entry<TPS>(SQ, g, S, (int)(&jb - J), s.pos, sr, queue_depth, queued);
                     ok = true;
                 }
             }
         }
     }
-    if (!ok) SQ.d[g] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (!ok) SQ.ref[g] = SH_NONE;
 }
 
-__global__ __launch_bounds__(RB) void k_direct_add(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P,
-                                                   int C, int chunk, ShadowQ SQ) {
+__global__ __launch_bounds__(RB) void k_direct_add(const Job *__restrict__ J, int nj, int64_t npx, PixBufs P, int C,
+                                                   int t0, ShadowQ SQ) {
     const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (gp >= total_px) return;
+    if (gp >= npx) return;
     const Job &jb = J[job_of(J, nj, gp)];
     const FrameView &F = jb.F;
     const int64_t p = gp - jb.pix_off;
     float4 *Dd = reinterpret_cast<float4 *>(F.Dd), *Ds = reinterpret_cast<float4 *>(F.Ds);
     Acc4 d{0, 0, 0, 0}, sp{0, 0, 0, 0};
-    if (chunk > 0) {
+    if (t0 > 0) {
         const float4 a = Dd[p], b = Ds[p];
         d = Acc4{a.x, a.y, a.z, a.w};
         sp = Acc4{b.x, b.y, b.z, b.w};
@@ -407,7 +453,7 @@ __global__ __launch_bounds__(RB) void k_direct_add(const Job *__restrict__ J, in
     const V3 base{b0.x, b0.y, b0.z};
     for (int t = 0; t < C; ++t) {
         const int64_t e = gp * C + t;
-        if (bu(SQ.d[e].w) == 0u || !SQ.vis[e]) continue;
+        if (!shadow_clear(SQ, SQ.ref[e])) continue;
         const float4 x = SQ.x[e], l = SQ.x2[e];
         accumulate(base, LSample{V3{x.x, x.y, x.z}, V3{l.x, l.y, l.z}, x.w}, d, sp);
     }
@@ -415,7 +461,7 @@ __global__ __launch_bounds__(RB) void k_direct_add(const Job *__restrict__ J, in
     Ds[p] = make_float4(sp.r, sp.g, sp.b, sp.v);
 }
 
-__global__ __launch_bounds__(RB) void k_direct_final(const Job *__restrict__ J, int nj, int64_t total_px, int chunks) {
+__global__ __launch_bounds__(RB) void k_direct_final(const Job *__restrict__ J, int nj, int64_t total_px) {
     const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
     if (gp >= total_px) return;
     const Job &jb = J[job_of(J, nj, gp)];
@@ -423,7 +469,7 @@ __global__ __launch_bounds__(RB) void k_direct_final(const Job *__restrict__ J, 
     const int64_t p = gp - jb.pix_off;
     float4 *Dd = reinterpret_cast<float4 *>(F.Dd), *Ds = reinterpret_cast<float4 *>(F.Ds);
     Acc4 d{0, 0, 0, 0}, sp{0, 0, 0, 0};
-    if (chunks > 0) {
+    if ((int)(float(jb.A.spp) * jb.A.p_direct) > 0) {  // an add pass wrote this pixel's sums
         const float4 a = Dd[p], b = Ds[p];
         d = Acc4{a.x, a.y, a.z, a.w};
         sp = Acc4{b.x, b.y, b.z, b.w};
@@ -434,18 +480,6 @@ __global__ __launch_bounds__(RB) void k_direct_final(const Job *__restrict__ J, 
     Ds[p] = make_float4(sp.r, sp.g, sp.b, sp.v);
 }
 
-// Wave-aggregated append: one atomic per wave.
-__device__ __forceinline__ uint32_t append(uint32_t *counter, bool want) {
-    uint64_t mask = __ballot(want);
-    if (!want) return 0xffffffffu;
-    uint32_t lane = threadIdx.x & 63u;
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader, 64);
-    return base + rank;
-}
 
 // The depth-0 vertex (sampleIndirectLightFromFirstIntersection, :314-397).
 __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj, PixBufs P, int round,
@@ -912,9 +946,9 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
   }
 }
 
-template <int MINW>
+template <int MINW, int TPS>
 __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J, NeeQ NQ, const uint32_t *count,
-                                                uint32_t *work, ShadowQ SQ) {
+                                                uint32_t *work, ShadowQ SQ, int queue_depth, uint32_t *queued) {
   const uint32_t n = *count;
   const uint32_t lane = threadIdx.x & 63u;
   while (true) {
@@ -950,7 +984,6 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
         const float factor = NQ.f[7][q].w;
         Stream rng{A.seed, TAG_INDIRECT, (uint32_t)(pix - jb.pix_off), smp, bu(f3.w)};
         const float total = direct_total(S, s, V);
-        const float4 none = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         for (int Ti = 0; Ti < NEE_MAX; ++Ti) {
             const int64_t e = (int64_t)q * NEE_MAX + Ti;
             ShadowRay sr;
@@ -958,11 +991,10 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
             if (total != 0.0f && Ti < cnt && direct_ray(S, s, V, total, cnt, rng, sr, x)) {
                 x.bsdf *= factor;  // scaling(samples, factor)
                 const V3 L = x.light * x.bsdf;
-                SQ.o[e] = make_float4(s.pos.x, s.pos.y, s.pos.z, sr.aim);
-                SQ.d[e] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, ub((uint32_t)jid + 1u));
                 SQ.x[e] = make_float4(L.x, L.y, L.z, x.weight);
+                shadow_entry<TPS>(SQ, e, S, jid, s.pos, sr, queue_depth, queued);
             } else {
-                SQ.d[e] = none;
+                SQ.ref[e] = SH_NONE;
             }
         }
     });
@@ -971,11 +1003,10 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
 
 // Any-hit walk of every shadow ray of the round (the k_extend scheme with
 // ANY: it stops once the running closest t is below aim).
-// Entries: *count * per when count is given (NEE records), else n_fixed.
 template <bool LDS, int TPS>
-__global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, ShadowQ SQ, const uint32_t *count, uint32_t per,
-                                               uint32_t n_fixed, uint32_t *work, int fetch_min, int lds_entries) {
-    const uint32_t n = count ? *count * per : n_fixed;
+__global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, ShadowQ SQ, const uint32_t *count,
+                                               uint32_t *work, int fetch_min, int lds_entries) {
+    const uint32_t n = *count;
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
     Walker<true, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
@@ -1004,21 +1035,15 @@ __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, Shadow
             if (idle) {
                 uint32_t slot = base + (uint32_t)__popcll(idle_mask & ((1ull << lane) - 1ull));
                 if (slot < n) {
-                    const float4 dd = SQ.d[slot];
-                    const uint32_t tag = bu(dd.w);
-                    if (tag == 0u) {
-                        SQ.vis[slot] = 0;  // no sample: nothing to trace
-                    } else {
-                        q = slot;
-                        const float4 oo = SQ.o[slot];
-                        const SceneView &SJ = J[tag - 1u].S;
-                        S.nodes = SJ.nodes;
-                        S.tris = SJ.tris;
-                        S.root_leaf = SJ.root_leaf;
-                        aim = oo.w;
-                        r = ray_pre(V3{oo.x, oo.y, oo.z}, V3{dd.x, dd.y, dd.z});
-                        w.init(S, r, aim + T_EPS);
-                    }
+                    q = slot;
+                    const float4 oo = SQ.o[slot], dd = SQ.d[slot];
+                    const SceneView &SJ = J[bu(dd.w)].S;
+                    S.nodes = SJ.nodes;
+                    S.tris = SJ.tris;
+                    S.root_leaf = SJ.root_leaf;
+                    aim = oo.w;
+                    r = ray_pre(V3{oo.x, oo.y, oo.z}, V3{dd.x, dd.y, dd.z});
+                    w.init(S, r, aim + T_EPS);
                 }
             }
         }
@@ -1055,7 +1080,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_add(const Job *__restrict__ J,
         uint32_t k = 0;
         for (int Ti = 0; Ti < cnt; ++Ti) {
             const int64_t e = (int64_t)q * NEE_MAX + Ti;
-            if (bu(SQ.d[e].w) == 0u || !SQ.vis[e]) continue;
+            if (!shadow_clear(SQ, SQ.ref[e])) continue;
             const float4 xx = SQ.x[e];
             // light * bsdf was formed when the sample was drawn; times 1 is exact
             const LSample x{V3{xx.x, xx.y, xx.z}, v3(1.0f), xx.w};
@@ -1247,10 +1272,21 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, int64_
 // sizes the work buffer).  Everything after it is a fixed launch sequence
 // with device-side queue counts; with `graph` it is captured once into a
 // hipGraph and launched as one graph.
-static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStream_t st, bool graph) {
-    const int nj = (int)hj.size();
+// Primary-vertex light samples per pixel of a job (spp * P_Direct).
+static int direct_count(const HostJob &h) {
+    return h.args ? std::max(0, (int)(float(h.args->spp) * h.args->p_direct)) : 0;
+}
+
+static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipStream_t st, bool graph) {
+    const int nj = (int)hj_in.size();
     if (nj <= 0) return RM_OK;
     if (nj > (1 << 23)) return fail(RM_ERR_INVALID, "too many jobs");
+    // Jobs in order of falling light samples per pixel, so that each chunk of
+    // the split primary-vertex pass covers a prefix of the pixels.  Every job's
+    // result is independent of the order.
+    std::vector<HostJob> hj(hj_in);
+    std::stable_sort(hj.begin(), hj.end(),
+                     [](const HostJob &a, const HostJob &b) { return direct_count(a) > direct_count(b); });
     std::vector<Job> J(nj);
     int R = 0, D = 1;
     int64_t all_px = 0;
@@ -1286,7 +1322,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     // counts[d]: paths entering depth d+1; counts[NCOUNT + d]: shading claims of
     // depth d; counts[2 NCOUNT + d]: traversal claims of depth d;
     // counts[3 NCOUNT]: NEE records of the round, counts[3 NCOUNT + 1..3]: claims of
-    // the NEE kernels (one pass, or generate / shadow / add)
+    // the NEE kernels (one pass, or generate / shadow / add), counts[3 NCOUNT + 4]:
+    // queued shadow rays; counts[3 NCOUNT + 8, 9]: the same for each chunk of
+    // the primary-vertex pass
     uint32_t *counts = carve<uint32_t>(cur, 4 * NCOUNT);
     // Trace variant 3 keeps the first pending entries of every walk in LDS:
     // each kernel of the launch that traverses gets nl entries per lane (at
@@ -1327,22 +1365,37 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     int max_direct = 0;
     for (int i = 0; i < nj; ++i) max_direct = std::max(max_direct, (int)(float(J[i].A.spp) * J[i].A.p_direct));
     const int dC = (int)std::max<int64_t>(1, std::min<int64_t>(32, ((int64_t)16 << 20) / std::max<int64_t>(1, total_px)));
-    const int dchunks = (max_direct + dC - 1) / dC;
-    const bool split = tuning().nee_split != 0 && total * NEE_MAX <= 0xffffffffLL && total_px * dC <= 0xffffffffLL;
+    // split: 1 queues every shadow ray; 2 queues those of jobs whose tree has
+    // SPLIT_DEPTH levels or more and traces the others where they are drawn
+    // (on shallow trees a walk costs less than queueing it), and keeps the
+    // single-kernel passes when no job is that deep
+    constexpr int SPLIT_DEPTH = 12;
+    const int split_mode = tuning().nee_split;
+    const bool split = (split_mode == 1 || (split_mode == 2 && max_depth >= SPLIT_DEPTH)) &&
+                       total * NEE_MAX <= 0xffffffffLL && total_px * dC <= 0xffffffffLL;
+    const int queue_depth = split_mode == 1 ? 0 : SPLIT_DEPTH;
     const int64_t ndirect = split ? total_px * dC : 0;
     const int64_t nshadow = split ? std::max<int64_t>(total * NEE_MAX, ndirect) : 0;
     const size_t set_bytes = pad(total * 4) + 5 * pad(total * 16) + 24 * pad(total * 16) + pad(total * 8) +
-                             9 * pad(total * 16) + 3 * pad(nshadow * 16) + pad(nshadow) + pad(ndirect * 16);
+                             9 * pad(total * 16) + 3 * pad(nshadow * 16) + pad(nshadow * 4) + pad(nshadow) +
+                             pad(ndirect * 16);
     const size_t need = pix_bytes + set_bytes;
-    const bool oversized = work.bytes > 2 * need + ((size_t)1 << 30);
-    work.slack_calls = oversized ? work.slack_calls + 1 : 0;
-    if (work.bytes < need || work.slack_calls >= 8) {
-        // grow, or give back a workspace far larger than the last 8 launches
-        // needed (a kept workspace would otherwise hold its largest launch's
-        // size for good, and one small call between large ones must not cost
-        // a reallocation); the buffer holds the primary pass's results, so
-        // start over
-        int32_t rc = work.alloc(need);
+    // Grow when short.  Give memory back only when the buffer exceeds twice
+    // the largest need of the last 64 launches (a kept workspace would
+    // otherwise hold its largest launch's size for good, while jobs of mixed
+    // sizes must not reallocate back and forth).  The buffer holds the
+    // primary pass's results, so a reallocation starts over.
+    work.window_max = std::max(work.window_max, need);
+    size_t target = 0;
+    if (work.bytes < need) {
+        target = need;
+    } else if (++work.window_calls >= 64) {
+        if (work.bytes > 2 * work.window_max + ((size_t)1 << 30)) target = work.window_max;
+        work.window_max = 0;
+        work.window_calls = 0;
+    }
+    if (target) {
+        int32_t rc = work.alloc(target);
         if (rc) return rc;
         return render_jobs(hj, work, st, graph);
     }
@@ -1366,6 +1419,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
     SQ.o = carve<float4>(cur, nshadow);
     SQ.d = carve<float4>(cur, nshadow);
     SQ.x = carve<float4>(cur, nshadow);
+    SQ.ref = carve<uint32_t>(cur, nshadow);
     SQ.vis = carve<uint8_t>(cur, nshadow);
     SQ.x2 = carve<float4>(cur, ndirect);
     hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum, P.set_off, P.set_pix);
@@ -1382,17 +1436,28 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
                            : (tps == 1 ? k_shadow<false, 1> : tps == 2 ? k_shadow<false, 2>
                               : tps == 4 ? k_shadow<false, 4> : k_shadow<false, 0>);
     if (split) {
-        uint32_t *claims = counts + 3 * NCOUNT + 8;  // zeroed before each chunk's shadow pass
-        const unsigned gd = (unsigned)((ndirect + RB - 1) / RB);
-        auto direct_gen = capped ? k_direct_gen<4> : k_direct_gen<1>;
-        for (int c = 0; c < dchunks; ++c) {
-            hipLaunchKernelGGL(direct_gen, dim3(gd), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, dC, c, SQ);
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, claims, 1);
-            hipLaunchKernelGGL(shadow, dim3(gpers), dim3(RB), walk_lds, st, (const Job *)dJ, SQ, (const uint32_t *)nullptr,
-                               0u, (uint32_t)ndirect, claims, fetch_min, nl);
-            hipLaunchKernelGGL(k_direct_add, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, dC, c, SQ);
+        uint32_t *claims = counts + 3 * NCOUNT + 8;  // and [9]: queued rays; both zeroed for each chunk
+        auto direct_gen = capped ? (tps ? k_direct_gen<4, 2> : k_direct_gen<4, 0>)
+                                 : (tps ? k_direct_gen<1, 2> : k_direct_gen<1, 0>);
+        // samples [t0, t0 + C) of the pixels [0, npx) whose jobs have more than
+        // t0 samples (jobs are in order of falling sample counts)
+        for (int t0 = 0; t0 < max_direct;) {
+            int64_t npx = total_px;
+            for (int i = 0; i < nj; ++i)
+                if (direct_count(hj[i]) <= t0) { npx = J[i].pix_off; break; }
+            const int C = (int)std::max<int64_t>(
+                1, std::min<int64_t>({(int64_t)max_direct - t0, ndirect / std::max<int64_t>(1, npx)}));
+            const int64_t n = npx * C;
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, claims, 2);
+            hipLaunchKernelGGL(direct_gen, dim3((unsigned)((n + RB - 1) / RB)), dim3(RB), walk_lds, st,
+                               (const Job *)dJ, nj, npx, P, C, t0, SQ, queue_depth, claims + 1);
+            hipLaunchKernelGGL(shadow, dim3(gpers), dim3(RB), walk_lds, st, (const Job *)dJ, SQ,
+                               (const uint32_t *)(claims + 1), claims, fetch_min, nl);
+            hipLaunchKernelGGL(k_direct_add, dim3((unsigned)((npx + RB - 1) / RB)), dim3(RB), 0, st, (const Job *)dJ, nj,
+                               npx, P, C, t0, SQ);
+            t0 += C;
         }
-        hipLaunchKernelGGL(k_direct_final, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, dchunks);
+        hipLaunchKernelGGL(k_direct_final, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px);
     } else {
         auto direct = capped ? (tps ? k_direct<4, 2> : k_direct<4, 0>) : (tps ? k_direct<1, 2> : k_direct<1, 0>);
         hipLaunchKernelGGL(direct, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
@@ -1405,7 +1470,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
         const unsigned ge = gpers;
         auto bounce = capped ? k_bounce<4> : k_bounce<1>;
         auto nee = capped ? (tps ? k_nee<4, 2> : k_nee<4, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
-        auto nee_gen = capped ? k_nee_gen<4> : k_nee_gen<1>;
+        auto nee_gen = capped ? (tps ? k_nee_gen<4, 2> : k_nee_gen<4, 0>) : (tps ? k_nee_gen<1, 2> : k_nee_gen<1, 0>);
         auto nee_add = capped ? k_nee_add<4> : k_nee_add<1>;
         auto extend_lds = tps == 1 ? k_extend<true, 1> : tps == 2 ? k_extend<true, 2>
                         : tps == 4 ? k_extend<true, 4> : k_extend<true, 0>;
@@ -1428,10 +1493,10 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
             }
             const uint32_t *n_nee = counts + 3 * NCOUNT;
             if (split) {
-                hipLaunchKernelGGL(nee_gen, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, NQ, n_nee,
-                                   counts + 3 * NCOUNT + 1, SQ);
-                hipLaunchKernelGGL(shadow, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ, SQ, n_nee,
-                                   (uint32_t)NEE_MAX, 0u, counts + 3 * NCOUNT + 2, fetch_min, nl);
+                hipLaunchKernelGGL(nee_gen, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, NQ, n_nee,
+                                   counts + 3 * NCOUNT + 1, SQ, queue_depth, counts + 3 * NCOUNT + 4);
+                hipLaunchKernelGGL(shadow, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ, SQ,
+                                   (const uint32_t *)(counts + 3 * NCOUNT + 4), counts + 3 * NCOUNT + 2, fetch_min, nl);
                 hipLaunchKernelGGL(nee_add, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, P, AB, NQ, n_nee,
                                    counts + 3 * NCOUNT + 3, SQ);
             } else {
@@ -1483,10 +1548,35 @@ static int32_t render_jobs(const std::vector<HostJob> &hj, DevBuf &work, hipStre
 
 using namespace rm;
 
+// Integrator work buffers (path queues, accumulator sets, shadow queues) of
+// single renders: one per (device, stream), reused by every render queued on
+// that stream, whose order makes the reuse safe; kept until the process ends
+// (render_jobs gives memory back when it stays oversized).  Renders on
+// different streams use different buffers and do not wait for each other.
+struct StreamWork {
+    std::mutex mu;
+    DevBuf buf;
+};
+static int32_t stream_work(hipStream_t st, StreamWork **out) {
+    int dev = 0;
+    RM_HIP(hipGetDevice(&dev));
+    static std::mutex mu;
+    static auto *table = new std::map<std::pair<int, uintptr_t>, StreamWork *>();
+    std::lock_guard<std::mutex> lock(mu);
+    StreamWork *&w = (*table)[{dev, (uintptr_t)st}];
+    if (!w) w = new StreamWork();
+    *out = w;
+    return RM_OK;
+}
+
 extern "C" int32_t rm_render(const rm_scene *scene, const rm_render_args *a, rm_frame *f, void *stream) {
     if (!f) return fail(RM_ERR_INVALID, "rm_render: frame is NULL");
     std::vector<HostJob> hj{HostJob{scene, a, f, 0}};
-    return render_jobs(hj, f->work, (hipStream_t)stream, false);
+    StreamWork *w = nullptr;
+    int32_t rc = stream_work((hipStream_t)stream, &w);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(w->mu);
+    return render_jobs(hj, w->buf, (hipStream_t)stream, false);
 }
 
 extern "C" int32_t rm_render_batch(const rm_batch_job *jobs, int32_t n, int32_t use_graph, void *stream) {

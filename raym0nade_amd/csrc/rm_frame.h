@@ -9,9 +9,6 @@ inline int frame_channels(int which) { return (which >= RM_BUF_DD && which <= RM
 struct rm_frame {
     int32_t w = 0, h = 0;
     rm::DevBuf buf[RM_NUM_BUFFERS];
-    // integrator scratch (path queues, per-pixel accumulators), grown on demand
-    rm::DevBuf work;
-    size_t work_bytes = 0;
     rm::DevBuf post;  // tone-mapped image read by FXAA
     rm::DevBuf drop;  // per pixel: the dropped outlier light sample (render -> adjoint)
     rm::DevBuf adj;   // adjoint scratch: texel gradients of every mip level, material gradients

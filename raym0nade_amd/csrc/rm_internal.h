@@ -28,7 +28,8 @@ int32_t ensure_device();
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
-    int slack_calls = 0;  // consecutive uses that needed far less (see render_jobs)
+    size_t window_max = 0;  // largest need of the current window of uses (see render_jobs)
+    int window_calls = 0;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -40,7 +41,8 @@ struct DevBuf {
     }
     int32_t alloc(size_t n) {
         release();
-        slack_calls = 0;
+        window_max = 0;
+        window_calls = 0;
         if (n == 0) n = 16;
         hipError_t e = hipMalloc(&p, n);
         if (e != hipSuccess) { p = nullptr; return hip_fail(e, "hipMalloc"); }
@@ -97,7 +99,7 @@ struct Tuning {
     int render_min_waves = 4;     // 1 or 4: register cap of the shading and NEE kernels
     int lds_stack_entries = 8;    // trace variant 3: walk-stack entries per lane kept in LDS
     int tris_per_step = 2;        // trace variant 3: triangles tested per step (0: a whole leaf)
-    int nee_split = 1;            // render: shadow rays of light samples traced in their own passes
+    int nee_split = 2;            // render: shadow rays of light samples in their own passes (2: deep trees)
 };
 Tuning &tuning();
 int device_cu_count();

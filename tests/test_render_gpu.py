@@ -154,14 +154,15 @@ def test_full_size_cornell_properties(cornell):
 
 
 @pytest.mark.parametrize("scene", ["cornell", "spheres"])
-@pytest.mark.parametrize("entries,tps", [(32, 0), (2, 0), (8, 1), (2, 2), (8, 4)])
-def test_lds_stack_walks_are_bit_identical(dev, scene, entries, tps):
+@pytest.mark.parametrize("entries,tps,split", [(32, 0, 1), (2, 0, 1), (8, 1, 1), (2, 2, 2), (8, 4, 2), (8, 2, 0)])
+def test_lds_stack_walks_are_bit_identical(dev, scene, entries, tps, split):
     # every walk of a render launch (primary, direct, extension, NEE) with its
     # stack in LDS (trace variant 3), wholly or split with scratch, and with a
     # leaf's triangles tested over several steps, visits the same nodes and
     # triangles in the same order as the scratch-stack, whole-leaf walker; the
-    # NEE pass split into generate / shadow / add gives the same sets as the
-    # single NEE kernel
+    # light-sample passes split into generate / shadow / add (every shadow ray
+    # queued, or only those of deep trees) give the same sums as the single
+    # kernels
     g = rm.generate(0, 1) if scene == "cornell" else rm.generate(2, 3, grid=2, freq=12, sky_width=256, sky_height=128)
     S = rm.Scene(g)
     a = small_args(g, 80, 48, 12, max_depth=16)
@@ -172,7 +173,7 @@ def test_lds_stack_walks_are_bit_identical(dev, scene, entries, tps):
         for k, v in zip(keys, (2, 32, 0, 0)):
             rm.set_tuning(k, v)
         gb2, r2 = RC.gpu_render(S, a)
-        for k, v in zip(keys, (3, entries, tps, 1)):
+        for k, v in zip(keys, (3, entries, tps, split)):
             rm.set_tuning(k, v)
         gb3, r3 = RC.gpu_render(S, a)
     finally:
