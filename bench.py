@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
     ap.add_argument("--adam-steps", type=int, default=0, help="also run an inverse-rendering Adam loop of n steps")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the denoise leg (on the 1080p render)")
+    ap.add_argument("--denoise-reps", type=int, default=5, help="timed denoise runs (after one warm-up)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -396,6 +398,65 @@ def adjoint_leg(args, torch, rm):
     return out
 
 
+# Algorithmic HBM bytes per pixel of the denoise path (four float4 radiance
+# buffers; G-buffer position, shape normal, surface normal, material as 3
+# floats each):
+#   spatialClamp   rows: read rgb+Var 16, write blur 4; cols: read blur 4,
+#                  read rgb+Var 16 (writes only for clamped pixels)  x 4 buffers
+#   filterVar      read 16, write 16                                x 4 buffers
+#   a-trous pass   read G-buffer 48 once + (read 16 + write 16)     x 4 buffers, x 5 passes
+DENOISE_BYTES_PER_PX = 4 * (16 + 4 + 4 + 16) + 4 * (16 + 16) + 5 * (48 + 4 * 32)
+
+
+def denoise_leg(args, torch, rm, fr, stream):
+    """rm_spatial_clamp + rm_filter (Photo::spatialClamp + Photo::filter) on
+    the rendered 1080p frame, its radiance restored before every run; HIP
+    events on the launch stream bracket each call."""
+    from raym0nade_amd import render_check as RC
+    saved = [fr.download(b) for b in RC.RAD_ORDER]
+
+    def restore():
+        for b, r in zip(RC.RAD_ORDER, saved):
+            fr.upload(b, r)
+
+    clamp_ms, filt_ms = [], []
+    for k in range(args.denoise_reps + 1):
+        restore()
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        fr.spatial_clamp(stream=stream)
+        e1.record(stream)
+        fr.filter(stream=stream)
+        e2.record(stream)
+        torch.cuda.synchronize()
+        if k > 0:
+            clamp_ms.append(e0.elapsed_time(e1))
+            filt_ms.append(e1.elapsed_time(e2))
+    c, f = float(np.median(clamp_ms)), float(np.median(filt_ms))
+    n = fr.width * fr.height
+    out = {"mpix_per_s": round(n / ((c + f) * 1e-3) / 1e6, 1), "ms": round(c + f, 3), "clamp_ms": round(c, 3),
+           "filter_ms": round(f, 3), "bytes_per_px": DENOISE_BYTES_PER_PX,
+           "algorithmic_gbs": round(n * DENOISE_BYTES_PER_PX / ((c + f) * 1e-3) / 1e9, 1), "runs": len(filt_ms)}
+    log(f"[bench] denoise 1080p: clamp {c:.3f} ms, filter {f:.3f} ms")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        restore()
+        gb, rad = RC.download_all(fr)
+        h0, w0 = fr.height // 4, fr.width // 4
+        gb, rad = gb[h0:h0 + fr.height // 2, w0:w0 + fr.width // 2], rad[:, h0:h0 + fr.height // 2, w0:w0 + fr.width // 2]
+        c0 = time.perf_counter()
+        O.denoise(gb, rad)
+        cdt = time.perf_counter() - c0
+        m = gb.shape[0] * gb.shape[1]
+        out["cpu_baseline"] = {
+            "value": round(m / cdt / 1e6, 4), "unit": "Mpixel/s", "cores": 4, "kind": "port",
+            "sample": f"the centre {gb.shape[1]}x{gb.shape[0]} crop of the same frame, {cdt:.1f} s; oracle "
+                      f"restatement of spatialClamp + filter, one std::thread per buffer as the reference runs it"}
+        out["gpu_over_cpu"] = round(out["mpix_per_s"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def render_leg(args, torch, rm, dev, stream):
     """Path samples/s of the 1080p workload: 64 displaced geodesic spheres
     (10,368,002 triangles) under a 2048x1024 fp32 sky, 1920x1080, 256 spp,
@@ -432,6 +493,8 @@ def render_leg(args, torch, rm, dev, stream):
     out = {"path_samples_per_s": round(samples / wall, 1), "frame_s": round(wall, 4), "frames": len(times),
            "width": a.width, "height": a.height, "spp": a.spp, "max_depth": a.max_depth, "n_tris": S.info.n_tris,
            "autotune": tuned}
+    if not args.no_denoise:
+        out["denoise"] = denoise_leg(args, torch, rm, fr, stream)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O

@@ -179,7 +179,10 @@ enum rm_buffer {
     RM_BUF_DS = 6,
     RM_BUF_ID = 7,
     RM_BUF_IS = 8,
-    RM_BUF_PIXELS = 9          /* last post-processed image           [h][w][3] */
+    RM_BUF_PIXELS = 9,         /* last post-processed image           [h][w][3] */
+    RM_BUF_MATERIAL = 10       /* G-buffer (metallic, roughness, opacity) of
+                                  the primary hit; HitInfo's defaults
+                                  (0, 0.8, 1) on a miss               [h][w][3] */
 };
 
 int32_t rm_frame_create(int32_t width, int32_t height, rm_frame **out);
@@ -221,6 +224,25 @@ int32_t rm_postprocess(rm_frame *frame, int32_t shade_options, float exposure, v
 /* Device-to-device copy of one frame buffer into `dst` (device memory of
  * width*height*channels floats), stream-ordered. */
 int32_t rm_frame_copy(const rm_frame *frame, int32_t which, float *dst, void *stream);
+
+/* Synchronous copy of host memory into one frame buffer (width*height*channels
+ * floats): a caller that produced its own G-buffer and radiance (the
+ * reference's Photo arrays) hands them to the denoiser this way. */
+int32_t rm_frame_upload(rm_frame *frame, int32_t which, const float *host);
+
+/* Photo::spatialClamp (src/image.cpp:30-82): in place on the four radiance
+ * buffers, each pixel brighter than 36x its blurred neighbourhood luminance
+ * scaled down to it.  Stream-ordered; launches only. */
+int32_t rm_spatial_clamp(rm_frame *frame, void *stream);
+
+/* Photo::filter (src/image.cpp:84-213): in place on the four radiance
+ * buffers (rgb and Var), filterVar then the five edge-aware a-trous passes at
+ * strides 1..16, guided by the G-buffer (position, normals, material); the
+ * specular buffers (Ds, Is) are weighted by the centre's roughness.  Pixels
+ * whose primary ray missed end at radiance 0, Var 0, as in the reference.
+ * The first call on a frame allocates its ping-pong buffer (64 bytes per
+ * pixel); later calls allocate nothing.  Stream-ordered. */
+int32_t rm_filter(rm_frame *frame, void *stream);
 
 /* ------------------------------------------------------- differentiation */
 

@@ -288,12 +288,41 @@ RenderArgs to_args(const rm_render_args *a) {
 }
 void put3(float *p, const vec3 &v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 vec3 get3(const float *p) { return vec3(p[0], p[1], p[2]); }
+
+// One G-buffer row: position, shapeNormal, surfaceNormal, emission,
+// baseColor, then (metallic, roughness, opacity).
+constexpr int GB = 18;
+void put_gbuf(float *g, const HitInfo &G) {
+    put3(g + 0, G.position);
+    put3(g + 3, G.shapeNormal);
+    put3(g + 6, G.surfaceNormal);
+    put3(g + 9, G.emission);
+    put3(g + 12, G.baseColor);
+    g[15] = G.metallic;
+    g[16] = G.roughness;
+    g[17] = G.opacity;
+}
+void get_gbuf(const float *g, HitInfo &G) {
+    G.position = get3(g);
+    G.shapeNormal = get3(g + 3);
+    G.surfaceNormal = get3(g + 6);
+    G.emission = get3(g + 9);
+    G.baseColor = get3(g + 12);
+    G.metallic = g[15];
+    G.roughness = g[16];
+    G.opacity = g[17];
+}
+std::vector<RadianceData> *photo_bufs(Photo &p, int b) {
+    std::vector<RadianceData> *bufs[4] = {&p.radiance_Dd, &p.radiance_Ds, &p.radiance_Id, &p.radiance_Is};
+    return bufs[b];
+}
 }  // namespace
 
 extern "C" {
 
 // Renders the rows listed in `rows` (all rows if NULL) with `threads` workers.
-// gbuf: [h][w][15] = position, shapeNormal, surfaceNormal, emission, baseColor.
+// gbuf: [h][w][18] = position, shapeNormal, surfaceNormal, emission,
+// baseColor, (metallic, roughness, opacity).
 // rad:  [4][h][w][4] = Dd, Ds, Id, Is as (rgb, Var).  Rows not rendered keep
 // the defaults of a fresh Photo.
 int32_t oracle_render(const oracle_scene *S, const rm_render_args *a, int32_t threads, const int32_t *rows,
@@ -308,30 +337,20 @@ int32_t oracle_render(const oracle_scene *S, const rm_render_args *a, int32_t th
     render_multiThread(S->model, args, photo, threads, rows ? &rs : nullptr);
     const size_t np = (size_t)args.width * args.height;
     if (gbuf)
-        for (size_t i = 0; i < np; ++i) {
-            const HitInfo &G = photo.Gbuffer[i];
-            put3(gbuf + i * 15 + 0, G.position);
-            put3(gbuf + i * 15 + 3, G.shapeNormal);
-            put3(gbuf + i * 15 + 6, G.surfaceNormal);
-            put3(gbuf + i * 15 + 9, G.emission);
-            put3(gbuf + i * 15 + 12, G.baseColor);
-        }
-    if (rad) {
-        const std::vector<RadianceData> *bufs[4] = {&photo.radiance_Dd, &photo.radiance_Ds, &photo.radiance_Id,
-                                                    &photo.radiance_Is};
+        for (size_t i = 0; i < np; ++i) put_gbuf(gbuf + i * GB, photo.Gbuffer[i]);
+    if (rad)
         for (int b = 0; b < 4; ++b)
             for (size_t i = 0; i < np; ++i) {
-                const RadianceData &R = (*bufs[b])[i];
+                const RadianceData &R = (*photo_bufs(photo, b))[i];
                 float *p = rad + ((size_t)b * np + i) * 4;
                 put3(p, R.radiance);
                 p[3] = R.Var;
             }
-    }
     return 0;
 }
 
 // Renders only the listed pixels (index y*width+x); outputs are compact:
-// gbuf [n][15], rad [4][n][4] in list order.
+// gbuf [n][18], rad [4][n][4] in list order.
 int32_t oracle_render_pixels(const oracle_scene *S, const rm_render_args *a, int32_t threads, const int32_t *pix,
                              int32_t n, float *gbuf, float *rad) {
     if (!S || !a || a->width <= 0 || a->height <= 0 || a->max_depth > 16 || !pix) return 1;
@@ -341,50 +360,92 @@ int32_t oracle_render_pixels(const oracle_scene *S, const rm_render_args *a, int
     std::vector<int> list(pix, pix + n);
     render_pixels(S->model, args, photo, threads, list);
     for (int k = 0; k < n; ++k) {
-        const HitInfo &G = photo.Gbuffer[list[k]];
-        if (gbuf) {
-            put3(gbuf + (size_t)k * 15 + 0, G.position);
-            put3(gbuf + (size_t)k * 15 + 3, G.shapeNormal);
-            put3(gbuf + (size_t)k * 15 + 6, G.surfaceNormal);
-            put3(gbuf + (size_t)k * 15 + 9, G.emission);
-            put3(gbuf + (size_t)k * 15 + 12, G.baseColor);
-        }
-        if (rad) {
-            const RadianceData *R[4] = {&photo.radiance_Dd[list[k]], &photo.radiance_Ds[list[k]],
-                                        &photo.radiance_Id[list[k]], &photo.radiance_Is[list[k]]};
+        if (gbuf) put_gbuf(gbuf + (size_t)k * GB, photo.Gbuffer[list[k]]);
+        if (rad)
             for (int b = 0; b < 4; ++b) {
+                const RadianceData &R = (*photo_bufs(photo, b))[list[k]];
                 float *p = rad + ((size_t)b * n + k) * 4;
-                put3(p, R[b]->radiance);
-                p[3] = R[b]->Var;
+                put3(p, R.radiance);
+                p[3] = R.Var;
             }
-        }
     }
     return 0;
 }
+
+namespace {
+void load_photo(Photo &photo, const float *gbuf, const float *rad) {
+    const size_t np = (size_t)photo.width * photo.height;
+    for (size_t i = 0; i < np; ++i) get_gbuf(gbuf + i * GB, photo.Gbuffer[i]);
+    for (int b = 0; b < 4; ++b)
+        for (size_t i = 0; i < np; ++i) {
+            const float *p = rad + ((size_t)b * np + i) * 4;
+            (*photo_bufs(photo, b))[i].radiance = get3(p);
+            (*photo_bufs(photo, b))[i].Var = p[3];
+        }
+}
+}  // namespace
 
 // Photo::postProcessing on given buffers (same layouts as oracle_render).
 void oracle_postprocess(int32_t w, int32_t h, const float *gbuf, const float *rad, int32_t options, float exposure,
                         float *pixels) {
     Photo photo(w, h);
     photo.exposure = exposure;
+    load_photo(photo, gbuf, rad);
+    photo.postProcessing(options);
     const size_t np = (size_t)w * h;
-    for (size_t i = 0; i < np; ++i) {
-        HitInfo &G = photo.Gbuffer[i];
-        G.position = get3(gbuf + i * 15);
-        G.shapeNormal = get3(gbuf + i * 15 + 3);
-        G.surfaceNormal = get3(gbuf + i * 15 + 6);
-        G.emission = get3(gbuf + i * 15 + 9);
-        G.baseColor = get3(gbuf + i * 15 + 12);
-    }
-    std::vector<RadianceData> *bufs[4] = {&photo.radiance_Dd, &photo.radiance_Ds, &photo.radiance_Id, &photo.radiance_Is};
+    for (size_t i = 0; i < np; ++i) put3(pixels + i * 3, photo.pixelarray[i]);
+}
+
+// The denoise path on given buffers (layouts as oracle_render), in place on
+// rad: steps & 1 runs Photo::spatialClamp, steps & 2 then Photo::filter
+// (four threads, one per buffer, as the reference runs it); steps & 4 runs
+// filterVar alone on each buffer.
+void oracle_denoise(int32_t w, int32_t h, const float *gbuf, float *rad, int32_t steps) {
+    Photo photo(w, h);
+    load_photo(photo, gbuf, rad);
+    if (steps & 1) photo.spatialClamp();
+    if (steps & 2) photo.filter();
+    if (steps & 4)
+        for (int b = 0; b < 4; ++b) filter_var(*photo_bufs(photo, b), w, h);
+    const size_t np = (size_t)w * h;
     for (int b = 0; b < 4; ++b)
         for (size_t i = 0; i < np; ++i) {
-            const float *p = rad + ((size_t)b * np + i) * 4;
-            (*bufs[b])[i].radiance = get3(p);
-            (*bufs[b])[i].Var = p[3];
+            float *p = rad + ((size_t)b * np + i) * 4;
+            put3(p, (*photo_bufs(photo, b))[i].radiance);
+            p[3] = (*photo_bufs(photo, b))[i].Var;
         }
-    photo.postProcessing(options);
-    for (size_t i = 0; i < np; ++i) put3(pixels + i * 3, photo.pixelarray[i]);
+}
+
+// One a-trous pass (src/image.cpp:158-189) of one buffer, in place on rad
+// [h][w][4], for the pass-level tests.
+void oracle_filter_pass(int32_t w, int32_t h, const float *gbuf, float *rad, int32_t specular, int32_t step) {
+    const size_t np = (size_t)w * h;
+    std::vector<HitInfo> G(np);
+    std::vector<RadianceData> R(np);
+    for (size_t i = 0; i < np; ++i) {
+        get_gbuf(gbuf + i * GB, G[i]);
+        R[i].radiance = get3(rad + i * 4);
+        R[i].Var = rad[i * 4 + 3];
+    }
+    filter_pass(R, G, w, h, specular != 0, step);
+    for (size_t i = 0; i < np; ++i) {
+        put3(rad + i * 4, R[i].radiance);
+        rad[i * 4 + 3] = R[i].Var;
+    }
+}
+
+// getWeight (src/image.cpp:109-156) of one neighbour: g* are G-buffer rows,
+// l* are (rgb, Var).
+float oracle_denoise_weight(const float *gp, const float *gq, const float *lp, const float *lq, int32_t specular) {
+    HitInfo Gp, Gq;
+    get_gbuf(gp, Gp);
+    get_gbuf(gq, Gq);
+    RadianceData Lp, Lq;
+    Lp.radiance = get3(lp);
+    Lp.Var = lp[3];
+    Lq.radiance = get3(lq);
+    Lq.Var = lq[3];
+    return denoise_weight(Gp, Gq, Lp, Lq, specular != 0);
 }
 
 // accumulateInwardRadiance on one sample, for its invariant tests.

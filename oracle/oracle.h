@@ -225,6 +225,8 @@ struct Photo {                                          // include/image.h:15-67
     void FXAA();                                        // src/image.cpp:363-452
     void gammaCorrection();                             // src/image.cpp:456-468
     void postProcessing(int shadeOptions);              // src/image.cpp:470-479 (no bloom / DoF)
+    void spatialClamp();                                // src/image.cpp:77-82   (oimage.cpp)
+    void filter();                                      // src/image.cpp:200-213 (oimage.cpp)
 };
 
 void calc_dPdxy(const Ray &ray, float hit_t, const vec3 &normal, const RayDifferential &base_diff, vec3 &hit_dPdx,
@@ -248,5 +250,16 @@ int adjoint_render(const Model &model, const RenderArgs &args, const AdjDirectio
 void render_multiThread(const Model &model, const RenderArgs &args, Photo &photo, int threads,
                         const std::vector<int> *rows = nullptr);                            // src/render.cpp:593-626
 void render_pixels(const Model &model, const RenderArgs &args, Photo &photo, int threads, const std::vector<int> &pix);
+// Denoise (oimage.cpp).  The reference's unqualified pow/exp/sqrt/abs on
+// floats resolve to the float overloads: model.h pulls in Python.h, whose
+// <math.h> is libstdc++'s wrapper that brings std's overloads into the
+// global namespace.
+void filter_var(std::vector<RadianceData> &rad, int width, int height);                     // src/image.cpp:84-107
+float denoise_weight(const HitInfo &Gp, const HitInfo &Gq, const RadianceData &Lp, const RadianceData &Lq,
+                     bool specular);                                                            // src/image.cpp:109-156
+void filter_pass(std::vector<RadianceData> &rad, const std::vector<HitInfo> &G, int width, int height, bool specular,
+                 int step);                                                                     // src/image.cpp:158-189
+void filter_radiance(std::vector<RadianceData> &rad, const std::vector<HitInfo> &G, int width, int height,
+                     bool specular);                                                            // src/image.cpp:191-198
 
 }  // namespace ref

@@ -23,6 +23,10 @@ f64p = C.POINTER(C.c_double)
 
 _lib = None
 
+# G-buffer row: position, shapeNormal, surfaceNormal, emission, baseColor,
+# (metallic, roughness, opacity)
+GB = 18
+
 
 def build() -> str:
     r = subprocess.run(["make", "-C", HERE, "-B", "liboracle.so"], capture_output=True, text=True)
@@ -61,6 +65,9 @@ def lib():
             "oracle_bsdf_eval": (None, [f32p, f32p, C.c_int32, f32p]),
             "oracle_bsdf_sample": (None, [f32p, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, f32p]),
             "oracle_adjoint_dir": (C.c_int32, [vp, vp, f32p, f64p, f64p, C.c_int32, f64p, f64p]),
+            "oracle_denoise": (None, [C.c_int32, C.c_int32, f32p, f32p, C.c_int32]),
+            "oracle_filter_pass": (None, [C.c_int32, C.c_int32, f32p, f32p, C.c_int32, C.c_int32]),
+            "oracle_denoise_weight": (C.c_float, [f32p, f32p, f32p, f32p, C.c_int32]),
         }
         for k, (res, args) in sig.items():
             fn = getattr(L, k)
@@ -146,9 +153,9 @@ class OracleScene:
         return t, ids
 
     def render(self, args, rows=None, threads=0):
-        """Returns (gbuf [h][w][15], rad [4][h][w][4]) for RmRenderArgs `args`."""
+        """Returns (gbuf [h][w][GB], rad [4][h][w][4]) for RmRenderArgs `args`."""
         w, h = args.width, args.height
-        g = np.empty((h, w, 15), dtype=np.float32)
+        g = np.empty((h, w, GB), dtype=np.float32)
         r = np.empty((4, h, w, 4), dtype=np.float32)
         rr = None
         nr = 0
@@ -163,10 +170,10 @@ class OracleScene:
 
 
 def render_pixels(scene, args, pixels, threads=0):
-    """Renders only `pixels` (indices y*width+x); returns gbuf [n][15], rad [4][n][4]."""
+    """Renders only `pixels` (indices y*width+x); returns gbuf [n][GB], rad [4][n][4]."""
     pix = np.ascontiguousarray(pixels, dtype=np.int32)
     n = pix.size
-    g = np.empty((n, 15), dtype=np.float32)
+    g = np.empty((n, GB), dtype=np.float32)
     r = np.empty((4, n, 4), dtype=np.float32)
     st = lib().oracle_render_pixels(scene.h, C.byref(args), threads or lib().oracle_hw_threads(),
                                     pix.ctypes.data_as(i32p), n, fp(g), fp(r))
@@ -217,6 +224,28 @@ def postprocess(gbuf, rad, options, exposure=1.0):
     out = np.empty((h, w, 3), dtype=np.float32)
     lib().oracle_postprocess(w, h, fp(f32(gbuf)), fp(f32(rad)), options, exposure, fp(out))
     return out
+
+
+def denoise(gbuf, rad, clamp=True, filter=True, var_only=False):
+    """Photo::spatialClamp then Photo::filter on copies of gbuf [h][w][GB] and
+    rad [4][h][w][4]; returns the new rad.  var_only: filterVar alone."""
+    h, w = gbuf.shape[:2]
+    out = f32(rad).copy()
+    steps = 4 if var_only else (1 if clamp else 0) | (2 if filter else 0)
+    lib().oracle_denoise(w, h, fp(f32(gbuf)), fp(out), steps)
+    return out
+
+
+def filter_pass(gbuf, rad1, specular, step):
+    """One a-trous pass of one buffer rad1 [h][w][4]; returns the new buffer."""
+    h, w = gbuf.shape[:2]
+    out = f32(rad1).copy()
+    lib().oracle_filter_pass(w, h, fp(f32(gbuf)), fp(out), int(specular), step)
+    return out
+
+
+def denoise_weight(gp, gq, lp, lq, specular):
+    return lib().oracle_denoise_weight(fp(f32(gp)), fp(f32(gq)), fp(f32(lp)), fp(f32(lq)), int(specular))
 
 
 def hw_threads():

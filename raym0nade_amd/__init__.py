@@ -10,8 +10,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (RM_BUF_BASE_COLOR, RM_BUF_DD, RM_BUF_DS, RM_BUF_EMISSION, RM_BUF_ID, RM_BUF_IS, RM_BUF_PIXELS,
-                   RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RmError, RmGenOptions, RmRenderArgs,
+from ._lib import (RM_BUF_BASE_COLOR, RM_BUF_DD, RM_BUF_DS, RM_BUF_EMISSION, RM_BUF_ID, RM_BUF_IS, RM_BUF_MATERIAL,
+                   RM_BUF_PIXELS, RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RmError, RmGenOptions, RmRenderArgs,
                    RmSceneDesc, RmSceneInfo, check, lib)
 
 __all__ = ["GeneratedScene", "Scene", "Frame", "generate", "rays", "philox", "RmError", "RmRenderArgs"]
@@ -181,7 +181,8 @@ class Frame:
     """Device frame buffers (rm_frame_create)."""
 
     CHANNELS = {RM_BUF_POSITION: 3, RM_BUF_SHAPE_NORMAL: 3, RM_BUF_SURFACE_NORMAL: 3, RM_BUF_EMISSION: 3,
-                RM_BUF_BASE_COLOR: 3, RM_BUF_DD: 4, RM_BUF_DS: 4, RM_BUF_ID: 4, RM_BUF_IS: 4, RM_BUF_PIXELS: 3}
+                RM_BUF_BASE_COLOR: 3, RM_BUF_DD: 4, RM_BUF_DS: 4, RM_BUF_ID: 4, RM_BUF_IS: 4, RM_BUF_PIXELS: 3,
+                RM_BUF_MATERIAL: 3}
 
     def __init__(self, width: int, height: int):
         h = C.c_void_p()
@@ -198,6 +199,23 @@ class Frame:
 
     def postprocess(self, options: int, exposure: float = 1.0, stream=None):
         check(lib().rm_postprocess(self._h, options, exposure, _stream(stream)))
+
+    def spatial_clamp(self, stream=None):
+        """Photo::spatialClamp on the four radiance buffers, in place."""
+        check(lib().rm_spatial_clamp(self._h, _stream(stream)))
+
+    def filter(self, stream=None):
+        """Photo::filter (filterVar + five a-trous passes) on the four radiance
+        buffers, in place."""
+        check(lib().rm_filter(self._h, _stream(stream)))
+
+    def upload(self, which: int, data: np.ndarray):
+        """Host array [h][w][channels] float32 into one buffer (synchronous)."""
+        ch = self.CHANNELS[which]
+        a = np.ascontiguousarray(data, dtype=np.float32)
+        if a.size != self.width * self.height * ch:
+            raise ValueError(f"upload: expected {self.height}x{self.width}x{ch} floats, got shape {a.shape}")
+        check(lib().rm_frame_upload(self._h, which, a.ctypes.data_as(C.c_void_p)))
 
     def download(self, which: int) -> np.ndarray:
         ch = self.CHANNELS[which]

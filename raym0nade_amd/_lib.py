@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libraym0nade_hip.so")
 RM_OK, RM_ERR_INVALID, RM_ERR_OOM, RM_ERR_HIP, RM_ERR_EMPTY, RM_ERR_UNSUPPORTED, RM_ERR_NO_DEVICE = range(7)
 
 (RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RM_BUF_EMISSION, RM_BUF_BASE_COLOR,
- RM_BUF_DD, RM_BUF_DS, RM_BUF_ID, RM_BUF_IS, RM_BUF_PIXELS) = range(10)
+ RM_BUF_DD, RM_BUF_DS, RM_BUF_ID, RM_BUF_IS, RM_BUF_PIXELS, RM_BUF_MATERIAL) = range(11)
 
 f32p = C.POINTER(C.c_float)
 i32p = C.POINTER(C.c_int32)
@@ -144,6 +144,9 @@ SIGNATURES = [
     ("rm_postprocess", C.c_int32, [C.c_void_p, C.c_int32, C.c_float, C.c_void_p]),
     ("rm_render_batch", C.c_int32, [C.POINTER(RmBatchJob), C.c_int32, C.c_int32, C.c_void_p]),
     ("rm_frame_copy", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("rm_frame_upload", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
+    ("rm_spatial_clamp", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("rm_filter", C.c_int32, [C.c_void_p, C.c_void_p]),
     ("rm_frame_set_recording", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rm_scene_update", C.c_int32, [C.c_void_p, C.POINTER(RmMaterial), C.c_void_p, C.c_void_p]),
     ("rm_adjoint", C.c_int32, [C.c_void_p, C.POINTER(RmRenderArgs), C.c_void_p, C.c_void_p, C.POINTER(RmGrads),

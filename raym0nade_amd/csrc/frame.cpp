@@ -54,6 +54,17 @@ extern "C" int32_t rm_frame_copy(const rm_frame *f, int32_t which, float *dst, v
     return RM_OK;
 }
 
+extern "C" int32_t rm_frame_upload(rm_frame *f, int32_t which, const float *host) {
+    if (!f || !host || which < 0 || which >= RM_NUM_BUFFERS) return fail(RM_ERR_INVALID, "rm_frame_upload: bad argument");
+    RM_HIP(hipDeviceSynchronize());
+    RM_HIP(hipMemcpy(f->buf[which].p, host, (size_t)f->w * f->h * frame_channels(which) * sizeof(float),
+                     hipMemcpyHostToDevice));
+    // the frame no longer holds the render the adjoint would differentiate
+    f->stamped = false;
+    f->rec_valid = false;
+    return RM_OK;
+}
+
 extern "C" int32_t rm_frame_download(const rm_frame *f, int32_t which, float *host) {
     if (!f || !host || which < 0 || which >= RM_NUM_BUFFERS) return fail(RM_ERR_INVALID, "rm_frame_download: bad argument");
     RM_HIP(hipDeviceSynchronize());

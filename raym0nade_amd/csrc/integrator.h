@@ -39,6 +39,7 @@ inline RArgs rargs_from(const rm_render_args *a) {
 
 struct FrameView {
     float *pos, *shapeN, *surfN, *emis, *base, *Dd, *Ds, *Id, *Is;
+    float *mat;     // per pixel (metallic, roughness, opacity) of the primary hit
     int32_t *drop;  // per pixel: name (s * 8 + k) of the dropped indirect light sample, or -1
     float4 *rec;    // null, or per (pixel, indirect sample): sum dd | sum ds | sum w L
     int32_t rec_stride;

@@ -263,6 +263,7 @@ __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int n
         st3(F.surfN, p, vnan());
         st3(F.emis, p, sky_get(S, Dir));
         st3(F.base, p, v3(0.0f));
+        st3(F.mat, p, V3{0.0f, 0.8f, 1.0f});  // HitInfo's defaults (src/model.cpp:4-7)
         P.pi0[gp] = zero;
         P.pi1[gp] = zero;
         P.pi2[gp] = zero;
@@ -282,6 +283,7 @@ __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int n
     st3(F.surfN, p, s.surfN);
     st3(F.emis, p, s.emission);
     st3(F.base, p, s.base);  // the nudged colour stays when no indirect sample exists (:529-530)
+    st3(F.mat, p, V3{s.metallic, s.roughness, s.opacity});
     P.pi0[gp] = make_float4(s.base.x, s.base.y, s.base.z, s.roughness);
     P.pi1[gp] = make_float4(s.metallic, s.opacity, s.eta, ub((uint32_t)s.id | (s.entering ? (1u << 30) : 0u)));
     P.pi2[gp] = make_float4(sav.x, sav.y, sav.z, ub((uint32_t)n_ind));
@@ -1238,6 +1240,7 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, int64_
     F.surfN = f->buf[RM_BUF_SURFACE_NORMAL].as<float>();
     F.emis = f->buf[RM_BUF_EMISSION].as<float>();
     F.base = f->buf[RM_BUF_BASE_COLOR].as<float>();
+    F.mat = f->buf[RM_BUF_MATERIAL].as<float>();
     F.Dd = f->buf[RM_BUF_DD].as<float>();
     F.Ds = f->buf[RM_BUF_DS].as<float>();
     F.Id = f->buf[RM_BUF_ID].as<float>();

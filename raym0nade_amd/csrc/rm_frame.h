@@ -2,7 +2,7 @@
 #pragma once
 #include "rm_internal.h"
 
-constexpr int RM_NUM_BUFFERS = 10;
+constexpr int RM_NUM_BUFFERS = 11;
 
 inline int frame_channels(int which) { return (which >= RM_BUF_DD && which <= RM_BUF_IS) ? 4 : 3; }
 
@@ -10,6 +10,7 @@ struct rm_frame {
     int32_t w = 0, h = 0;
     rm::DevBuf buf[RM_NUM_BUFFERS];
     rm::DevBuf post;  // tone-mapped image read by FXAA
+    rm::DevBuf filt;  // denoise: the four radiance buffers' other half of the ping-pong
     rm::DevBuf drop;  // per pixel: the dropped outlier light sample (render -> adjoint)
     rm::DevBuf adj;   // adjoint scratch: texel gradients of every mip level, material gradients
     // per indirect sample (rm_frame_set_recording): sums over its light samples

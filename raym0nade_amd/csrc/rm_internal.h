@@ -10,6 +10,15 @@
 
 namespace rm {
 
+// Bijection blockIdx -> logical block over a grid of G blocks, giving each
+// XCD (the hardware deals blocks round-robin over the 8 XCDs) one contiguous
+// range, so that neighbouring work shares that XCD's L2.
+__host__ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t G) {
+    uint32_t q = G >> 3, r = G & 7u, x = b & 7u;
+    uint32_t start = x * q + (x < r ? x : r);
+    return start + (b >> 3);
+}
+
 // Records a failure for rm_last_error() and returns `code`.
 int32_t fail(int32_t code, const char *fmt, ...);
 // Maps a HIP error to RM_ERR_HIP / RM_ERR_OOM with context.

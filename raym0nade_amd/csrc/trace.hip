@@ -17,13 +17,6 @@ namespace rm {
 constexpr int TRACE_BLOCK = 256;
 static_assert(TRACE_BLOCK == WALK_BLOCK, "LDS walk stacks assume blocks of WALK_BLOCK lanes");
 
-// Bijection blockIdx -> logical block with contiguous ranges per XCD.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
-    uint32_t q = G >> 3, r = G & 7u, x = b & 7u;
-    uint32_t start = x * q + (x < r ? x : r);
-    return start + (b >> 3);
-}
-
 __device__ __forceinline__ void load_ray(const float *rays, int64_t i, V3 &o, V3 &d) {
     const float2 *p = reinterpret_cast<const float2 *>(rays + i * 6);
     float2 a = p[0], b = p[1], c = p[2];

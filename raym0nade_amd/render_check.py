@@ -1,15 +1,16 @@
 """Helpers that run the GPU renderer into host arrays laid out like the
-oracle's (G-buffer [h][w][15], radiance [4][h][w][4]) and compare the two.
+oracle's (G-buffer [h][w][18], radiance [4][h][w][4]) and compare the two.
 Used by the GPU tests and by the smoke check; the oracle itself is passed in
 by the caller, never imported here."""
 from __future__ import annotations
 
 import numpy as np
 
-from . import (RM_BUF_BASE_COLOR, RM_BUF_DD, RM_BUF_DS, RM_BUF_EMISSION, RM_BUF_ID, RM_BUF_IS, RM_BUF_POSITION,
-               RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, Frame)
+from . import (RM_BUF_BASE_COLOR, RM_BUF_DD, RM_BUF_DS, RM_BUF_EMISSION, RM_BUF_ID, RM_BUF_IS, RM_BUF_MATERIAL,
+               RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, Frame)
 
-GB_ORDER = [RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RM_BUF_EMISSION, RM_BUF_BASE_COLOR]
+GB_ORDER = [RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RM_BUF_EMISSION, RM_BUF_BASE_COLOR,
+            RM_BUF_MATERIAL]
 RAD_ORDER = [RM_BUF_DD, RM_BUF_DS, RM_BUF_ID, RM_BUF_IS]
 
 

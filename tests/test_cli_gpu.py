@@ -1,7 +1,9 @@
 """The console host (reference grammar, src/myconsole.cpp:128-160) driving the
 GPU path: a scripted session renders the Cornell workload and writes the
-export list as PNGs; each decodes, and the Raw export equals the library's
-post-processed pixels truncated like Photo::save (src/image.cpp:518-524)."""
+export list as PNGs (src/render.cpp:641-663 without bloom); each decodes, and
+the Raw and Filter exports equal the library's post-processed pixels after
+spatialClamp, and after spatialClamp + filter, truncated like Photo::save
+(src/image.cpp:518-524)."""
 import os
 import struct
 import subprocess
@@ -45,16 +47,27 @@ def test_console_session_writes_exports(tmp_path, dev):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Rendering completed" in r.stdout, r.stdout
     tags = ["DiffuseColor", "DiffuseColor_FXAA", "shapeNormal", "surfaceNormal", "Direct_Diffuse", "Direct_Specular",
-            "Indirect_Diffuse", "Indirect_Specular", "Raw", "Raw_FXAA"]
+            "Indirect_Diffuse", "Indirect_Specular", "Raw", "Raw_FXAA", "Direct_Diffuse_Filter",
+            "Direct_Specular_Filter", "Indirect_Diffuse_Filter", "Indirect_Specular_Filter", "Filter", "Filter_FXAA"]
     imgs = {t: read_png(tmp_path / "output" / f"box({t}).png") for t in tags}
     assert all(im.shape == (512, 512, 3) for im in imgs.values())
-    # same render through the library: Raw = shade(Full) -> gamma, truncated to 8 bits
+    # same render through the library: Raw = spatialClamp -> shade(Full) -> gamma,
+    # Filter = spatialClamp -> filter -> shade(Full) -> gamma, truncated to 8 bits
     g = rm.generate(0, 1)
     S = rm.Scene(g)
     fr = rm.Frame(512, 512)
     fr.render(S, g.args)
-    fr.postprocess(63, g.args.exposure)
-    torch.cuda.synchronize()
-    px = fr.download(rm.RM_BUF_PIXELS)
-    expect = (px * np.float32(255)).astype(np.uint8)
-    assert np.array_equal(imgs["Raw"], expect)
+    fr.spatial_clamp()
+    for tag in ("Raw", "Filter"):
+        if tag == "Filter":
+            fr.filter()
+        fr.postprocess(63, g.args.exposure)
+        torch.cuda.synchronize()
+        px = fr.download(rm.RM_BUF_PIXELS)
+        expect = (px * np.float32(255)).astype(np.uint8)
+        assert np.array_equal(imgs[tag], expect), tag
+    # the filter removes noise: the filtered image is smoother than the raw one
+    def roughness(im):
+        f = im.astype(np.float64)
+        return np.abs(np.diff(f, axis=0)).mean() + np.abs(np.diff(f, axis=1)).mean()
+    assert roughness(imgs["Filter"]) < roughness(imgs["Raw"])
