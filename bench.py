@@ -457,6 +457,98 @@ def denoise_leg(args, torch, rm, fr, stream):
     return out
 
 
+# the reference's export list (src/render.cpp:641-673) as (shade options,
+# step run before it); the depth-of-field five run when CoC > 1e-4
+EXPORTS = [(1, None), (1 | 512, None), (64, None), (128, None),
+           (4 | 16, "clamp"), (4 | 32, None), (8 | 16, None), (8 | 32, None),
+           (63, None), (63 | 256, None), (63 | 512, None), (63 | 768, None),
+           (4 | 16, "filter"), (4 | 32, None), (8 | 16, None), (8 | 32, None),
+           (63, None), (63 | 256, None), (63 | 512, None), (63 | 768, None),
+           (1 | 1024, None), (63 | 1024, None), (63 | 1280, None), (63 | 1536, None), (63 | 1792, None)]
+
+
+def exports_leg(args, torch, rm, fr, a, stream):
+    """Bloom, depth of field and the whole export list of the reference's
+    render command on the rendered 1080p frame (GPU time of the library
+    calls; PNG encoding excluded).  Depth of field with CoC 8 focused at the
+    median distance of the frame's hits."""
+    from raym0nade_amd import render_check as RC
+    saved = [fr.download(b) for b in RC.RAD_ORDER]
+    pos = fr.download(rm.RM_BUF_POSITION)
+    cam = np.array([a.position[k] for k in range(3)], np.float32)
+    d = np.linalg.norm(pos - cam, axis=-1)
+    focus = float(np.median(d[np.isfinite(d)]))
+    coc = 8.0
+    fr.set_dof(focus, coc, cam)
+
+    def timed(fn, reps=3):
+        ts = []
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts[1:]))
+
+    out = {"focus": round(focus, 3), "coc": coc}
+    out["shade_ms"] = round(timed(lambda: fr.postprocess(63, a.exposure, stream=stream)), 3)
+    out["bloom_ms"] = round(timed(lambda: fr.postprocess(63 | 256, a.exposure, stream=stream)), 3)
+    out["dof_ms"] = round(timed(lambda: fr.postprocess(63 | 1024, a.exposure, stream=stream)), 3)
+
+    def all_exports():
+        for opt, step in EXPORTS:
+            if step == "clamp":
+                fr.spatial_clamp(stream=stream)
+            elif step == "filter":
+                fr.filter(stream=stream)
+            fr.postprocess(opt, a.exposure, stream=stream)
+
+    def restore():
+        for b, r in zip(RC.RAD_ORDER, saved):
+            fr.upload(b, r)
+
+    ts = []
+    for k in range(3):
+        restore()
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        all_exports()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - c0)
+    out["all_exports_ms"] = round(1e3 * float(np.median(ts[1:])), 2)
+    out["n_exports"] = len(EXPORTS)
+    log(f"[bench] exports 1080p: bloom {out['bloom_ms']} ms, dof {out['dof_ms']} ms, "
+        f"all {len(EXPORTS)} exports {out['all_exports_ms']} ms")
+    restore()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        gb, rad = RC.download_all(fr)
+        h0, w0 = fr.height // 4, fr.width // 4
+        gb, rad = gb[h0:h0 + fr.height // 2, w0:w0 + fr.width // 2], rad[:, h0:h0 + fr.height // 2, w0:w0 + fr.width // 2]
+        m = gb.shape[0] * gb.shape[1]
+        cpu = {}
+        for name, opt in (("bloom", 63 | 256), ("dof", 63 | 1024)):
+            c0 = time.perf_counter()
+            O.postprocess(gb, rad, opt, a.exposure, focus=focus, coc=coc, cam=cam)
+            cpu[name] = time.perf_counter() - c0
+        out["cpu_baseline"] = {
+            "value": {k: round(m / v / 1e6, 3) for k, v in cpu.items()}, "unit": "Mpixel/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the centre {gb.shape[1]}x{gb.shape[0]} crop of the same frame: shade + bloom + gamma "
+                      f"{cpu['bloom']:.1f} s, shade + depth of field + gamma {cpu['dof']:.1f} s; oracle restatement, "
+                      f"one thread as the reference"}
+        out["gpu_over_cpu"] = {"bloom": round(n_px(fr) / (out["bloom_ms"] * 1e-3) / 1e6 / out["cpu_baseline"]["value"]["bloom"], 1),
+                               "dof": round(n_px(fr) / (out["dof_ms"] * 1e-3) / 1e6 / out["cpu_baseline"]["value"]["dof"], 1)}
+    return out
+
+
+def n_px(fr):
+    return fr.width * fr.height
+
+
 def render_leg(args, torch, rm, dev, stream):
     """Path samples/s of the 1080p workload: 64 displaced geodesic spheres
     (10,368,002 triangles) under a 2048x1024 fp32 sky, 1920x1080, 256 spp,
@@ -495,6 +587,7 @@ def render_leg(args, torch, rm, dev, stream):
            "autotune": tuned}
     if not args.no_denoise:
         out["denoise"] = denoise_leg(args, torch, rm, fr, stream)
+        out["exports"] = exports_leg(args, torch, rm, fr, a, stream)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O

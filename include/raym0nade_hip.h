@@ -212,14 +212,25 @@ typedef struct rm_batch_job {
 } rm_batch_job;
 int32_t rm_render_batch(const rm_batch_job *jobs, int32_t n, int32_t use_graph, void *stream);
 
-/* Photo::postProcessing (src/image.cpp:470-479) without bloom / depth of field:
- * shade(options) -> gammaCorrection -> FXAA if options & 512.  Options are
- * bit-compatible with Photo::ShadeOption (include/image.h:17-34); the extra bit
- * RM_SHADE_LINEAR skips gammaCorrection (the linear image the adjoint
- * differentiates).  Writes RM_BUF_PIXELS. */
+/* Photo::postProcessing (src/image.cpp:470-479): shade(options) ->
+ * depthFeildBlur if options & 1024 -> bloom if options & 256 ->
+ * gammaCorrection -> FXAA if options & 512.  Options are bit-compatible with
+ * Photo::ShadeOption (include/image.h:17-34); the extra bit RM_SHADE_LINEAR
+ * skips gammaCorrection (the linear image the adjoint differentiates).
+ * Writes RM_BUF_PIXELS.  Stream-ordered, except that depth of field waits
+ * once on the stream for the size of its work lists (so it is refused in an
+ * rm_render_batch job's post_options).  A pixel whose primary ray missed is
+ * at distance +inf for depth of field (undefined in the reference). */
 #define RM_SHADE_FULL 63
 #define RM_SHADE_LINEAR 65536
 int32_t rm_postprocess(rm_frame *frame, int32_t shade_options, float exposure, void *stream);
+
+/* Photo's depth-of-field fields (include/image.h:36-37, set from the render
+ * arguments at src/render.cpp:666-668).  rm_render sets them from its
+ * arguments (focus, coc, position); this call overrides them, e.g. for a
+ * frame filled with rm_frame_upload.  Used by rm_postprocess with option
+ * 1024 (DoDepthFieldBlur); CoC must be finite and >= 0. */
+int32_t rm_frame_set_dof(rm_frame *frame, float focus, float coc, const float camera_position[3]);
 
 /* Device-to-device copy of one frame buffer into `dst` (device memory of
  * width*height*channels floats), stream-ordered. */

@@ -385,11 +385,16 @@ void load_photo(Photo &photo, const float *gbuf, const float *rad) {
 }
 }  // namespace
 
-// Photo::postProcessing on given buffers (same layouts as oracle_render).
+// Photo::postProcessing on given buffers (same layouts as oracle_render);
+// focus, coc and cam (3 floats, may be NULL) are Photo's depth-of-field
+// fields (src/render.cpp:666-668).
 void oracle_postprocess(int32_t w, int32_t h, const float *gbuf, const float *rad, int32_t options, float exposure,
-                        float *pixels) {
+                        float focus, float coc, const float *cam, float *pixels) {
     Photo photo(w, h);
     photo.exposure = exposure;
+    photo.focus = focus;
+    photo.CoC = coc;
+    if (cam) photo.cameraPosition = get3(cam);
     load_photo(photo, gbuf, rad);
     photo.postProcessing(options);
     const size_t np = (size_t)w * h;

@@ -146,6 +146,107 @@ void filter_radiance(std::vector<RadianceData> &rad, const std::vector<HitInfo> 
     for (int step = 1; step <= 16; step *= 2) filter_pass(rad, G, width, height, specular, step);
 }
 
+// src/image.cpp:248-283: the bright part of the image (luminance >= 1,
+// compressed by Clum^0.65, minus 1 per channel) blurred by five a-trous
+// B3-spline passes (zero padding, weights not renormalised); after every
+// pass the image gains a sixth of that level.
+void Photo::bloom() {
+    const int R = 2;
+    const float h[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    const size_t n = (size_t)width * height;
+    std::vector<vec3> level(n, vec3(0.0f));  // value-initialised like std::vector<vec3>(n)
+    const float omega_bloom = 0.65f;
+    for (size_t id = 0; id < n; ++id) {
+        float Clum = dot(pixelarray[id], RGB_Weight);
+        if (Clum < 1.0f) continue;
+        level[id] = pixelarray[id] / std::pow(Clum, omega_bloom);
+        level[id].x = gmax(level[id].x - 1.0f, 0.0f);
+        level[id].y = gmax(level[id].y - 1.0f, 0.0f);
+        level[id].z = gmax(level[id].z - 1.0f, 0.0f);
+    }
+    for (int step = 1; step <= 16; step *= 2) {
+        const std::vector<vec3> prev = level;
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x) {
+                vec3 col(0.0f);
+                for (int ky = -R; ky <= R; ++ky)
+                    for (int kx = -R; kx <= R; ++kx) {
+                        int nx = x + kx * step, ny = y + ky * step;
+                        if (nx < 0 || nx >= width || ny < 0 || ny >= height) continue;
+                        col += prev[(size_t)ny * width + nx] * h[ky + R] * h[kx + R];
+                    }
+                level[(size_t)y * width + x] = col;
+            }
+        for (size_t id = 0; id < n; ++id) pixelarray[id] += level[id] / 6.0f;
+    }
+}
+
+// src/image.cpp:285-356: every pixel, nearest first (stable order by distance
+// to the camera), spreads its colour over a disc of radius
+// CoC0 = min(CoC |1 - focus / depth| + 1e-4, 96) with weights
+// min(CoC0 - r, 1) normalised over the whole disc; a receiving pixel takes at
+// most 0.99 of weight in total, first come first served, and ends as its
+// weighted mean.
+// EXT: a pixel whose primary ray missed (NaN position) is at depth +inf
+// (CoC0 = min(CoC + 1e-4, 96)).  In the reference its NaN depth breaks the
+// sort's ordering and int(NaN) is undefined.
+void Photo::depthFeildBlur() {
+    struct PixelPos {
+        int x, y;
+        float depth;
+    };
+    const size_t n = (size_t)width * height;
+    std::vector<PixelPos> pixels;
+    pixels.reserve(n);
+    std::vector<vec3> blurred(n, vec3(0.0f));
+    std::vector<float> gained(n, 0.0f);
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            float depth = length(Gbuffer[(size_t)y * width + x].position - cameraPosition);
+            if (depth != depth) depth = INFINITY;
+            pixels.push_back(PixelPos{x, y, depth});
+        }
+    std::stable_sort(pixels.begin(), pixels.end(), [](const PixelPos &a, const PixelPos &b) { return a.depth < b.depth; });
+    const float gainThreshold = 0.99f, MaxCoC = 96.0f;
+    for (size_t i = 0; i < n; ++i) {
+        const PixelPos &p = pixels[i];
+        float CoC0 = std::min(CoC * std::fabs(1.0f - focus / p.depth) + eps_zero, MaxCoC);
+        int radius = int(CoC0);
+        vec3 color = pixelarray[(size_t)p.y * width + p.x];
+        float totalWeight = 0.0f;
+        for (int dy = -radius; dy <= radius; ++dy) {
+            int xLen = int(std::sqrt(CoC0 * CoC0 - dy * dy));
+            int xl, xr;
+            for (xl = -xLen; xl <= 0; xl++) {
+                float w = std::min(CoC0 - std::sqrt(float(xl * xl + dy * dy)), 1.0f);
+                if (w == 1.0f) break;
+                totalWeight += w;
+            }
+            for (xr = xLen; xr > 0; xr--) {
+                float w = std::min(CoC0 - std::sqrt(float(xr * xr + dy * dy)), 1.0f);
+                if (w == 1.0f) break;
+                totalWeight += w;
+            }
+            totalWeight += xr - xl + 1;
+        }
+        for (int dy = -radius; dy <= radius; ++dy) {
+            int xLen = int(std::sqrt(CoC0 * CoC0 - dy * dy));
+            for (int dx = -xLen; dx <= xLen; ++dx) {
+                int nx = p.x + dx, ny = p.y + dy;
+                if (nx < 0 || nx >= width || ny < 0 || ny >= height) continue;
+                float w = std::min(CoC0 - std::sqrt(float(dx * dx + dy * dy)), 1.0f) / totalWeight;
+                if (w < eps_zero) continue;
+                size_t id = (size_t)ny * width + nx;
+                if (gained[id] + w > gainThreshold) w = gainThreshold - gained[id];
+                if (w < eps_zero) continue;
+                blurred[id] += color * w;
+                gained[id] += w;
+            }
+        }
+    }
+    for (size_t i = 0; i < n; ++i) pixelarray[i] = blurred[i] / gained[i];
+}
+
 // src/image.cpp:77-82
 void Photo::spatialClamp() {
     clamp_buffer(radiance_Dd, width, height);

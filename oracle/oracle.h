@@ -217,6 +217,8 @@ struct Photo {                                          // include/image.h:15-67
     };
     int width, height;
     float exposure = 1.0f;
+    float focus = 0.0f, CoC = 0.0f;                     // include/image.h:36-37
+    vec3 cameraPosition{0.0f};
     std::vector<HitInfo> Gbuffer;
     std::vector<RadianceData> radiance_Dd, radiance_Ds, radiance_Id, radiance_Is;
     std::vector<vec3> pixelarray;
@@ -224,8 +226,10 @@ struct Photo {                                          // include/image.h:15-67
     void shade(int options);                            // src/image.cpp:215-246
     void FXAA();                                        // src/image.cpp:363-452
     void gammaCorrection();                             // src/image.cpp:456-468
-    void postProcessing(int shadeOptions);              // src/image.cpp:470-479 (no bloom / DoF)
+    void postProcessing(int shadeOptions);              // src/image.cpp:470-479
     void spatialClamp();                                // src/image.cpp:77-82   (oimage.cpp)
+    void bloom();                                       // src/image.cpp:248-283 (oimage.cpp)
+    void depthFeildBlur();                              // src/image.cpp:285-356 (oimage.cpp)
     void filter();                                      // src/image.cpp:200-213 (oimage.cpp)
 };
 

@@ -59,7 +59,8 @@ def lib():
             "oracle_gen": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32]),
             "oracle_hw_threads": (C.c_int32, []),
             "oracle_render": (C.c_int32, [vp, vp, C.c_int32, i32p, C.c_int32, f32p, f32p]),
-            "oracle_postprocess": (None, [C.c_int32, C.c_int32, f32p, f32p, C.c_int32, C.c_float, f32p]),
+            "oracle_postprocess": (None, [C.c_int32, C.c_int32, f32p, f32p, C.c_int32, C.c_float, C.c_float, C.c_float,
+                                          f32p, f32p]),
             "oracle_render_pixels": (C.c_int32, [vp, vp, C.c_int32, i32p, C.c_int32, f32p, f32p]),
             "oracle_accumulate": (None, [f32p, f32p, f32p, C.c_float, f32p, f32p]),
             "oracle_bsdf_eval": (None, [f32p, f32p, C.c_int32, f32p]),
@@ -219,10 +220,12 @@ def philox(ctr, key):
     return tuple(o)
 
 
-def postprocess(gbuf, rad, options, exposure=1.0):
+def postprocess(gbuf, rad, options, exposure=1.0, focus=0.0, coc=0.0, cam=(0.0, 0.0, 0.0)):
+    """Photo::postProcessing(options) with Photo's exposure, focus, CoC and
+    cameraPosition; returns the pixel array [h][w][3]."""
     h, w = gbuf.shape[:2]
     out = np.empty((h, w, 3), dtype=np.float32)
-    lib().oracle_postprocess(w, h, fp(f32(gbuf)), fp(f32(rad)), options, exposure, fp(out))
+    lib().oracle_postprocess(w, h, fp(f32(gbuf)), fp(f32(rad)), options, exposure, focus, coc, fp(f32(cam)), fp(out))
     return out
 
 

@@ -487,10 +487,13 @@ void Photo::gammaCorrection() {
     }
 }
 
-// src/image.cpp:470-479 (bloom and depth of field are not part of this path)
+// src/image.cpp:470-479 (bloom and depthFeildBlur in oimage.cpp).
+// EXT: the bit 65536 (the product's RM_SHADE_LINEAR) skips gammaCorrection.
 void Photo::postProcessing(int shadeOptions) {
     shade(shadeOptions);
-    gammaCorrection();
+    if (shadeOptions & DoDepthFieldBlur) depthFeildBlur();
+    if (shadeOptions & DoBloom) bloom();
+    if (!(shadeOptions & 65536)) gammaCorrection();
     if (shadeOptions & DoFXAA) FXAA();
 }
 

@@ -209,6 +209,11 @@ class Frame:
         buffers, in place."""
         check(lib().rm_filter(self._h, _stream(stream)))
 
+    def set_dof(self, focus: float, coc: float, camera_position):
+        """Photo's focus, CoC and cameraPosition for option 1024 (depth of field)."""
+        cam = (C.c_float * 3)(*[float(v) for v in camera_position])
+        check(lib().rm_frame_set_dof(self._h, focus, coc, cam))
+
     def upload(self, which: int, data: np.ndarray):
         """Host array [h][w][channels] float32 into one buffer (synchronous)."""
         ch = self.CHANNELS[which]

@@ -146,6 +146,7 @@ SIGNATURES = [
     ("rm_frame_copy", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("rm_frame_upload", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("rm_spatial_clamp", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("rm_frame_set_dof", C.c_int32, [C.c_void_p, C.c_float, C.c_float, C.POINTER(C.c_float)]),
     ("rm_filter", C.c_int32, [C.c_void_p, C.c_void_p]),
     ("rm_frame_set_recording", C.c_int32, [C.c_void_p, C.c_int32]),
     ("rm_scene_update", C.c_int32, [C.c_void_p, C.POINTER(RmMaterial), C.c_void_p, C.c_void_p]),

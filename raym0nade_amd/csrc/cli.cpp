@@ -258,8 +258,7 @@ public:
             save_png(save + "(" + tag + ").png", px, a.width, a.height);
         };
         // Photo::ShadeOption bits (include/image.h:17-34), in the order of
-        // src/render.cpp:641-663; the bloom and depth-of-field exports are
-        // not part of this build
+        // src/render.cpp:641-673 (bloom 256, FXAA 512, depth of field 1024)
         exportImage("DiffuseColor", 1);
         exportImage("DiffuseColor_FXAA", 1 | 512);
         exportImage("shapeNormal", 64);
@@ -270,14 +269,25 @@ public:
         exportImage("Indirect_Diffuse", 8 | 16);
         exportImage("Indirect_Specular", 8 | 32);
         exportImage("Raw", 63);
+        exportImage("Raw_Bloom", 63 | 256);
         exportImage("Raw_FXAA", 63 | 512);
+        exportImage("Raw_Bloom_FXAA", 63 | 256 | 512);
         if (rm_filter(frame, nullptr)) std::cout << "Error: " << rm_last_error() << std::endl;
         exportImage("Direct_Diffuse_Filter", 4 | 16);
         exportImage("Direct_Specular_Filter", 4 | 32);
         exportImage("Indirect_Diffuse_Filter", 8 | 16);
         exportImage("Indirect_Specular_Filter", 8 | 32);
         exportImage("Filter", 63);
+        exportImage("Filter_Bloom", 63 | 256);
         exportImage("Filter_FXAA", 63 | 512);
+        exportImage("Filter_Bloom_FXAA", 63 | 256 | 512);
+        if (a.coc > 1e-4f) {  // the frame holds focus, CoC and the camera position from rm_render
+            exportImage("BaseColor_DepthFieldBlur", 1 | 1024);
+            exportImage("Filter_DepthFieldBlur", 63 | 1024);
+            exportImage("Filter_DepthFieldBlur_Bloom", 63 | 1024 | 256);
+            exportImage("Filter_DepthFieldBlur_FXAA", 63 | 1024 | 512);
+            exportImage("Filter_DepthFieldBlur_Bloom_FXAA", 63 | 1024 | 256 | 512);
+        }
         rm_frame_destroy(frame);
         auto t2 = std::chrono::steady_clock::now();
         std::cout << "Post processing finished. Total: " << std::chrono::duration<double, std::milli>(t2 - t0).count()

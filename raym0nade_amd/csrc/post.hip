@@ -1,8 +1,10 @@
 // Post-processing behind rm_postprocess: Photo::shade (src/image.cpp:215-246),
+// Photo::depthFeildBlur (:285-356, dof.hip), Photo::bloom (:248-283),
 // Photo::gammaCorrection (:454-468) and Photo::FXAA (:358-452), one lane per
-// pixel.  shade and the tone curve are fused in one pass; FXAA reads the
-// tone-mapped image from a second buffer because every output pixel looks at
-// up to 12 neighbours along the edge normal.
+// pixel.  Without bloom and depth of field, shade and the tone curve are
+// fused in one pass; with them the linear image goes to a frame buffer
+// first.  FXAA reads the tone-mapped image from a second buffer because
+// every output pixel looks at up to 12 neighbours along the edge normal.
 #include <hip/hip_runtime.h>
 #include "rm_frame.h"
 #include "rm_vec.h"
@@ -22,6 +24,23 @@ __device__ __forceinline__ V3 ld4p(const float *p, int64_t i) { return V3{p[i * 
 struct PostIn {
     const float *shapeN, *surfN, *emis, *base, *Dd, *Ds, *Id, *Is;
 };
+
+// fp32 tanh and pow evaluated in fp64 and rounded once: the correctly
+// rounded fp32 result except within ~2^-50 of a rounding midpoint, which is
+// what glibc's tanhf / powf return nearly always.
+__device__ __forceinline__ float tanh_f(float x) { return (float)tanh((double)x); }
+__device__ __forceinline__ float pow_f(float x, float y) { return (float)pow((double)x, (double)y); }
+
+// gammaCorrection (src/image.cpp:456-468): max(0), tanh luminance soft clip
+// above 0.75, min(1), ^(1/2.2)
+__device__ __forceinline__ V3 gamma(V3 px) {
+    px = vmax(px, v3(0.0f));
+    float c = dot(px, V3{0.3f, 0.6f, 0.1f});
+    if (c > 0.75f) px = px / c * (tanh_f(3.0f * (c - 0.75f)) / 3.0f + 0.75f);
+    px = vmin(px, v3(1.0f));
+    const float e = 1.0f / 2.2f;
+    return V3{pow_f(px.x, e), pow_f(px.y, e), pow_f(px.z, e)};
+}
 
 __global__ __launch_bounds__(PB) void k_shade_gamma(PostIn I, int64_t n, int opt, float exposure, float *out) {
     int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
@@ -46,18 +65,68 @@ __global__ __launch_bounds__(PB) void k_shade_gamma(PostIn I, int64_t n, int opt
         px = dc * rd + rs;
         if (opt & O_Emission) px += ld3p(I.emis, i) * exposure;
     }
-    if (!(opt & O_Linear)) {
-        // gammaCorrection: max(0), tanh luminance soft clip above 0.75, min(1), ^(1/2.2)
-        px = vmax(px, v3(0.0f));
-        float c = dot(px, V3{0.3f, 0.6f, 0.1f});
-        if (c > 0.75f) px = px / c * (tanhf(3.0f * (c - 0.75f)) / 3.0f + 0.75f);
-        px = vmin(px, v3(1.0f));
-        const float e = 1.0f / 2.2f;
-        px = V3{powf(px.x, e), powf(px.y, e), powf(px.z, e)};
-    }
+    if (!(opt & O_Linear)) px = gamma(px);
     out[i * 3] = px.x;
     out[i * 3 + 1] = px.y;
     out[i * 3 + 2] = px.z;
+}
+
+// gammaCorrection of a linear image (or a plain copy with apply = 0).
+__global__ __launch_bounds__(PB) void k_gamma(const float *__restrict__ in, int64_t n, int apply, float *__restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (i >= n) return;
+    V3 px = ld3p(in, i);
+    if (apply) px = gamma(px);
+    out[i * 3] = px.x;
+    out[i * 3 + 1] = px.y;
+    out[i * 3 + 2] = px.z;
+}
+
+// bloom, first half (src/image.cpp:254-262): the compressed highlights,
+// Clum >= 1 only (the level starts at zero elsewhere).
+__global__ __launch_bounds__(PB) void k_bloom_level0(const float *__restrict__ img, int64_t n, float *__restrict__ lvl) {
+    int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (i >= n) return;
+    V3 p = ld3p(img, i);
+    float c = dot(p, V3{0.3f, 0.6f, 0.1f});
+    V3 b = v3(0.0f);
+    if (!(c < 1.0f)) {
+        b = p / pow_f(c, 0.65f);
+        b = V3{gmax(b.x - 1.0f, 0.0f), gmax(b.y - 1.0f, 0.0f), gmax(b.z - 1.0f, 0.0f)};
+    }
+    lvl[i * 3] = b.x;
+    lvl[i * 3 + 1] = b.y;
+    lvl[i * 3 + 2] = b.z;
+}
+
+// bloom, one level (src/image.cpp:264-282): the previous level blurred by the
+// 5x5 B3-spline at stride `step` (zero padding, no renormalisation), and the
+// image gains a sixth of it.  Blocks are renumbered per XCD band of rows.
+__global__ __launch_bounds__(PB) void k_bloom_pass(const float *__restrict__ prev, int w, int h, int step,
+                                                  float *__restrict__ next, float *__restrict__ img) {
+    uint32_t i = xcd_block(blockIdx.x, gridDim.x) * PB + threadIdx.x;
+    if (i >= (uint32_t)w * (uint32_t)h) return;
+    const float hk[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    int x = (int)(i % (uint32_t)w), y = (int)(i / (uint32_t)w);
+    V3 col = v3(0.0f);
+#pragma unroll
+    for (int ky = -2; ky <= 2; ++ky) {
+        int ny = y + ky * step;
+        if (ny < 0 || ny >= h) continue;
+#pragma unroll
+        for (int kx = -2; kx <= 2; ++kx) {
+            int nx = x + kx * step;
+            if (nx < 0 || nx >= w) continue;
+            col += ld3p(prev, (int64_t)ny * w + nx) * hk[ky + 2] * hk[kx + 2];
+        }
+    }
+    next[(size_t)i * 3] = col.x;
+    next[(size_t)i * 3 + 1] = col.y;
+    next[(size_t)i * 3 + 2] = col.z;
+    V3 p = ld3p(img, i) + col / 6.0f;
+    img[(size_t)i * 3] = p.x;
+    img[(size_t)i * 3 + 1] = p.y;
+    img[(size_t)i * 3 + 2] = p.z;
 }
 
 __device__ __forceinline__ float luma(const float *d, int64_t i) {
@@ -132,15 +201,21 @@ __global__ __launch_bounds__(PB) void k_fxaa(const float *__restrict__ d, int w,
 using namespace rm;
 
 // Allocation half of rm_postprocess (kept out of any graph capture).
-int32_t rm::post_prepare(rm_frame *f, int opt) {
+int32_t rm::post_prepare(rm_frame *f, int opt, bool in_batch) {
     if (!f) return fail(RM_ERR_INVALID, "rm_postprocess: frame is NULL");
-    if (opt & (O_Bloom | O_DoF)) return fail(RM_ERR_UNSUPPORTED, "rm_postprocess: bloom and depth of field are not built");
-    const size_t need = (size_t)f->w * f->h * 12;
-    if ((opt & O_FXAA) && f->post.bytes < need) return f->post.alloc(need);
+    if ((opt & O_DoF) && in_batch)
+        return fail(RM_ERR_UNSUPPORTED, "depth of field in a batch's post options: it sizes its work from the image "
+                                        "(a host wait); run rm_postprocess on the frame after the batch");
+    const size_t plane = (size_t)f->w * f->h * 12;
+    int32_t rc;
+    if ((opt & O_FXAA) && f->post.bytes < plane && (rc = f->post.alloc(plane))) return rc;
+    if ((opt & (O_Bloom | O_DoF)) && f->lin.bytes < 2 * plane && (rc = f->lin.alloc(2 * plane))) return rc;
+    if ((opt & O_Bloom) && f->lvl.bytes < 2 * plane && (rc = f->lvl.alloc(2 * plane))) return rc;
     return RM_OK;
 }
 
-// Launch half: only kernel launches on `st` (capturable).
+// Launch half: only kernel launches on `st` (capturable), except depth of
+// field (dof.hip), which waits for its tile-list sizes.
 int32_t rm::post_launch(rm_frame *f, int opt, float exposure, hipStream_t st) {
     const int64_t n = (int64_t)f->w * f->h;
     PostIn I;
@@ -153,15 +228,33 @@ int32_t rm::post_launch(rm_frame *f, int opt, float exposure, hipStream_t st) {
     I.Id = f->buf[RM_BUF_ID].as<float>();
     I.Is = f->buf[RM_BUF_IS].as<float>();
     float *pixels = f->buf[RM_BUF_PIXELS].as<float>();
-    unsigned g = (unsigned)((n + PB - 1) / PB);
-    if (opt & O_FXAA) {
-        if (f->post.bytes < (size_t)n * 12) return fail(RM_ERR_INVALID, "rm_postprocess: FXAA buffer not prepared");
-        float *tmp = f->post.as<float>();
-        hipLaunchKernelGGL(k_shade_gamma, dim3(g), dim3(PB), 0, st, I, n, (int)opt, exposure, tmp);
-        hipLaunchKernelGGL(k_fxaa, dim3(g), dim3(PB), 0, st, (const float *)tmp, f->w, f->h, pixels);
+    const unsigned g = (unsigned)((n + PB - 1) / PB);
+    if ((opt & O_FXAA) && f->post.bytes < (size_t)n * 12) return fail(RM_ERR_INVALID, "rm_postprocess: FXAA buffer not prepared");
+    float *tone_out = (opt & O_FXAA) ? f->post.as<float>() : pixels;  // where the tone-mapped image goes
+    if (!(opt & (O_Bloom | O_DoF))) {
+        hipLaunchKernelGGL(k_shade_gamma, dim3(g), dim3(PB), 0, st, I, n, (int)opt, exposure, tone_out);
     } else {
-        hipLaunchKernelGGL(k_shade_gamma, dim3(g), dim3(PB), 0, st, I, n, (int)opt, exposure, pixels);
+        if (f->lin.bytes < (size_t)n * 24 || ((opt & O_Bloom) && f->lvl.bytes < (size_t)n * 24))
+            return fail(RM_ERR_INVALID, "rm_postprocess: bloom / depth-of-field buffers not prepared");
+        float *img = f->lin.as<float>(), *img2 = img + n * 3;
+        hipLaunchKernelGGL(k_shade_gamma, dim3(g), dim3(PB), 0, st, I, n, (int)(opt | O_Linear), exposure, img);
+        if (opt & O_DoF) {
+            int32_t rc = dof_launch(f, img, img2, st);
+            if (rc) return rc;
+            std::swap(img, img2);
+        }
+        if (opt & O_Bloom) {
+            float *la = f->lvl.as<float>(), *lb = la + n * 3;
+            hipLaunchKernelGGL(k_bloom_level0, dim3(g), dim3(PB), 0, st, (const float *)img, n, la);
+            for (int step = 1; step <= 16; step *= 2) {
+                hipLaunchKernelGGL(k_bloom_pass, dim3(g), dim3(PB), 0, st, (const float *)la, f->w, f->h, step, lb, img);
+                std::swap(la, lb);
+            }
+        }
+        hipLaunchKernelGGL(k_gamma, dim3(g), dim3(PB), 0, st, (const float *)img, n, (opt & O_Linear) ? 0 : 1, tone_out);
     }
+    if (opt & O_FXAA)
+        hipLaunchKernelGGL(k_fxaa, dim3(g), dim3(PB), 0, st, (const float *)f->post.as<float>(), f->w, f->h, pixels);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }
@@ -170,4 +263,12 @@ extern "C" int32_t rm_postprocess(rm_frame *f, int32_t opt, float exposure, void
     int32_t rc = post_prepare(f, opt);
     if (rc) return rc;
     return post_launch(f, opt, exposure, (hipStream_t)stream);
+}
+
+extern "C" int32_t rm_frame_set_dof(rm_frame *f, float focus, float coc, const float camera_position[3]) {
+    if (!f || !camera_position) return fail(RM_ERR_INVALID, "rm_frame_set_dof: NULL argument");
+    f->dof_focus = focus;
+    f->dof_coc = coc;
+    for (int k = 0; k < 3; ++k) f->dof_cam[k] = camera_position[k];
+    return RM_OK;
 }

@@ -1404,7 +1404,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     }
     for (int i = 0; i < nj; ++i)
         if (hj[i].post) {
-            int32_t rc = post_prepare(hj[i].frame, hj[i].post);
+            int32_t rc = post_prepare(hj[i].frame, hj[i].post, true);
             if (rc) return rc;
         }
     RM_HIP(hipMemcpyAsync(bsum, hs.data(), nblk * 8, hipMemcpyHostToDevice, st));
@@ -1542,6 +1542,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
         f->stamp_version = hj[i].scene->version;
         f->stamp_args = *hj[i].args;
         f->stamped = true;
+        f->dof_focus = hj[i].args->focus;
+        f->dof_coc = hj[i].args->coc;
+        for (int k = 0; k < 3; ++k) f->dof_cam[k] = hj[i].args->position[k];
         f->rec_valid = J[i].F.rec != nullptr;
     }
     return RM_OK;

@@ -1,6 +1,7 @@
 """The console host (reference grammar, src/myconsole.cpp:128-160) driving the
 GPU path: a scripted session renders the Cornell workload and writes the
-export list as PNGs (src/render.cpp:641-663 without bloom); each decodes, and
+export list as PNGs (src/render.cpp:641-663; the workload has CoC 0, so no
+depth-of-field exports); each decodes, and
 the Raw and Filter exports equal the library's post-processed pixels after
 spatialClamp, and after spatialClamp + filter, truncated like Photo::save
 (src/image.cpp:518-524)."""
@@ -47,8 +48,9 @@ def test_console_session_writes_exports(tmp_path, dev):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Rendering completed" in r.stdout, r.stdout
     tags = ["DiffuseColor", "DiffuseColor_FXAA", "shapeNormal", "surfaceNormal", "Direct_Diffuse", "Direct_Specular",
-            "Indirect_Diffuse", "Indirect_Specular", "Raw", "Raw_FXAA", "Direct_Diffuse_Filter",
-            "Direct_Specular_Filter", "Indirect_Diffuse_Filter", "Indirect_Specular_Filter", "Filter", "Filter_FXAA"]
+            "Indirect_Diffuse", "Indirect_Specular", "Raw", "Raw_Bloom", "Raw_FXAA", "Raw_Bloom_FXAA",
+            "Direct_Diffuse_Filter", "Direct_Specular_Filter", "Indirect_Diffuse_Filter", "Indirect_Specular_Filter",
+            "Filter", "Filter_Bloom", "Filter_FXAA", "Filter_Bloom_FXAA"]
     imgs = {t: read_png(tmp_path / "output" / f"box({t}).png") for t in tags}
     assert all(im.shape == (512, 512, 3) for im in imgs.values())
     # same render through the library: Raw = spatialClamp -> shade(Full) -> gamma,
