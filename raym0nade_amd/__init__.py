@@ -117,6 +117,19 @@ class Scene:
         self.info = RmSceneInfo()
         check(lib().rm_scene_get_info(h, C.byref(self.info)))
 
+    @classmethod
+    def load(cls, model_folder: str, model_name: str, sky_name: str = "null") -> "Scene":
+        """Model::Model(model_folder, model_name, skyMap_name) for Wavefront OBJ
+        models (rm_scene_load)."""
+        self = cls.__new__(cls)
+        self._keep = None
+        h = C.c_void_p()
+        check(lib().rm_scene_load(model_folder.encode(), model_name.encode(), sky_name.encode(), C.byref(h)))
+        self._h = h
+        self.info = RmSceneInfo()
+        check(lib().rm_scene_get_info(h, C.byref(self.info)))
+        return self
+
     @property
     def handle(self):
         return self._h
@@ -175,6 +188,17 @@ def render_batch(jobs, use_graph: bool = True, stream=None) -> None:
     if stream is not None:
         st = C.c_void_p(stream.cuda_stream)
     check(lib().rm_render_batch(arr, len(jobs), int(use_graph), st))
+
+
+def load_image(path: str) -> np.ndarray:
+    """PNG (c / 255) or Radiance HDR image as float32 [h][w][3] (rm_image_load)."""
+    w, h = C.c_int32(), C.c_int32()
+    ptr = C.POINTER(C.c_float)()
+    check(lib().rm_image_load(path.encode(), C.byref(w), C.byref(h), C.byref(ptr)))
+    try:
+        return np.ctypeslib.as_array(ptr, shape=(h.value, w.value, 3)).copy()
+    finally:
+        lib().rm_image_free(ptr)
 
 
 class Frame:

@@ -21,7 +21,7 @@ OBJDIR = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-SOURCES = ["api.cpp", "scene.cpp", "scenegen.cpp", "frame.cpp", "trace.hip", "render.hip", "post.hip", "dof.hip", "denoise.hip", "adjoint.hip"]
+SOURCES = ["api.cpp", "scene.cpp", "scenegen.cpp", "assets.cpp", "frame.cpp", "trace.hip", "render.hip", "post.hip", "dof.hip", "denoise.hip", "adjoint.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -59,7 +59,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread"])
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + ["-lpthread", "-lz"])
     os.replace(tmp, LIB)
     # the console host (reference CLI grammar) linked against the library
     tmp = CLI + ".tmp"

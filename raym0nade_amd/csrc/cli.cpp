@@ -1,6 +1,6 @@
 // raym0nade_cli -- the reference's console (main.cpp:5-25, src/myconsole.cpp)
 // on top of the C ABI.  Same command grammar and prompts:
-//   create model <id>   (asset import is not part of this build: reported)
+//   create model <id>   Wavefront OBJ + MTL, PNG albedo maps, Radiance HDR sky
 //   create scene <id> <config> <seed>   procedural workload 0/1/2/4
 //   create args <id>    prompts exactly like MyConsole::createRenderArgs
 //   delete model|args <id>, view model|args <id>, render <model> <args>, exit
@@ -105,8 +105,18 @@ public:
         std::cout << "Enter the sky map name (e.g., sky.hdr): ";
         std::cin >> sky;
         std::cin.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
-        std::cout << "Asset import (assimp) is not part of this build; use `create scene " << id
-                  << " <config> <seed>`." << std::endl;
+        ModelEntry m;
+        if (rm_scene_load(folder.c_str(), name.c_str(), sky.c_str(), &m.scene)) {
+            std::cout << "Error loading model: " << rm_last_error() << std::endl;
+            return;
+        }
+        rm_scene_info info;
+        rm_scene_get_info(m.scene, &info);
+        std::cout << "faces: " << info.n_tris << std::endl;
+        std::cout << "BVH has built with size " << info.n_nodes << std::endl;
+        m.path = folder + name;
+        m.faces = info.n_tris;
+        models[id] = m;
     }
 
     void createScene(const std::string &id, int config, unsigned seed) {
