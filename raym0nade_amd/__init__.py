@@ -48,7 +48,10 @@ class GeneratedScene:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().rm_generated_destroy(h)
+            try:
+                lib().rm_generated_destroy(h)
+            except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
+                pass
             self._h = None
 
 
@@ -170,7 +173,10 @@ class Scene:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().rm_scene_destroy(h)
+            try:
+                lib().rm_scene_destroy(h)
+            except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
+                pass
             self._h = None
 
 
@@ -255,5 +261,8 @@ class Frame:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            lib().rm_frame_destroy(h)
+            try:
+                lib().rm_frame_destroy(h)
+            except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
+                pass
             self._h = None
