@@ -243,6 +243,13 @@ int32_t rm_render_batch(const rm_batch_job *jobs, int32_t n, int32_t use_graph, 
  * at distance +inf for depth of field (undefined in the reference). */
 #define RM_SHADE_FULL 63
 #define RM_SHADE_LINEAR 65536
+/* Extra option bits (no Photo::ShadeOption counterpart): run spatialClamp
+ * and/or filter on the frame's radiance buffers, in place, before shading --
+ * what the reference does between its exports (src/render.cpp:646,655).
+ * Launch-only, so they may be given in an rm_render_batch job's
+ * post_options, where they run inside the batch's graph. */
+#define RM_POST_SPATIAL_CLAMP (1 << 17)
+#define RM_POST_FILTER (1 << 18)
 int32_t rm_postprocess(rm_frame *frame, int32_t shade_options, float exposure, void *stream);
 
 /* Photo's depth-of-field fields (include/image.h:36-37, set from the render

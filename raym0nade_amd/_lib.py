@@ -111,6 +111,8 @@ class RmGrads(C.Structure):
 
 RM_SHADE_FULL = 63
 RM_SHADE_LINEAR = 65536
+RM_POST_SPATIAL_CLAMP = 1 << 17
+RM_POST_FILTER = 1 << 18
 
 
 class RmGenOptions(C.Structure):

@@ -18,7 +18,7 @@
 //                 material distance) are computed once and shared, only the
 //                 radiance-distance term is per buffer.
 // Every arithmetic step keeps the reference's operation order; the only
-// rounding differences left are the device powf / expf against the host's.
+// rounding difference left is the device expf against the host's.
 #include <hip/hip_runtime.h>
 #include "rm_frame.h"
 #include "rm_vec.h"
@@ -231,43 +231,39 @@ Quad frame_quad(rm_frame *f) {
     return q;
 }
 
-// The ping-pong half: four planes of float4 per pixel.
-int32_t filt_quad(rm_frame *f, Quad &q) {
+// The ping-pong half: four planes of float4 per pixel (allocated by
+// denoise_prepare).
+Quad filt_quad(rm_frame *f) {
     const size_t n = (size_t)f->w * f->h;
-    const size_t need = n * 4 * sizeof(float4);
-    if (f->filt.bytes < need) {
-        int32_t rc = f->filt.alloc(need);
-        if (rc) return rc;
-    }
+    Quad q;
     for (int b = 0; b < 4; ++b) q.b[b] = f->filt.as<float4>() + (size_t)b * n;
-    return RM_OK;
+    return q;
 }
 
 }  // namespace
 
-extern "C" int32_t rm_spatial_clamp(rm_frame *f, void *stream) {
-    if (!f) return fail(RM_ERR_INVALID, "rm_spatial_clamp: frame is NULL");
-    Quad scratch;
-    int32_t rc = filt_quad(f, scratch);
-    if (rc) return rc;
-    hipStream_t st = (hipStream_t)stream;
+int32_t rm::denoise_prepare(rm_frame *f) {
+    const size_t need = (size_t)f->w * f->h * 4 * sizeof(float4);
+    return f->filt.bytes < need ? f->filt.alloc(need) : RM_OK;
+}
+
+int32_t rm::clamp_launch(rm_frame *f, hipStream_t st) {
+    if (f->filt.bytes < (size_t)f->w * f->h * 4 * sizeof(float4))
+        return fail(RM_ERR_INVALID, "spatial clamp: work buffer not prepared");
     const uint32_t n = (uint32_t)((size_t)f->w * f->h), nb = (n + DB - 1) / DB;
     Quad R = frame_quad(f);
-    float *b1 = reinterpret_cast<float *>(scratch.b[0]);  // 4 planes of n floats
+    float *b1 = reinterpret_cast<float *>(filt_quad(f).b[0]);  // 4 planes of n floats
     hipLaunchKernelGGL(k_clamp_rows, dim3(4 * nb), dim3(DB), 0, st, R, f->w, n, nb, b1);
     hipLaunchKernelGGL(k_clamp_cols, dim3(4 * nb), dim3(DB), 0, st, R, f->w, f->h, n, nb, (const float *)b1);
     RM_HIP(hipGetLastError());
     return RM_OK;
 }
 
-extern "C" int32_t rm_filter(rm_frame *f, void *stream) {
-    if (!f) return fail(RM_ERR_INVALID, "rm_filter: frame is NULL");
-    Quad B;
-    int32_t rc = filt_quad(f, B);
-    if (rc) return rc;
-    hipStream_t st = (hipStream_t)stream;
+int32_t rm::filter_launch(rm_frame *f, hipStream_t st) {
+    if (f->filt.bytes < (size_t)f->w * f->h * 4 * sizeof(float4))
+        return fail(RM_ERR_INVALID, "filter: work buffer not prepared");
     const uint32_t n = (uint32_t)((size_t)f->w * f->h), nb = (n + DB - 1) / DB;
-    Quad A = frame_quad(f);
+    Quad A = frame_quad(f), B = filt_quad(f);
     GView G{f->buf[RM_BUF_POSITION].as<float>(), f->buf[RM_BUF_SHAPE_NORMAL].as<float>(),
             f->buf[RM_BUF_SURFACE_NORMAL].as<float>(), f->buf[RM_BUF_MATERIAL].as<float>()};
     hipLaunchKernelGGL(k_filter_var, dim3(4 * nb), dim3(DB), 0, st, A, B, f->w, f->h, n, nb);
@@ -279,4 +275,16 @@ extern "C" int32_t rm_filter(rm_frame *f, void *stream) {
     }
     RM_HIP(hipGetLastError());
     return RM_OK;
+}
+
+extern "C" int32_t rm_spatial_clamp(rm_frame *f, void *stream) {
+    if (!f) return fail(RM_ERR_INVALID, "rm_spatial_clamp: frame is NULL");
+    int32_t rc = denoise_prepare(f);
+    return rc ? rc : clamp_launch(f, (hipStream_t)stream);
+}
+
+extern "C" int32_t rm_filter(rm_frame *f, void *stream) {
+    if (!f) return fail(RM_ERR_INVALID, "rm_filter: frame is NULL");
+    int32_t rc = denoise_prepare(f);
+    return rc ? rc : filter_launch(f, (hipStream_t)stream);
 }

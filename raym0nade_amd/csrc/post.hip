@@ -15,7 +15,8 @@ constexpr int PB = 256;
 enum : int {
     O_BaseColor = 1, O_Emission = 2, O_Direct = 4, O_Indirect = 8, O_Diffuse = 16, O_Specular = 32,
     O_ShapeNormal = 64, O_SurfaceNormal = 128, O_Bloom = 256, O_FXAA = 512, O_DoF = 1024,
-    O_Linear = RM_SHADE_LINEAR  // no gammaCorrection (not a Photo::ShadeOption)
+    O_Linear = RM_SHADE_LINEAR,  // no gammaCorrection (not a Photo::ShadeOption)
+    O_Clamp = RM_POST_SPATIAL_CLAMP, O_Filter = RM_POST_FILTER
 };
 
 __device__ __forceinline__ V3 ld3p(const float *p, int64_t i) { return V3{p[i * 3], p[i * 3 + 1], p[i * 3 + 2]}; }
@@ -211,6 +212,7 @@ int32_t rm::post_prepare(rm_frame *f, int opt, bool in_batch) {
     if ((opt & O_FXAA) && f->post.bytes < plane && (rc = f->post.alloc(plane))) return rc;
     if ((opt & (O_Bloom | O_DoF)) && f->lin.bytes < 2 * plane && (rc = f->lin.alloc(2 * plane))) return rc;
     if ((opt & O_Bloom) && f->lvl.bytes < 2 * plane && (rc = f->lvl.alloc(2 * plane))) return rc;
+    if ((opt & (O_Clamp | O_Filter)) && (rc = denoise_prepare(f))) return rc;
     return RM_OK;
 }
 
@@ -229,6 +231,9 @@ int32_t rm::post_launch(rm_frame *f, int opt, float exposure, hipStream_t st) {
     I.Is = f->buf[RM_BUF_IS].as<float>();
     float *pixels = f->buf[RM_BUF_PIXELS].as<float>();
     const unsigned g = (unsigned)((n + PB - 1) / PB);
+    int32_t drc;
+    if ((opt & O_Clamp) && (drc = clamp_launch(f, st))) return drc;
+    if ((opt & O_Filter) && (drc = filter_launch(f, st))) return drc;
     if ((opt & O_FXAA) && f->post.bytes < (size_t)n * 12) return fail(RM_ERR_INVALID, "rm_postprocess: FXAA buffer not prepared");
     float *tone_out = (opt & O_FXAA) ? f->post.as<float>() : pixels;  // where the tone-mapped image goes
     if (!(opt & (O_Bloom | O_DoF))) {

@@ -40,6 +40,10 @@ namespace rm {
 // it is refused where the launch half will be captured (in_batch).
 int32_t post_prepare(rm_frame *f, int opt, bool in_batch = false);
 int32_t post_launch(rm_frame *f, int opt, float exposure, hipStream_t st);
+// spatialClamp / filter in two halves, like post_prepare / post_launch.
+int32_t denoise_prepare(rm_frame *f);
+int32_t clamp_launch(rm_frame *f, hipStream_t st);
+int32_t filter_launch(rm_frame *f, hipStream_t st);
 // depthFeildBlur of the linear image `in` into `out` ([h][w][3] each).
 int32_t dof_launch(rm_frame *f, const float *in, float *out, hipStream_t st);
 }  // namespace rm
