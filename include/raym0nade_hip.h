@@ -119,21 +119,22 @@ int32_t rm_scene_get_perm(const rm_scene *scene, int32_t *perm);
 
 /* Model::Model(model_folder, model_name, skyMap_name) (src/model.cpp:172-215)
  * for Wavefront OBJ models: reads model_folder + model_name with its MTL
- * libraries, PNG albedo maps (map_Kd) and, unless sky_name is "null", the
+ * libraries, PNG or DDS albedo maps (map_Kd) and, unless sky_name is "null", the
  * Radiance HDR sky model_folder + sky_name, then calls rm_scene_create.
  * Faces are fan-triangulated; meshes are the (group, material) pairs in order
  * of appearance.  MTL d / Tr below 0.99 makes the reference's glass
  * (src/material.cpp:300-333); Kd, Ke, Pr and Pm set the constant albedo,
- * emission, roughness and metallic (extensions).  Other model formats and
- * DDS textures are RM_ERR_UNSUPPORTED (the reference reads them through
- * assimp and Python). */
+ * emission, roughness and metallic (extensions).  Other model formats are
+ * RM_ERR_UNSUPPORTED (the reference reads them through assimp). */
 int32_t rm_scene_load(const char *model_folder, const char *model_name, const char *sky_name, rm_scene **out);
 
 /* An image as linear floats [h][w][3] (malloc'd; free with rm_image_free):
  * PNG as c / 255 after libpng's conversions in the reference's loader
  * (src/material.cpp:214-267: 16-bit to the high byte, grey and palette to
- * RGB, alpha dropped), Radiance HDR (RGBE) as stored (SkyBox::load,
- * src/component.cpp:69-114). */
+ * RGB, alpha dropped); DDS (uncompressed RGB(A), BC1/BC2/BC3 by FourCC or
+ * DX10 header; the reference's loadImageFromDDS, src/material.cpp:190-212)
+ * as c / 255 of the decoded 8-bit texels, alpha dropped; Radiance HDR (RGBE)
+ * as stored (SkyBox::load, src/component.cpp:69-114). */
 int32_t rm_image_load(const char *path, int32_t *width, int32_t *height, float **rgb);
 void rm_image_free(float *rgb);
 

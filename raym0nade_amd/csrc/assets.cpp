@@ -5,6 +5,7 @@
 //   - a PNG decoder (zlib inflate + the five PNG row filters), with libpng's
 //     conversions as the reference requests them (16-bit stripped to the
 //     high byte, grey and palette expanded to RGB, alpha dropped);
+//   - a DDS decoder (uncompressed, BC1-BC3) for textures;
 //   - a Radiance HDR (RGBE) reader for the sky, flat and run-length scanlines.
 // assimp, libpng, FreeImage and imageio are absent from this image; the
 // formats are implemented from their published specifications.
@@ -217,6 +218,93 @@ int32_t decode_hdr(const std::vector<uint8_t> &f, int &W, int &H, std::vector<fl
     return RM_OK;
 }
 
+// DDS (DirectDraw Surface) top level as RGB bytes: uncompressed 24/32-bit
+// RGB(A) with channel masks, and the S3TC block formats BC1 (DXT1), BC2
+// (DXT3) and BC3 (DXT5), by FourCC or DX10 header -- the formats the
+// reference's imageio/FreeImage reader decodes to 8-bit RGBA.  Alpha is
+// dropped (the scene format has RGB albedo only).
+int32_t decode_dds(const std::vector<uint8_t> &f, int &W, int &H, std::vector<uint8_t> &rgb) {
+    auto le32 = [&](size_t o) { return (uint32_t)f[o] | (uint32_t)f[o + 1] << 8 | (uint32_t)f[o + 2] << 16 | (uint32_t)f[o + 3] << 24; };
+    if (f.size() < 128 || memcmp(f.data(), "DDS ", 4) != 0 || le32(4) != 124) return fail(RM_ERR_INVALID, "DDS: bad header");
+    H = (int)le32(12);
+    W = (int)le32(16);
+    if (W <= 0 || H <= 0 || W > (1 << 15) || H > (1 << 15)) return fail(RM_ERR_INVALID, "DDS: bad size");
+    const uint32_t pf_flags = le32(80), fourcc = le32(84), bits = le32(88);
+    size_t data = 128;
+    int bc = 0;  // 1, 2, 3 = BC1..BC3; 0 = uncompressed
+    if (pf_flags & 0x4) {  // DDPF_FOURCC
+        if (fourcc == 0x31545844u) bc = 1;         // "DXT1"
+        else if (fourcc == 0x33545844u) bc = 2;    // "DXT3"
+        else if (fourcc == 0x35545844u) bc = 3;    // "DXT5"
+        else if (fourcc == 0x30315844u) {          // "DX10": DXGI format follows
+            if (f.size() < 148) return fail(RM_ERR_INVALID, "DDS: short DX10 header");
+            const uint32_t dxgi = le32(128);
+            data = 148;
+            if (dxgi == 71 || dxgi == 72) bc = 1;       // BC1_UNORM(_SRGB)
+            else if (dxgi == 74 || dxgi == 75) bc = 2;  // BC2
+            else if (dxgi == 77 || dxgi == 78) bc = 3;  // BC3
+            else return fail(RM_ERR_UNSUPPORTED, "DDS: DXGI format %u", dxgi);
+        } else {
+            return fail(RM_ERR_UNSUPPORTED, "DDS: FourCC %08x", fourcc);
+        }
+    } else if (!(pf_flags & 0x40) || (bits != 24 && bits != 32)) {  // DDPF_RGB
+        return fail(RM_ERR_UNSUPPORTED, "DDS: pixel format flags %x, %u bits", pf_flags, bits);
+    }
+    rgb.assign((size_t)W * H * 3, 0);
+    if (bc == 0) {
+        const uint32_t mask[3] = {le32(92), le32(96), le32(100)};
+        const size_t bpp = bits / 8, row = (size_t)W * bpp;
+        if (f.size() < data + row * H) return fail(RM_ERR_INVALID, "DDS: truncated");
+        for (int c = 0; c < 3; ++c)
+            if (!mask[c]) return fail(RM_ERR_UNSUPPORTED, "DDS: missing channel mask");
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const uint8_t *p = &f[data + (size_t)y * row + (size_t)x * bpp];
+                const uint32_t v = (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (bpp == 4 ? (uint32_t)p[3] << 24 : 0u);
+                for (int c = 0; c < 3; ++c) {
+                    const int sh = __builtin_ctz(mask[c]);
+                    const uint32_t m = mask[c] >> sh, q = (v & mask[c]) >> sh;
+                    rgb[((size_t)y * W + x) * 3 + c] = (uint8_t)(m == 255 ? q : (q * 255 + m / 2) / m);
+                }
+            }
+        return RM_OK;
+    }
+    const int bw = (W + 3) / 4, bh = (H + 3) / 4;
+    const size_t bsz = bc == 1 ? 8 : 16;
+    if (f.size() < data + (size_t)bw * bh * bsz) return fail(RM_ERR_INVALID, "DDS: truncated block data");
+    for (int by = 0; by < bh; ++by)
+        for (int bx = 0; bx < bw; ++bx) {
+            const uint8_t *b = &f[data + ((size_t)by * bw + bx) * bsz];
+            const uint8_t *cb = bc == 1 ? b : b + 8;  // the colour half
+            const uint32_t c0 = cb[0] | cb[1] << 8, c1 = cb[2] | cb[3] << 8;
+            int pal[4][3];
+            auto expand = [](uint32_t c, int *o) {
+                const uint32_t r = (c >> 11) & 31, g = (c >> 5) & 63, bl = c & 31;
+                o[0] = (int)(r << 3 | r >> 2), o[1] = (int)(g << 2 | g >> 4), o[2] = (int)(bl << 3 | bl >> 2);
+            };
+            expand(c0, pal[0]);
+            expand(c1, pal[1]);
+            for (int k = 0; k < 3; ++k) {
+                if (bc != 1 || c0 > c1) {  // four colours (always for BC2/BC3)
+                    pal[2][k] = (2 * pal[0][k] + pal[1][k]) / 3;
+                    pal[3][k] = (pal[0][k] + 2 * pal[1][k]) / 3;
+                } else {  // three colours and black (transparent in BC1)
+                    pal[2][k] = (pal[0][k] + pal[1][k]) / 2;
+                    pal[3][k] = 0;
+                }
+            }
+            const uint32_t idx = cb[4] | cb[5] << 8 | cb[6] << 16 | (uint32_t)cb[7] << 24;
+            for (int t = 0; t < 16; ++t) {
+                const int x = bx * 4 + (t & 3), y = by * 4 + (t >> 2);
+                if (x >= W || y >= H) continue;
+                const int* c = pal[(idx >> (2 * t)) & 3];
+                uint8_t *o = &rgb[((size_t)y * W + x) * 3];
+                o[0] = (uint8_t)c[0], o[1] = (uint8_t)c[1], o[2] = (uint8_t)c[2];
+            }
+        }
+    return RM_OK;
+}
+
 std::string ext_of(const std::string &path) {
     size_t d = path.find_last_of('.');
     std::string e = d == std::string::npos ? "" : path.substr(d + 1);
@@ -238,11 +326,20 @@ int32_t load_image(const std::string &path, int &W, int &H, std::vector<float> &
         is_hdr = false;
         return RM_OK;
     }
+    if (e == "dds") {
+        std::vector<uint8_t> rgb;
+        int32_t rc = decode_dds(f, W, H, rgb);
+        if (rc) return rc;
+        out.resize(rgb.size());
+        for (size_t i = 0; i < rgb.size(); ++i) out[i] = (float)rgb[i] / 255.0f;
+        is_hdr = false;
+        return RM_OK;
+    }
     if (e == "hdr") {
         is_hdr = true;
         return decode_hdr(f, W, H, out);
     }
-    return fail(RM_ERR_UNSUPPORTED, "%s: only .png and .hdr images are read by this build (DDS is not)", path.c_str());
+    return fail(RM_ERR_UNSUPPORTED, "%s: only .png, .dds and .hdr images are read by this build", path.c_str());
 }
 
 // ------------------------------------------------------------ OBJ + MTL

@@ -5,6 +5,9 @@ forward row filters of the PNG specification, and every colour type and bit
 depth. Each is decoded and checked against the conversions the reference's
 loader asks libpng for (src/material.cpp:231-260): 16 bits to the high byte,
 grey 1/2/4 bits scaled to 0..255, palette and grey to RGB, alpha dropped.
+DDS: hand-built S3TC blocks whose palettes are worked out by hand (four-
+and three-colour BC1, BC2 and BC3 colour halves, the DX10 header, partial
+blocks) and a masked 32-bit BGRA image.
 HDR: RGBE files written flat, with the per-component run-length scanlines,
 and with the original repeat texels, against (r, g, b) * 2^(e - 136)."""
 import struct
@@ -122,6 +125,11 @@ def test_png_errors(tmp_path):
     q.write_bytes(b"DDS ")
     with pytest.raises(RmError) as e:
         rm.load_image(str(q))
+    assert e.value.code == RM_ERR_INVALID
+    q = tmp_path / "x.tga"
+    q.write_bytes(b"\0" * 32)
+    with pytest.raises(RmError) as e:
+        rm.load_image(str(q))
     assert e.value.code == RM_ERR_UNSUPPORTED
 
 
@@ -178,3 +186,60 @@ def test_hdr_decodes_rgbe(tmp_path, mode):
     scale = np.where(e == 0, 0.0, np.ldexp(1.0, e - 136)).astype(np.float32)
     want = rgbe[..., :3].astype(np.float32) * scale[..., None]
     assert np.array_equal(got, want)
+
+
+def dds_file(w, h, payload, fourcc=None, dxgi=None, rgb_masks=None, bits=32):
+    hdr = bytearray(128)
+    hdr[0:4] = b"DDS "
+    struct.pack_into("<IIII", hdr, 4, 124, 0x1007, h, w)
+    if fourcc is not None:
+        struct.pack_into("<II4s", hdr, 76, 32, 0x4, fourcc)
+    else:
+        struct.pack_into("<IIIIIIII", hdr, 76, 32, 0x40, 0, bits, *rgb_masks, 0)
+    ext = struct.pack("<IIIII", dxgi, 3, 0, 1, 0) if dxgi is not None else b""
+    return bytes(hdr) + ext + payload
+
+
+RED, BLUE = 0xF800, 0x001F
+# c0 = red > c1 = blue: red, blue, (2 red + blue) / 3, (red + 2 blue) / 3
+PAL4 = np.uint8([[255, 0, 0], [0, 0, 255], [170, 0, 85], [85, 0, 170]])
+# c0 = blue < c1 = red: blue, red, (blue + red) / 2, black
+PAL3 = np.uint8([[0, 0, 255], [255, 0, 0], [127, 0, 127], [0, 0, 0]])
+IDX = sum(((t & 3) << (2 * t)) for t in range(16))  # texel (x, y) uses palette entry x
+
+
+def bc1(c0, c1):
+    return struct.pack("<HHI", c0, c1, IDX)
+
+
+@pytest.mark.parametrize("kind", ["DXT1", "DXT1-3", "DXT3", "DXT5", "DX10-BC1", "partial"])
+def test_dds_s3tc_blocks(tmp_path, kind):
+    w = h = 4
+    if kind == "DXT1":
+        data, pal = dds_file(4, 4, bc1(RED, BLUE), fourcc=b"DXT1"), PAL4
+    elif kind == "DXT1-3":
+        data, pal = dds_file(4, 4, bc1(BLUE, RED), fourcc=b"DXT1"), PAL3
+    elif kind == "DXT3":  # explicit alpha half first; colours always four-entry
+        data, pal = dds_file(4, 4, b"\xff" * 8 + bc1(BLUE, RED), fourcc=b"DXT3"), \
+            np.uint8([[0, 0, 255], [255, 0, 0], [85, 0, 170], [170, 0, 85]])
+    elif kind == "DXT5":
+        data, pal = dds_file(4, 4, bytes([255, 0]) + b"\0" * 6 + bc1(RED, BLUE), fourcc=b"DXT5"), PAL4
+    elif kind == "DX10-BC1":
+        data, pal = dds_file(4, 4, bc1(RED, BLUE), fourcc=b"DX10", dxgi=71), PAL4
+    else:  # 5 x 3: two blocks across, one down; the texels past the edge are dropped
+        w, h = 5, 3
+        data, pal = dds_file(5, 3, bc1(RED, BLUE) * 2, fourcc=b"DXT1"), PAL4
+    p = tmp_path / "t.dds"
+    p.write_bytes(data)
+    got = rm.load_image(str(p))
+    want = np.stack([np.stack([pal[x & 3] for x in range(w)]) for _ in range(h)])
+    assert np.array_equal(got, want.astype(np.float32) / np.float32(255.0))
+
+
+def test_dds_uncompressed_bgra(tmp_path):
+    rng = np.random.default_rng(4)
+    px = rng.integers(0, 256, (3, 5, 4)).astype(np.uint8)  # stored B, G, R, A
+    p = tmp_path / "u.dds"
+    p.write_bytes(dds_file(5, 3, px.tobytes(), rgb_masks=(0x00FF0000, 0x0000FF00, 0x000000FF)))
+    got = rm.load_image(str(p))
+    assert np.array_equal(got, px[..., [2, 1, 0]].astype(np.float32) / np.float32(255.0))
