@@ -243,3 +243,29 @@ def test_dds_uncompressed_bgra(tmp_path):
     p.write_bytes(dds_file(5, 3, px.tobytes(), rgb_masks=(0x00FF0000, 0x0000FF00, 0x000000FF)))
     got = rm.load_image(str(p))
     assert np.array_equal(got, px[..., [2, 1, 0]].astype(np.float32) / np.float32(255.0))
+
+
+def test_scene_load_rejects_bad_models_before_touching_a_device(tmp_path):
+    """rm_scene_load's parse errors (no device needed: they come before
+    rm_scene_create)."""
+    def load(name, sky="null"):
+        with pytest.raises(RmError) as e:
+            rm.Scene.load(str(tmp_path) + "/", name, sky)
+        return e.value.code
+    assert load("missing.obj") == RM_ERR_INVALID
+    (tmp_path / "m.fbx").write_bytes(b"")
+    assert load("m.fbx") == RM_ERR_UNSUPPORTED
+    (tmp_path / "idx.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 4\n")
+    assert load("idx.obj") == RM_ERR_INVALID          # vertex 4 does not exist
+    (tmp_path / "zero.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 0 1 2\n")
+    assert load("zero.obj") == RM_ERR_INVALID         # OBJ indices start at 1
+    (tmp_path / "mat.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl nothing\nf 1 2 3\n")
+    assert load("mat.obj") == RM_ERR_INVALID          # unknown material
+    (tmp_path / "lib.obj").write_text("mtllib nofile.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    assert load("lib.obj") == RM_ERR_INVALID          # missing material library
+    (tmp_path / "empty.obj").write_text("v 0 0 0\n")
+    from raym0nade_amd._lib import RM_ERR_EMPTY
+    assert load("empty.obj") == RM_ERR_EMPTY
+    (tmp_path / "ok.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    (tmp_path / "sky.hdr").write_bytes(b"not an hdr")
+    assert load("ok.obj", "sky.hdr") == RM_ERR_INVALID
