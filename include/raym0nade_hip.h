@@ -362,7 +362,9 @@ enum rm_tune_key {
     RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variants 2, 3 (default 8) */
     RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 16) */
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive
-                                           origins are far apart (default), 2 always */
+                                           origins are far apart, 2 always, 3 as 1 with a radix sort
+                                           of the bins (rocPRIM) instead of the counting sort
+                                           (default) */
     RM_TUNE_RENDER_BLOCKS_PER_CU = 4,   /* blocks per CU of the persistent bounce kernel (default 4) */
     RM_TUNE_ADJOINT_SPLIT = 5,          /* lanes per pixel of the adjoint, each replaying a share of
                                            the pixel's samples (default 4) */

@@ -101,7 +101,8 @@ struct Tuning {
     int trace_variant = 3;        // 0 stack walk, 1 step walk, 2 persistent step walk, 3 same with LDS stacks
     int persistent_blocks_per_cu = 8;
     int fetch_min = 16;           // idle lanes before a wave claims new rays
-    int ray_sort = 1;             // 0 never, 1 when consecutive origins are far apart, 2 always
+    int ray_sort = 3;             // 0 never, 1 when consecutive origins are far apart, 2 always,
+                                  // 3 as 1 with a radix sort (rocPRIM) instead of the counting sort
     int render_blocks_per_cu = 4; // resident blocks per CU of the persistent bounce kernel
     int adjoint_split = 4;        // lanes per pixel of the adjoint kernel
     int kernel_timing = 0;        // record HIP events around the trace kernels (rm_last_trace_ms)

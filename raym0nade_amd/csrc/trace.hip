@@ -6,8 +6,11 @@
 // the 8 XCDs) works on one contiguous range of rays: neighbouring rays of a
 // coherent batch then share that XCD's L2.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <algorithm>
 #include <atomic>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include "rm_internal.h"
 #include "traverse.h"
@@ -331,6 +334,23 @@ __global__ __launch_bounds__(256) void k_bin_hist(SceneView S, const float *__re
     atomicAdd(hist + b, 1u);
 }
 
+// Keys for the radix-sort reordering (ray_sort 3): the bin when the batch is
+// incoherent, else 0 (a stable sort then keeps the batch's order).
+__global__ __launch_bounds__(256) void k_bin_keys(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                  const int *__restrict__ flag, uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = 0;
+    if (*flag) {
+        V3 o, d;
+        load_ray(rays, i, o, d);
+        b = ray_bin(S, o, d);
+    }
+    keys[i] = b;
+    vals[i] = (uint32_t)i;
+}
+
 // exclusive scan of NBINS counters in place, one block of 1024 lanes
 __global__ __launch_bounds__(1024) void k_bin_scan(const int *__restrict__ flag, uint32_t *__restrict__ hist) {
     if (!*flag) return;
@@ -442,6 +462,23 @@ static int32_t mark(int k, hipStream_t st) {
     return RM_OK;
 }
 
+// Reordering workspace per (device, stream): calls queued on one stream run in
+// order, so they can share it; the lock covers the enqueueing only.
+struct TraceWork {
+    std::mutex mu;
+    DevBuf buf;
+};
+static TraceWork *trace_work(hipStream_t st) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    static std::mutex mu;
+    static auto *table = new std::map<std::pair<int, uintptr_t>, TraceWork *>();
+    std::lock_guard<std::mutex> lock(mu);
+    TraceWork *&w = (*table)[{dev, (uintptr_t)st}];
+    if (!w) w = new TraceWork();
+    return w;
+}
+
 template <bool ANY>
 static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, float tmin, float t0, const float *aim_dev,
                             float *t_out, int32_t *id_out, uint8_t *occ, hipStream_t st) {
@@ -450,13 +487,46 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
     const bool timing = T.kernel_timing != 0;
     int32_t mrc;
     if (timing && (mrc = mark(0, st))) return mrc;
-    // reordering workspace (stream-ordered): keys[n], order[n], bins, flag
+    // reordering workspace: keys[n], order[n], bins, flag
     uint32_t *order = nullptr;
     int *flag = nullptr;
     char *ws = nullptr;
-    if (T.ray_sort != 0 && n >= 4096) {
+    std::unique_lock<std::mutex> ws_lock;
+    if (T.ray_sort == 3 && n >= 4096) {
+        // radix sort of (bin, index) pairs: keys, sorted keys, indices, order, flag, sort storage
+        size_t tmp_bytes = 0;
+        RM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                         (const uint32_t *)nullptr, (uint32_t *)nullptr, (size_t)n, 0u,
+                                         (unsigned)(3 * CELL_BITS + 3), st));
+        const size_t plane = ((size_t)n * 4 + 255) & ~(size_t)255;
+        const size_t bytes = 4 * plane + 256 + tmp_bytes;
+        TraceWork *w = trace_work(st);
+        ws_lock = std::unique_lock<std::mutex>(w->mu);
+        if (w->buf.bytes < bytes) {
+            if (w->buf.p) RM_HIP(hipStreamSynchronize(st));  // earlier calls on this stream may still read it
+            int32_t rc = w->buf.alloc(bytes);
+            if (rc) return rc;
+        }
+        char *base = (char *)w->buf.p;
+        uint32_t *keys = (uint32_t *)base, *keys2 = (uint32_t *)(base + plane), *vals = (uint32_t *)(base + 2 * plane);
+        order = (uint32_t *)(base + 3 * plane);
+        flag = (int *)(base + 4 * plane);
+        void *tmp = base + 4 * plane + 256;
+        unsigned gn = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_coherence, dim3(1), dim3(256), 0, st, s->view, rays, n, 0.02f, flag);
+        hipLaunchKernelGGL(k_bin_keys, dim3(gn), dim3(256), 0, st, s->view, rays, n, (const int *)flag, keys, vals);
+        RM_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, (const uint32_t *)keys, keys2, (const uint32_t *)vals, order,
+                                         (size_t)n, 0u, (unsigned)(3 * CELL_BITS + 3), st));
+    } else if (T.ray_sort != 0 && n >= 4096) {
         size_t bytes = (size_t)n * 8 + (size_t)NBINS * 4 + 256;
-        RM_HIP(hipMallocAsync((void **)&ws, bytes, st));
+        TraceWork *w = trace_work(st);
+        ws_lock = std::unique_lock<std::mutex>(w->mu);
+        if (w->buf.bytes < bytes) {
+            if (w->buf.p) RM_HIP(hipStreamSynchronize(st));  // earlier calls on this stream may still read it
+            int32_t rc = w->buf.alloc(bytes);
+            if (rc) return rc;
+        }
+        ws = (char *)w->buf.p;
         uint32_t *keys = reinterpret_cast<uint32_t *>(ws);
         order = keys + n;
         uint32_t *hist = order + n;
@@ -509,7 +579,6 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
         if ((mrc = mark(2, st))) return mrc;
         trace_events().recorded = true;
     }
-    if (ws) RM_HIP(hipFreeAsync(ws, st));
     return RM_OK;
 }
 

@@ -31,10 +31,11 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
 
 @pytest.fixture(params=[(0, 0, 32, 0), (0, 2, 32, 0), (1, 1, 32, 0), (2, 1, 32, 0), (2, 2, 32, 0), (3, 1, 32, 0),
                         (3, 0, 32, 0), (3, 1, 2, 0), (3, 1, 8, 1), (3, 1, 2, 2), (3, 0, 8, 4), (4, 1, 8, 0),
-                        (4, 0, 2, 0)],
+                        (4, 0, 2, 0), (3, 3, 8, 2), (0, 3, 32, 0)],
                 ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted",
                      "persistent-lds-adaptive", "persistent-lds", "persistent-lds2-spill", "lds8-1tri",
-                     "lds2-spill-2tri", "lds8-4tri", "two-walks-lds8", "two-walks-lds2-spill"])
+                     "lds2-spill-2tri", "lds8-4tri", "two-walks-lds8", "two-walks-lds2-spill",
+                     "lds8-2tri-radix-sorted", "stack-radix-sorted"])
 def variant(request):
     """Every traversal kernel variant (one or two walks per lane), with and
     without ray reordering, with the walk stack in scratch, in LDS, or split
