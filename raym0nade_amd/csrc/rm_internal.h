@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdint.h>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 #include "../../include/raym0nade_hip.h"
 #include "rm_layout.h"
@@ -72,6 +74,24 @@ struct DevBuf {
 
 // Host-built, flattened scene before upload (also used by host-only tests of
 // the builder through rm_scene_create's info).
+// Allocator whose resize(n) leaves trivially constructible elements
+// uninitialised, for record arrays whose every byte the builder writes: the
+// zero fill std::vector would do first is a serial pass over gigabytes.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <class U>
+    void construct(U *p) noexcept { ::new ((void *)p) U; }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
+};
+template <class T>
+using HostArray = std::vector<T, NoInitAlloc<T>>;
+
 struct HostScene {
     int64_t n = 0;
     int32_t depth = 0;
@@ -80,8 +100,8 @@ struct HostScene {
     float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
     std::vector<int32_t> perm;          // slot -> original id
     std::vector<PairNode> pairs;        // indexed by interior node u
-    std::vector<TriRec> tris;           // slot order
-    std::vector<TriShade> shade;        // slot order
+    HostArray<TriRec> tris;             // slot order
+    HostArray<TriShade> shade;          // slot order
     std::vector<MatRec> mats;
     std::vector<TexRec> texs;
     std::vector<float> texels;

@@ -12,7 +12,6 @@
 #include <math.h>
 #include <string.h>
 #include <algorithm>
-#include <chrono>
 #include <thread>
 #include "rm_internal.h"
 #include "rm_vec.h"
@@ -26,8 +25,16 @@ constexpr float PI_F = 3.14159265358979323846f;
 struct Builder {
     int64_t n;
     const float *verts;             // original order [n][9]
-    std::vector<float> cent;        // original order [n][3]
-    std::vector<int32_t> idx;       // slot -> original id (permuted in place)
+    // One record per slot: the centroid travels with the face id, so the
+    // selection below compares contiguous records instead of looking the
+    // centroid up through the id.  The comparator sees the same floats in
+    // the same order either way, so std::nth_element makes the same moves
+    // and the id permutation is the one the reference's index vector gets.
+    struct Rec {
+        float c[3];
+        int32_t id;
+    };
+    HostArray<Rec> rec;             // slot -> (centroid, original id), permuted in place
     std::vector<float> lo, hi;      // per heap node [size][3]
     std::vector<int32_t> fl, fr;    // per heap node face range (fr==0: interior)
     std::vector<uint8_t> kind;      // 0 unused, 1 interior, 2 leaf
@@ -56,7 +63,7 @@ struct Builder {
             fr[u] = (int32_t)R;
             V3 a = v3(INFINITY), b = v3(-INFINITY);
             for (int64_t i = L; i < R; ++i) {
-                int32_t f = idx[i];
+                int32_t f = rec[i].id;
                 V3 v0 = vert(f, 0), v1 = vert(f, 1), v2 = vert(f, 2);
                 // Face::aabb (src/component.cpp:41-46): glm::min/max
                 V3 flo = vmin(v0, vmin(v1, v2)), fhi = vmax(v0, vmax(v1, v2));
@@ -69,7 +76,7 @@ struct Builder {
         kind[u] = 1;
         V3 Em = v3(0.0f), Em2 = v3(0.0f);
         for (int64_t i = L; i < R; ++i) {
-            const float *c = &cent[(int64_t)idx[i] * 3];
+            const float *c = rec[i].c;
             V3 m{c[0], c[1], c[2]};
             Em += m;
             Em2 += m * m;
@@ -79,9 +86,8 @@ struct Builder {
         if (D.y > D.x) axis = 1;
         if (D.z > D.x && D.z > D.y) axis = 2;
         int64_t M = (R + L) / 2;
-        const float *cb = cent.data();
-        std::nth_element(idx.begin() + L, idx.begin() + M, idx.begin() + R,
-                         [cb, axis](int32_t A, int32_t B) { return cb[(int64_t)A * 3 + axis] < cb[(int64_t)B * 3 + axis]; });
+        std::nth_element(rec.begin() + L, rec.begin() + M, rec.begin() + R,
+                         [axis](const Rec &A, const Rec &B) { return A.c[axis] < B.c[axis]; });
         if (par_depth > 0 && R - L > 65536) {
             std::thread t([&] { dfs(u << 1, L, M, par_depth - 1); });
             dfs(u << 1 | 1, M, R, par_depth - 1);
@@ -94,6 +100,19 @@ struct Builder {
     }
 };
 
+// fn(i0, i1) over [0, n) cut into one contiguous range per hardware thread
+// (one range below 65536 items); for loops whose iterations are independent.
+template <class F>
+void parallel_ranges(int64_t n, F fn) {
+    int64_t parts = n >= 65536 ? (int64_t)std::max(1u, std::thread::hardware_concurrency()) : 1;
+    int64_t step = (n + parts - 1) / parts;
+    std::vector<std::thread> pool;
+    for (int64_t p = 1; p < parts && p * step < n; ++p)
+        pool.emplace_back(fn, p * step, std::min(n, (p + 1) * step));
+    fn((int64_t)0, std::min(n, step));
+    for (std::thread &t : pool) t.join();
+}
+
 // nodeCount (src/bvh.cpp:46-48).
 int64_t node_count(int64_t u, int64_t n) { return n <= LEAF_BAG ? u : node_count(u << 1 | 1, (n + 1) >> 1); }
 
@@ -101,19 +120,28 @@ int64_t node_count(int64_t u, int64_t n) { return n <= LEAF_BAG ? u : node_count
 // monotone in x, so  fl(|a| / L) < 1e-4f  <=>  |a| < thr  for every |a|,
 // including NaN (both false).  L == 0 (degenerate edge1) gives a NaN or an
 // out-of-range u in the reference, never a hit; thr = inf rejects it outright.
+// The search starts at fl(EPS * L), within an ulp or two of the answer, and
+// walks one representable value at a time in the direction the test gives;
+// monotonicity makes the first change of the test the boundary.
 float reject_threshold(float L) {
     if (L != L) return L;
     if (!(L > 0.0f) || isinf(L)) return INFINITY;
-    uint32_t lo = 0, hi = 0x7f800000u;
-    while (lo < hi) {
-        uint32_t mid = lo + (hi - lo) / 2;
+    auto passes = [L](uint32_t bits) {
         float x;
-        memcpy(&x, &mid, 4);
+        memcpy(&x, &bits, 4);
         volatile float q = x / L;
-        if (q >= EPS) hi = mid; else lo = mid + 1;
+        return q >= EPS;
+    };
+    float x0 = EPS * L;
+    uint32_t b;
+    memcpy(&b, &x0, 4);
+    if (passes(b)) {
+        while (b > 0 && passes(b - 1)) --b;
+    } else {
+        while (!passes(b)) ++b;  // ends at +inf at the latest (inf / L = inf)
     }
     float r;
-    memcpy(&r, &lo, 4);
+    memcpy(&r, &b, 4);
     return r;
 }
 
@@ -213,13 +241,13 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
     Builder B;
     B.n = n;
     B.verts = d->verts;
-    B.cent.resize(n * 3);
-    for (int64_t i = 0; i < n; ++i) {
-        V3 c = (B.vert((int32_t)i, 0) + B.vert((int32_t)i, 1) + B.vert((int32_t)i, 2)) / 3.0f;  // Face::center
-        B.cent[i * 3] = c.x; B.cent[i * 3 + 1] = c.y; B.cent[i * 3 + 2] = c.z;
-    }
-    B.idx.resize(n);
-    for (int64_t i = 0; i < n; ++i) B.idx[i] = (int32_t)i;
+    B.rec.resize(n);
+    parallel_ranges(n, [&B](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            V3 c = (B.vert((int32_t)i, 0) + B.vert((int32_t)i, 1) + B.vert((int32_t)i, 2)) / 3.0f;  // Face::center
+            B.rec[i] = Builder::Rec{{c.x, c.y, c.z}, (int32_t)i};
+        }
+    });
     B.size = node_count(1, n) + 1;
     B.lo.assign(B.size * 3, 0.0f);
     B.hi.assign(B.size * 3, 0.0f);
@@ -231,7 +259,9 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
     while ((1u << par_depth) < hw && par_depth < 6) ++par_depth;
     B.dfs(1, 0, n, par_depth);
     hs.n_nodes = B.size;
-    hs.perm = B.idx;
+    hs.perm.resize(n);
+    for (int64_t i = 0; i < n; ++i) hs.perm[i] = B.rec[i].id;
+    HostArray<Builder::Rec>().swap(B.rec);
 
     // ---- flatten: pair records per interior node
     int32_t depth = 0;
@@ -260,39 +290,42 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
         hs.root_hi[k] = B.hi[3 + k];
     }
 
-    // ---- triangle records in slot order
+    // ---- triangle records in slot order (every slot independent)
     hs.tris.resize(n);
     hs.shade.resize(n);
-    for (int64_t s = 0; s < n; ++s) {
-        int32_t f = B.idx[s];
-        V3 v0 = B.vert(f, 0), v1 = B.vert(f, 1), v2 = B.vert(f, 2);
-        V3 e1 = v1 - v0, e2 = v2 - v0;
-        TriRec &T = hs.tris[s];
-        T.p[0] = v0.x; T.p[1] = v0.y; T.p[2] = v0.z;
-        T.p[3] = reject_threshold(sqrtf(dot(e1, e1)));
-        T.e1[0] = e1.x; T.e1[1] = e1.y; T.e1[2] = e1.z;
-        memcpy(&T.e1[3], &f, 4);
-        T.e2[0] = e2.x; T.e2[1] = e2.y; T.e2[2] = e2.z; T.e2[3] = 0.0f;
-        TriShade &S = hs.shade[s];
-        memset(&S, 0, sizeof(S));
-        for (int k = 0; k < 3; ++k) {
-            V3 v = B.vert(f, k);
-            S.v[k][0] = v.x; S.v[k][1] = v.y; S.v[k][2] = v.z;
-            if (d->normals) {
-                const float *q = d->normals + (int64_t)f * 9 + k * 3;
-                S.n[k][0] = q[0]; S.n[k][1] = q[1]; S.n[k][2] = q[2];
-            } else {
-                V3 g = normalize(cross(e1, e2));
-                S.n[k][0] = g.x; S.n[k][1] = g.y; S.n[k][2] = g.z;
+    auto tri_range = [&](int64_t s0, int64_t s1) {
+        for (int64_t s = s0; s < s1; ++s) {
+            int32_t f = hs.perm[s];
+            V3 v0 = B.vert(f, 0), v1 = B.vert(f, 1), v2 = B.vert(f, 2);
+            V3 e1 = v1 - v0, e2 = v2 - v0;
+            TriRec &T = hs.tris[s];
+            T.p[0] = v0.x; T.p[1] = v0.y; T.p[2] = v0.z;
+            T.p[3] = reject_threshold(sqrtf(dot(e1, e1)));
+            T.e1[0] = e1.x; T.e1[1] = e1.y; T.e1[2] = e1.z;
+            memcpy(&T.e1[3], &f, 4);
+            T.e2[0] = e2.x; T.e2[1] = e2.y; T.e2[2] = e2.z; T.e2[3] = 0.0f;
+            TriShade &S = hs.shade[s];
+            memset(&S, 0, sizeof(S));
+            for (int k = 0; k < 3; ++k) {
+                V3 v = B.vert(f, k);
+                S.v[k][0] = v.x; S.v[k][1] = v.y; S.v[k][2] = v.z;
+                if (d->normals) {
+                    const float *q = d->normals + (int64_t)f * 9 + k * 3;
+                    S.n[k][0] = q[0]; S.n[k][1] = q[1]; S.n[k][2] = q[2];
+                } else {
+                    V3 g = normalize(cross(e1, e2));
+                    S.n[k][0] = g.x; S.n[k][1] = g.y; S.n[k][2] = g.z;
+                }
+                if (d->uvs) {
+                    S.uv[k][0] = d->uvs[(int64_t)f * 6 + k * 2];
+                    S.uv[k][1] = d->uvs[(int64_t)f * 6 + k * 2 + 1];
+                }
             }
-            if (d->uvs) {
-                S.uv[k][0] = d->uvs[(int64_t)f * 6 + k * 2];
-                S.uv[k][1] = d->uvs[(int64_t)f * 6 + k * 2 + 1];
-            }
+            S.material = d->material[f];
+            S.mesh = d->mesh ? d->mesh[f] : 0;
         }
-        S.material = d->material[f];
-        S.mesh = d->mesh ? d->mesh[f] : 0;
-    }
+    };
+    parallel_ranges(n, tri_range);
 
     // ---- materials and textures
     hs.mats.resize(d->n_materials);
@@ -499,8 +532,8 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     I.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
     // keep only what host calls need
     std::vector<PairNode>().swap(h.pairs);
-    std::vector<TriRec>().swap(h.tris);
-    std::vector<TriShade>().swap(h.shade);
+    HostArray<TriRec>().swap(h.tris);
+    HostArray<TriShade>().swap(h.shade);
     std::vector<float>().swap(h.texels);
     *out = s;
     return RM_OK;
