@@ -64,10 +64,22 @@ typedef struct rm_material {
 } rm_material;
 
 /* fp32 linear RGB texture, row-major [height][width][3].  A mip chain of 2x2
- * box averages is built like ImageData::generateMipmaps (src/material.cpp:113-148). */
+ * box averages is built like ImageData::generateMipmaps (src/material.cpp:113-148).
+ *
+ * alpha (optional, NULL for none): [height][width] coverage, a / 255 for an
+ * 8-bit alpha byte a.  A material whose albedo texture has an alpha value
+ * below 1 is an alpha cut-out (Material::hasFullyTransparentPart,
+ * src/material.cpp:330-333).  Every closest-hit and any-hit walk of the
+ * library (rm_trace_closest, rm_trace_any, rm_render, rm_adjoint) then skips
+ * its faces where the bilinear level-0 alpha at the hit is below 1e-4
+ * (TransparentTest, src/model.cpp:217-230): it walks again from t + 1e-4, at
+ * most 8 walks per ray.  A closest-hit ray whose 8 walks all end on such
+ * points is a miss (Model::rayHit, src/model.cpp:332-341), and a shadow ray
+ * that meets 8 of them is blocked (Model::rayHit_test, src/model.cpp:343-354). */
 typedef struct rm_texture {
     int32_t width, height;
     const float *rgb;
+    const float *alpha;
 } rm_texture;
 
 /* Flat scene.  Replaces what Model's constructor assembles from assimp

@@ -112,6 +112,8 @@ oracle_scene *oracle_scene_create(const rm_scene_desc *d) {
             X.diffuse.height = T.height;
             X.diffuse.data[0].assign(T.rgb, T.rgb + (size_t)T.width * T.height * 3);
             X.diffuse.generateMipmaps();
+            if (T.alpha) X.diffuse.alpha.assign(T.alpha, T.alpha + (size_t)T.width * T.height);
+            X.hasFullyTransparentPart = X.diffuse.hasTransparentPart();
         }
     }
     M.vertexDatas.reserve(n * 3);
@@ -190,16 +192,15 @@ void oracle_scene_get_light(const oracle_scene *S, int32_t k, float *out7, float
     if (cdf) memcpy(cdf, L.faceDist.prefixSums.data(), L.faceDist.prefixSums.size() * sizeof(float));
 }
 
-// Closest hit for a batch (BVH::rayHit on HitRecord(tmin, tmax)).  counts:
+// Closest hit for a batch (Model::rayHit's loop from HitRecord(tmin, tmax)).  counts:
 // uint64[4] = interior visits, leaf visits, box tests, triangle tests.
 void oracle_trace_closest(const oracle_scene *S, const float *rays, int64_t n, float tmin, float tmax, float *t_out,
                           int32_t *id_out, int32_t threads, uint64_t *counts) {
     std::vector<Counters> per(std::max(1, threads));
     parallel_for(n, threads, [&](int64_t i, int t) {
-        HitRecord hit(tmin, tmax);
-        S->model.bvh.rayHit(load_ray(rays, i), hit, counts ? &per[t] : nullptr);
+        HitRecord hit = S->model.rayHitFrom(load_ray(rays, i), tmin, tmax, counts ? &per[t] : nullptr);
         t_out[i] = hit.t_max;
-        id_out[i] = hit.face ? hit.face->id : -1;
+        id_out[i] = hit.t_max != tmax && hit.face ? hit.face->id : -1;
     });
     if (counts) {
         memset(counts, 0, 4 * sizeof(uint64_t));
@@ -213,9 +214,7 @@ void oracle_trace_any(const oracle_scene *S, const float *rays, int64_t n, float
     std::vector<Counters> per(std::max(1, threads));
     parallel_for(n, threads, [&](int64_t i, int t) {
         float a = aim ? aim[i] : aim_depth;
-        HitRecord hit(tmin, a + eps_zero);
-        S->model.bvh.rayHit(load_ray(rays, i), hit, counts ? &per[t] : nullptr);
-        occ[i] = hit.t_max >= a ? 0 : 1;
+        occ[i] = S->model.rayHitTestFrom(load_ray(rays, i), tmin, a, counts ? &per[t] : nullptr) ? 1 : 0;
     });
     if (counts) {
         memset(counts, 0, 4 * sizeof(uint64_t));

@@ -231,19 +231,44 @@ void BVH::dfs_rayHit(int u, const Ray &ray, HitRecord &closest_hit, Counters *c)
 
 void BVH::rayHit(const Ray &ray, HitRecord &closest_hit, Counters *c) const { dfs_rayHit(1, ray, closest_hit, c); }
 
-// src/model.cpp:332-341 (no material has an alpha cut-out: TransparentTest is false)
-HitRecord Model::rayHit(Ray ray, Counters *c) const {
-    HitRecord hit(eps_zero, INFINITY);
-    bvh.rayHit(ray, hit, c);
+// src/model.cpp:217-230
+static bool TransparentTest(const Ray &ray, const HitRecord &hit) {
+    const Face &face = *hit.face;
+    const Material &material = *face.material;
+    if (!material.hasFullyTransparentPart) return false;
+    vec3 intersection = ray.origin + ray.direction * hit.t_max;
+    vec3 b = barycentric(face.v[0], face.v[1], face.v[2], intersection);
+    vec2 texUV = b.x * face.data[0]->uv + b.y * face.data[1]->uv + b.z * face.data[2]->uv;
+    return material.diffuse.getAlpha(texUV.x, texUV.y) < eps_zero;
+}
+
+// src/model.cpp:330-341
+static const int maxRayDepth_hit = 8;
+
+HitRecord Model::rayHitFrom(Ray ray, float tmin, float tmax, Counters *c) const {
+    HitRecord hit(tmin, tmax);
+    for (int T = 0; T < maxRayDepth_hit; T++) {
+        bvh.rayHit(ray, hit, c);
+        if (hit.t_max == tmax || !TransparentTest(ray, hit)) return hit;
+        hit = HitRecord(hit.t_max + eps_zero, tmax);
+    }
     return hit;
 }
 
-// src/model.cpp:343-354 (same)
-bool Model::rayHit_test(Ray ray, float aimDepth, Counters *c) const {
-    HitRecord hit(eps_zero, aimDepth + eps_zero);
-    bvh.rayHit(ray, hit, c);
-    if (hit.t_max >= aimDepth) return false;
+HitRecord Model::rayHit(Ray ray, Counters *c) const { return rayHitFrom(ray, eps_zero, INFINITY, c); }
+
+// src/model.cpp:343-354
+bool Model::rayHitTestFrom(Ray ray, float tmin, float aimDepth, Counters *c) const {
+    HitRecord hit(tmin, aimDepth + eps_zero);
+    for (int T = 0; T < maxRayDepth_hit; T++) {
+        bvh.rayHit(ray, hit, c);
+        if (hit.t_max >= aimDepth) return false;
+        if (!TransparentTest(ray, hit)) return true;
+        hit = HitRecord(hit.t_max + eps_zero, aimDepth + eps_zero);
+    }
     return true;
 }
+
+bool Model::rayHit_test(Ray ray, float aimDepth, Counters *c) const { return rayHitTestFrom(ray, eps_zero, aimDepth, c); }
 
 }  // namespace ref

@@ -69,6 +69,37 @@ vec3 ImageData::get(float u, float v, float depth) const {
     return ret1 * (1.0f - levelBlend) + ret2 * levelBlend;
 }
 
+// src/material.cpp:102-107: some alpha byte below 255.
+bool ImageData::hasTransparentPart() const {
+    for (float a : alpha)
+        if (a < 1.0f) return true;
+    return false;
+}
+
+// The alpha component of get<vec4>(u, v, 0) (src/material.cpp:48-94): v is
+// flipped, level 0 is sampled bilinearly with wrap-around, and the level-1
+// sample enters with blend weight 0, which leaves the level-0 value as it is.
+float ImageData::getAlpha(float u, float v) const {
+    v = 1.0f - v;
+    float x = u * float(width), y = v * float(height);
+    int x0 = int(std::floor(x)), y0 = int(std::floor(y));
+    float dx = x - float(x0), dy = y - float(y0);
+    auto wrap = [](int &q, int m) {
+        if (q < 0 || q >= m) {
+            q %= m;
+            if (q < 0) q += m;
+        }
+    };
+    wrap(x0, width);
+    wrap(y0, height);
+    int x1 = (x0 + 1) % width, y1 = (y0 + 1) % height;
+    float a00 = alpha[(size_t)y0 * width + x0], a01 = alpha[(size_t)y0 * width + x1];
+    float a10 = alpha[(size_t)y1 * width + x0], a11 = alpha[(size_t)y1 * width + x1];
+    float a0 = a00 * (1.0f - dx) + a01 * dx;
+    float a1 = a10 * (1.0f - dx) + a11 * dx;
+    return a0 * (1.0f - dy) + a1 * dy;
+}
+
 // src/material.cpp:349-356.  EXT: constant albedo without a texture; fp32
 // textures are linear, so no gamma.
 vec4 Material::getDiffuseColor(float u, float v, float duv) const {

@@ -47,6 +47,11 @@ struct ImageData {                                      // include/material.h:12
     void generateMipmaps();                             // src/material.cpp:113-148
     vec3 get(float u, float v, float depth) const;      // src/material.cpp:81-94
     bool empty() const { return data[0].empty(); }
+    // EXT: the alpha channel as its own level-0 plane of a / 255 values (the
+    // reference keeps RGBA bytes); only TransparentTest reads it, at level 0.
+    std::vector<float> alpha;
+    bool hasTransparentPart() const;                    // src/material.cpp:102-107
+    float getAlpha(float u, float v) const;             // component 3 of get<vec4>(u, v, 0), :81-94
 };
 
 struct Material {                                       // include/material.h:29-52
@@ -57,6 +62,7 @@ struct Material {                                       // include/material.h:29
     vec3 emission{0.0f};      // EXT constant emission
     float metallic = 0.0f;    // EXT constant metallic
     ImageData diffuse;        // EXT fp32 albedo texture
+    bool hasFullyTransparentPart = false;               // src/material.cpp:330-333
     bool hasEmission() const { return emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f; }
     vec4 getDiffuseColor(float u, float v, float duv) const;             // src/material.cpp:349-356
     vec3 getNormal(float u, float v, float duv) const;                   // src/material.cpp:358-363
@@ -131,6 +137,10 @@ struct Model {                                          // include/model.h:25-40
     SkyBox skyMap;
     HitRecord rayHit(Ray ray, Counters *c = nullptr) const;                       // src/model.cpp:332-341
     bool rayHit_test(Ray ray, float aimDepth, Counters *c = nullptr) const;       // src/model.cpp:343-354
+    // The same loops with the first t_min (and rayHit's t_max) as arguments;
+    // the reference's are eps_zero and INFINITY.
+    HitRecord rayHitFrom(Ray ray, float tmin, float tmax, Counters *c = nullptr) const;
+    bool rayHitTestFrom(Ray ray, float tmin, float aimDepth, Counters *c = nullptr) const;
 };
 
 void getHitNormals(const Face &face, const vec3 &inDir, const vec3 &baryCoords, vec3 &shapeNormal,

@@ -34,7 +34,7 @@ class RmMaterial(C.Structure):
 
 
 class RmTexture(C.Structure):
-    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", f32p)]
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", f32p), ("alpha", f32p)]
 
 
 class RmSceneDesc(C.Structure):

@@ -330,7 +330,7 @@ __device__ __forceinline__ void replay_path(const AdjArgs &G, const Surf &s0, V3
     for (;;) {
         RayPre r = ray_pre(o, d);
         float t = INFINITY;
-        int32_t slot = trace_walk<false, false>(S, r, T_EPS, t, -INFINITY);
+        int32_t slot = model_closest<false, 0>(S, o, d, r, t);
         if (slot < 0) {
             if (!exclude && S.sky) {
                 V3 L = sky_get(S, d) * v3(1.0f);
@@ -472,7 +472,7 @@ __device__ __forceinline__ void adjoint_pixel(const AdjArgs &G, int64_t p, int c
     camera_ray(A, x, y, Dir, cam);
     RayPre r = ray_pre(A.pos, Dir);
     float t = INFINITY;
-    int32_t slot = trace_walk<false, false>(S, r, T_EPS, t, -INFINITY);
+    int32_t slot = model_closest<false, 0>(S, A.pos, Dir, r, t);
     const V3 dimg = ld3(G.dimg + p * 3);
     const float ex = A.exposure;
     V3 rep = v3(0.0f);

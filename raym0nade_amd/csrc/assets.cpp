@@ -223,7 +223,11 @@ int32_t decode_hdr(const std::vector<uint8_t> &f, int &W, int &H, std::vector<fl
 // (DXT3) and BC3 (DXT5), by FourCC or DX10 header -- the formats the
 // reference's imageio/FreeImage reader decodes to 8-bit RGBA.  Alpha is
 // dropped (the scene format has RGB albedo only).
-int32_t decode_dds(const std::vector<uint8_t> &f, int &W, int &H, std::vector<uint8_t> &rgb) {
+// alpha receives one byte per pixel: 255 for formats without alpha; BC1's
+// punch-through black is 0; BC2 expands its 4-bit values (a * 17); BC3
+// interpolates its two end points with integer division, as for the colours.
+int32_t decode_dds(const std::vector<uint8_t> &f, int &W, int &H, std::vector<uint8_t> &rgb,
+                   std::vector<uint8_t> &alpha, bool &has_alpha) {
     auto le32 = [&](size_t o) { return (uint32_t)f[o] | (uint32_t)f[o + 1] << 8 | (uint32_t)f[o + 2] << 16 | (uint32_t)f[o + 3] << 24; };
     if (f.size() < 128 || memcmp(f.data(), "DDS ", 4) != 0 || le32(4) != 124) return fail(RM_ERR_INVALID, "DDS: bad header");
     H = (int)le32(12);
@@ -251,8 +255,10 @@ int32_t decode_dds(const std::vector<uint8_t> &f, int &W, int &H, std::vector<ui
         return fail(RM_ERR_UNSUPPORTED, "DDS: pixel format flags %x, %u bits", pf_flags, bits);
     }
     rgb.assign((size_t)W * H * 3, 0);
+    alpha.assign((size_t)W * H, 255);
+    has_alpha = bc != 0 || (pf_flags & 0x1);  // DDPF_ALPHAPIXELS
     if (bc == 0) {
-        const uint32_t mask[3] = {le32(92), le32(96), le32(100)};
+        const uint32_t mask[3] = {le32(92), le32(96), le32(100)}, amask = (pf_flags & 0x1) ? le32(104) : 0u;
         const size_t bpp = bits / 8, row = (size_t)W * bpp;
         if (f.size() < data + row * H) return fail(RM_ERR_INVALID, "DDS: truncated");
         for (int c = 0; c < 3; ++c)
@@ -265,6 +271,11 @@ int32_t decode_dds(const std::vector<uint8_t> &f, int &W, int &H, std::vector<ui
                     const int sh = __builtin_ctz(mask[c]);
                     const uint32_t m = mask[c] >> sh, q = (v & mask[c]) >> sh;
                     rgb[((size_t)y * W + x) * 3 + c] = (uint8_t)(m == 255 ? q : (q * 255 + m / 2) / m);
+                }
+                if (amask) {
+                    const int sh = __builtin_ctz(amask);
+                    const uint32_t m = amask >> sh, q = (v & amask) >> sh;
+                    alpha[(size_t)y * W + x] = (uint8_t)(m == 255 ? q : (q * 255 + m / 2) / m);
                 }
             }
         return RM_OK;
@@ -294,12 +305,29 @@ int32_t decode_dds(const std::vector<uint8_t> &f, int &W, int &H, std::vector<ui
                 }
             }
             const uint32_t idx = cb[4] | cb[5] << 8 | cb[6] << 16 | (uint32_t)cb[7] << 24;
+            int apal[8] = {b[0], b[1], 0, 0, 0, 0, 0, 0};  // BC3 alpha end points and steps
+            if (bc == 3) {
+                if (apal[0] > apal[1]) {
+                    for (int i = 2; i < 8; ++i) apal[i] = ((8 - i) * apal[0] + (i - 1) * apal[1]) / 7;
+                } else {
+                    for (int i = 2; i < 6; ++i) apal[i] = ((6 - i) * apal[0] + (i - 1) * apal[1]) / 5;
+                    apal[6] = 0;
+                    apal[7] = 255;
+                }
+            }
+            uint64_t abits = 0;
+            for (int k = 0; k < 6; ++k) abits |= (uint64_t)b[2 + k] << (8 * k);
             for (int t = 0; t < 16; ++t) {
                 const int x = bx * 4 + (t & 3), y = by * 4 + (t >> 2);
                 if (x >= W || y >= H) continue;
-                const int* c = pal[(idx >> (2 * t)) & 3];
+                const int sel = (idx >> (2 * t)) & 3;
+                const int* c = pal[sel];
                 uint8_t *o = &rgb[((size_t)y * W + x) * 3];
                 o[0] = (uint8_t)c[0], o[1] = (uint8_t)c[1], o[2] = (uint8_t)c[2];
+                uint8_t &a = alpha[(size_t)y * W + x];
+                if (bc == 1) a = (c0 <= c1 && sel == 3) ? 0 : 255;
+                else if (bc == 2) a = (uint8_t)(((b[t >> 1] >> (4 * (t & 1))) & 15) * 17);
+                else a = (uint8_t)apal[(abits >> (3 * t)) & 7];
             }
         }
     return RM_OK;
@@ -313,7 +341,11 @@ std::string ext_of(const std::string &path) {
 }
 
 // An image file as linear floats: PNG as c / 255, HDR as stored.
-int32_t load_image(const std::string &path, int &W, int &H, std::vector<float> &out, bool &is_hdr) {
+// alpha (optional) receives a / 255 per pixel for a DDS format with an alpha
+// channel and is left empty otherwise (the reference strips PNG alpha,
+// src/material.cpp:264-266).
+int32_t load_image(const std::string &path, int &W, int &H, std::vector<float> &out, bool &is_hdr,
+                   std::vector<float> *alpha = nullptr) {
     std::vector<uint8_t> f;
     if (!read_file(path, f)) return fail(RM_ERR_INVALID, "cannot read %s", path.c_str());
     const std::string e = ext_of(path);
@@ -327,11 +359,16 @@ int32_t load_image(const std::string &path, int &W, int &H, std::vector<float> &
         return RM_OK;
     }
     if (e == "dds") {
-        std::vector<uint8_t> rgb;
-        int32_t rc = decode_dds(f, W, H, rgb);
+        std::vector<uint8_t> rgb, a8;
+        bool has_alpha = false;
+        int32_t rc = decode_dds(f, W, H, rgb, a8, has_alpha);
         if (rc) return rc;
         out.resize(rgb.size());
         for (size_t i = 0; i < rgb.size(); ++i) out[i] = (float)rgb[i] / 255.0f;
+        if (alpha && has_alpha) {
+            alpha->resize(a8.size());
+            for (size_t i = 0; i < a8.size(); ++i) (*alpha)[i] = (float)a8[i] / 255.0f;
+        }
         is_hdr = false;
         return RM_OK;
     }
@@ -430,7 +467,7 @@ struct ObjScene {
     std::vector<float> verts, uvs, normals;
     std::vector<int32_t> material, mesh;
     std::vector<rm_material> mats;
-    std::vector<std::vector<float>> tex_data;
+    std::vector<std::vector<float>> tex_data, tex_alpha;
     std::vector<rm_texture> texs;
     std::vector<float> sky;
     int sky_w = 0, sky_h = 0;
@@ -557,11 +594,13 @@ int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &
                 int w = 0, h = 0;
                 bool hdr = false;
                 std::vector<float> px;
-                int32_t rc = load_image(path, w, h, px, hdr);
+                std::vector<float> alpha;
+                int32_t rc = load_image(path, w, h, px, hdr, &alpha);
                 if (rc) return rc;
                 if (!hdr)
                     for (auto &v : px) v = std::pow(v, 2.2f);
                 S.tex_data.push_back(std::move(px));
+                S.tex_alpha.push_back(std::move(alpha));
                 S.texs.push_back(rm_texture{w, h, nullptr});
                 it = tex_of.emplace(path, (int)S.texs.size() - 1).first;
             }
@@ -569,7 +608,10 @@ int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &
         }
         S.mats.push_back(d.m);
     }
-    for (size_t i = 0; i < S.texs.size(); ++i) S.texs[i].rgb = S.tex_data[i].data();
+    for (size_t i = 0; i < S.texs.size(); ++i) {
+        S.texs[i].rgb = S.tex_data[i].data();
+        S.texs[i].alpha = S.tex_alpha[i].empty() ? nullptr : S.tex_alpha[i].data();
+    }
     return RM_OK;
 }
 

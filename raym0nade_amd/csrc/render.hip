@@ -253,7 +253,7 @@ __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int n
     camera_ray(A, x, y, Dir, base);
     RayPre r = ray_pre(A.pos, Dir);
     float t = INFINITY;
-    int32_t slot = trace_walk<false, true, TPS>(S, r, T_EPS, t, -INFINITY);
+    int32_t slot = model_closest<true, TPS>(S, A.pos, Dir, r, t);
     Surf s = surf_default();
     float4 zero = make_float4(0, 0, 0, 0);
     if (slot < 0) {
@@ -379,7 +379,9 @@ __device__ __forceinline__ uint32_t append(uint32_t *counter, bool want) {
 template <int TPS>
 __device__ __forceinline__ void shadow_entry(const ShadowQ &Q, int64_t e, const SceneView &S, int job, V3 o,
                                              const ShadowRay &sr, int queue_depth, uint32_t *queued) {
-    const bool q = S.depth >= queue_depth;
+    // k_shadow carries only the walk's part of the scene, so shadow rays of a
+    // scene with alpha cut-outs are traced here, through Model::rayHit_test.
+    const bool q = S.depth >= queue_depth && !S.cutout;
     const uint32_t slot = append(queued, q);
     if (q) {
         Q.o[slot] = make_float4(o.x, o.y, o.z, sr.aim);
@@ -387,9 +389,7 @@ __device__ __forceinline__ void shadow_entry(const ShadowQ &Q, int64_t e, const 
         Q.ref[e] = slot;
     } else {
         RayPre r = ray_pre(o, sr.dir);
-        float tmax = sr.aim + T_EPS;
-        trace_walk<true, true, TPS>(S, r, T_EPS, tmax, sr.aim);
-        Q.ref[e] = (tmax < sr.aim) ? SH_BLOCKED : SH_CLEAR;
+        Q.ref[e] = model_clear<true, TPS>(S, o, sr.dir, r, sr.aim) ? SH_CLEAR : SH_BLOCKED;
     }
 }
 
@@ -778,7 +778,10 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
         const int maxD = A.max_depth;
         float t = ub(hq.x);
         TraceCounts tc{};
-        const int32_t slot = (int32_t)hq.y;
+        int32_t slot = (int32_t)hq.y;
+        // k_extend walks once; the rest of Model::rayHit for alpha cut-outs
+        // (this launch has no walk LDS)
+        if (S.cutout) slot = model_hit_rest<false, 0>(S, path.o, path.d, t, slot);
         V3 extra = v3(1.0f);  // refract-side NEE multiplies by absorb after the samples (passBsdf)
         bool extra_on = false;
         // an emitter or sky hit ends the path with one light sample

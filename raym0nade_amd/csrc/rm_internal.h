@@ -112,6 +112,7 @@ struct HostScene {
     std::vector<float> sky_cdf;
     std::vector<int32_t> sky_guide;
     int32_t sky_w = 0, sky_h = 0;
+    int32_t cutout = 0;                 // some material is an alpha cut-out
 };
 
 int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs);

@@ -76,13 +76,15 @@ struct alignas(16) MatRec {
     float opacity;
     float ior;
     int32_t tex;       // -1 or texture index
-    int32_t pad[2];
+    int32_t cutout;    // 1: the texture has an alpha plane with a value below 1
+    int32_t pad;
 };
 
 // fp32 texture with a mip chain (level l at offset[l], size w>>l x h>>l).
 struct TexRec {
     int32_t width, height, levels, pad;
     int64_t offset[8];  // in floats, into the scene's texel pool
+    int64_t alpha;      // level-0 alpha plane [h][w] in the pool, or -1
 };
 
 // One light object (LightObject, include/component.h:47-54).
@@ -121,6 +123,8 @@ struct SceneView {
     // table's copies of a view for launches that allocate that LDS, 0 in the
     // scene's own view.
     int32_t walk_lds;
+    // 1 when a material is an alpha cut-out: walks then apply TransparentTest.
+    int32_t cutout;
 };
 
 }  // namespace rm

@@ -360,7 +360,23 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
                         cur[((int64_t)y * cw + x) * 3 + c] = s / 4.0f;
                     }
         }
+        // TransparentTest reads level 0 only (duv = NaN), so the alpha plane
+        // has no mip chain.
+        R.alpha = -1;
+        if (X.alpha) {
+            R.alpha = (int64_t)hs.texels.size();
+            hs.texels.insert(hs.texels.end(), X.alpha, X.alpha + (int64_t)X.width * X.height);
+        }
         hs.texs.push_back(R);
+    }
+    // Material::hasFullyTransparentPart (src/material.cpp:330-333): an alpha
+    // value below 1 (an 8-bit alpha below 255) anywhere in the albedo texture.
+    for (MatRec &M : hs.mats) {
+        if (M.tex < 0 || !d->textures[M.tex].alpha) continue;
+        const rm_texture &X = d->textures[M.tex];
+        const int64_t np = (int64_t)X.width * X.height;
+        for (int64_t i = 0; i < np && !M.cutout; ++i) M.cutout = X.alpha[i] < 1.0f ? 1 : 0;
+        hs.cutout |= M.cutout;
     }
 
     // ---- light objects (Model::checkLightObject, src/model.cpp:23-82), per
@@ -501,6 +517,7 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     v.mats = s->mats.as<MatRec>();
     v.texs = s->texs.as<TexRec>();
     v.texels = s->texels.as<float>();
+    v.cutout = h.cutout;
     v.lights = s->lights.as<LightRec>();
     v.light_faces = s->light_faces.as<TriShade>();
     v.light_cdf = s->light_cdf.as<float>();
@@ -591,6 +608,7 @@ extern "C" int32_t rm_scene_update(rm_scene *scene, const rm_material *materials
             next[m] = mat_rec(materials[m], has_sky);
             const MatRec &o = h.mats[m], &n = next[m];
             if (n.tex != o.tex) return fail(RM_ERR_INVALID, "rm_scene_update: material %zu changes its texture", m);
+            next[m].cutout = o.cutout;  // the alpha planes do not change
             bool e0 = o.emission[0] != 0.0f || o.emission[1] != 0.0f || o.emission[2] != 0.0f;
             bool e1 = n.emission[0] != 0.0f || n.emission[1] != 0.0f || n.emission[2] != 0.0f;
             if (e0 != e1) return fail(RM_ERR_INVALID, "rm_scene_update: material %zu changes whether it emits", m);

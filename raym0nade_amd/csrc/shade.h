@@ -135,6 +135,38 @@ __device__ __forceinline__ V3 tex_get(const SceneView &S, const TexRec &T, float
     return a * (1.0f - blend) + b * blend;
 }
 
+// Component 3 of ImageData::get<vec4> at level 0 (get_bilinear,
+// src/material.cpp:48-94) on an alpha plane of a / 255 values.
+__device__ __forceinline__ float alpha_bilinear(const float *data, int w, int h, float u, float v) {
+    float x = u * float(w), y = v * float(h);
+    int x0 = (int)floorf(x), y0 = (int)floorf(y);
+    float dx = x - float(x0), dy = y - float(y0);
+    if (x0 < 0 || x0 >= w) { x0 %= w; if (x0 < 0) x0 += w; }
+    if (y0 < 0 || y0 >= h) { y0 %= h; if (y0 < 0) y0 += h; }
+    int x1 = (x0 + 1) % w, y1 = (y0 + 1) % h;
+    float c00 = data[(int64_t)y0 * w + x0], c01 = data[(int64_t)y0 * w + x1];
+    float c10 = data[(int64_t)y1 * w + x0], c11 = data[(int64_t)y1 * w + x1];
+    float c0 = c00 * (1.0f - dx) + c01 * dx;
+    float c1 = c10 * (1.0f - dx) + c11 * dx;
+    return c0 * (1.0f - dy) + c1 * dy;
+}
+
+// TransparentTest (src/model.cpp:217-230): the hit (t, slot) of the ray
+// (o, d) lies on a fully transparent texel of an alpha cut-out.  The level-1
+// term of get() is multiplied by a blend of 0, so the level-0 value is the
+// result exactly.
+__device__ __forceinline__ bool transparent_at(const SceneView &S, V3 o, V3 d, float t, int32_t slot) {
+    const TriShade &F = S.shade[slot];
+    const MatRec &M = S.mats[F.material];
+    if (!M.cutout) return false;
+    V3 P = o + d * t;
+    V3 bc = barycentric(ld3(F.v[0]), ld3(F.v[1]), ld3(F.v[2]), P);
+    V2 u0{F.uv[0][0], F.uv[0][1]}, u1{F.uv[1][0], F.uv[1][1]}, u2{F.uv[2][0], F.uv[2][1]};
+    V2 uv = (bc.x * u0 + bc.y * u1) + bc.z * u2;
+    const TexRec &T = S.texs[M.tex];
+    return alpha_bilinear(S.texels + T.alpha, T.width, T.height, uv.x, 1.0f - uv.y) < T_EPS;
+}
+
 // getDuv (src/model.cpp:272-277)
 __device__ __forceinline__ V2 get_duv(const TriShade &F, V3 dP) {
     V3 v0 = ld3(F.v[0]);
@@ -697,6 +729,80 @@ __device__ __forceinline__ bool direct_ray(const SceneView &S, const Surf &s, V3
     return true;
 }
 
+// The rest of Model::rayHit (src/model.cpp:332-341) for a scene with alpha
+// cut-outs, given the first walk's result (t, slot) over [tmin, tmax0]: while
+// the hit is transparent, walk again over [t + T_EPS, tmax0]; 8 walks in all,
+// after which the ray is a miss.  Returns the slot (-1: miss, t = tmax0).
+// The reference's tmax0 is inf.
+template <bool LDSW = true, int TPS = 0>
+__device__ __noinline__ int32_t model_hit_rest(const SceneView &S, V3 o, V3 d, float &t, int32_t slot,
+                                               float tmax0 = INFINITY) {
+    const RayPre r = ray_pre(o, d);
+    for (int T = 0; T < 8; ++T) {
+        if (slot < 0 || !transparent_at(S, o, d, t, slot)) return slot;
+        if (T == 7) break;
+        const float tmin = t + T_EPS;
+        t = tmax0;
+        slot = trace_walk<false, LDSW, TPS>(S, r, tmin, t, -INFINITY);
+    }
+    t = tmax0;
+    return -1;
+}
+
+// Model::rayHit (src/model.cpp:332-341) from the first walk on, around the
+// caller's one inlined walk: without alpha cut-outs the loop ends after that
+// walk, so kernels keep one copy of the traversal and no call.
+template <bool LDSW = true, int TPS = 0>
+__device__ __forceinline__ int32_t model_closest(const SceneView &S, V3 o, V3 d, const RayPre &r, float &t) {
+    float tmin = T_EPS;
+    for (int T = 0;; ++T) {
+        t = INFINITY;
+        const int32_t slot = trace_walk<false, LDSW, TPS>(S, r, tmin, t, -INFINITY);
+        if (!S.cutout || slot < 0 || !transparent_at(S, o, d, t, slot)) return slot;
+        if (T == 7) break;
+        tmin = t + T_EPS;
+    }
+    t = INFINITY;
+    return -1;
+}
+
+// Model::rayHit_test (src/model.cpp:343-354) as "the shadow ray is clear",
+// around the caller's one inlined any-hit walk.  Without alpha cut-outs that
+// walk stops at the first t below aim and decides.  With them it runs with
+// aim -inf, which never stops it early, so it is the closest walk the loop
+// needs (the walk's ANY flag only adds those stops).
+template <bool LDSW = true, int TPS = 0>
+__device__ __forceinline__ bool model_clear(const SceneView &S, V3 o, V3 d, const RayPre &r, float aim) {
+    float tmin = T_EPS;
+    const float stop = S.cutout ? -INFINITY : aim;
+    for (int T = 0; T < 8; ++T) {
+        float tmax = aim + T_EPS;
+        const int32_t slot = trace_walk<true, LDSW, TPS>(S, r, tmin, tmax, stop);
+        if (!(tmax < aim)) return true;
+        if (!S.cutout || !transparent_at(S, o, d, tmax, slot)) return false;
+        tmin = tmax + T_EPS;
+    }
+    return false;
+}
+
+// Model::rayHit_test (src/model.cpp:343-354) in full, for a shadow ray that
+// the early-exit any-hit walk found blocked in a scene with alpha cut-outs:
+// closest walks over [tmin, aim + T_EPS], each from the last hit + T_EPS,
+// until a hit at or beyond aim (clear), an opaque hit (blocked) or 8 walks
+// (blocked).
+template <bool LDSW = true, int TPS = 0>
+__device__ __noinline__ bool model_blocked(const SceneView &S, V3 o, V3 d, float tmin, float aim) {
+    const RayPre r = ray_pre(o, d);
+    for (int T = 0; T < 8; ++T) {
+        float t = aim + T_EPS;
+        const int32_t slot = trace_walk<false, LDSW, TPS>(S, r, tmin, t, -INFINITY);
+        if (t >= aim) return false;
+        if (!transparent_at(S, o, d, t, slot)) return true;
+        tmin = t + T_EPS;
+    }
+    return true;
+}
+
 // One light sample with its shadow ray traced here.  Returns false when the
 // reference `continue`s (no sample, or the shadow ray is blocked).
 template <bool COUNT, bool LDSW = true, int TPS = 0>
@@ -705,9 +811,7 @@ __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s,
     ShadowRay sr;
     if (!direct_ray(S, s, V, total, cnt, g, sr, out, info)) return false;
     RayPre r = ray_pre(s.pos, sr.dir);
-    float tmax = sr.aim + T_EPS;
-    trace_walk<true, LDSW, TPS>(S, r, T_EPS, tmax, sr.aim);
-    return !(tmax < sr.aim);
+    return model_clear<LDSW, TPS>(S, s.pos, sr.dir, r, sr.aim);
 }
 
 // Total light-object weight of sampleDirectLight (0 -> no samples); 1 with a sky.

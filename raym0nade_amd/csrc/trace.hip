@@ -13,6 +13,7 @@
 #include <mutex>
 #include <type_traits>
 #include "rm_internal.h"
+#include "shade.h"
 #include "traverse.h"
 
 namespace rm {
@@ -71,6 +72,33 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_any(SceneView S, const fl
     TraceCounts c{};
     trace<true, false>(S, r, tmin, tmax, aim, c);
     occ[i] = (tmax < aim) ? 1 : 0;
+}
+
+// Alpha cut-outs after the walk of a batch (TransparentTest,
+// src/model.cpp:217-230): a ray with a closest hit is walked again through
+// the whole of Model::rayHit (src/model.cpp:332-341); a blocked shadow ray
+// runs Model::rayHit_test (src/model.cpp:343-354).  Launched only for scenes
+// with a cut-out material, so the traversal kernels stay as they are.
+template <bool ANY>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_cutout(SceneView S, const float *__restrict__ rays, int64_t n,
+                                                              float tmin, float t0, const float *__restrict__ aim_arr,
+                                                              float *t_out, int32_t *id_out, uint8_t *occ) {
+    const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    V3 o, d;
+    load_ray(rays, i, o, d);
+    if constexpr (ANY) {
+        if (!occ[i]) return;
+        const float aim = aim_arr ? aim_arr[i] : t0;
+        occ[i] = model_blocked<false, 0>(S, o, d, tmin, aim) ? 1 : 0;
+    } else {
+        if (id_out[i] < 0) return;
+        float t = t0;
+        int32_t slot = trace_walk<false, false>(S, ray_pre(o, d), tmin, t, -INFINITY);
+        slot = model_hit_rest<false, 0>(S, o, d, t, slot, t0);
+        t_out[i] = t;
+        id_out[i] = slot >= 0 ? __float_as_int(S.tris[slot].e1[3]) : -1;
+    }
 }
 
 // One ray per lane, step-wise walker (one node per loop iteration).
@@ -575,6 +603,11 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                            id_out, (const uint32_t *)order, (const int *)flag);
     }
     RM_HIP(hipGetLastError());
+    if (s->host.cutout) {
+        hipLaunchKernelGGL(k_trace_cutout<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,
+                           t_out, id_out, occ);
+        RM_HIP(hipGetLastError());
+    }
     if (timing) {
         if ((mrc = mark(2, st))) return mrc;
         trace_events().recorded = true;
