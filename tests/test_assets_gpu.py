@@ -156,3 +156,47 @@ def test_console_create_model_renders_the_obj(dev, tmp_path):
         assert r.returncode == 0 and "Rendering completed" in r.stdout, r.stdout + r.stderr
         runs[kind] = read_png(tmp_path / "output" / "box(Raw).png")
     assert np.array_equal(runs["obj"], runs["gen"])
+
+
+def dds_block_texture(fmt):
+    """A 4x4 DDS whose left two texel columns have alpha 0 and right two 255:
+    BC1 punch-through (c0 <= c1, index 3), BC2 explicit nibbles 0 / 15, BC3
+    in its six-step mode (a0 = 0 <= a1 = 255: index 6 is 0, index 7 is 255)."""
+    hdr = bytearray(128)
+    hdr[0:4] = b"DDS "
+    struct.pack_into("<IIII", hdr, 4, 124, 0x1007, 4, 4)
+    struct.pack_into("<II4s", hdr, 76, 32, 0x4, {1: b"DXT1", 2: b"DXT3", 3: b"DXT5"}[fmt])
+    left = [(t & 3) < 2 for t in range(16)]
+    if fmt == 1:
+        idx = sum((3 if left[t] else 1) << (2 * t) for t in range(16))
+        return bytes(hdr) + struct.pack("<HHI", 0x0000, 0xFFFF, idx)
+    colour = struct.pack("<HHI", 0xFFFF, 0x0000, 0)  # every texel c0 (white)
+    if fmt == 2:
+        nib = [0 if left[t] else 15 for t in range(16)]
+        alpha = bytes(nib[2 * k] | nib[2 * k + 1] << 4 for k in range(8))
+    else:
+        bits = sum((6 if left[t] else 7) << (3 * t) for t in range(16))
+        alpha = bytes([0, 255]) + bits.to_bytes(6, "little")
+    return bytes(hdr) + alpha + colour
+
+
+@pytest.mark.parametrize("fmt", [1, 2, 3])
+def test_dds_alpha_makes_a_cutout(dev, tmp_path, fmt):
+    """Rays at u = 0.125 (transparent texels) pass the textured quad at z = 0
+    and hit the back quad at z = -2; rays at u = 0.625 stop on the quad."""
+    (tmp_path / "leaf.dds").write_bytes(dds_block_texture(fmt))
+    (tmp_path / "c.mtl").write_text("newmtl Leaf\nmap_Kd leaf.dds\nnewmtl Wall\nKd 0.5 0.5 0.5\n")
+    (tmp_path / "c.obj").write_text(
+        "mtllib c.mtl\nv -1 -1 0\nv 1 -1 0\nv 1 1 0\nv -1 1 0\nv -3 -3 -2\nv 3 -3 -2\nv 3 3 -2\nv -3 3 -2\n"
+        "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nvn 0 0 1\n"
+        "usemtl Leaf\nf 1/1/1 2/2/1 3/3/1 4/4/1\nusemtl Wall\nf 5/1/1 6/2/1 7/3/1 8/4/1\n")
+    S = rm.Scene.load(str(tmp_path) + "/", "c.obj")
+    xs = np.float32([-0.75, -0.8, 0.25, 0.3])
+    rays = np.zeros((4, 6), np.float32)
+    rays[:, 0], rays[:, 1], rays[:, 2], rays[:, 5] = xs, 0.3, 1.0, -1.0
+    t, ids = S.trace_closest(torch.from_numpy(rays).to(dev))
+    occ = S.trace_any(torch.from_numpy(rays).to(dev), aim=2.0)
+    torch.cuda.synchronize()
+    t = t.cpu().numpy()
+    np.testing.assert_allclose(t, [3.0, 3.0, 1.0, 1.0], atol=1e-5)
+    assert np.array_equal(occ.cpu().numpy(), [0, 0, 1, 1])
