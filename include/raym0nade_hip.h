@@ -177,7 +177,9 @@ int32_t rm_trace_any(const rm_scene *scene, const float *rays_dev, int64_t n,
 
 /* Instrumented closest/any traversal (same results) that also counts, in the
  * reference traversal order, interior visits, leaf visits, box tests and
- * triangle tests: counts_dev = uint64[4].  Used to price algorithmic bytes. */
+ * triangle tests: counts_dev = uint64[4].  Used to price algorithmic bytes.
+ * It counts the first walk of each ray: the further walks past alpha
+ * cut-outs (rm_texture) are not counted. */
 int32_t rm_trace_count(const rm_scene *scene, const float *rays_dev, int64_t n,
                        float tmin, float tmax, int32_t any_hit,
                        uint64_t *counts_dev, void *stream);
