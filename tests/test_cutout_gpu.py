@@ -85,3 +85,23 @@ def test_cornell_with_cutout_grille_renders_like_the_oracle(dev):
     on_grille = np.isin(ids.cpu().numpy(), [n - 2, n - 1])
     print("grille rays stopped:", on_grille.mean())
     assert 0.2 < on_grille.mean() < 0.8
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_mixing_cutout_and_plain_scenes_equals_single_renders(dev, use_graph):
+    """The cut-out flag travels per job: a batch of the Cornell box and the
+    Cornell box with the grille gives each job its single render, bit for bit."""
+    g = rm.generate(0, 1)
+    d, keep = cornell_with_grille(g)
+    scenes = [rm.Scene(g), rm.Scene(d)]
+    a = rm.RmRenderArgs.from_buffer_copy(g.args)
+    a.accuracy = np.float32(a.accuracy * a.width / 64)
+    a.width = a.height = 64
+    a.spp = 16
+    jobs = [(S, a, rm.Frame(64, 64), 0) for S in scenes]
+    rm.render_batch(jobs, use_graph=use_graph)
+    for S, _, fr, _ in jobs:
+        gb, rad = RC.download_all(fr)
+        gb1, rad1 = RC.gpu_render(S, a)
+        assert np.array_equal(gb.view(np.uint32), gb1.view(np.uint32))
+        assert np.array_equal(rad.view(np.uint32), rad1.view(np.uint32))
