@@ -46,6 +46,14 @@ def cpu_info():
     return model
 
 
+def host_threads(O) -> int:
+    """Every host core this process may run on (the reference's thread pool
+    uses them all, src/render.cpp:593-626)."""
+    if hasattr(os, "sched_getaffinity"):
+        return max(1, len(os.sched_getaffinity(0)))
+    return max(1, O.hw_threads())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,8 +238,7 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else O.hw_threads()
-        threads = max(1, min(threads, 16))
+        threads = host_threads(O)
         log(f"[bench] CPU baseline: oracle BVH build + BVH::rayHit on {threads} threads")
         OS = O.OracleScene(g.desc)
         k = max(1, min(args.cpu_sample, n_coh, n_inc))
@@ -246,7 +253,7 @@ def main():
             reps.append(time.perf_counter() - c0)
         cdt = float(np.median(reps))
         cpu = {"value": round((sc.shape[0] + si.shape[0]) / cdt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "host_cpus": os.cpu_count(),
                "sample": f"{sc.shape[0]} coherent (every {max(1, n_coh // k)}th) + {si.shape[0]} incoherent rays of "
                          f"the same batches, closest hit, median of 3 runs of {cdt:.1f} s; CPU {cpu_info()}; "
                          f"oracle restatement of BVH::rayHit, g++ -O2 -ffp-contract=off"}
@@ -591,8 +598,7 @@ def render_leg(args, torch, rm, dev, stream):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else O.hw_threads()
-        threads = max(1, min(threads, 16))
+        threads = host_threads(O)
         log("[bench] CPU baseline: oracle scene build (single thread, reference builder)")
         c0 = time.perf_counter()
         OS = O.OracleScene(g.desc)
@@ -605,6 +611,7 @@ def render_leg(args, torch, rm, dev, stream):
         cpu_samples = len(pix) * a.spp
         out["cpu_baseline"] = {
             "value": round(cpu_samples / cdt, 1), "unit": "path samples/s", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(),
             "sample": f"{len(pix)} pixels on a regular grid (every {rs}th row x every {cs}th column) of the same "
                       f"frame at the full {a.spp} spp, {cdt:.1f} s; oracle restatement of renderPixel on a "
                       f"row-pool of std::threads"}

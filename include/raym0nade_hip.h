@@ -232,6 +232,41 @@ int32_t rm_frame_download(const rm_frame *frame, int32_t which, float *host);
 int32_t rm_render(const rm_scene *scene, const rm_render_args *args, rm_frame *frame,
                   void *stream);
 
+/* Counters of the last render of a frame (no reference counterpart; the
+ * reference prints only its wall time, src/render.cpp:632).  Every render
+ * resets and fills them:
+ *   PRIMARY_RAYS     camera rays (one closest-hit walk per pixel)
+ *   EXTENSION_RAYS   closest-hit walks of indirect path vertices
+ *   SHADOW_RAYS      any-hit walks of light samples (primary vertex and NEE)
+ *   HITS             shading vertices: primary and extension hits
+ *   PATHS            indirect samples started at the primary vertex
+ *   MEDIUM_OVERFLOW  dielectric entries a path's medium stack could not hold:
+ *                    it keeps 16 entered media, which holds the reference's
+ *                    unbounded Medium (src/render.cpp:13-42) for every path
+ *                    of max_depth <= 16, except an entry made at a path's
+ *                    last vertex, which nothing reads afterwards
+ *   MEDIUM_MAX       the deepest medium stack (entered media) of any path
+ *   WALK_*           with RM_TUNE_RENDER_COUNT on: interior visits, leaf
+ *                    visits, box tests and triangle tests of the extension and
+ *                    shadow walks, in the reference traversal order (as
+ *                    rm_trace_count counts them); 0 otherwise */
+enum rm_render_stat {
+    RM_STAT_PRIMARY_RAYS = 0,
+    RM_STAT_EXTENSION_RAYS = 1,
+    RM_STAT_SHADOW_RAYS = 2,
+    RM_STAT_HITS = 3,
+    RM_STAT_PATHS = 4,
+    RM_STAT_MEDIUM_OVERFLOW = 5,
+    RM_STAT_WALK_INTERIOR = 6,
+    RM_STAT_WALK_LEAVES = 7,
+    RM_STAT_WALK_BOXES = 8,
+    RM_STAT_WALK_TRIS = 9,
+    RM_STAT_MEDIUM_MAX = 10,
+    RM_STAT_COUNT = 16
+};
+/* Synchronous copy of the frame's RM_STAT_COUNT counters (waits for the device). */
+int32_t rm_frame_get_stats(const rm_frame *frame, uint64_t *stats);
+
 /* Batched form (no reference counterpart; the reference renders one frame per
  * render_multiThread call).  All jobs run through one set of kernel launches
  * over their concatenated pixels -- each launch carries paths of every job --
@@ -310,9 +345,9 @@ int32_t rm_filter(rm_frame *frame, void *stream);
 int32_t rm_scene_update(rm_scene *scene, const rm_material *materials, const float *texels, void *stream);
 
 /* With recording on, rm_render also keeps per indirect sample of every pixel
- * the sums rm_adjoint would otherwise recompute by a first replay of the
- * sample (48 B per sample, scenes without transmission only), so the adjoint
- * replays each sample once.  Off by default; off frees the record. */
+ * the sums of its light samples (48 B per sample, scenes without transmission
+ * only), so the adjoint replays each sample's path once.  rm_adjoint needs it
+ * for every frame with indirect samples.  Off by default; off frees the record. */
 int32_t rm_frame_set_recording(rm_frame *frame, int32_t on);
 
 /* Gradient outputs of rm_adjoint, all device memory, overwritten; any may be NULL. */
@@ -326,9 +361,11 @@ typedef struct rm_grads {
 
 /* Path-replay adjoint of the linear Full image (RM_SHADE_FULL | RM_SHADE_LINEAR)
  * of rm_render(scene, args, frame): given d_image = d loss / d image (device,
- * [h][w][3]), writes d loss / d parameters.  `frame` must hold that render.
- * Every sample path is replayed from its counter-RNG stream, twice (once for
- * its contribution, once for the derivatives), so memory stays O(pixels).
+ * [h][w][3]), writes d loss / d parameters.  `frame` must hold that render,
+ * made with recording on (rm_frame_set_recording; RM_ERR_UNSUPPORTED
+ * otherwise).  Every sample path is recomputed from its counter-RNG stream,
+ * not stored: the record keeps only each sample's sums (48 B), and the
+ * adjoint replays the path once to form the derivatives.
  * Albedo textures and emission get the exact derivative of the seeded
  * estimator; roughness gets the pathwise derivative plus the score of every
  * roughness-dependent draw (GGX half vectors, the cosine/GGX pick), an unbiased
@@ -393,10 +430,15 @@ enum rm_tune_key {
     RM_TUNE_TRIS_PER_STEP = 9           /* trace variant 3 and render walks: triangles one walk step
                                            tests (0 = a whole leaf; 1, 2 (default) or 4: a leaf over
                                            several steps; the render's shading kernels use 0 or 2) */,
-    RM_TUNE_NEE_SPLIT = 10              /* render: 1 traces the shadow rays of light samples (the
+    RM_TUNE_NEE_SPLIT = 10,             /* render: 1 traces the shadow rays of light samples (the
                                            primary vertex's and the NEE records') in their own
                                            persistent passes, 0 inside the sampling kernels, 2 (default)
                                            splits when the launch's deepest tree has >= 12 levels */
+    RM_TUNE_RENDER_COUNT = 11           /* 1: renders count the extension and shadow walks' node
+                                           visits, box and triangle tests into the frame's
+                                           RM_STAT_WALK_* counters (every shadow ray then goes
+                                           through the queued pass; results are unchanged, the
+                                           render is slower); default 0 */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

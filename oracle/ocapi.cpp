@@ -552,3 +552,156 @@ int32_t oracle_adjoint_dir(const oracle_scene *S, const rm_render_args *a, const
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------- probes
+// Single functions of the restatement, for the tests that pin them against
+// closed forms, quadrature and independent recomputations.
+namespace {
+HitInfo surface_from(const float *p) {
+    // position(3) shapeNormal(3) surfaceNormal(3) baseColor(3) roughness metallic opacity eta entering
+    HitInfo h;
+    h.position = get3(p);
+    h.shapeNormal = get3(p + 3);
+    h.surfaceNormal = get3(p + 6);
+    h.baseColor = get3(p + 9);
+    h.roughness = p[12];
+    h.metallic = p[13];
+    h.opacity = p[14];
+    h.eta = p[15];
+    h.entering = p[16] != 0.0f;
+    return h;
+}
+}  // namespace
+
+extern "C" {
+
+// RandomDistribution (src/component.cpp:12-31) over weights[n]: `count` picks
+// from stream (seed, TAG_INDIRECT, 0, 0) into out, the uniform draw of each
+// pick into u_out, and pdf(i) of every index into pdf_out (each may be NULL).
+void oracle_dist_probe(const float *weights, int32_t n, uint32_t seed, int32_t count, int32_t *out, float *u_out,
+                       float *pdf_out) {
+    RandomDistribution d;
+    d.Init(std::vector<float>(weights, weights + n));
+    Generator g(seed, TAG_INDIRECT, 0, 0), gu(seed, TAG_INDIRECT, 0, 0);
+    for (int32_t i = 0; i < count; ++i) {
+        if (out) out[i] = d(g);
+        if (u_out) u_out[i] = gu();
+    }
+    if (pdf_out)
+        for (int32_t i = 0; i < n; ++i) pdf_out[i] = d.pdf(i);
+}
+
+// The scene's sky (SkyBox, src/component.cpp:54-67,120-140).  what 0:
+// get(dir) for n directions (in: [n][3]) into out [n][3]; 1: dist.pdf(i) of
+// every pixel into out [h*w]; 2: the premultiplied data into out [h*w][3].
+// Returns the pixel count (0 without a sky).
+int64_t oracle_sky_probe(const oracle_scene *S, int32_t what, const float *in, int64_t n, float *out) {
+    const SkyBox &sky = S->model.skyMap;
+    const int64_t np = (int64_t)sky.width * sky.height;
+    if (sky.empty()) return 0;
+    if (what == 0)
+        for (int64_t i = 0; i < n; ++i) put3(out + i * 3, sky.get(get3(in + i * 3)));
+    else if (what == 1)
+        for (int64_t i = 0; i < np; ++i) out[i] = sky.dist.pdf((int)i);
+    else
+        for (int64_t i = 0; i < np; ++i) put3(out + i * 3, sky.data[i]);
+    return np;
+}
+
+// sampleSkyBox (src/sampling.cpp:450-465) about `normal`, one call per
+// stream (seed, TAG_INDIRECT, s, 0): out [count][6] = Dir, light (NaN Dir
+// when all 16 tries fell below the normal's plane).
+void oracle_sky_sample(const oracle_scene *S, const float *normal, uint32_t seed, int32_t count, float *out) {
+    for (int32_t s = 0; s < count; ++s) {
+        Generator g(seed, TAG_INDIRECT, (uint32_t)s, 0);
+        vec3 dir, light(0.0f);
+        probe_sky_sample(S->model.skyMap, get3(normal), g, dir, light);
+        put3(out + s * 6, dir);
+        put3(out + s * 6 + 3, light);
+    }
+}
+
+// sampleDirectLight (src/sampling.cpp:467-527) at a surface (surface_from
+// layout, 17 floats) seen along -in_dir, `cnt` samples from one stream
+// (seed, TAG_INDIRECT, s, 0) per s < streams: out [streams][3] = the sum over
+// the samples of weight * light * bsdfPdf.
+void oracle_direct_light(const oracle_scene *S, const float *surface, const float *in_dir, int32_t cnt, uint32_t seed,
+                         int32_t streams, int32_t threads, float *out) {
+    const HitInfo h = surface_from(surface);
+    const BSDF bsdf(get3(in_dir), h);
+    parallel_for(streams, threads, [&](int64_t s, int) {
+        Generator g(seed, TAG_INDIRECT, (uint32_t)s, 0);
+        std::vector<LightSample> v = sampleDirectLight(bsdf, S->model, OneGen{&g}, cnt);
+        vec3 sum(0.0f);
+        for (const LightSample &x : v) sum += x.weight * x.light * x.bsdfPdf;
+        put3(out + s * 3, sum);
+    });
+}
+
+// getBRDF / getBTDF / getBSDF (which 0 / 1 / 2) at a surface (surface_from
+// layout) seen along -in_dir, for n directions L [n][3].
+void oracle_bsdf_eval_n(const float *surface, const float *in_dir, const float *L, int64_t n, int32_t which,
+                        float *out) {
+    const BSDF b(get3(in_dir), surface_from(surface));
+    for (int64_t i = 0; i < n; ++i) {
+        const vec3 l = get3(L + i * 3);
+        put3(out + i * 3, which == 0 ? b.getBRDF(l) : (which == 1 ? b.getBTDF(l) : b.getBSDF(l)));
+    }
+}
+
+// GTR1 / GTR2 / smithG_GGX (which 0 / 1 / 2, src/sampling.cpp:30-47) at n points.
+void oracle_brdf_term(int32_t which, const float *x, int64_t n, float a, float *out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = brdf_term(which, x[i], a);
+}
+
+// renderPixel's camera ray and initRayDiff (src/render.cpp:439-477) for
+// pixel (x, y): out = Dir(3), dDdx(3), dDdy(3).
+void oracle_camera(const rm_render_args *a, int32_t x, int32_t y, float *out9) {
+    const RenderArgs args = to_args(a);
+    float rayX = float(x) - float(args.width) / 2.0f, rayY = float(y) - float(args.height) / 2.0f;
+    vec3 d = args.direction + args.accuracy * (rayX * args.right + rayY * args.up);
+    RayDifferential diff;
+    initRayDiff(d, args, diff);
+    put3(out9, normalize(d));
+    put3(out9 + 3, diff.dDdx);
+    put3(out9 + 6, diff.dDdy);
+}
+
+// barycentric (src/geometry.cpp:89-103) of P in the triangle tri9, and
+// getHitNormals (src/model.cpp:240-270) with vertex normals n9 for a ray
+// along dir3 at barycentric coordinates bc3: out = bary(3), shapeNormal(3),
+// surfaceNormal(3), entering.
+void oracle_hit_normals(const float *tri9, const float *n9, const float *P3, const float *dir3, const float *bc3,
+                        float *out10) {
+    VertexData vd[3];
+    Face f;
+    for (int k = 0; k < 3; ++k) {
+        f.v[k] = get3(tri9 + 3 * k);
+        vd[k].uv = vec2(0.0f);
+        vd[k].normal = get3(n9 + 3 * k);
+        f.data[k] = &vd[k];
+    }
+    f.material = nullptr;
+    f.id = 0;
+    put3(out10, barycentric(f.v[0], f.v[1], f.v[2], get3(P3)));
+    vec3 sn, un;
+    bool entering = false;
+    getHitNormals(f, get3(dir3), get3(bc3), sn, un, entering);
+    put3(out10 + 3, sn);
+    put3(out10 + 6, un);
+    out10[9] = entering ? 1.0f : 0.0f;
+}
+
+// ImageData::get (src/material.cpp:81-94, fp32 texels) of a w x h texture
+// with its mip chain (generateMipmaps, :113-148) at n points uvd [n][3] =
+// (u, v, depth): out [n][3].
+void oracle_tex_get(int32_t w, int32_t h, const float *rgb, const float *uvd, int64_t n, float *out) {
+    ImageData img;
+    img.width = w;
+    img.height = h;
+    img.data[0].assign(rgb, rgb + (size_t)w * h * 3);
+    img.generateMipmaps();
+    for (int64_t i = 0; i < n; ++i) put3(out + i * 3, img.get(uvd[i * 3], uvd[i * 3 + 1], uvd[i * 3 + 2]));
+}
+
+}  // extern "C"

@@ -191,6 +191,11 @@ void clampPdf(vec3 &pdf);                               // src/sampling.cpp:238-
 // its own counter stream so samples can be computed independently.
 template <class GenFor>
 std::vector<LightSample> sampleDirectLight(const BSDF &bsdf, const Model &model, GenFor genFor, int sampleCnt);
+// Probes of file-local pieces for the tests: brdf_term(0|1|2, x, a) is
+// GTR1(x, a) | GTR2(x, a) | smithG_GGX(x, a) (src/sampling.cpp:30-47);
+// probe_sky_sample is sampleSkyBox (src/sampling.cpp:450-465).
+float brdf_term(int which, float x, float a);
+void probe_sky_sample(const SkyBox &sky, const vec3 &normal, Generator &gen, vec3 &Dir, vec3 &light);
 struct OneGen {          // every light sample continues the caller's stream
     Generator *g;
     Generator &operator()(int) const { return *g; }

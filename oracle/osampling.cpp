@@ -402,6 +402,14 @@ std::vector<LightSample> sampleDirectLight(const BSDF &bsdf, const Model &model,
 }
 
 template std::vector<LightSample> sampleDirectLight<OneGen>(const BSDF &, const Model &, OneGen, int);
+
+// Probes for the tests: the file-local helpers above, one call each.
+float brdf_term(int which, float x, float a) {
+    return which == 0 ? GTR1(x, a) : (which == 1 ? GTR2(x, a) : smithG_GGX(x, a));
+}
+void probe_sky_sample(const SkyBox &sky, const vec3 &normal, Generator &gen, vec3 &Dir, vec3 &light) {
+    sampleSkyBox(sky, normal, gen, Dir, light);
+}
 template std::vector<LightSample> sampleDirectLight<PerSampleGen>(const BSDF &, const Model &, PerSampleGen, int);
 
 // src/image.cpp:615-628

@@ -76,7 +76,7 @@ def philox(ctr, key) -> tuple:
 
 (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT,
  TUNE_RENDER_BLOCKS_PER_CU, TUNE_ADJOINT_SPLIT, TUNE_KERNEL_TIMING, TUNE_RENDER_MIN_WAVES,
- TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP, TUNE_NEE_SPLIT) = range(11)
+ TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP, TUNE_NEE_SPLIT, TUNE_RENDER_COUNT) = range(12)
 
 
 def last_trace_ms() -> tuple:
@@ -222,6 +222,13 @@ class Frame:
 
     def render(self, scene: Scene, args: RmRenderArgs, stream=None):
         check(lib().rm_render(scene.handle, C.byref(args), self._h, _stream(stream)))
+
+    def stats(self) -> dict:
+        """Counters of the frame's last render (rm_frame_get_stats), by name."""
+        from ._lib import RM_STAT_COUNT, STAT_NAMES
+        v = (C.c_uint64 * RM_STAT_COUNT)()
+        check(lib().rm_frame_get_stats(self._h, v))
+        return {k: int(v[i]) for i, k in enumerate(STAT_NAMES)}
 
     def set_recording(self, on: bool = True):
         """Keep per-sample sums for rm_adjoint (one replay per sample instead of two)."""

@@ -109,6 +109,13 @@ class RmGrads(C.Structure):
     ]
 
 
+(RM_STAT_PRIMARY_RAYS, RM_STAT_EXTENSION_RAYS, RM_STAT_SHADOW_RAYS, RM_STAT_HITS, RM_STAT_PATHS,
+ RM_STAT_MEDIUM_OVERFLOW, RM_STAT_WALK_INTERIOR, RM_STAT_WALK_LEAVES, RM_STAT_WALK_BOXES, RM_STAT_WALK_TRIS,
+ RM_STAT_MEDIUM_MAX) = range(11)
+RM_STAT_COUNT = 16
+STAT_NAMES = ["primary_rays", "extension_rays", "shadow_rays", "hits", "paths", "medium_overflow", "walk_interior",
+              "walk_leaves", "walk_boxes", "walk_tris", "medium_max"]
+
 RM_SHADE_FULL = 63
 RM_SHADE_LINEAR = 65536
 RM_POST_SPATIAL_CLAMP = 1 << 17
@@ -165,6 +172,7 @@ SIGNATURES = [
     ("rm_set_tuning", C.c_int32, [C.c_int32, C.c_int32]),
     ("rm_get_tuning", C.c_int32, [C.c_int32, C.POINTER(C.c_int32)]),
     ("rm_last_trace_ms", C.c_int32, [C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("rm_frame_get_stats", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64)]),
 ]
 
 _lib = None

@@ -624,6 +624,15 @@ extern "C" int32_t rm_adjoint(const rm_scene *scene, const rm_render_args *a, rm
     for (size_t m = 0; m < h.mats.size(); ++m)
         if (!(h.mats[m].opacity > 1.0f - T_EPS))
             return fail(RM_ERR_UNSUPPORTED, "rm_adjoint: material %zu transmits (opaque scenes only)", m);
+    // Every indirect sample's sums come from the render's record.  A frame
+    // rendered without recording would need the two-replay form (each sample
+    // replayed once for its sums, once for its derivatives); that form is
+    // refused, because one GPU run of it ended in a memory fault whose cause
+    // has not been found.
+    const int n_ind = a->spp - (int)(float(a->spp) * a->p_direct);
+    if (n_ind > 0 && !f->rec_valid)
+        return fail(RM_ERR_UNSUPPORTED, "rm_adjoint: the frame was rendered without recording "
+                                        "(rm_frame_set_recording(frame, 1) before rm_render)");
     const int n_mats = (int)h.mats.size();
     if (n_mats > 4096) return fail(RM_ERR_UNSUPPORTED, "rm_adjoint: %d materials (at most 4096)", n_mats);
     hipStream_t st = (hipStream_t)stream;
@@ -643,6 +652,9 @@ extern "C" int32_t rm_adjoint(const rm_scene *scene, const rm_render_args *a, rm
     if (g->replay && (rc = fill(g->replay, np * 3, st))) return rc;
     AdjArgs G;
     G.S = scene->view;
+    // k_adjoint's dynamic LDS holds the material-gradient sums: its walks keep
+    // their stacks in scratch (trace_walk<..., false>), and the view says so
+    G.S.walk_lds = 0;
     G.A = rargs_from(a);
     G.Fbase = f->buf[RM_BUF_BASE_COLOR].as<float>();
     G.drop = f->drop.as<int32_t>();

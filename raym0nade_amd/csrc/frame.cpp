@@ -22,6 +22,11 @@ extern "C" int32_t rm_frame_create(int32_t width, int32_t height, rm_frame **out
         if ((st = f->buf[b].alloc(bytes))) { delete f; return st; }
         if (hipMemset(f->buf[b].p, 0, bytes) != hipSuccess) { delete f; return fail(RM_ERR_HIP, "hipMemset"); }
     }
+    if ((st = f->stats.alloc(RM_STAT_COUNT * sizeof(uint64_t)))) { delete f; return st; }
+    if (hipMemset(f->stats.p, 0, RM_STAT_COUNT * sizeof(uint64_t)) != hipSuccess) {
+        delete f;
+        return fail(RM_ERR_HIP, "hipMemset");
+    }
     *out = f;
     return RM_OK;
 }
@@ -62,6 +67,13 @@ extern "C" int32_t rm_frame_upload(rm_frame *f, int32_t which, const float *host
     // the frame no longer holds the render the adjoint would differentiate
     f->stamped = false;
     f->rec_valid = false;
+    return RM_OK;
+}
+
+extern "C" int32_t rm_frame_get_stats(const rm_frame *f, uint64_t *stats) {
+    if (!f || !stats) return fail(RM_ERR_INVALID, "rm_frame_get_stats: NULL argument");
+    RM_HIP(hipDeviceSynchronize());
+    RM_HIP(hipMemcpy(stats, f->stats.p, RM_STAT_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return RM_OK;
 }
 

@@ -43,6 +43,7 @@ struct FrameView {
     int32_t *drop;  // per pixel: name (s * 8 + k) of the dropped indirect light sample, or -1
     float4 *rec;    // null, or per (pixel, indirect sample): sum dd | sum ds | sum w L
     int32_t rec_stride;
+    unsigned long long *stats;  // RM_STAT_COUNT counters of the render (rm_frame_get_stats)
 };
 
 __device__ __forceinline__ void st3(float *p, int64_t i, V3 v) {

@@ -46,41 +46,89 @@ constexpr int NCOUNT = 32;   // queue counters per round (>= max_depth + 1)
 // Medium (src/render.cpp:13-42): a multimap id -> (ior, absorb) holding air
 // (-1, 1.0, white) plus the entered dielectrics, kept sorted by id with equal
 // ids in insertion order.  Air contributes exact factors, so only the entered
-// media are stored (at most MED_CAP; deeper nesting is dropped).
-constexpr int MED_CAP = 3;
+// media are stored.  Every entry's ior is its material's ior and its absorb
+// the material's transmitting colour -- the baseColor of a transmitting hit
+// (src/material.cpp:300-335) -- nudged like the primary albedo
+// (src/render.cpp:492-495) when the medium was entered at the primary vertex,
+// whose baseColor is the nudged one.  So an entry is the material id, with
+// MED_NUDGED for the nudge, and ior and absorb are read from the material.
+// A path enters at most one medium per vertex that continues, and at most
+// max_depth <= 16 vertices continue, so 16 entries hold the reference's
+// stack exactly; an insert into a full stack (only possible at the last
+// vertex, where nothing reads the stack afterwards) is dropped and counted.
+// The entries live in registers: every loop below is unrolled, so each index
+// is a constant.
+constexpr int MED_CAP = 16;
+constexpr uint32_t MED_NUDGED = 0x80000000u;
 struct Med {
     int n;
-    int id[MED_CAP];
-    float ior[MED_CAP];
-    V3 ab[MED_CAP];
+    uint32_t e[MED_CAP];
 };
-__device__ __forceinline__ float med_ior(const Med &m) {
-    float r = 1.0f;
-    for (int i = 0; i < m.n; ++i) r = gmax(r, m.ior[i]);  // std::max(ior, medium.ior)
-    return r;
+__device__ __forceinline__ int med_key(uint32_t e) { return (int)(e & ~MED_NUDGED); }
+__device__ __forceinline__ void med_clear(Med &m) {
+    m.n = 0;
+#pragma unroll
+    for (int i = 0; i < MED_CAP; ++i) m.e[i] = 0u;
 }
-__device__ __forceinline__ V3 med_absorb(const Med &m) {
-    V3 a = v3(1.0f);
-    for (int i = 0; i < m.n; ++i) a *= m.ab[i];
+__device__ __forceinline__ V3 med_entry_absorb(const SceneView &S, uint32_t e) {
+    V3 a = ld3(S.mats[med_key(e)].trans);
+    if (e & MED_NUDGED) nudge_base(a);
     return a;
 }
-__device__ __forceinline__ void med_erase(Med &m, int id) {
-    int k = 0;
-    for (int i = 0; i < m.n; ++i)
-        if (m.id[i] != id) {
-            m.id[k] = m.id[i]; m.ior[k] = m.ior[i]; m.ab[k] = m.ab[i]; ++k;
-        }
-    m.n = k;
+__device__ __forceinline__ float med_ior(const SceneView &S, const Med &m) {
+    float r = 1.0f;
+#pragma unroll
+    for (int i = 0; i < MED_CAP; ++i)
+        if (i < m.n) r = gmax(r, S.mats[med_key(m.e[i])].ior);  // std::max(ior, medium.ior)
+    return r;
 }
-__device__ __forceinline__ void med_insert(Med &m, int id, float ior, V3 ab) {
-    if (m.n >= MED_CAP) return;
-    int pos = m.n;
-    while (pos > 0 && m.id[pos - 1] > id) {
-        m.id[pos] = m.id[pos - 1]; m.ior[pos] = m.ior[pos - 1]; m.ab[pos] = m.ab[pos - 1];
-        --pos;
+__device__ __forceinline__ V3 med_absorb(const SceneView &S, const Med &m) {
+    V3 a = v3(1.0f);
+#pragma unroll
+    for (int i = 0; i < MED_CAP; ++i)
+        if (i < m.n) a *= med_entry_absorb(S, m.e[i]);
+    return a;
+}
+// multimap::erase(id): the entries of `id` form one run [lo, lo + c) of the
+// sorted stack; the tail moves down by c in shifts of 1, 2, 4, 8 and 16.
+__device__ __forceinline__ void med_erase(Med &m, int id) {
+    int lo = 0, c = 0;
+#pragma unroll
+    for (int i = 0; i < MED_CAP; ++i)
+        if (i < m.n) {
+            const int k = med_key(m.e[i]);
+            lo += k < id ? 1 : 0;
+            c += k == id ? 1 : 0;
+        }
+    if (c == 0) return;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+        const int sft = 1 << b;
+        if (c & sft) {
+#pragma unroll
+            for (int i = 0; i < MED_CAP; ++i)
+                if (i >= lo) m.e[i] = i + sft < MED_CAP ? m.e[i + sft] : 0u;
+        }
     }
-    m.id[pos] = id; m.ior[pos] = ior; m.ab[pos] = ab;
+    m.n -= c;
+}
+// multimap::emplace(id, ...): after the entries with keys <= id.  Returns
+// false when the stack is full and the entry is dropped (counted as
+// RM_STAT_MEDIUM_OVERFLOW by the caller).
+__device__ __forceinline__ bool med_insert(Med &m, int id, bool nudged) {
+    if (m.n >= MED_CAP) return false;
+    int pos = 0;
+#pragma unroll
+    for (int i = 0; i < MED_CAP; ++i)
+        if (i < m.n && med_key(m.e[i]) <= id) ++pos;
+    const uint32_t x = (uint32_t)id | (nudged ? MED_NUDGED : 0u);
+#pragma unroll
+    for (int i = MED_CAP - 1; i >= 0; --i) {
+        const uint32_t below = i > 0 ? m.e[i - 1] : 0u;
+        m.e[i] = i > pos ? below : (i == pos ? x : m.e[i]);
+    }
     ++m.n;
+    return true;
 }
 
 // getAbsorb (src/render.cpp:83-87)
@@ -92,17 +140,21 @@ __device__ __forceinline__ V3 get_absorb(V3 a, float dis) {
     return V3{powf(b.x, k), powf(b.y, k), powf(b.z, k)};
 }
 
-// calcEta (src/render.cpp:89-99)
-__device__ __forceinline__ void calc_eta(Med &m, Surf &s) {
-    float e1 = med_ior(m), e2;
+// calcEta (src/render.cpp:89-99) at a transmitting hit, whose eta and
+// baseColor are its material's ior and transmitting colour; returns false
+// when the re-inserted entry did not fit.
+__device__ __forceinline__ bool calc_eta(const SceneView &S, Med &m, Surf &s) {
+    float e1 = med_ior(S, m), e2;
+    bool ok = true;
     if (s.entering) {
         e2 = gmax(e1, s.eta);
     } else {
         med_erase(m, s.id);
-        e2 = med_ior(m);
-        med_insert(m, s.id, s.eta, s.base);
+        e2 = med_ior(S, m);
+        ok = med_insert(m, s.id, false);
     }
     s.eta = e1 / e2;
+    return ok;
 }
 
 // ---------------------------------------------------------------- buffers
@@ -111,7 +163,8 @@ struct PathQ {
 };
 // f0 o.xyz | pixel      f1 d.xyz | sample     f2 T.xyz | W        f3 p0.xyz | rf
 // f4 dPdx  | rng n      f5 dPdy  | set        f6 dDdx  | depth+exclude<<8
-// f7 dDdy  | medium n + job<<8              f8-f11 medium entries (id, ior, absorb) x 3
+// f7 dDdy  | medium n + job<<8              f8-f11 medium entries 0..15
+// (written and read only when the path is inside a medium)
 
 struct Path {
     V3 o, d, T, p0;
@@ -135,11 +188,12 @@ __device__ __forceinline__ void store_path(const PathQ &Q, uint32_t q, const Pat
     Q.f[5][q] = make_float4(P.diff.dPdy.x, P.diff.dPdy.y, P.diff.dPdy.z, ub(P.set));
     Q.f[6][q] = make_float4(P.diff.dDdx.x, P.diff.dDdx.y, P.diff.dDdx.z, ub((uint32_t)P.depth | (P.exclude ? 256u : 0u)));
     Q.f[7][q] = make_float4(P.diff.dDdy.x, P.diff.dDdy.y, P.diff.dDdy.z, ub((uint32_t)P.med.n | ((uint32_t)P.job << 8)));
-    const Med &m = P.med;
-    Q.f[8][q] = make_float4(ub((uint32_t)m.id[0]), m.ior[0], m.ab[0].x, m.ab[0].y);
-    Q.f[9][q] = make_float4(m.ab[0].z, ub((uint32_t)m.id[1]), m.ior[1], m.ab[1].x);
-    Q.f[10][q] = make_float4(m.ab[1].y, m.ab[1].z, ub((uint32_t)m.id[2]), m.ior[2]);
-    Q.f[11][q] = make_float4(m.ab[2].x, m.ab[2].y, m.ab[2].z, 0.0f);
+    if (P.med.n > 0) {
+        const uint32_t *e = P.med.e;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            Q.f[8 + k][q] = make_float4(ub(e[4 * k]), ub(e[4 * k + 1]), ub(e[4 * k + 2]), ub(e[4 * k + 3]));
+    }
 }
 
 __device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
@@ -157,13 +211,18 @@ __device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
     P.depth = (int)(dx & 255u);
     P.exclude = (dx & 256u) != 0;
     P.diff.dDdy = V3{h.x, h.y, h.z};
+    med_clear(P.med);
     P.med.n = (int)(bu(h.w) & 255u);
     P.job = (int)(bu(h.w) >> 8);
     if (P.med.n > 0) {
-        float4 m0 = Q.f[8][q], m1 = Q.f[9][q], m2 = Q.f[10][q], m3 = Q.f[11][q];
-        P.med.id[0] = (int)bu(m0.x); P.med.ior[0] = m0.y; P.med.ab[0] = V3{m0.z, m0.w, m1.x};
-        P.med.id[1] = (int)bu(m1.y); P.med.ior[1] = m1.z; P.med.ab[1] = V3{m1.w, m2.x, m2.y};
-        P.med.id[2] = (int)bu(m2.z); P.med.ior[2] = m2.w; P.med.ab[2] = V3{m3.x, m3.y, m3.z};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 m = Q.f[8 + k][q];
+            P.med.e[4 * k] = bu(m.x);
+            P.med.e[4 * k + 1] = bu(m.y);
+            P.med.e[4 * k + 2] = bu(m.z);
+            P.med.e[4 * k + 3] = bu(m.w);
+        }
     }
     return P;
 }
@@ -218,6 +277,25 @@ __device__ __forceinline__ int job_of(const Job *J, int nj, int64_t p) {
     return lo;
 }
 
+// Adds per-lane counts v (v < 2^BITS) to counter k of the frame of each
+// lane's job (RM_STAT_*): one atomic per distinct job among the counting lanes
+// of the wave.  Every active lane of the wave calls it; a lane with v == 0
+// may pass any job.
+template <int BITS>
+__device__ __forceinline__ void job_stat_add(const Job *J, int myjob, uint32_t v, int k) {
+    uint64_t pend = __ballot(v != 0u);
+    while (pend) {
+        const int lead = (int)__ffsll((unsigned long long)pend) - 1;
+        const int j = __builtin_amdgcn_readfirstlane(__shfl(myjob, lead, 64));
+        const bool mine = v != 0u && myjob == j;
+        unsigned long long sum = 0;
+        for (int b = 0; b < BITS; ++b)
+            sum += (unsigned long long)__popcll(__ballot(mine && ((v >> b) & 1u))) << b;
+        if ((int)(threadIdx.x & 63u) == lead) atomicAdd(J[j].F.stats + k, sum);
+        pend &= ~__ballot(mine);
+    }
+}
+
 // p: global pixel (work buffers), lp: job-local pixel (frame buffers)
 __device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F, int64_t p, int64_t lp) {
     Surf s = surf_default();
@@ -254,6 +332,9 @@ __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int n
     RayPre r = ray_pre(A.pos, Dir);
     float t = INFINITY;
     int32_t slot = model_closest<true, TPS>(S, A.pos, Dir, r, t);
+    const int ji = (int)(&jb - J);
+    job_stat_add<1>(J, ji, 1u, RM_STAT_PRIMARY_RAYS);
+    job_stat_add<1>(J, ji, slot >= 0 ? 1u : 0u, RM_STAT_HITS);
     Surf s = surf_default();
     float4 zero = make_float4(0, 0, 0, 0);
     if (slot < 0) {
@@ -311,6 +392,7 @@ __global__ __launch_bounds__(RB, MINW) void k_direct(const Job *__restrict__ J, 
     Acc4 d{0, 0, 0, 0}, sp{0, 0, 0, 0};
     V3 pos = ld3(F.pos + p * 3);
     V3 em = ld3(F.emis + p * 3);
+    uint32_t nrays = 0;
     if (any_finite(pos) && !(length(em) > 0.0f)) {
         Surf s = pixel_surf(P, F, gp, p);
         V3 in = normalize(s.pos - A.pos);
@@ -322,10 +404,12 @@ __global__ __launch_bounds__(RB, MINW) void k_direct(const Job *__restrict__ J, 
             for (int T = 0; T < cnt; ++T) {
                 Stream g{A.seed, TAG_DIRECT, (uint32_t)p, (uint32_t)T, 0u};
                 LSample ls;
-                if (direct_sample<false, true, TPS>(S, s, V, total, cnt, g, ls, tc)) accumulate(s.base, ls, d, sp);
+                if (direct_sample<false, true, TPS>(S, s, V, total, cnt, g, ls, tc, nullptr, &nrays))
+                    accumulate(s.base, ls, d, sp);
             }
         }
     }
+    job_stat_add<31>(J, (int)(&jb - J), nrays, RM_STAT_SHADOW_RAYS);
     calc_var(d, A.exposure);
     calc_var(sp, A.exposure);
     reinterpret_cast<float4 *>(F.Dd)[p] = make_float4(d.r, d.g, d.b, d.v);
@@ -435,6 +519,7 @@ __global__ __launch_bounds__(RB, MINW) void k_direct_gen(const Job *__restrict__
         }
     }
     if (!ok) SQ.ref[g] = SH_NONE;
+    job_stat_add<1>(J, (int)(&jb - J), ok ? 1u : 0u, RM_STAT_SHADOW_RAYS);
 }
 
 __global__ __launch_bounds__(RB) void k_direct_add(const Job *__restrict__ J, int nj, int64_t npx, PixBufs P, int C,
@@ -493,6 +578,7 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
     int64_t gp = 0;
     uint32_t s_idx = 0;
     int myjob = -1;
+    bool ovf = false;
     if (g < total_sets) {
         gp = P.set_pix[g];
         int M = P.M[gp];
@@ -515,7 +601,6 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
             float hit_t = length(s.pos - A.pos);
             V3 V = -in;
             float rf = s.roughness * 1.0f;
-            float ior = s.eta;
             s.eta = 1.0f / s.eta;
             float P_reflect = 1.0f, Fr = 0.0f;
             if (s.opacity < T_EPS) {
@@ -525,7 +610,7 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
             }
             Stream rng{A.seed, TAG_INDIRECT, (uint32_t)p, s_idx, 0u};
             Med med;
-            med.n = 0;
+            med_clear(med);
             V3 nd, bp;
             int fails = 0;
             Diff next;
@@ -539,8 +624,9 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
                 bp *= (1.0f - Fr);
                 bp /= (1.0f - P_reflect);
                 next = cam;
+                // the primary vertex's baseColor is the nudged one (k_primary)
                 if (s.entering)
-                    med_insert(med, s.id, ior, s.base);
+                    ovf = !med_insert(med, s.id, true);
                 else
                     med_erase(med, s.id);
             }
@@ -565,6 +651,9 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
     });
     uint32_t q = append(count, want);
     if (want) store_path(Q, q, path);
+    job_stat_add<1>(J, myjob, want ? 1u : 0u, RM_STAT_PATHS);
+    job_stat_add<1>(J, myjob, ovf ? 1u : 0u, RM_STAT_MEDIUM_OVERFLOW);
+    if (want && path.med.n > 0) atomicMax(J[myjob].F.stats + RM_STAT_MEDIUM_MAX, (unsigned long long)path.med.n);
 }
 
 // Light sample of a finished path -> its accumulator set.
@@ -694,14 +783,23 @@ __device__ __forceinline__ float4 *rec_slot(const Job &jb, uint32_t gp, uint32_t
 // as everywhere else, so the hits are bit-identical.
 // LDS: the walk's first lds_entries pending entries in the block's dynamic
 // LDS (lds_entries * RB * 8 bytes), deeper ones in per-lane scratch.
-template <bool LDS, int TPS>
+// COUNT: each walk's node visits and tests go to its frame's RM_STAT_WALK_*.
+__device__ __forceinline__ void add_walk_counts(unsigned long long *stats, TraceCounts &c) {
+    atomicAdd(stats + RM_STAT_WALK_INTERIOR, (unsigned long long)c.interior);
+    atomicAdd(stats + RM_STAT_WALK_LEAVES, (unsigned long long)c.leaves);
+    atomicAdd(stats + RM_STAT_WALK_BOXES, (unsigned long long)c.boxes);
+    atomicAdd(stats + RM_STAT_WALK_TRIS, (unsigned long long)c.tris);
+    c = TraceCounts{};
+}
+
+template <bool LDS, int TPS, bool COUNT = false>
 __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ Q, const uint32_t *count_in,
                                                uint32_t *work, uint2 *__restrict__ hit, int fetch_min,
                                                int lds_entries) {
     const uint32_t n = *count_in;
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
-    Walker<false, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
+    Walker<false, COUNT, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
     if constexpr (LDS)
         w.stk.bind(&ent, lds_entries);
     else
@@ -709,6 +807,7 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
     RayPre r;
     SceneView S;  // per lane: the nodes, triangles and root of its path's scene
     int64_t q = -1;
+    int job = 0;
     bool exhausted = false;
     TraceCounts c{};
     while (true) {
@@ -728,7 +827,7 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
                 if (slot < n) {
                     q = slot;
                     float4 a = Q.f[0][slot], b = Q.f[1][slot];
-                    int job = (int)(bu(Q.f[7][slot].w) >> 8);
+                    job = (int)(bu(Q.f[7][slot].w) >> 8);
                     const SceneView &SJ = J[job].S;
                     S.nodes = SJ.nodes;
                     S.tris = SJ.tris;
@@ -740,6 +839,7 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
         }
         if (q >= 0 && w.step(S, r, T_EPS, -INFINITY, c)) {
             hit[q] = make_uint2(bu(w.tmax), (uint32_t)w.best);
+            if (COUNT) add_walk_counts(J[job].F.stats, c);
             q = -1;
         }
     }
@@ -768,6 +868,8 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
         hq = hit[q];
     }
     bool nee_out = false;
+    bool hit_any = false;
+    uint32_t ovf = 0;
     NeeRec nee_rec;
     for_each_job(live ? path.job : -1, live, [&](int jid) {
         const Job &jb = J[jid];
@@ -792,23 +894,23 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
         if (slot < 0) {
             if (!(path.exclude || !S.sky)) emit(LSample{v3(1.0f), sky_get(S, path.d), 1.0f});
         } else {
+            hit_any = true;
             Surf s = surf_default();
             s.pos = path.o + path.d * t;
             V3 dPdx, dPdy;
             hit_info(S, slot, path.d, t, path.diff, dPdx, dPdy, s);
             V3 V = -path.d;
-            float ior = s.eta;
             path.rf = gmax(path.rf, 1.0f * s.roughness);
             s.roughness = gmax(s.roughness, path.rf);
             const float P_RR = 1.0f + (0.5f - 1.0f) * sqrtf(s.roughness);
             bool doNEE = P_RR < 0.9f && depth >= A.rr_min;
-            V3 absorb = get_absorb(med_absorb(path.med), t);
+            V3 absorb = get_absorb(med_absorb(S, path.med), t);
             if (length(s.emission) > T_EPS) {
                 if (!path.exclude) emit(LSample{absorb, s.emission, 1.0f});
             } else {
                 float P_reflect = 1.0f, Fr = 0.0f;
                 if (s.opacity < T_EPS) {
-                    calc_eta(path.med, s);
+                    if (!calc_eta(S, path.med, s)) ++ovf;
                     V3 rd;
                     precise_refraction(s, V, rd, Fr);
                     P_reflect = path.med.n == 0 ? 0.24f + (1.0f - 0.24f) * Fr : Fr;
@@ -852,10 +954,11 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
                         next = path.diff;
                         bp /= (1.0f - P_reflect);
                         if (doNEE) bp /= P_RR;
-                        if (s.entering)
-                            med_insert(path.med, s.id, ior, s.base);
-                        else
+                        if (s.entering) {
+                            if (!med_insert(path.med, s.id, false)) ++ovf;
+                        } else {
                             med_erase(path.med, s.id);
+                        }
                     }
                 }
                 if (nee) {
@@ -886,8 +989,14 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
     });
     uint32_t slot_out = append(count_out, cont);
     if (cont) store_path(Qout, slot_out, path);
+    if (cont && path.med.n > 1)  // rare: a path inside two or more media
+        atomicMax(J[path.job].F.stats + RM_STAT_MEDIUM_MAX, (unsigned long long)path.med.n);
     uint32_t slot_nee = append(nee_count, nee_out);
     if (nee_out) store_nee(NQ, slot_nee, path, nee_rec);
+    const int myjob = live ? path.job : -1;
+    job_stat_add<1>(J, myjob, live ? 1u : 0u, RM_STAT_EXTENSION_RAYS);
+    job_stat_add<1>(J, myjob, hit_any ? 1u : 0u, RM_STAT_HITS);
+    job_stat_add<2>(J, myjob, ovf, RM_STAT_MEDIUM_OVERFLOW);
   }
 }
 
@@ -910,6 +1019,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
     float4 f5 = make_float4(0, 0, 0, 0);
     if (live) f5 = NQ.f[5][q];
     const uint32_t flags = bu(f5.w);
+    uint32_t nrays = 0;
     for_each_job(live ? (int)(flags >> 8) : -1, live, [&](int jid) {
         const Job &jb = J[jid];
         const SceneView &S = jb.S;
@@ -940,7 +1050,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
             TraceCounts tc{};
             for (int Ti = 0; Ti < cnt; ++Ti) {
                 LSample x;
-                if (direct_sample<false, true, TPS>(S, s, V, total, cnt, rng, x, tc)) {
+                if (direct_sample<false, true, TPS>(S, s, V, total, cnt, rng, x, tc, nullptr, &nrays)) {
                     x.bsdf *= factor;  // scaling(samples, factor)
                     add_light_sample(P, AB, pix, set, smp, k, T, W, p0, x, extra_on, extra, rec_slot(jb, pix, smp));
                     ++k;
@@ -948,6 +1058,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
             }
         }
     });
+    job_stat_add<3>(J, live ? (int)(flags >> 8) : -1, nrays, RM_STAT_SHADOW_RAYS);
   }
 }
 
@@ -966,6 +1077,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
     float4 f5 = make_float4(0, 0, 0, 0);
     if (live) f5 = NQ.f[5][q];
     const uint32_t flags = bu(f5.w);
+    uint32_t nrays = 0;
     for_each_job(live ? (int)(flags >> 8) : -1, live, [&](int jid) {
         const Job &jb = J[jid];
         const SceneView &S = jb.S;
@@ -998,23 +1110,25 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
                 const V3 L = x.light * x.bsdf;
                 SQ.x[e] = make_float4(L.x, L.y, L.z, x.weight);
                 shadow_entry<TPS>(SQ, e, S, jid, s.pos, sr, queue_depth, queued);
+                ++nrays;
             } else {
                 SQ.ref[e] = SH_NONE;
             }
         }
     });
+    job_stat_add<3>(J, live ? (int)(flags >> 8) : -1, nrays, RM_STAT_SHADOW_RAYS);
   }
 }
 
 // Any-hit walk of every shadow ray of the round (the k_extend scheme with
 // ANY: it stops once the running closest t is below aim).
-template <bool LDS, int TPS>
+template <bool LDS, int TPS, bool COUNT = false>
 __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, ShadowQ SQ, const uint32_t *count,
                                                uint32_t *work, int fetch_min, int lds_entries) {
     const uint32_t n = *count;
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
-    Walker<true, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
+    Walker<true, COUNT, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
     if constexpr (LDS)
         w.stk.bind(&ent, lds_entries);
     else
@@ -1023,6 +1137,7 @@ __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, Shadow
     SceneView S;  // per lane: the nodes, triangles and root of its ray's scene
     float aim = 0.0f;
     int64_t q = -1;
+    uint32_t job = 0;
     bool exhausted = false;
     TraceCounts c{};
     while (true) {
@@ -1042,7 +1157,8 @@ __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, Shadow
                 if (slot < n) {
                     q = slot;
                     const float4 oo = SQ.o[slot], dd = SQ.d[slot];
-                    const SceneView &SJ = J[bu(dd.w)].S;
+                    job = bu(dd.w);
+                    const SceneView &SJ = J[job].S;
                     S.nodes = SJ.nodes;
                     S.tris = SJ.tris;
                     S.root_leaf = SJ.root_leaf;
@@ -1054,6 +1170,7 @@ __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, Shadow
         }
         if (q >= 0 && w.step(S, r, T_EPS, aim, c)) {
             SQ.vis[q] = (w.tmax < aim) ? 0 : 1;
+            if (COUNT) add_walk_counts(J[job].F.stats, c);
             q = -1;
         }
     }
@@ -1105,6 +1222,11 @@ __global__ __launch_bounds__(RB) void k_clear_sets(AccBufs AB, uint32_t n) {
     AB.a[2][g] = make_float4(0.0f, -1.0f, 0.0f, 0.0f);
     AB.a[3][g] = z;
     AB.a[4][g] = z;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n) {
@@ -1249,6 +1371,7 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, int64_
     F.Id = f->buf[RM_BUF_ID].as<float>();
     F.Is = f->buf[RM_BUF_IS].as<float>();
     F.drop = f->drop.as<int32_t>();
+    F.stats = f->stats.as<unsigned long long>();
     F.rec = nullptr;
     F.rec_stride = 0;
     f->rec_valid = false;
@@ -1340,8 +1463,15 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     for (int i = 0; i < nj; ++i) max_depth = std::max(max_depth, (int)J[i].S.depth);
     const int nl = lds_walk ? std::min(max_depth, tuning().lds_stack_entries) : 0;
     const size_t walk_lds = (size_t)nl * RB * 8;
+    // LdsStack's launch contract: every kernel below that walks is launched
+    // with blocks of RB lanes and walk_lds bytes of dynamic LDS
+    if (RB != WALK_BLOCK || walk_lds < (size_t)nl * WALK_BLOCK * 8 || nl > MAX_STACK)
+        return fail(RM_ERR_INVALID, "render: LDS walk launch (%d lanes, %zu bytes) does not hold %d entries", RB,
+                    walk_lds, nl);
     for (int i = 0; i < nj; ++i) J[i].S.walk_lds = nl;
     RM_HIP(hipMemcpyAsync(dJ, J.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st));
+    for (int i = 0; i < nj; ++i)
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(RM_STAT_COUNT), 0, st, J[i].F.stats, (int)RM_STAT_COUNT);
     for (int i = 0; i < nj; ++i)
         if (J[i].F.rec) {
             const int64_t n4 = J[i].np * J[i].F.rec_stride * 3;
@@ -1376,7 +1506,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     // (on shallow trees a walk costs less than queueing it), and keeps the
     // single-kernel passes when no job is that deep
     constexpr int SPLIT_DEPTH = 12;
-    const int split_mode = tuning().nee_split;
+    // counting renders send every shadow ray through the counting k_shadow
+    const bool counting = tuning().render_count != 0;
+    const int split_mode = counting ? 1 : tuning().nee_split;
     const bool split = (split_mode == 1 || (split_mode == 2 && max_depth >= SPLIT_DEPTH)) &&
                        total * NEE_MAX <= 0xffffffffLL && total_px * dC <= 0xffffffffLL;
     const int queue_depth = split_mode == 1 ? 0 : SPLIT_DEPTH;
@@ -1437,10 +1569,14 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     const bool capped = tuning().render_min_waves == 4;
     const unsigned gpers = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
     const int fetch_min = tuning().fetch_min;
-    auto shadow = lds_walk ? (tps == 1 ? k_shadow<true, 1> : tps == 2 ? k_shadow<true, 2>
-                              : tps == 4 ? k_shadow<true, 4> : k_shadow<true, 0>)
-                           : (tps == 1 ? k_shadow<false, 1> : tps == 2 ? k_shadow<false, 2>
-                              : tps == 4 ? k_shadow<false, 4> : k_shadow<false, 0>);
+    auto shadow = counting ? (lds_walk ? (tps == 1 ? k_shadow<true, 1, true> : tps == 2 ? k_shadow<true, 2, true>
+                                          : tps == 4 ? k_shadow<true, 4, true> : k_shadow<true, 0, true>)
+                                       : (tps == 1 ? k_shadow<false, 1, true> : tps == 2 ? k_shadow<false, 2, true>
+                                          : tps == 4 ? k_shadow<false, 4, true> : k_shadow<false, 0, true>))
+                  : (lds_walk ? (tps == 1 ? k_shadow<true, 1> : tps == 2 ? k_shadow<true, 2>
+                                 : tps == 4 ? k_shadow<true, 4> : k_shadow<true, 0>)
+                              : (tps == 1 ? k_shadow<false, 1> : tps == 2 ? k_shadow<false, 2>
+                                 : tps == 4 ? k_shadow<false, 4> : k_shadow<false, 0>));
     if (split) {
         uint32_t *claims = counts + 3 * NCOUNT + 8;  // and [9]: queued rays; both zeroed for each chunk
         auto direct_gen = capped ? (tps ? k_direct_gen<4, 2> : k_direct_gen<4, 0>)
@@ -1478,10 +1614,14 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
         auto nee = capped ? (tps ? k_nee<4, 2> : k_nee<4, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
         auto nee_gen = capped ? (tps ? k_nee_gen<4, 2> : k_nee_gen<4, 0>) : (tps ? k_nee_gen<1, 2> : k_nee_gen<1, 0>);
         auto nee_add = capped ? k_nee_add<4> : k_nee_add<1>;
-        auto extend_lds = tps == 1 ? k_extend<true, 1> : tps == 2 ? k_extend<true, 2>
-                        : tps == 4 ? k_extend<true, 4> : k_extend<true, 0>;
-        auto extend_scratch = tps == 1 ? k_extend<false, 1> : tps == 2 ? k_extend<false, 2>
-                            : tps == 4 ? k_extend<false, 4> : k_extend<false, 0>;
+        auto extend_lds = counting ? (tps == 1 ? k_extend<true, 1, true> : tps == 2 ? k_extend<true, 2, true>
+                                      : tps == 4 ? k_extend<true, 4, true> : k_extend<true, 0, true>)
+                                   : (tps == 1 ? k_extend<true, 1> : tps == 2 ? k_extend<true, 2>
+                                      : tps == 4 ? k_extend<true, 4> : k_extend<true, 0>);
+        auto extend_scratch = counting ? (tps == 1 ? k_extend<false, 1, true> : tps == 2 ? k_extend<false, 2, true>
+                                          : tps == 4 ? k_extend<false, 4, true> : k_extend<false, 0, true>)
+                                       : (tps == 1 ? k_extend<false, 1> : tps == 2 ? k_extend<false, 2>
+                                          : tps == 4 ? k_extend<false, 4> : k_extend<false, 0>);
         for (int j = 0; j < R; ++j) {
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, counts, 4 * NCOUNT);
             hipLaunchKernelGGL(k_spawn, dim3(gs), dim3(RB), 0, st, (const Job *)dJ, nj, P, j, nsets, Q[0], counts);

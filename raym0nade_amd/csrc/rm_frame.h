@@ -20,6 +20,7 @@ struct rm_frame {
     float dof_focus = 0.0f, dof_coc = 0.0f;
     float dof_cam[3] = {0.0f, 0.0f, 0.0f};
     rm::DevBuf drop;  // per pixel: the dropped outlier light sample (render -> adjoint)
+    rm::DevBuf stats; // RM_STAT_COUNT uint64 counters of the last render
     rm::DevBuf adj;   // adjoint scratch: texel gradients of every mip level, material gradients
     // per indirect sample (rm_frame_set_recording): sums over its light samples
     // of the diffuse and specular increments and of w L, [pixel][sample] x 3 float4

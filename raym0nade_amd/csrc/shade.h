@@ -804,12 +804,15 @@ __device__ __noinline__ bool model_blocked(const SceneView &S, V3 o, V3 d, float
 }
 
 // One light sample with its shadow ray traced here.  Returns false when the
-// reference `continue`s (no sample, or the shadow ray is blocked).
+// reference `continue`s (no sample, or the shadow ray is blocked).  `nrays`,
+// when given, counts the shadow rays traced.
 template <bool COUNT, bool LDSW = true, int TPS = 0>
 __device__ __forceinline__ bool direct_sample(const SceneView &S, const Surf &s, V3 V, float total, int cnt, Stream &g,
-                                              LSample &out, TraceCounts &tc, DirectInfo *info = nullptr) {
+                                              LSample &out, TraceCounts &tc, DirectInfo *info = nullptr,
+                                              uint32_t *nrays = nullptr) {
     ShadowRay sr;
     if (!direct_ray(S, s, V, total, cnt, g, sr, out, info)) return false;
+    if (nrays) ++*nrays;
     RayPre r = ray_pre(s.pos, sr.dir);
     return model_clear<LDSW, TPS>(S, s.pos, sr.dir, r, sr.aim);
 }

@@ -72,6 +72,26 @@ def zscores(rad, orad, spp, p_direct):
     return (tot(rad) - tot(orad)) / s
 
 
+def linear_image(gbuf, rad, exposure=1.0):
+    """Photo::shade(Full) without the tone curve, from G-buffer rows [..., GB]
+    and radiance [4][..., 4] (any leading pixel shape)."""
+    base, emis = gbuf[..., 12:15].astype(np.float64), gbuf[..., 9:12].astype(np.float64)
+    r = rad[..., :3].astype(np.float64)
+    return base * (r[0] + r[2]) + r[1] + r[3] + np.nan_to_num(emis) * exposure
+
+
+def image_metrics(gb, rad, ogb, orad, spp, p_direct, exposure=1.0):
+    """Per-pixel z-scores of the demodulated radiance (zscores) and the
+    relative MSE of the linear shaded image, sum (x - y)^2 / sum y^2, of a GPU
+    render against the oracle's render of the same pixels."""
+    z = zscores(rad, orad, spp, p_direct)
+    a = np.nan_to_num(linear_image(gb, rad, exposure))
+    b = np.nan_to_num(linear_image(ogb, orad, exposure))
+    rel_mse = float(((a - b) ** 2).sum() / max(1e-30, (b ** 2).sum()))
+    return {"max_abs_z": float(np.abs(z).max()), "frac_abs_z_lt_3": float((np.abs(z) < 3).mean()),
+            "rel_mse": rel_mse, "pixels": int(z.size)}
+
+
 def smoke(scene, oracle_scene, gen, dev):
     """Small render on cuda:0 checked against the oracle."""
     from . import RmRenderArgs

@@ -18,6 +18,7 @@ from raym0nade_amd import render_check as RC
 from raym0nade_amd.diff import DiffRenderer
 
 from adj_util import linear_image, perturbed_desc, small_args, texel_count
+from raym0nade_amd._lib import RM_ERR_UNSUPPORTED
 
 pytestmark = pytest.mark.gpu
 LIGHT = 5
@@ -46,7 +47,8 @@ def test_replay_reproduces_image(dev, gen):
 
 
 @pytest.mark.skip(reason="one run of this test ended in a GPU memory fault (raised as RmError, box ended) after the "
-                         "LDS walk-stack change; its cause is not yet shown, so the case stays off the GPU")
+                         "LDS walk-stack change; its cause is not yet shown, so the two-replay adjoint it exercises "
+                         "is refused by rm_adjoint (test_unrecorded_frame_is_refused) and the case stays off the GPU")
 def test_recorded_first_pass_equals_replayed(dev, gen):
     """With recording the render keeps each sample's sums and the adjoint
     replays it once; without, it replays twice.  Same gradients."""
@@ -207,6 +209,66 @@ def test_autograd_inverse_rendering_reduces_loss(dev, gen):
     # so individual texels are not expected to converge in 40 steps.)
     assert np.mean(losses[-5:]) < 0.8 * np.mean(losses[:5]), losses
     assert float(emis[LIGHT].mean()) > float(emis0[LIGHT].mean())
+
+
+def test_unrecorded_frame_is_refused(dev, gen):
+    """rm_adjoint needs the render's per-sample record; a frame rendered with
+    recording off is refused before any launch."""
+    R = DiffRenderer(gen, small_args(gen, 24, 8), record=False)
+    p = R.initial_params()
+    R.forward_image(*p)
+    with pytest.raises(rm.RmError) as e:
+        R.adjoint(torch.ones((24, 24, 3), device=dev), p[0].numel())
+    assert e.value.code == RM_ERR_UNSUPPORTED and "recording" in str(e.value)
+
+
+class _DescScene(rm.GeneratedScene):
+    """A scene description built in the test, in the shape DiffRenderer takes."""
+
+    def __init__(self, desc, args, keep):
+        self._h = None
+        self.desc, self.args, self._keep_arrays = desc, args, keep
+
+
+def test_cutout_scene_adjoint_matches_oracle(dev):
+    """Cornell box with an alpha cut-out grille: every walk of the replay skips
+    the grille's transparent cells (TransparentTest), so the replay equals the
+    render, and the gradients equal the FP64 oracle adjoint's for the same seed."""
+    from cutout_util import cornell_with_grille
+    g0 = rm.generate(0, 1)
+    desc, keep = cornell_with_grille(g0)
+    G = _DescScene(desc, g0.args, keep)
+    args = small_args(G, 32, 16)
+    R = DiffRenderer(G, args)
+    p = R.initial_params()
+    img = R.forward_image(*p)
+    nm = desc.n_materials
+    light = [m for m in range(nm) if max(desc.materials[m].emission) > 0][0]
+    rng = np.random.default_rng(11)
+    delta = rng.standard_normal((32, 32, 3)).astype(np.float32)
+    gt, gr, ge, rep = R.adjoint(torch.from_numpy(delta).to(dev), p[0].numel(), replay=True)
+    a, b = rep.cpu().numpy(), img.cpu().numpy()
+    assert np.isclose(a, b, rtol=1e-4, atol=1e-5 * np.abs(b).max()).mean() >= 0.999
+    gt, gr, ge = gt.cpu().numpy().astype(np.float64), gr.cpu().numpy(), ge.cpu().numpy()
+    S = O.OracleScene(desc)
+    oimg, _ = O.adjoint_dir(S, args, None, np.zeros(nm), np.zeros((nm, 3)))
+    np.testing.assert_allclose(img.cpu().numpy(), oimg, rtol=2e-3, atol=2e-3 * np.abs(oimg).max())
+
+    def check(tt, tr, te, gpu_val, what):
+        _, dimg = O.adjoint_dir(S, args, tt, tr, te)
+        ref = float((delta * dimg).sum())
+        scale = float(np.abs(delta * dimg).sum()) + 1e-12
+        assert abs(gpu_val - ref) <= 2e-2 * scale, (what, gpu_val, ref, scale)
+
+    tt = rng.standard_normal(p[0].numel()).astype(np.float32)
+    check(tt, np.zeros(nm), np.zeros((nm, 3)), float(gt @ tt), "grille texture")
+    for m in (0, nm - 1):
+        tr = np.zeros(nm)
+        tr[m] = 1.0
+        check(None, tr, np.zeros((nm, 3)), float(gr[m]), f"roughness {m}")
+    te = np.zeros((nm, 3))
+    te[light] = rng.standard_normal(3)
+    check(None, np.zeros(nm), te, float((ge * te).sum()), "emission")
 
 
 def test_adjoint_argument_errors(dev, gen):

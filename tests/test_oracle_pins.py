@@ -1,0 +1,511 @@
+"""Pins of the oracle's sampling, light, camera, hit and texture functions
+against what the mathematics fixes (CPU only).
+
+Each check computes its expected value in a way the restatement does not:
+exact CDF inversion with numpy, deterministic quadrature of the expectation
+of an estimator as the reference defines it, closed forms from the
+literature, or values worked out by hand.
+
+  * RandomDistribution (src/component.cpp:12-31): every pick is the
+    lower_bound of total * u in the fp32 prefix sums (numpy searchsorted), the
+    picks follow the weights (chi-square), pdf(i) is the weight's share.
+  * SkyBox (src/component.cpp:54-67,120-140): the weights are luminance x
+    pixel solid angle, the pdf sums to 1, get() of a pixel's centre
+    direction returns that pixel's radiance, sampleSkyBox draws pixels in
+    proportion to the pdf restricted to the normal's hemisphere.
+  * sampleDirectLight (src/sampling.cpp:396-527): the mean of the estimator
+    over many streams equals its expectation, computed by quadrature -- over
+    the sky's pixels, and over the light quad's surface (face pick, uniform
+    point, acceptance with probability cos, weight 1 / (fails + 1)).
+  * GTR1, GTR2 (src/sampling.cpp:30-41): normalised, int D cos(theta_h) = 1;
+    smithG_GGX (:43-47) equals Walter et al.'s G1 / (2 cos); sampleBRDF and
+    sampleCos (:203-269): the mean of brdfPdf over accepted draws equals its
+    expectation by quadrature (for sampleBRDF that keeps the reference's pdf,
+    which lacks the half vector's cos(theta_h)), and the tries follow
+    P(accept).
+  * renderPixel's camera ray and initRayDiff (src/render.cpp:439-477): the
+    normalised direction and its exact derivative in float64.
+  * barycentric (src/geometry.cpp:89-103) and getHitNormals
+    (src/model.cpp:240-270) on hand-built triangles.
+  * ImageData::get (src/material.cpp:48-94): texels at integer coordinates,
+    wrap-around, v flip, the mip blend and the depth clamp.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle_py as O
+import raym0nade_amd as rm
+from raym0nade_amd._lib import RmMaterial
+
+from cutout_util import make_desc
+
+L = O.lib()
+f32p, i32p = O.f32p, O.i32p
+W_RGB = np.array([0.3, 0.6, 0.1])
+
+
+def _sig():
+    vp = C.c_void_p
+    for name, res, args in [
+        ("oracle_dist_probe", None, [f32p, C.c_int32, C.c_uint32, C.c_int32, i32p, f32p, f32p]),
+        ("oracle_sky_probe", C.c_int64, [vp, C.c_int32, f32p, C.c_int64, f32p]),
+        ("oracle_sky_sample", None, [vp, f32p, C.c_uint32, C.c_int32, f32p]),
+        ("oracle_direct_light", None, [vp, f32p, f32p, C.c_int32, C.c_uint32, C.c_int32, C.c_int32, f32p]),
+        ("oracle_bsdf_eval_n", None, [f32p, f32p, f32p, C.c_int64, C.c_int32, f32p]),
+        ("oracle_brdf_term", None, [C.c_int32, f32p, C.c_int64, C.c_float, f32p]),
+        ("oracle_camera", None, [vp, C.c_int32, C.c_int32, f32p]),
+        ("oracle_hit_normals", None, [f32p, f32p, f32p, f32p, f32p, f32p]),
+        ("oracle_tex_get", None, [C.c_int32, C.c_int32, f32p, f32p, C.c_int64, f32p]),
+    ]:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+_sig()
+
+
+def fp(a):
+    return a.ctypes.data_as(f32p)
+
+
+def surface(pos, shape_n, surf_n, base, rough=0.8, metal=0.0, opacity=1.0, eta=1.0, entering=True):
+    return np.float32(list(pos) + list(shape_n) + list(surf_n) + list(base) +
+                      [rough, metal, opacity, eta, 1.0 if entering else 0.0])
+
+
+def bsdf_n(surf, in_dir, dirs, which=0):
+    dirs = np.ascontiguousarray(dirs, np.float32)
+    out = np.empty_like(dirs)
+    L.oracle_bsdf_eval_n(fp(surf), fp(np.float32(in_dir)), fp(dirs), dirs.shape[0], which, fp(out))
+    return out
+
+
+def brdf_term(which, x, a):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    L.oracle_brdf_term(which, fp(x), x.size, a, fp(out))
+    return out
+
+
+# ------------------------------------------------------- RandomDistribution
+def test_random_distribution_is_lower_bound_of_the_fp32_prefix_sums():
+    rng = np.random.default_rng(1)
+    w = rng.random(257).astype(np.float32)
+    w[[3, 40, 41, 200]] = 0.0  # zero weights are never picked
+    n = 200000
+    picks = np.empty(n, np.int32)
+    us = np.empty(n, np.float32)
+    pdf = np.empty(w.size, np.float32)
+    L.oracle_dist_probe(fp(w), w.size, 7, n, picks.ctypes.data_as(i32p), fp(us), fp(pdf))
+    prefix = np.cumsum(w, dtype=np.float32)  # sequential fp32 sums, as Init forms them
+    want = np.searchsorted(prefix, np.float32(prefix[-1]) * us, side="left")
+    assert np.array_equal(picks, want)
+    assert not np.isin(picks, [3, 40, 41, 200]).any()
+    counts = np.bincount(picks, minlength=w.size)
+    keep = w > 0
+    expected = n * w[keep].astype(np.float64) / w.sum(dtype=np.float64)
+    assert stats.chisquare(counts[keep], expected * counts.sum() / expected.sum()).pvalue > 1e-4
+    diff = np.concatenate([[prefix[0]], (prefix[1:] - prefix[:-1]).astype(np.float32)])
+    assert np.array_equal(pdf, (diff / prefix[-1]).astype(np.float32))
+    assert abs(pdf.astype(np.float64).sum() - 1.0) < 1e-5
+
+
+# ------------------------------------------------------------------ sky
+SKY_W, SKY_H = 48, 24
+
+
+def sky_rgb(seed=3):
+    rng = np.random.default_rng(seed)
+    v = np.arange(SKY_H)[:, None]
+    base = 0.2 + 2.0 * np.exp(-((v - 5) / 4.0) ** 2)  # a bright band near the zenith
+    return (base[..., None] * (0.5 + rng.random((SKY_H, SKY_W, 3)))).astype(np.float32)
+
+
+def plane_under_sky(rgb):
+    """One ground quad (y = 0, 20 x 20, facing +y) under the sky `rgb`."""
+    h = 10.0
+    verts = np.float32([[[-h, 0, -h], [-h, 0, h], [h, 0, h]], [[-h, 0, -h], [h, 0, h], [h, 0, -h]]])
+    m = RmMaterial()
+    for c in range(3):
+        m.base_color[c] = 0.5
+    m.roughness, m.metallic, m.opacity, m.ior, m.albedo_tex = 0.8, 0.0, 1.0, 1.0, -1
+    d, keep = make_desc(verts, np.zeros((2, 3, 2), np.float32), [0, 0], [0, 0], [m], [])
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    keep.append(rgb)
+    d.sky_width, d.sky_height = rgb.shape[1], rgb.shape[0]
+    d.sky_rgb = fp(rgb)
+    return d, keep
+
+
+def pixel_dirs(w, h):
+    """sampleSkyBox's direction of every pixel centre, float64 [h*w][3]."""
+    v, u = np.divmod(np.arange(w * h), w)
+    phi = np.pi * (v + 0.5) / h
+    th = 2 * np.pi * (u + 0.5) / w
+    return np.stack([-np.sin(phi) * np.sin(th), np.cos(phi), np.sin(phi) * np.cos(th)], 1)
+
+
+@pytest.fixture(scope="module")
+def sky_scene():
+    rgb = sky_rgb()
+    d, keep = plane_under_sky(rgb)
+    return rgb, O.OracleScene(d), keep
+
+
+def test_sky_weights_pdf_and_get(sky_scene):
+    rgb, S, _ = sky_scene
+    n = SKY_W * SKY_H
+    data = np.empty((n, 3), np.float32)
+    assert L.oracle_sky_probe(S.h, 2, None, 0, fp(data)) == n
+    v = np.arange(n) // SKY_W
+    area = np.sin(np.pi * (v + 0.5) / SKY_H) * 2 * np.pi / n
+    np.testing.assert_allclose(data, rgb.reshape(-1, 3) * area[:, None], rtol=1e-5)  # fp32 sin and products
+    pdf = np.empty(n, np.float32)
+    L.oracle_sky_probe(S.h, 1, None, 0, fp(pdf))
+    lum = (rgb.reshape(-1, 3).astype(np.float64) * area[:, None]) @ W_RGB
+    # the pdf is a difference of fp32 prefix sums: exact up to their rounding
+    prefix = np.cumsum(lum)
+    tol = 1e-5 * lum / lum.sum() + 4 * 2.0 ** -24 * prefix / lum.sum()
+    assert (np.abs(pdf - lum / lum.sum()) <= tol).all()
+    assert abs(pdf.astype(np.float64).sum() - 1.0) < 1e-5
+    # SkyBox::get of a pixel's centre direction is that pixel's radiance
+    dirs = pixel_dirs(SKY_W, SKY_H).astype(np.float32)
+    got = np.empty_like(dirs)
+    L.oracle_sky_probe(S.h, 0, fp(dirs), n, fp(got))
+    np.testing.assert_allclose(got, rgb.reshape(-1, 3), rtol=1e-5)
+    # and of any direction, the pixel the direction falls in (directions kept
+    # away from pixel borders)
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=(20000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    th = np.arctan2(-d[:, 0], d[:, 2]) % (2 * np.pi)
+    ph = np.arccos(d[:, 1])
+    fu, fv = th / (2 * np.pi) * SKY_W, ph / np.pi * SKY_H
+    ok = (np.abs(fu - np.round(fu)) > 1e-3) & (np.abs(fv - np.round(fv)) > 1e-3)
+    d = np.ascontiguousarray(d[ok], np.float32)
+    got = np.empty_like(d)
+    L.oracle_sky_probe(S.h, 0, fp(d), d.shape[0], fp(got))
+    want = rgb[np.minimum(fv[ok].astype(int), SKY_H - 1), np.minimum(fu[ok].astype(int), SKY_W - 1)]
+    np.testing.assert_allclose(got, want, rtol=1e-5)
+
+
+def test_sky_sampling_follows_the_pdf_in_the_upper_hemisphere(sky_scene):
+    rgb, S, _ = sky_scene
+    n = SKY_W * SKY_H
+    pdf = np.empty(n, np.float32)
+    L.oracle_sky_probe(S.h, 1, None, 0, fp(pdf))
+    data = np.empty((n, 3), np.float32)
+    L.oracle_sky_probe(S.h, 2, None, 0, fp(data))
+    normal = np.float32([0.3, 0.9, -0.3])
+    normal /= np.linalg.norm(normal)
+    m = 200000
+    out = np.empty((m, 6), np.float32)
+    L.oracle_sky_sample(S.h, fp(normal), 5, m, fp(out))
+    dirs = pixel_dirs(SKY_W, SKY_H)
+    upper = dirs @ normal > 0
+    drawn = np.isfinite(out[:, 0])
+    # which pixel each accepted direction is the centre of
+    idx = np.argmin(np.linalg.norm(out[drawn, None, :3].astype(np.float64) - dirs[None, upper], axis=2), axis=1)
+    cand = np.flatnonzero(upper)
+    pix = cand[idx]
+    assert np.allclose(out[drawn, :3], dirs[pix], atol=1e-5)
+    counts = np.bincount(idx, minlength=cand.size)
+    p = pdf[upper].astype(np.float64)
+    big = p * drawn.sum() / p.sum() > 5
+    assert stats.chisquare(counts[big], (p[big] / p[big].sum()) * counts[big].sum()).pvalue > 1e-4
+    # light = premultiplied radiance / pdf
+    np.testing.assert_allclose(out[drawn, 3:], data[pix] / pdf[pix, None], rtol=1e-5)
+    # a draw fails only after 16 tries below the hemisphere
+    U = p.sum()
+    assert abs((~drawn).mean() - (1 - U) ** 16) < 5 * np.sqrt((1 - U) ** 16 / m) + 1e-6
+
+
+def test_sky_light_estimator_expectation(sky_scene):
+    """E[sum w L bsdfPdf] of sampleDirectLight with a sky = (1 - (1 - U)^16) / U
+    * sum over pixels above the surface of clamp(getBSDF(dir_i)) data_i, U the
+    pdf mass above the surface (the draw is retried up to 16 times below it
+    and divided by the unrestricted pdf)."""
+    rgb, S, _ = sky_scene
+    n = SKY_W * SKY_H
+    pdf = np.empty(n, np.float32)
+    L.oracle_sky_probe(S.h, 1, None, 0, fp(pdf))
+    data = np.empty((n, 3), np.float32)
+    L.oracle_sky_probe(S.h, 2, None, 0, fp(data))
+    nrm = np.float32([0, 1, 0])
+    surf = surface((0.3, 0.0, -0.2), nrm, nrm, (0.5, 0.5, 0.5), rough=0.6)
+    in_dir = np.float32([-0.4, -0.8, 0.2])
+    in_dir /= np.linalg.norm(in_dir)
+    dirs = pixel_dirs(SKY_W, SKY_H)
+    upper = dirs @ nrm > 0
+    U = pdf[upper].astype(np.float64).sum()
+    b = bsdf_n(surf, -in_dir, dirs[upper]).astype(np.float64)
+    lum = b @ W_RGB
+    b = np.where(lum[:, None] > 64, b * 64 / lum[:, None], b)
+    expect = (1 - (1 - U) ** 16) / U * (b * data[upper]).sum(0)
+    m = 400000
+    out = np.empty((m, 3), np.float32)
+    L.oracle_direct_light(S.h, fp(surf), fp(-in_dir), 1, 9, m, 0, fp(out))
+    mean, se = out.mean(0, dtype=np.float64), out.std(0, dtype=np.float64) / np.sqrt(m)
+    assert (np.abs(mean - expect) < 4 * se + 1e-6).all(), (mean, expect, se)
+
+
+def light_scene(emission=(15.0, 12.0, 9.0)):
+    """Floor quad (y = 0) and a light quad 0.5 x 0.5 at y = 1 facing down
+    (two faces, each split unevenly), nothing between them."""
+    h = 4.0
+    floor = [[[-h, 0, -h], [-h, 0, h], [h, 0, h]], [[-h, 0, -h], [h, 0, h], [h, 0, -h]]]
+    a, b = -0.25, 0.25
+    light = [[[a, 1, a], [b, 1, a], [b, 1, 0.1]], [[a, 1, a], [b, 1, 0.1], [a, 1, b]], [[a, 1, b], [b, 1, 0.1], [b, 1, b]]]
+    verts = np.float32(floor + light)
+    mats = []
+    for em in ((0.0, 0.0, 0.0), emission):
+        m = RmMaterial()
+        for c in range(3):
+            m.base_color[c] = 0.6
+            m.emission[c] = em[c]
+        m.roughness, m.metallic, m.opacity, m.ior, m.albedo_tex = 0.5, 0.0, 1.0, 1.0, -1
+        mats.append(m)
+    return make_desc(verts, np.zeros((5, 3, 2), np.float32), [0, 0, 1, 1, 1], [0, 0, 1, 1, 1], mats, []), verts
+
+
+def test_light_object_estimator_expectation():
+    """E[sum w L bsdfPdf] of sampleDirectLight with one light object, by
+    quadrature over the light's faces: a face is picked in proportion to
+    area * luminance, a point uniformly on it, accepted with probability
+    cos(phi) (the light's normal turned toward the point), else redrawn up to
+    16 times; an accepted sample after k failures counts with weight
+    1 / (k + 1) and light 2 pi power colour / (d^2 + 1e-3) / P, P = 1."""
+    (d, keep), verts = light_scene()
+    S = O.OracleScene(d)
+    pos = np.float32([0.4, 0.0, -0.3])
+    nrm = np.float32([0, 1, 0])
+    surf = surface(pos, nrm, nrm, (0.6, 0.6, 0.6), rough=0.5)
+    in_dir = np.float32([0.3, -0.9, 0.1])
+    in_dir /= np.linalg.norm(in_dir)
+    Le = np.array([15.0, 12.0, 9.0])
+    faces = verts[2:].astype(np.float64)
+    areas = np.linalg.norm(np.cross(faces[:, 1] - faces[:, 0], faces[:, 2] - faces[:, 0]), axis=1) / 2
+    fw = areas * (Le @ W_RGB)
+    power = fw.sum()
+    colour = (Le * areas.sum()) / ((Le * areas.sum()) @ W_RGB)
+    # quadrature points: centroids of a 60 x 60 subdivision of each face
+    M = 60
+    i, j = np.meshgrid(np.arange(M), np.arange(M), indexing="ij")
+    keep_ij = i + j < M
+    up = np.stack([(i + 1 / 3) / M, (j + 1 / 3) / M], -1)[keep_ij]
+    dn = np.stack([(i + 2 / 3) / M, (j + 2 / 3) / M], -1)[(i + j) < M - 1]
+    bary = np.concatenate([up, dn])
+    pts, wts = [], []
+    for f in range(3):
+        x = faces[f, 0] + bary[:, :1] * (faces[f, 1] - faces[f, 0]) + bary[:, 1:] * (faces[f, 2] - faces[f, 0])
+        pts.append(x)
+        # face pick probability / face area = density per unit area; each point covers area / M^2
+        wts.append(np.full(len(x), (fw[f] / power) / areas[f] * areas[f] / M ** 2))
+    x, q = np.concatenate(pts), np.concatenate(wts)
+    v = x - pos
+    dist = np.linalg.norm(v, axis=1)
+    ldir = v / dist[:, None]
+    lnorm = np.array([0.0, -1.0, 0.0])  # every light face's normal, turned toward the floor
+    c = np.clip(ldir @ -lnorm, 0, 1)    # acceptance probability cos(phi)
+    A = (q * c).sum()
+    b = bsdf_n(surf, -in_dir, ldir).astype(np.float64)
+    lum = b @ W_RGB
+    b = np.where(lum[:, None] > 64, b * 64 / lum[:, None], b)
+    g = b * (2 * np.pi * power * colour)[None] / (dist ** 2 + 1e-3)[:, None]
+    harmonic = sum((1 - A) ** k / (k + 1) for k in range(16))
+    expect = harmonic * ((q * c)[:, None] * g).sum(0)
+    m = 400000
+    out = np.empty((m, 3), np.float32)
+    L.oracle_direct_light(S.h, fp(surf), fp(-in_dir), 1, 13, m, 0, fp(out))
+    mean, se = out.mean(0, dtype=np.float64), out.std(0, dtype=np.float64) / np.sqrt(m)
+    # quadrature error of the midpoint-like rule is far below the Monte Carlo error
+    assert (np.abs(mean - expect) < 4 * se + 2e-4 * np.abs(expect)).all(), (mean, expect, se)
+
+
+# ------------------------------------------------------------------ BRDF
+@pytest.mark.parametrize("a", [0.05, 0.2, 0.5, 0.8, 1.0])
+def test_gtr2_is_normalised(a):
+    # 2 pi int_0^1 D(x) x dx = 1; x = cos(theta_h), fine grid near x = 1
+    x = np.concatenate([np.linspace(0, 0.9, 200001), 1 - np.geomspace(0.1, 1e-9, 400001)])
+    x = np.unique(x)
+    D = brdf_term(1, x.astype(np.float32), a).astype(np.float64)
+    assert abs(2 * np.pi * np.trapezoid(D * x, x) - 1.0) < 2e-4
+
+
+@pytest.mark.parametrize("a", [0.05, 0.1, 0.3, 0.7])
+def test_gtr1_is_normalised(a):
+    x = np.unique(np.concatenate([np.linspace(0, 0.9, 200001), 1 - np.geomspace(0.1, 1e-9, 400001)]))
+    D = brdf_term(0, x.astype(np.float32), a).astype(np.float64)
+    assert abs(2 * np.pi * np.trapezoid(D * x, x) - 1.0) < 2e-4
+    assert np.allclose(brdf_term(0, np.float32([0.2, 0.9]), 1.0), 1 / np.pi)
+
+
+@pytest.mark.parametrize("alpha", [0.05, 0.25, 0.6, 1.0])
+def test_smith_ggx_is_walter_g1_over_two_cos(alpha):
+    c = np.linspace(0.02, 1.0, 500)
+    g = brdf_term(2, c.astype(np.float32), alpha).astype(np.float64)
+    tan2 = (1 - c * c) / (c * c)
+    g1 = 2 / (1 + np.sqrt(1 + alpha * alpha * tan2))  # Walter et al. 2007, eq. 34
+    np.testing.assert_allclose(2 * c * g, g1, rtol=2e-6)
+
+
+def reflect_quadrature(surf, V, which, a, n_t=1200, n_p=600):
+    """(the expected brdfPdf of an accepted draw times P(accept), P(accept)) for
+    sampleBRDF (which 1) or sampleCos (which 0); accepted: L above the surface,
+    and for half vectors L.H > 0.
+
+    sampleCos draws L with density cos / pi and returns getBRDF / (cos / pi):
+    the first value is int getBRDF dw_L.  sampleBRDF draws the half vector H
+    with density D(H) cos(theta_h) (the GGX inversion of src/sampling.cpp:209-
+    212), so L = reflect(V, H) has density D cos(theta_h) / (4 L.H); but the
+    pdf it divides by is GTR2(cos theta_h) / (4 L.H) (:223), without the
+    cos(theta_h) of the half-vector density, so the first value is
+    int getBRDF cos(theta_h) dw_L."""
+    N = np.array([0.0, 0.0, 1.0])
+    t = (np.arange(n_t) + 0.5) * (np.pi / 2) / n_t
+    p = (np.arange(n_p) + 0.5) * 2 * np.pi / n_p
+    T, P = np.meshgrid(t, p, indexing="ij")
+    dw = np.sin(T) * (np.pi / 2 / n_t) * (2 * np.pi / n_p)
+    d = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    dw = dw.reshape(-1)
+    if which == 1:
+        H = d
+        Lr = 2 * (H @ V)[:, None] * H - V
+        dens_h = brdf_term(1, H[:, 2].astype(np.float32), a).astype(np.float64) * H[:, 2]
+        ok = ((Lr * H).sum(1) > 0) & (Lr @ N > 0)
+        f = bsdf_n(surf, V, Lr[ok]).astype(np.float64)
+        jac = 4 * np.abs((Lr * H).sum(1))[ok]           # dw_L = 4 |L.H| dw_H
+        integral = (f * (dw[ok] * jac * H[ok, 2])[:, None]).sum(0)
+        pacc = (dens_h * dw)[ok].sum()
+    else:
+        ok = d @ N > 0
+        f = bsdf_n(surf, V, d[ok]).astype(np.float64)
+        integral = (f * dw[ok, None]).sum(0)
+        pacc = ((d[:, 2] / np.pi) * dw)[ok].sum()
+    return integral, pacc
+
+
+@pytest.mark.parametrize("rough", [0.2, 0.5, 0.9])
+@pytest.mark.parametrize("which", [0, 1])
+def test_reflection_samplers_mean_is_the_albedo_integral(rough, which):
+    nrm = np.float32([0, 0, 1])
+    surf = surface((0, 0, 0), nrm, nrm, (0.7, 0.5, 0.3), rough=rough, metal=0.3)
+    V = np.array([0.5, 0.1, 0.86])
+    V /= np.linalg.norm(V)
+    integral, pacc = reflect_quadrature(surf, V, which, rough)
+    params = np.zeros(17, np.float32)
+    params[0:3] = nrm
+    params[3:6] = nrm
+    params[6:9] = V
+    params[9:12] = (0.7, 0.5, 0.3)
+    params[12:17] = (rough, 0.3, 1.0, 1.0, 1.0)
+    m = 60000
+    out = np.empty((m, 8), np.float32)
+    for k in range(m):
+        L.oracle_bsdf_sample(fp(params), which, 3, k, 0, fp(out[k]))
+    ok = np.isfinite(out[:, 0])
+    vals = out[ok, 3:6].astype(np.float64)
+    mean, se = vals.mean(0), vals.std(0) / np.sqrt(ok.sum())
+    want = integral / pacc
+    assert (np.abs(mean - want) < 4.5 * se + 1e-3 * np.abs(want)).all(), (mean, want, se)
+    tries = 1 + out[ok, 7]
+    assert abs(tries.mean() - 1 / pacc) < 5 * tries.std() / np.sqrt(ok.sum()) + 1e-3
+
+
+# ------------------------------------------------------------- camera
+def test_camera_ray_and_differentials_closed_form():
+    g = rm.generate(0, 1)
+    a = rm.RmRenderArgs.from_buffer_copy(g.args)
+    pos = np.array(a.position[:], np.float64)
+    dirv = np.array(a.direction[:], np.float64)
+    right, up = np.array(a.right[:], np.float64), np.array(a.up[:], np.float64)
+    acc = float(np.float32(a.accuracy))
+    out = np.empty(9, np.float32)
+    for x, y in [(0, 0), (511, 0), (256, 256), (17, 403), (511, 511)]:
+        L.oracle_camera(C.cast(C.pointer(a), C.c_void_p), x, y, fp(out))
+        d = dirv + acc * ((x - a.width / 2) * right + (y - a.height / 2) * up)
+        n = d / np.linalg.norm(d)
+        np.testing.assert_allclose(out[:3], n, rtol=0, atol=2e-7)
+        # d/dx normalize(d(x)) = (I - n n^T) (acc right) / |d|
+        for k, e in ((3, right), (6, up)):
+            de = acc * e
+            want = (de - n * (n @ de)) / np.linalg.norm(d)
+            np.testing.assert_allclose(out[k:k + 3], want, rtol=0, atol=2e-6 * np.abs(want).max())
+    assert pos.shape == (3,)
+
+
+# ---------------------------------------------------------- hit normals
+def hit_normals(tri, nrm, P, d, bc):
+    out = np.empty(10, np.float32)
+    L.oracle_hit_normals(fp(np.float32(tri)), fp(np.float32(nrm)), fp(np.float32(P)), fp(np.float32(d)),
+                         fp(np.float32(bc)), fp(out))
+    return out[:3], out[3:6], out[6:9], bool(out[9])
+
+
+def test_barycentric_and_hit_normals_on_hand_built_triangles():
+    big = np.array([[0, 0, 0], [2, 0, 0], [0, 1, 0]], np.float64)           # area 1 > 0.01: flat
+    small = np.array([[0, 0, 0], [0.1, 0, 0], [0, 0.1, 0]], np.float64)     # area 0.005: smoothed
+    for tri in (big, small):
+        for bc in ((0.2, 0.3, 0.5), (1, 0, 0), (0.25, 0.25, 0.5)):
+            P = np.array(bc) @ tri
+            got, *_ = hit_normals(tri, np.tile([0, 0, 1.0], (3, 1)), P, (0, 0, -1), bc)
+            np.testing.assert_allclose(got, bc, atol=2e-6)
+    tilt = np.array([[0.3, 0, 0.954], [0, 0.3, 0.954], [0, 0, 1.0]])        # dot with (0,0,1) > 0.85
+    far = np.array([[0.9, 0, 0.436], [0, 0, 1.0], [0, 0, 1.0]])             # first normal beyond 0.85
+    bc = np.array([0.2, 0.3, 0.5])
+    # big face: surface normal = shape normal, turned against the ray
+    _, sn, un, ent = hit_normals(big, tilt, bc @ big, (0, 0, -1), bc)
+    assert np.allclose(sn, [0, 0, 1]) and np.allclose(un, sn) and ent
+    _, sn, un, ent = hit_normals(big, tilt, bc @ big, (0, 0, 1), bc)
+    assert np.allclose(sn, [0, 0, -1]) and np.allclose(un, sn) and not ent
+    # small face: barycentric blend of the vertex normals
+    _, sn, un, ent = hit_normals(small, tilt, bc @ small, (0, 0, -1), bc)
+    w = bc @ tilt
+    assert np.allclose(un, w / np.linalg.norm(w), atol=1e-6) and ent
+    # a vertex normal more than acos(0.85) from the face normal is replaced by it
+    _, sn, un, _ = hit_normals(small, far, bc @ small, (0, 0, -1), bc)
+    w = bc[0] * np.array([0, 0, 1.0]) + bc[1] * far[1] + bc[2] * far[2]
+    assert np.allclose(un, w / np.linalg.norm(w), atol=1e-6)
+    # seen from below, the vertex normals are turned with the shape normal
+    _, sn, un, ent = hit_normals(small, tilt, bc @ small, (0, 0, 1), bc)
+    w = bc @ -tilt
+    assert np.allclose(sn, [0, 0, -1]) and np.allclose(un, w / np.linalg.norm(w), atol=1e-6) and not ent
+
+
+# ------------------------------------------------------------ textures
+def tex_get(rgb, uvd):
+    uvd = np.ascontiguousarray(uvd, np.float32)
+    out = np.empty((uvd.shape[0], 3), np.float32)
+    h, w = rgb.shape[:2]
+    L.oracle_tex_get(w, h, fp(np.ascontiguousarray(rgb, np.float32)), fp(uvd), uvd.shape[0], fp(out))
+    return out
+
+
+def test_texture_fetch_texels_wrap_flip_and_mip_blend():
+    rng = np.random.default_rng(8)
+    w, h = 8, 4
+    rgb = rng.random((h, w, 3)).astype(np.float32)
+    # integer texel coordinates: u = i / w, 1 - v = j / h -> texel (j, i) exactly
+    pts = [(i / w, 1 - j / h, 0.0) for i in range(w) for j in range(1, h)]
+    got = tex_get(rgb, pts)
+    want = np.array([rgb[j, i] for i in range(w) for j in range(1, h)])
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+    # wrap-around in u and v, and v = 1 -> row 0
+    assert np.allclose(tex_get(rgb, [(1 + 3 / w, 1.0, 0)]), rgb[0, 3])
+    assert np.allclose(tex_get(rgb, [(-5 / w, 1 - 2 / h, 0)]), rgb[2, 3])
+    # halfway between texels: the mean of the two
+    assert np.allclose(tex_get(rgb, [(2.5 / w, 1 - 1 / h, 0)]), (rgb[1, 2] + rgb[1, 3]) / 2, atol=1e-6)
+    # mip level 1 = 2x2 box means; depth 0.5 blends the levels; depth is clamped
+    lv1 = rgb.reshape(h // 2, 2, w // 2, 2, 3).mean((1, 3))
+    assert np.allclose(tex_get(rgb, [(2 / w, 1 - 2 / h, 1.0)]), lv1[1, 1], atol=1e-6)
+    assert np.allclose(tex_get(rgb, [(2 / w, 1 - 2 / h, 0.5)]), (rgb[2, 2] + lv1[1, 1]) / 2, atol=1e-6)
+    # an 8x4 texture has levels 8x4, 4x2 and 2x1; depth is clamped to [0, 2]
+    lv2 = lv1.reshape(1, 2, 2, 2, 3).mean((1, 3))
+    assert np.allclose(tex_get(rgb, [(0, 1, 50.0)]), lv2[0, 0], atol=1e-6)
+    assert np.allclose(tex_get(rgb, [(0.3, 0.6, -3.0)]), tex_get(rgb, [(0.3, 0.6, 0.0)]))
