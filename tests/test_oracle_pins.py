@@ -509,3 +509,35 @@ def test_texture_fetch_texels_wrap_flip_and_mip_blend():
     lv2 = lv1.reshape(1, 2, 2, 2, 3).mean((1, 3))
     assert np.allclose(tex_get(rgb, [(0, 1, 50.0)]), lv2[0, 0], atol=1e-6)
     assert np.allclose(tex_get(rgb, [(0.3, 0.6, -3.0)]), tex_get(rgb, [(0.3, 0.6, 0.0)]))
+
+
+# -------------------------------------------------------- integrator
+def test_indirect_light_of_a_lone_plane_under_the_sky_is_zero(sky_scene):
+    """Every indirect sample of a lone plane leaves for the sky at depth 1,
+    where sampleIndirectLightFromFirstIntersection's paths exclude emitters
+    and the sky (the primary vertex's light samples cover them,
+    src/render.cpp:314-423 with :121-312): Id = Is = 0, while the direct
+    buffers are positive.  The one exception is a grazing continuation from
+    a hit point that rounded to just below the plane, which meets the plane
+    again beyond t_min = 1e-4 and gathers light there: a few pixels, at a
+    negligible level.  With P_Direct = 1 every sample is direct and the
+    indirect sums stay 0."""
+    rgb, S, _ = sky_scene
+    a = rm.RmRenderArgs()
+    a.position[:] = [0.0, 3.0, 0.0]
+    a.direction[:] = [0.0, -1.0, 0.0]
+    a.up[:] = [0.0, 0.0, 1.0]
+    a.right[:] = [1.0, 0.0, 0.0]
+    a.accuracy = np.float32(2 * np.tan(np.radians(20)) / 16)
+    a.exposure, a.p_direct = 1.0, 0.5
+    a.width = a.height = 16
+    a.spp, a.seed, a.max_depth, a.rr_min_depth = 32, 3, 16, 1
+    gb, rad = S.render(a)
+    assert np.isfinite(gb[..., 0]).all()
+    zero = (rad[2:] == 0).all(axis=(0, 3))
+    assert zero.mean() >= 0.98
+    assert rad[2:, ..., :3].sum() < 1e-3 * rad[:2, ..., :3].sum()
+    assert (rad[0, ..., :3].sum(-1) > 0).all()
+    a.p_direct = 1.0
+    _, rad1 = S.render(a)
+    assert (rad1[2:] == 0).all() and (rad1[0, ..., :3].sum(-1) > 0).all()
