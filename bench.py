@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
-    ap.add_argument("--adam-steps", type=int, default=0, help="also run an inverse-rendering Adam loop of n steps")
+    ap.add_argument("--adam-steps", type=int, default=200, help="inverse-rendering Adam steps (0 skips the loop)")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoise leg (on the 1080p render)")
     ap.add_argument("--denoise-reps", type=int, default=5, help="timed denoise runs (after one warm-up)")
     args = ap.parse_args()
@@ -331,11 +331,15 @@ def batch_leg(args, torch, rm):
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     log(f"[bench] batch: one graph launch {tb:.2f} s")
-    def sequential():
+    def sequential(per_job=None):
         t0 = time.perf_counter()
         for S, a, fr, post in jobs:
+            j0 = time.perf_counter()
             fr.render(S, a)
             fr.postprocess(post, a.exposure)
+            if per_job is not None:
+                torch.cuda.synchronize()
+                per_job.append(time.perf_counter() - j0)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
@@ -346,16 +350,28 @@ def batch_leg(args, torch, rm):
     autotune.restore(LIBRARY_DEFAULTS)
     for _, a, _, _ in jobs:
         a.paths_per_round = 0
-    ts = sequential()
+    per_job = []
+    ts = sequential(per_job)
     autotune.restore(saved)
     log(f"[bench] batch: sequential plain rm_render {ts:.2f} s")
+    # where the time goes, job by job (plain launches): each job's samples/s
+    # alone, and the share of the sequential time of the jobs under 20M samples
+    table = [{"job": i, "config": sp.config, "tris": tris[i], "side": sp.side, "spp": sp.spp, "depth": sp.max_depth,
+              "seconds": round(t, 4), "samples_per_s": round(sp.samples / t, 1)}
+             for i, (sp, t) in enumerate(zip(specs, per_job))]
+    small = [r for r, sp in zip(table, specs) if sp.samples < 20_000_000]
+    small_s = sum(r["seconds"] for r in small)
+    small_samples = sum(sp.samples for sp in specs if sp.samples < 20_000_000)
     return {"jobs": len(jobs), "samples": total, "batched_samples_per_s": round(total / tb, 1),
             "sequential_samples_per_s": round(total / ts, 1), "batched_over_sequential": round(ts / tb, 3),
             "sequential_tuned_samples_per_s": round(total / ts_tuned, 1),
             "batched_over_sequential_tuned": round(ts_tuned / tb, 3),
             "batched_s": round(tb, 3), "sequential_s": round(ts, 3), "sequential_tuned_s": round(ts_tuned, 3),
             "autotune": tuned, "tris_min": min(tris), "tris_max": max(tris),
-            "library_defaults": LIBRARY_DEFAULTS,
+            "library_defaults": LIBRARY_DEFAULTS, "per_job": table,
+            "small_jobs": {"count": len(small), "seconds": round(small_s, 3), "samples": small_samples,
+                           "share_of_sequential_time": round(small_s / max(1e-9, sum(per_job)), 3),
+                           "share_of_samples": round(small_samples / total, 3)},
             "note": "sequential = one rm_render + rm_postprocess per job with the library's default launch "
                     "parameters (library_defaults, automatic paths per round); sequential_tuned = the same with "
                     "the batch's autotuned parameters; every frame's buffers exist beforehand"}
@@ -377,28 +393,41 @@ def adjoint_leg(args, torch, rm):
     if not args.no_autotune:
         from raym0nade_amd import autotune
         tuned = autotune.tune_adjoint(R, (tex, rough, emis), d, log=log)
-    fw, rd, ad = [], [], []
+    plain = rm.Frame(R.args.width, R.args.height)  # recording off: the plain forward render
+    plain.render(R.scene, R.args)
+    fw, rd, pr, ad = [], [], [], []
     for _ in range(3):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        R.frame.render(R.scene, R.args)
+        plain.render(R.scene, R.args)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        R.forward_image(tex, rough, emis)  # parameter upload + render + linear shade
+        R.frame.render(R.scene, R.args)
+        torch.cuda.synchronize()
         t2 = time.perf_counter()
+        R.forward_image(tex, rough, emis)  # parameter upload + recording render + linear shade
+        t3 = time.perf_counter()
         R.adjoint(d, tex.numel())
         torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        rd.append(t1 - t0)
-        fw.append(t2 - t1)
-        ad.append(t3 - t2)
-    r, f, a = float(np.median(rd)), float(np.median(fw)), float(np.median(ad))
-    out = {"render_s": round(r, 4), "forward_s": round(f, 4), "adjoint_s": round(a, 4),
-           "adjoint_over_forward": round(a / r, 3), "adjoint_over_forward_with_upload": round(a / f, 3),
-           "width": R.args.width, "height": R.args.height, "spp": R.args.spp, "texels": int(tex.numel()),
+        t4 = time.perf_counter()
+        pr.append(t1 - t0)
+        rd.append(t2 - t1)
+        fw.append(t3 - t2)
+        ad.append(t4 - t3)
+    p, r, f, a = (float(np.median(v)) for v in (pr, rd, fw, ad))
+    a_args = R.args
+    stride = a_args.spp - int(np.float32(a_args.spp) * np.float32(a_args.p_direct))
+    out = {"plain_render_s": round(p, 4), "recording_render_s": round(r, 4), "forward_s": round(f, 4),
+           "adjoint_s": round(a, 4), "adjoint_over_forward": round(a / p, 3),
+           "adjoint_over_recording_forward": round(a / r, 3), "adjoint_over_forward_with_upload": round(a / f, 3),
+           "record_bytes": int(a_args.width * a_args.height * stride * 48),
+           "recompute_only": "refused by rm_adjoint (the two-replay form is switched off)",
+           "width": a_args.width, "height": a_args.height, "spp": a_args.spp, "texels": int(tex.numel()),
            "split": rm.get_tuning(rm.TUNE_ADJOINT_SPLIT), "autotune": tuned,
-           "note": "render with per-sample recording on (the adjoint replays each sample once)"}
-    log(f"[bench] adjoint: render {r:.3f} s, forward {f:.3f} s, adjoint {a:.3f} s, ratio {a / r:.2f}")
+           "note": "adjoint_over_forward divides by a plain rm_render (recording off); the adjoint needs the "
+                   "recording render (48 B per indirect sample) and replays each sample's path once"}
+    log(f"[bench] adjoint: plain render {p:.3f} s, recording render {r:.3f} s, forward {f:.3f} s, "
+        f"adjoint {a:.3f} s, ratio {a / p:.2f}")
     if args.adam_steps > 0:
         from raym0nade_amd.inverse import inverse_render
         out["adam"] = inverse_render(g, steps=args.adam_steps, log=log)
@@ -556,6 +585,41 @@ def n_px(fr):
     return fr.width * fr.height
 
 
+def render_roofline(rm, torch, dev, S, a, st, counted, wall):
+    """Total rays/s of a render and its roofline: every traced ray priced as
+    the headline prices it (24 B in, 8 B out for closest hits and 1 B for
+    shadow rays, 24 B per box test, 8 B per node visit, 36 B per triangle
+    test), plus 60 B of vertex data per shading hit and 152 B per pixel
+    (G-buffer and radiance).  Extension and shadow walks are counted by the
+    counting render; the primary walks (camera rays, closest hit from 1e-4) by
+    rm_trace_count on the same camera rays."""
+    rays = torch.from_numpy(rm.rays(0, a.width * a.height, cam=a)).to(dev)
+    pi, pl, pb, pt = (float(v) for v in S.trace_count(rays).cpu().numpy())
+    del rays
+    keys = ("primary_rays", "extension_rays", "shadow_rays", "hits", "paths")
+    same = all(st[k] == counted[k] for k in keys)
+    n_pri, n_ext, n_sh = st["primary_rays"], st["extension_rays"], st["shadow_rays"]
+    walk_bytes = (24 * (pb + counted["walk_boxes"]) + 8 * (pi + pl + counted["walk_interior"] + counted["walk_leaves"])
+                  + 36 * (pt + counted["walk_tris"]))
+    ray_bytes = 32 * (n_pri + n_ext) + 25 * n_sh
+    other = 60 * st["hits"] + 152 * a.width * a.height
+    total = walk_bytes + ray_bytes + other
+    achieved = total / wall / 1e9
+    rays_total = n_pri + n_ext + n_sh
+    log(f"[bench] 1080p rays: {rays_total / 1e9:.2f} G ({n_pri} primary, {n_ext} extension, {n_sh} shadow), "
+        f"{rays_total / wall / 1e6:.0f} Mrays/s; algorithmic {total / 1e9:.1f} GB, {achieved:.0f} GB/s")
+    return {"rays_per_s": round(rays_total / wall, 1), "rays": {"primary": n_pri, "extension": n_ext, "shadow": n_sh},
+            "stats": st, "counting_render_matches": same,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "kernel": "whole render (k_primary ... k_finish, FXAA post)",
+                         "algorithmic_bytes_per_frame": round(total),
+                         "bytes": {"walks": round(walk_bytes), "rays": round(ray_bytes), "vertices_pixels": round(other)},
+                         "walk_counts": {"primary": [pi, pl, pb, pt],
+                                         "extension_shadow": [counted["walk_interior"], counted["walk_leaves"],
+                                                              counted["walk_boxes"], counted["walk_tris"]]}}}
+
+
 def render_leg(args, torch, rm, dev, stream):
     """Path samples/s of the 1080p workload: 64 displaced geodesic spheres
     (10,368,002 triangles) under a 2048x1024 fp32 sky, 1920x1080, 256 spp,
@@ -577,21 +641,30 @@ def render_leg(args, torch, rm, dev, stream):
                                      paths_per_round=(0, 4, 8), log=log)
         a.paths_per_round = tuned["paths_per_round"]
     times = []
+    counted = None
     for k in range(args.render_frames + 1):
+        # the untimed warm-up frame is a counting render (same image): the
+        # node visits and tests of its extension and shadow walks price them
+        rm.set_tuning(rm.TUNE_RENDER_COUNT, 1 if k == 0 else 0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         fr.render(S, a, stream=stream)
         fr.postprocess(63 | 512, a.exposure, stream=stream)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        log(f"[bench] frame {k}: {dt:.3f} s" + (" (warm-up)" if k == 0 else ""))
-        if k > 0:
+        log(f"[bench] frame {k}: {dt:.3f} s" + (" (warm-up, counting walks)" if k == 0 else ""))
+        if k == 0:
+            counted = fr.stats()
+        else:
             times.append(dt)
+    rm.set_tuning(rm.TUNE_RENDER_COUNT, 0)
     wall = float(np.median(times))
     samples = a.width * a.height * a.spp
+    st = fr.stats()
     out = {"path_samples_per_s": round(samples / wall, 1), "frame_s": round(wall, 4), "frames": len(times),
            "width": a.width, "height": a.height, "spp": a.spp, "max_depth": a.max_depth, "n_tris": S.info.n_tris,
            "autotune": tuned}
+    out.update(render_roofline(rm, torch, dev, S, a, st, counted, wall))
     if not args.no_denoise:
         out["denoise"] = denoise_leg(args, torch, rm, fr, stream)
         out["exports"] = exports_leg(args, torch, rm, fr, a, stream)
