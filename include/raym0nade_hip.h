@@ -51,7 +51,21 @@ int32_t rm_abi_version(void);
  *   metallic     constant metallic (reference: 0 without a specular map,
  *                src/material.cpp:374-378)
  *   opacity/ior/roughness/transmitting_color as Material (src/material.cpp:109-111,300-335)
- *   albedo_tex   index of an fp32 linear RGB texture (rm_texture) or -1 */
+ *   albedo_tex   diffuse texture (rm_texture) or -1: fp32 linear RGB (extension),
+ *                or 8-bit as the reference's Material::getDiffuseColor reads it
+ *                (src/material.cpp:349-356: bilinear x trilinear filtering of
+ *                bytes / 255, then the power 2.2)
+ * and the reference's other maps, each an 8-bit texture or -1 (every -1 when
+ * the material has none; the fields are read, so a caller sets them):
+ *   normal_tex   aiTextureType_NORMALS: getNormal = sample * 2 - 1, tilting
+ *                the shading normal in the face's uv tangent frame
+ *                (src/material.cpp:358-363, src/model.cpp:279-298)
+ *   emissive_tex aiTextureType_EMISSIVE: emission = sample ^ 2.2, and the
+ *                mesh becomes a light object whose colour and power are the
+ *                texture averaged over an 8 x 8 grid of each face
+ *                (src/material.cpp:365-372, src/model.cpp:23-82)
+ *   specular_tex aiTextureType_SPECULAR: metallic = min(b, 0.99), roughness =
+ *                max(g, 1e-3) of the level-0 sample (src/material.cpp:374-383) */
 typedef struct rm_material {
     float base_color[3];
     float emission[3];
@@ -61,6 +75,9 @@ typedef struct rm_material {
     float opacity;
     float ior;
     int32_t albedo_tex;
+    int32_t normal_tex;
+    int32_t emissive_tex;
+    int32_t specular_tex;
 } rm_material;
 
 /* fp32 linear RGB texture, row-major [height][width][3].  A mip chain of 2x2
@@ -75,11 +92,26 @@ typedef struct rm_material {
  * (TransparentTest, src/model.cpp:217-230): it walks again from t + 1e-4, at
  * most 8 walks per ray.  A closest-hit ray whose 8 walks all end on such
  * points is a miss (Model::rayHit, src/model.cpp:332-341), and a shadow ray
- * that meets 8 of them is blocked (Model::rayHit_test, src/model.cpp:343-354). */
+ * that meets 8 of them is blocked (Model::rayHit_test, src/model.cpp:343-354).
+ *
+ * bytes (optional): an 8-bit texture in the reference's own storage
+ * (ImageData, include/material.h:12-24): [height][width][channels] bytes as its
+ * loaders produce them (channels 3 for PNG, 4 for DDS).  rgb and alpha are
+ * then unused.  Mips are integer 2 x 2 means (sum / 4, src/material.cpp:113-148)
+ * and a sample reads channels-strided texels with the sampler's own stride
+ * (4 bytes for colour, emissive and specular lookups, 3 for normal maps,
+ * src/material.cpp:48-79) as the reference does; a texel read that runs past
+ * the level's last byte (a 4-byte read of 3-channel data, undefined in the
+ * reference) reads zeros.  A diffuse texture with some alpha byte (every 4th
+ * byte from the 4th) below 255 makes its material an alpha cut-out
+ * (src/material.cpp:102-107,330-333), tested on the level-0 sample's 4th
+ * component (src/model.cpp:217-230). */
 typedef struct rm_texture {
     int32_t width, height;
     const float *rgb;
     const float *alpha;
+    const uint8_t *bytes;
+    int32_t channels;
 } rm_texture;
 
 /* Flat scene.  Replaces what Model's constructor assembles from assimp

@@ -30,11 +30,22 @@ void checkLightObject(Model &M, const std::vector<Face> &meshFaces, const Materi
     vec3 color(0.0f);
     std::vector<float> faceWeights;
     for (const Face &face : meshFaces) {
-        // getAverageEmissiveColor (src/model.cpp:23-42) on a constant emission
+        // getAverageEmissiveColor (src/model.cpp:23-42)
         vec3 averageColor(0.0f);
-        for (int i = 0; i < 8; ++i)
-            for (int j = 0; j < 8; ++j) averageColor += material.getEmissiveColor(0, 0, NAN);
-        vec3 textureColor = averageColor / static_cast<float>(64);
+        const int gridResolution = 8;
+        for (int i = 0; i < gridResolution; ++i)
+            for (int j = 0; j < gridResolution; ++j) {
+                float a = static_cast<float>(i) / (gridResolution - 1);
+                float b = static_cast<float>(j) / (gridResolution - 1);
+                if (a + b > 1.0f) {
+                    a = 1.0f - a;
+                    b = 1.0f - b;
+                }
+                float c = 1.0f - a - b;
+                vec2 uv = a * face.data[0]->uv + b * face.data[1]->uv + c * face.data[2]->uv;
+                averageColor += material.getEmissiveColor(uv[0], uv[1], NAN);
+            }
+        vec3 textureColor = averageColor / static_cast<float>(gridResolution * gridResolution);
         float Clum = dot(textureColor, RGB_Weight),
               area = length(cross(face.v[1] - face.v[0], face.v[2] - face.v[0])) / 2.0f;
         if (Clum < eps_zero || area < eps_zero) continue;
@@ -106,15 +117,32 @@ oracle_scene *oracle_scene_create(const rm_scene_desc *d) {
         X.metallic = R.metallic;
         X.opacity = R.opacity;
         X.ior = R.ior;
+        auto load8 = [&](int32_t t, ImageData8 &img) {
+            if (t < 0 || t >= d->n_textures || !d->textures[t].bytes) return;
+            const rm_texture &T = d->textures[t];
+            img.width = T.width;
+            img.height = T.height;
+            img.channels = T.channels;
+            img.data[0].assign(T.bytes, T.bytes + (size_t)T.width * T.height * T.channels);
+            img.generateMipmaps();
+        };
         if (R.albedo_tex >= 0 && R.albedo_tex < d->n_textures) {
             const rm_texture &T = d->textures[R.albedo_tex];
-            X.diffuse.width = T.width;
-            X.diffuse.height = T.height;
-            X.diffuse.data[0].assign(T.rgb, T.rgb + (size_t)T.width * T.height * 3);
-            X.diffuse.generateMipmaps();
-            if (T.alpha) X.diffuse.alpha.assign(T.alpha, T.alpha + (size_t)T.width * T.height);
-            X.hasFullyTransparentPart = X.diffuse.hasTransparentPart();
+            if (T.bytes) {
+                load8(R.albedo_tex, X.diffuse8);
+                X.hasFullyTransparentPart = X.diffuse8.hasTransparentPart();
+            } else {
+                X.diffuse.width = T.width;
+                X.diffuse.height = T.height;
+                X.diffuse.data[0].assign(T.rgb, T.rgb + (size_t)T.width * T.height * 3);
+                X.diffuse.generateMipmaps();
+                if (T.alpha) X.diffuse.alpha.assign(T.alpha, T.alpha + (size_t)T.width * T.height);
+                X.hasFullyTransparentPart = X.diffuse.hasTransparentPart();
+            }
         }
+        load8(R.normal_tex, X.normals8);
+        if (!has_sky) load8(R.emissive_tex, X.emissive8);  // src/model.cpp:140: no emissive maps with a sky
+        load8(R.specular_tex, X.specular8);
     }
     M.vertexDatas.reserve(n * 3);
     M.faces.reserve(n);
@@ -702,6 +730,93 @@ void oracle_tex_get(int32_t w, int32_t h, const float *rgb, const float *uvd, in
     img.data[0].assign(rgb, rgb + (size_t)w * h * 3);
     img.generateMipmaps();
     for (int64_t i = 0; i < n; ++i) put3(out + i * 3, img.get(uvd[i * 3], uvd[i * 3 + 1], uvd[i * 3 + 2]));
+}
+
+// 8-bit ImageData (src/material.cpp:26-148) of a w x h texture with
+// `channels` bytes per texel: get<vec3> (bw 3) or get<vec4> (bw 4) at n points
+// uvd [n][3] = (u, v, depth): out [n][4].
+void oracle_tex8_get(int32_t w, int32_t h, int32_t channels, const uint8_t *bytes, int32_t bw, const float *uvd,
+                     int64_t n, float *out) {
+    ImageData8 img;
+    img.width = w;
+    img.height = h;
+    img.channels = channels;
+    img.data[0].assign(bytes, bytes + (size_t)w * h * channels);
+    img.generateMipmaps();
+    for (int64_t i = 0; i < n; ++i) {
+        if (bw == 3)
+            img.get<3>(uvd[i * 3], uvd[i * 3 + 1], uvd[i * 3 + 2], out + i * 4);
+        else
+            img.get<4>(uvd[i * 3], uvd[i * 3 + 1], uvd[i * 3 + 2], out + i * 4);
+    }
+}
+
+// generateMipmaps of the same texture: the level count; when `out` is given,
+// every level's bytes, level after level.  transparent: hasTransparentPart.
+int32_t oracle_tex8_mips(int32_t w, int32_t h, int32_t channels, const uint8_t *bytes, uint8_t *out,
+                         int32_t *transparent) {
+    ImageData8 img;
+    img.width = w;
+    img.height = h;
+    img.channels = channels;
+    img.data[0].assign(bytes, bytes + (size_t)w * h * channels);
+    img.generateMipmaps();
+    if (out)
+        for (int l = 0; l < img.map_depth; ++l) {
+            memcpy(out, img.data[l].data(), img.data[l].size());
+            out += img.data[l].size();
+        }
+    if (transparent) *transparent = img.hasTransparentPart() ? 1 : 0;
+    return img.map_depth;
+}
+
+// A material whose one 8-bit map is the given texture: which 0
+// getDiffuseColor (vec4), 1 getNormal, 2 getEmissiveColor, 3 getSurfaceData
+// (roughness, metallic) at n points uvd [n][3] = (u, v, duv; NaN duv: level
+// 0): out [n][4].
+void oracle_material8(int32_t w, int32_t h, int32_t channels, const uint8_t *bytes, int32_t which, const float *uvd,
+                      int64_t n, float *out) {
+    Material m;
+    ImageData8 &img = which == 0 ? m.diffuse8 : which == 1 ? m.normals8 : which == 2 ? m.emissive8 : m.specular8;
+    img.width = w;
+    img.height = h;
+    img.channels = channels;
+    img.data[0].assign(bytes, bytes + (size_t)w * h * channels);
+    img.generateMipmaps();
+    for (int64_t i = 0; i < n; ++i) {
+        const float u = uvd[i * 3], v = uvd[i * 3 + 1], duv = uvd[i * 3 + 2];
+        float *o = out + i * 4;
+        o[3] = 0.0f;
+        if (which == 0) {
+            vec4 c = m.getDiffuseColor(u, v, duv);
+            for (int k = 0; k < 4; ++k) o[k] = c[k];
+        } else if (which == 1 || which == 2) {
+            put3(o, which == 1 ? m.getNormal(u, v, duv) : m.getEmissiveColor(u, v, duv));
+        } else {
+            m.getSurfaceData(u, v, o[0], o[1]);
+            o[2] = 0.0f;
+        }
+    }
+}
+
+// calcSurfaceNormal (src/model.cpp:279-298): the surface normal surfN tilted
+// by the normal-map value nm in the uv tangent frame of the triangle tri9
+// with uvs uv6.
+void oracle_surface_normal(const float *tri9, const float *uv6, const float *nm3, const float *shapeN3,
+                           const float *surfN3, float *out3) {
+    VertexData vd[3];
+    Face f;
+    for (int k = 0; k < 3; ++k) {
+        f.v[k] = get3(tri9 + 3 * k);
+        vd[k].uv = vec2(uv6[2 * k], uv6[2 * k + 1]);
+        vd[k].normal = vec3(0.0f);
+        f.data[k] = &vd[k];
+    }
+    f.material = nullptr;
+    f.id = 0;
+    vec3 sn = get3(surfN3);
+    calcSurfaceNormal(f, get3(nm3), get3(shapeN3), sn);
+    put3(out3, sn);
 }
 
 }  // extern "C"

@@ -239,6 +239,7 @@ static bool TransparentTest(const Ray &ray, const HitRecord &hit) {
     vec3 intersection = ray.origin + ray.direction * hit.t_max;
     vec3 b = barycentric(face.v[0], face.v[1], face.v[2], intersection);
     vec2 texUV = b.x * face.data[0]->uv + b.y * face.data[1]->uv + b.z * face.data[2]->uv;
+    if (!material.diffuse8.empty()) return material.getDiffuseColor(texUV.x, texUV.y, NAN)[3] < eps_zero;
     return material.diffuse.getAlpha(texUV.x, texUV.y) < eps_zero;
 }
 

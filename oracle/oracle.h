@@ -54,6 +54,20 @@ struct ImageData {                                      // include/material.h:12
     float getAlpha(float u, float v) const;             // component 3 of get<vec4>(u, v, 0), :81-94
 };
 
+// The reference's 8-bit texture (include/material.h:12-24): bytes as its
+// loaders store them ([h][w][channels]), integer mips, samples of bytes / 255
+// read with the sampler's own stride (4 for vec4 lookups, 3 for vec3).  The
+// reference's reads past a level's last byte are undefined; here they read 0.
+struct ImageData8 {
+    int width = 0, height = 0, channels = 0, map_depth = 0;
+    std::vector<uint8_t> data[8];
+    void generateMipmaps();                             // src/material.cpp:113-148
+    template <int BitWidth>
+    void get(float u, float v, float depth, float out[4]) const;   // src/material.cpp:48-94
+    bool empty() const { return data[0].empty(); }
+    bool hasTransparentPart() const;                    // src/material.cpp:102-107
+};
+
 struct Material {                                       // include/material.h:29-52
     int id = 0;
     float opacity = 1.0f, ior = 1.0f, roughness = 0.8f; // src/material.cpp:109-111
@@ -62,8 +76,13 @@ struct Material {                                       // include/material.h:29
     vec3 emission{0.0f};      // EXT constant emission
     float metallic = 0.0f;    // EXT constant metallic
     ImageData diffuse;        // EXT fp32 albedo texture
+    // the reference's textures: aiTextureType_DIFFUSE, NORMALS, EMISSIVE, SPECULAR
+    ImageData8 diffuse8, normals8, emissive8, specular8;
     bool hasFullyTransparentPart = false;               // src/material.cpp:330-333
-    bool hasEmission() const { return emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f; }
+    // EXT: constant emission, or (the reference's light objects) an emissive texture
+    bool hasEmission() const {
+        return !emissive8.empty() || emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f;
+    }
     vec4 getDiffuseColor(float u, float v, float duv) const;             // src/material.cpp:349-356
     vec3 getNormal(float u, float v, float duv) const;                   // src/material.cpp:358-363
     vec3 getEmissiveColor(float u, float v, float duv) const;            // src/material.cpp:365-372
@@ -147,6 +166,8 @@ void getHitNormals(const Face &face, const vec3 &inDir, const vec3 &baryCoords, 
                    vec3 &surfaceNormal_raw, bool &entering);                      // src/model.cpp:240-270
 void getHitMaterial(const Face &face, const vec3 &baryCoords, const vec3 &hit_dPdx, const vec3 &hit_dPdy,
                     HitInfo &hitInfo);                                            // src/model.cpp:300-328
+void calcSurfaceNormal(const Face &face, const vec3 &normalMap, const vec3 &shapeNormal,
+                       vec3 &surfaceNormal);                                      // src/model.cpp:279-298
 
 // ---------------------------------------------------------------- sampling
 class BSDF {                                            // include/sampling.h:8-26

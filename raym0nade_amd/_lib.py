@@ -30,11 +30,21 @@ class RmMaterial(C.Structure):
         ("opacity", C.c_float),
         ("ior", C.c_float),
         ("albedo_tex", C.c_int32),
+        ("normal_tex", C.c_int32),
+        ("emissive_tex", C.c_int32),
+        ("specular_tex", C.c_int32),
     ]
+
+    def __init__(self, *a, **k):
+        super().__init__(*a, **k)
+        for f in ("albedo_tex", "normal_tex", "emissive_tex", "specular_tex"):
+            if f not in k:
+                setattr(self, f, -1)
 
 
 class RmTexture(C.Structure):
-    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", f32p), ("alpha", f32p)]
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", f32p), ("alpha", f32p),
+                ("bytes", C.POINTER(C.c_uint8)), ("channels", C.c_int32)]
 
 
 class RmSceneDesc(C.Structure):

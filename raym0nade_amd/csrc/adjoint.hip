@@ -621,9 +621,13 @@ extern "C" int32_t rm_adjoint(const rm_scene *scene, const rm_render_args *a, rm
         memcmp(&f->stamp_args, a, sizeof(rm_render_args)) != 0)
         return fail(RM_ERR_INVALID, "rm_adjoint: the frame does not hold rm_render(scene, args)");
     const HostScene &h = scene->host;
-    for (size_t m = 0; m < h.mats.size(); ++m)
-        if (!(h.mats[m].opacity > 1.0f - T_EPS))
+    for (size_t m = 0; m < h.mats.size(); ++m) {
+        const MatRec &M = h.mats[m];
+        if (!(M.opacity > 1.0f - T_EPS))
             return fail(RM_ERR_UNSUPPORTED, "rm_adjoint: material %zu transmits (opaque scenes only)", m);
+        if (M.ntex >= 0 || M.etex >= 0 || M.stex >= 0 || (M.tex >= 0 && h.texs[M.tex].channels))
+            return fail(RM_ERR_UNSUPPORTED, "rm_adjoint: material %zu has 8-bit texture maps (fp32 albedo only)", m);
+    }
     // Every indirect sample's sums come from the render's record.  A frame
     // rendered without recording would need the two-replay form (each sample
     // replayed once for its sums, once for its derivatives); that form is

@@ -128,7 +128,7 @@ extern "C" int32_t rm_get_tuning(int32_t key, int32_t *value) {
 
 extern "C" const char *rm_last_error(void) { return rm::g_err; }
 
-extern "C" int32_t rm_abi_version(void) { return 1; }
+extern "C" int32_t rm_abi_version(void) { return 2; }
 
 extern "C" void rm_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     rm::U4 r = rm::philox4x32_10(rm::U4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);

@@ -10,6 +10,7 @@
 // assimp, libpng, FreeImage and imageio are absent from this image; the
 // formats are implemented from their published specifications.
 #include <zlib.h>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -379,12 +380,63 @@ int32_t load_image(const std::string &path, int &W, int &H, std::vector<float> &
     return fail(RM_ERR_UNSUPPORTED, "%s: only .png, .dds and .hdr images are read by this build", path.c_str());
 }
 
+// An 8-bit texture as the reference's loaders store it (ImageData::data[0]):
+// PNG as 3 bytes per pixel after libpng's conversions
+// (src/material.cpp:214-267); DDS as FreeImage decodes it for the imageio
+// script (scripts/dds_to_array.py): 4 bytes per pixel (RGBA) for
+// block-compressed and 32-bit formats, 3 for 24-bit RGB.
+int32_t load_image8(const std::string &path, int &W, int &H, int &channels, std::vector<uint8_t> &bytes) {
+    std::vector<uint8_t> f;
+    if (!read_file(path, f)) return fail(RM_ERR_INVALID, "cannot read %s", path.c_str());
+    const std::string e = ext_of(path);
+    if (e == "png") {
+        channels = 3;
+        return decode_png(f, W, H, bytes);
+    }
+    if (e == "dds") {
+        std::vector<uint8_t> rgb, a8;
+        bool has_alpha = false;
+        int32_t rc = decode_dds(f, W, H, rgb, a8, has_alpha);
+        if (rc) return rc;
+        const uint32_t pf_flags = f[80] | f[81] << 8 | f[82] << 16 | (uint32_t)f[83] << 24;
+        const uint32_t bits = f[88] | f[89] << 8 | f[90] << 16 | (uint32_t)f[91] << 24;
+        channels = ((pf_flags & 0x4) || bits == 32) ? 4 : 3;
+        const size_t np = (size_t)W * H;
+        bytes.resize(np * channels);
+        for (size_t i = 0; i < np; ++i) {
+            for (int c = 0; c < 3; ++c) bytes[i * channels + c] = rgb[i * 3 + c];
+            if (channels == 4) bytes[i * 4 + 3] = a8[i];
+        }
+        return RM_OK;
+    }
+    return fail(RM_ERR_UNSUPPORTED, "%s: 8-bit texture maps are .png or .dds", path.c_str());
+}
+
 // ------------------------------------------------------------ OBJ + MTL
 
+// Texture paths are URL-decoded (urlDecode, src/material.cpp:9-24): each
+// "%XX" becomes the byte of hex value XX.
+std::string url_decode(const std::string &s) {
+    std::string out;
+    for (size_t i = 0; i < s.size(); ++i) {
+        auto hex = [](char c) { return std::isxdigit((unsigned char)c) != 0; };
+        if (s[i] == '%' && i + 2 < s.size() && hex(s[i + 1]) && hex(s[i + 2])) {
+            out += (char)std::stoi(s.substr(i + 1, 2), nullptr, 16);
+            i += 2;
+        } else {
+            out += s[i];
+        }
+    }
+    return out;
+}
+
+// Texture maps of a material under the keywords assimp's OBJ importer maps
+// to the reference's texture types: map_Kd diffuse, norm / map_Kn normals,
+// map_Ke / map_emissive emissive, map_Ks specular.
 struct MtlDef {
     std::string name;
     rm_material m{};
-    std::string map_kd;
+    std::string map_kd, map_norm, map_ke, map_ks;
     bool has_emission = false;
 };
 
@@ -394,7 +446,7 @@ rm_material default_material() {
     m.roughness = 0.8f;                                   // Material() (src/material.cpp:109-111)
     m.opacity = 1.0f;
     m.ior = 1.0f;
-    m.albedo_tex = -1;
+    m.albedo_tex = m.normal_tex = m.emissive_tex = m.specular_tex = -1;
     return m;
 }
 
@@ -450,12 +502,17 @@ int32_t parse_mtl(const std::string &path, std::vector<MtlDef> &mats) {
             float t = 0;
             ss >> t;
             opacity.back() = 1.0f - t;
-        } else if (k == "map_Kd") {
+        } else if (k == "map_Kd" || k == "norm" || k == "map_Kn" || k == "map_Ke" || k == "map_emissive" ||
+                   k == "map_Ks") {
             std::string rest;
             std::getline(ss >> std::ws, rest);
             while (!rest.empty() && (rest.back() == '\r' || rest.back() == ' ')) rest.pop_back();
             size_t sp = rest.find_last_of(' ');  // options before the file name are ignored
-            cur->map_kd = sp == std::string::npos ? rest : rest.substr(sp + 1);
+            std::string file = sp == std::string::npos ? rest : rest.substr(sp + 1);
+            if (k == "map_Kd") cur->map_kd = file;
+            else if (k == "norm" || k == "map_Kn") cur->map_norm = file;
+            else if (k == "map_Ks") cur->map_ks = file;
+            else cur->map_ke = file;
         }
     }
     for (size_t i = 0; i < mats.size(); ++i)
@@ -468,6 +525,7 @@ struct ObjScene {
     std::vector<int32_t> material, mesh;
     std::vector<rm_material> mats;
     std::vector<std::vector<float>> tex_data, tex_alpha;
+    std::vector<std::vector<uint8_t>> tex_bytes;
     std::vector<rm_texture> texs;
     std::vector<float> sky;
     int sky_w = 0, sky_h = 0;
@@ -580,37 +638,58 @@ int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &
             S.material.push_back(t.mat);
             S.mesh.push_back(t.mesh);
         }
-    // materials and their albedo textures (EXT: decoded to linear fp32 per
-    // texel as pow(c / 255, 2.2); the reference filters the 8-bit texels and
-    // applies the power afterwards, src/material.cpp:337-356)
+    // materials and their texture maps (Material::loadImageFromFile,
+    // src/material.cpp:290-298): PNG and DDS maps as the reference's 8-bit
+    // texels; an HDR diffuse map (EXT) as linear fp32 texels.  Each file is
+    // loaded once.
     std::map<std::string, int> tex_of;
-    for (auto &d : defs) {
-        if (!d.map_kd.empty()) {
-            std::string path = folder + d.map_kd;
-            for (auto &c : path)
-                if (c == '\\') c = '/';
-            auto it = tex_of.find(path);
-            if (it == tex_of.end()) {
-                int w = 0, h = 0;
-                bool hdr = false;
-                std::vector<float> px;
-                std::vector<float> alpha;
-                int32_t rc = load_image(path, w, h, px, hdr, &alpha);
-                if (rc) return rc;
-                if (!hdr)
-                    for (auto &v : px) v = std::pow(v, 2.2f);
-                S.tex_data.push_back(std::move(px));
-                S.tex_alpha.push_back(std::move(alpha));
-                S.texs.push_back(rm_texture{w, h, nullptr});
-                it = tex_of.emplace(path, (int)S.texs.size() - 1).first;
-            }
-            d.m.albedo_tex = it->second;
+    auto texture = [&](const std::string &file, bool allow_hdr, int32_t &index) -> int32_t {
+        index = -1;
+        if (file.empty()) return RM_OK;
+        std::string path = url_decode(folder + file);
+        for (auto &c : path)
+            if (c == '\\') c = '/';
+        auto it = tex_of.find(path);
+        if (it != tex_of.end()) {
+            index = it->second;
+            return RM_OK;
         }
+        int w = 0, h = 0;
+        if (ext_of(path) == "hdr") {
+            if (!allow_hdr) return fail(RM_ERR_UNSUPPORTED, "%s: an HDR file as a normal, emissive or specular map", path.c_str());
+            bool hdr = false;
+            std::vector<float> px;
+            int32_t rc = load_image(path, w, h, px, hdr);
+            if (rc) return rc;
+            S.tex_data.push_back(std::move(px));
+            S.tex_bytes.emplace_back();
+            S.texs.push_back(rm_texture{w, h, nullptr, nullptr, nullptr, 0});
+        } else {
+            int ch = 0;
+            std::vector<uint8_t> bytes;
+            int32_t rc = load_image8(path, w, h, ch, bytes);
+            if (rc) return rc;
+            S.tex_data.emplace_back();
+            S.tex_bytes.push_back(std::move(bytes));
+            S.texs.push_back(rm_texture{w, h, nullptr, nullptr, nullptr, ch});
+        }
+        index = (int)S.texs.size() - 1;
+        tex_of.emplace(path, index);
+        return RM_OK;
+    };
+    for (auto &d : defs) {
+        int32_t rc;
+        if ((rc = texture(d.map_kd, true, d.m.albedo_tex)) || (rc = texture(d.map_norm, false, d.m.normal_tex)) ||
+            (rc = texture(d.map_ke, false, d.m.emissive_tex)) || (rc = texture(d.map_ks, false, d.m.specular_tex)))
+            return rc;
         S.mats.push_back(d.m);
     }
     for (size_t i = 0; i < S.texs.size(); ++i) {
-        S.texs[i].rgb = S.tex_data[i].data();
-        S.texs[i].alpha = S.tex_alpha[i].empty() ? nullptr : S.tex_alpha[i].data();
+        if (S.texs[i].channels) {
+            S.texs[i].bytes = S.tex_bytes[i].data();
+        } else {
+            S.texs[i].rgb = S.tex_data[i].data();
+        }
     }
     return RM_OK;
 }

@@ -9,6 +9,7 @@
 #include <vector>
 #include "../../include/raym0nade_hip.h"
 #include "rm_layout.h"
+#include "rm_vec.h"
 
 namespace rm {
 
@@ -105,6 +106,8 @@ struct HostScene {
     std::vector<MatRec> mats;
     std::vector<TexRec> texs;
     std::vector<float> texels;
+    std::vector<uint8_t> bytes;         // 8-bit textures, every mip level
+    std::vector<V3> light_face_avg;     // per light face: getAverageEmissiveColor (src/model.cpp:23-42)
     std::vector<LightRec> lights;
     std::vector<TriShade> light_faces;
     std::vector<float> light_cdf;
@@ -141,7 +144,7 @@ int device_cu_count();
 struct rm_scene {
     rm::HostScene host;  // perm + sizes kept; big arrays released after upload
     rm::SceneView view{};
-    rm::DevBuf nodes, tris, shade, mats, texs, texels, lights, light_faces, light_cdf, sky, sky_cdf, sky_guide;
+    rm::DevBuf nodes, tris, shade, mats, texs, texels, bytes, lights, light_faces, light_cdf, sky, sky_cdf, sky_guide;
     rm_scene_info info{};
     uint64_t version = 0;  // bumped by rm_scene_update
 };

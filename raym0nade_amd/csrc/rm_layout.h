@@ -75,16 +75,23 @@ struct alignas(16) MatRec {
     float trans[3];
     float opacity;
     float ior;
-    int32_t tex;       // -1 or texture index
-    int32_t cutout;    // 1: the texture has an alpha plane with a value below 1
-    int32_t pad;
+    int32_t tex;       // -1 or the diffuse texture (fp32 or 8-bit)
+    int32_t cutout;    // 1: the diffuse texture has an alpha below 1 (an alpha byte below 255)
+    int32_t ntex;      // -1 or the 8-bit normal map (aiTextureType_NORMALS)
+    int32_t etex;      // -1 or the 8-bit emissive map (aiTextureType_EMISSIVE)
+    int32_t stex;      // -1 or the 8-bit specular map (aiTextureType_SPECULAR)
+    int32_t pad[2];
 };
 
-// fp32 texture with a mip chain (level l at offset[l], size w>>l x h>>l).
+// Texture with a mip chain (level l at offset[l], size w>>l x h>>l).
+// channels 0: fp32 RGB texels, offsets in floats into the texel pool;
+// channels 3 or 4: the reference's 8-bit texels in that channel layout,
+// offsets in bytes into the byte pool.
 struct TexRec {
-    int32_t width, height, levels, pad;
-    int64_t offset[8];  // in floats, into the scene's texel pool
-    int64_t alpha;      // level-0 alpha plane [h][w] in the pool, or -1
+    int32_t width, height, levels, channels;
+    int64_t offset[8];
+    int64_t alpha;      // fp32 textures: level-0 alpha plane [h][w] in the texel pool, or -1
+    int64_t level_bytes[8];  // 8-bit textures: bytes of each level (reads past them read 0)
 };
 
 // One light object (LightObject, include/component.h:47-54).
@@ -125,6 +132,7 @@ struct SceneView {
     int32_t walk_lds;
     // 1 when a material is an alpha cut-out: walks then apply TransparentTest.
     int32_t cutout;
+    const uint8_t *bytes;         // 8-bit texels of every 8-bit texture, all mip levels
 };
 
 }  // namespace rm

@@ -15,6 +15,7 @@
 #include <thread>
 #include "rm_internal.h"
 #include "rm_vec.h"
+#include "tex8.h"
 
 namespace rm {
 namespace {
@@ -161,17 +162,53 @@ MatRec mat_rec(const rm_material &M, bool has_sky) {
     R.opacity = M.opacity;
     R.ior = M.ior;
     R.tex = M.albedo_tex;
+    R.ntex = M.normal_tex;
+    R.etex = has_sky ? -1 : M.emissive_tex;  // no emissive maps with a sky (src/model.cpp:140)
+    R.stex = M.specular_tex;
     return R;
 }
 
-// getAverageEmissiveColor (src/model.cpp:23-42): the mean of an 8x8
-// barycentric grid of emission lookups, constant emission here.
+// getAverageEmissiveColor (src/model.cpp:23-42) with a constant emission:
+// the mean of 64 identical lookups.
 V3 emissive_avg(V3 em) {
     V3 avg = v3(0.0f);
     for (int q = 0; q < 64; ++q) avg += em;
     return avg / 64.0f;
 }
 float emissive_lum(V3 em) { return dot(emissive_avg(em), v3(0.3f, 0.6f, 0.1f)); }
+
+// getAverageEmissiveColor (src/model.cpp:23-42) with an emissive map: the
+// mean of getEmissiveColor(uv, NaN) (level 0, then the power 2.2 with glibc's
+// powf, src/material.cpp:337-347,365-372) over an 8 x 8 barycentric grid of
+// the face's uvs.
+V3 emissive_avg_tex(const HostScene &hs, const TexRec &T, const TriShade &F) {
+    V3 avg = v3(0.0f);
+    const int g = 8;
+    for (int i = 0; i < g; ++i)
+        for (int j = 0; j < g; ++j) {
+            float a = static_cast<float>(i) / (g - 1), b = static_cast<float>(j) / (g - 1);
+            if (a + b > 1.0f) {
+                a = 1.0f - a;
+                b = 1.0f - b;
+            }
+            float c = 1.0f - a - b;
+            float u = (a * F.uv[0][0] + b * F.uv[1][0]) + c * F.uv[2][0];
+            float v = (a * F.uv[0][1] + b * F.uv[1][1]) + c * F.uv[2][1];
+            float t[4];
+            tex8_get<4>(hs.bytes.data(), T, u, v, 0.0f, t);
+            avg += V3{powf(t[0], 2.2f), powf(t[1], 2.2f), powf(t[2], 2.2f)};
+        }
+    return avg / static_cast<float>(g * g);
+}
+
+// The per-face mean emission of light face k (set at build; faces of
+// constant-emission materials follow rm_scene_update's new values).
+V3 light_face_avg(const HostScene &hs, int k) {
+    const TriShade &F = hs.light_faces[k];
+    const MatRec &M = hs.mats[F.material];
+    if (M.etex >= 0) return hs.light_face_avg[k];
+    return emissive_avg(V3{M.emission[0], M.emission[1], M.emission[2]});
+}
 
 // The rest of checkLightObject (src/model.cpp:44-82) for every light object
 // whose faces are chosen: colour = area-weighted mean emission normalised to
@@ -183,8 +220,7 @@ void light_params(HostScene &hs) {
         std::vector<float> w(L.n_faces);
         for (int k = 0; k < L.n_faces; ++k) {
             const TriShade &F = hs.light_faces[L.first_face + k];
-            const MatRec &M = hs.mats[F.material];
-            V3 avg = emissive_avg(V3{M.emission[0], M.emission[1], M.emission[2]});
+            V3 avg = light_face_avg(hs, L.first_face + k);
             float Clum = dot(avg, v3(0.3f, 0.6f, 0.1f));
             V3 v0{F.v[0][0], F.v[0][1], F.v[0][2]}, v1{F.v[1][0], F.v[1][1], F.v[1][2]},
                 v2{F.v[2][0], F.v[2][1], F.v[2][2]};
@@ -227,13 +263,21 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
     for (int64_t i = 0; i < n; ++i)
         if (d->material[i] < 0 || d->material[i] >= d->n_materials)
             return fail(RM_ERR_INVALID, "rm_scene_create: material id %d of triangle %lld out of range", d->material[i], (long long)i);
-    for (int32_t m = 0; m < d->n_materials; ++m) {
-        int32_t t = d->materials[m].albedo_tex;
-        if (t >= d->n_textures || t < -1) return fail(RM_ERR_INVALID, "rm_scene_create: material %d texture %d out of range", m, t);
-    }
-    for (int32_t t = 0; t < d->n_textures; ++t)
-        if (d->textures[t].width <= 0 || d->textures[t].height <= 0 || !d->textures[t].rgb)
+    for (int32_t t = 0; t < d->n_textures; ++t) {
+        const rm_texture &X = d->textures[t];
+        if (X.width <= 0 || X.height <= 0 || (!X.rgb && !X.bytes))
             return fail(RM_ERR_INVALID, "rm_scene_create: texture %d is empty", t);
+        if (X.bytes && X.channels != 3 && X.channels != 4)
+            return fail(RM_ERR_INVALID, "rm_scene_create: 8-bit texture %d has %d channels (3 or 4)", t, X.channels);
+    }
+    for (int32_t m = 0; m < d->n_materials; ++m) {
+        const rm_material &M = d->materials[m];
+        int32_t t = M.albedo_tex;
+        if (t >= d->n_textures || t < -1) return fail(RM_ERR_INVALID, "rm_scene_create: material %d texture %d out of range", m, t);
+        for (int32_t q : {M.normal_tex, M.emissive_tex, M.specular_tex})
+            if (q != -1 && (q < 0 || q >= d->n_textures || !d->textures[q].bytes))
+                return fail(RM_ERR_INVALID, "rm_scene_create: material %d map %d is not an 8-bit texture of the scene", m, q);
+    }
     const bool has_sky = d->sky_rgb && d->sky_width > 0 && d->sky_height > 0;
 
     hs.n = n;
@@ -331,13 +375,46 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
     hs.mats.resize(d->n_materials);
     for (int32_t m = 0; m < d->n_materials; ++m) hs.mats[m] = mat_rec(d->materials[m], has_sky);
     for (int32_t t = 0; t < d->n_textures; ++t) {
-        // ImageData::generateMipmaps (src/material.cpp:113-148) on fp32 texels:
-        // up to 8 levels, 2x2 box average with wrap, stop when a side reaches 0.
         const rm_texture &X = d->textures[t];
         TexRec R{};
         R.width = X.width;
         R.height = X.height;
         R.levels = 8;
+        R.alpha = -1;
+        if (X.bytes) {
+            // ImageData::generateMipmaps (src/material.cpp:113-148) on the
+            // reference's bytes: integer 2x2 sums / 4 in the channel layout
+            const int ch = X.channels;
+            R.channels = ch;
+            R.offset[0] = (int64_t)hs.bytes.size();
+            R.level_bytes[0] = (int64_t)X.width * X.height * ch;
+            hs.bytes.insert(hs.bytes.end(), X.bytes, X.bytes + R.level_bytes[0]);
+            for (int lv = 1; lv < 8; ++lv) {
+                int pw = X.width >> (lv - 1), ph = X.height >> (lv - 1);
+                int cw = X.width >> lv, chh = X.height >> lv;
+                if (cw == 0 || chh == 0) { R.levels = lv; break; }
+                R.offset[lv] = (int64_t)hs.bytes.size();
+                R.level_bytes[lv] = (int64_t)cw * chh * ch;
+                hs.bytes.resize(hs.bytes.size() + (size_t)R.level_bytes[lv]);
+                const uint8_t *prev = hs.bytes.data() + R.offset[lv - 1];
+                uint8_t *cur = hs.bytes.data() + R.offset[lv];
+                for (int y = 0; y < chh; ++y)
+                    for (int x = 0; x < cw; ++x)
+                        for (int c = 0; c < ch; ++c) {
+                            int sum = 0;
+                            for (int dy = 0; dy < 2; ++dy)
+                                for (int dx = 0; dx < 2; ++dx) {
+                                    int px = (x * 2 + dx) % pw, py = (y * 2 + dy) % ph;
+                                    sum += prev[((int64_t)py * pw + px) * ch + c];
+                                }
+                            cur[((int64_t)y * cw + x) * ch + c] = (uint8_t)(sum / 4);
+                        }
+            }
+            hs.texs.push_back(R);
+            continue;
+        }
+        // ImageData::generateMipmaps (src/material.cpp:113-148) on fp32 texels:
+        // up to 8 levels, 2x2 box average with wrap, stop when a side reaches 0.
         R.offset[0] = (int64_t)hs.texels.size();
         hs.texels.insert(hs.texels.end(), X.rgb, X.rgb + (int64_t)X.width * X.height * 3);
         for (int lv = 1; lv < 8; ++lv) {
@@ -370,8 +447,17 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
         hs.texs.push_back(R);
     }
     // Material::hasFullyTransparentPart (src/material.cpp:330-333): an alpha
-    // value below 1 (an 8-bit alpha below 255) anywhere in the albedo texture.
+    // value below 1 (an 8-bit alpha below 255) anywhere in the albedo texture;
+    // for the reference's bytes, every 4th byte from the 4th
+    // (ImageData::hasTransparentPart, src/material.cpp:102-107).
     for (MatRec &M : hs.mats) {
+        if (M.tex >= 0 && d->textures[M.tex].bytes) {
+            const TexRec &T = hs.texs[M.tex];
+            for (int64_t i = 3; i < T.level_bytes[0] && !M.cutout; i += 4)
+                M.cutout = hs.bytes[T.offset[0] + i] < 255 ? 1 : 0;
+            hs.cutout |= M.cutout;
+            continue;
+        }
         if (M.tex < 0 || !d->textures[M.tex].alpha) continue;
         const rm_texture &X = d->textures[M.tex];
         const int64_t np = (int64_t)X.width * X.height;
@@ -387,19 +473,32 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
             int64_t j = i;
             int32_t mesh = d->mesh ? d->mesh[i] : 0;
             while (j < n && (d->mesh ? d->mesh[j] : 0) == mesh) ++j;
-            // a mesh has one material in the reference (aiMesh::mMaterialIndex)
+            // a mesh has one material in the reference (aiMesh::mMaterialIndex);
+            // it is a light object with an emissive map (src/model.cpp:121) or,
+            // as an extension, a constant emission
             const MatRec &M = hs.mats[d->material[i]];
-            bool emissive = M.emission[0] != 0.0f || M.emission[1] != 0.0f || M.emission[2] != 0.0f;
+            bool emissive = M.etex >= 0 || M.emission[0] != 0.0f || M.emission[1] != 0.0f || M.emission[2] != 0.0f;
             if (emissive) {
                 LightRec L{};
                 std::vector<int64_t> faces;
+                std::vector<V3> avgs;
                 V3 em{M.emission[0], M.emission[1], M.emission[2]};
                 for (int64_t f = i; f < j; ++f) {
-                    float Clum = emissive_lum(em);
+                    V3 avg = emissive_avg(em);
+                    if (M.etex >= 0) {
+                        TriShade F{};
+                        for (int k = 0; k < 3 && d->uvs; ++k) {
+                            F.uv[k][0] = d->uvs[f * 6 + k * 2];
+                            F.uv[k][1] = d->uvs[f * 6 + k * 2 + 1];
+                        }
+                        avg = emissive_avg_tex(hs, hs.texs[M.etex], F);
+                    }
+                    float Clum = dot(avg, v3(0.3f, 0.6f, 0.1f));
                     V3 v0 = B.vert((int32_t)f, 0), v1 = B.vert((int32_t)f, 1), v2 = B.vert((int32_t)f, 2);
                     float area = length(cross(v1 - v0, v2 - v0)) / 2.0f;
                     if (Clum < EPS || area < EPS) continue;
                     faces.push_back(f);
+                    avgs.push_back(avg);
                 }
                 if (!faces.empty()) {
                     // the faces only; power, colour, centre and face CDF come from light_params
@@ -408,6 +507,7 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
                     for (size_t k = 0; k < faces.size(); ++k) {
                         hs.light_cdf.push_back(0.0f);
                         hs.light_faces.push_back(TriShade{});
+                        hs.light_face_avg.push_back(avgs[k]);
                     }
                     // fill copies from original-order data
                     for (size_t k = 0; k < faces.size(); ++k) {
@@ -502,6 +602,7 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     UP(mats, mats);
     UP(texs, texs);
     UP(texels, texels);
+    UP(bytes, bytes);
     UP(lights, lights);
     UP(light_faces, light_faces);
     UP(light_cdf, light_cdf);
@@ -517,6 +618,7 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     v.mats = s->mats.as<MatRec>();
     v.texs = s->texs.as<TexRec>();
     v.texels = s->texels.as<float>();
+    v.bytes = s->bytes.as<uint8_t>();
     v.cutout = h.cutout;
     v.lights = s->lights.as<LightRec>();
     v.light_faces = s->light_faces.as<TriShade>();
@@ -543,7 +645,8 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     I.n_light_faces = (int32_t)h.light_faces.size();
     I.has_sky = h.sky.empty() ? 0 : 1;
     I.device_bytes = (int64_t)(s->nodes.bytes + s->tris.bytes + s->shade.bytes + s->mats.bytes + s->texs.bytes +
-                               s->texels.bytes + s->lights.bytes + s->light_faces.bytes + s->light_cdf.bytes +
+                               s->texels.bytes + s->bytes.bytes + s->lights.bytes + s->light_faces.bytes +
+                               s->light_cdf.bytes +
                                s->sky.bytes + s->sky_cdf.bytes + s->sky_guide.bytes);
     I.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     I.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
@@ -552,6 +655,7 @@ extern "C" int32_t rm_scene_create(const rm_scene_desc *desc, rm_scene **out) {
     HostArray<TriRec>().swap(h.tris);
     HostArray<TriShade>().swap(h.shade);
     std::vector<float>().swap(h.texels);
+    std::vector<uint8_t>().swap(h.bytes);
     *out = s;
     return RM_OK;
 }
@@ -564,7 +668,7 @@ extern "C" int32_t rm_bvh_build(const float *verts, int64_t n, int32_t *perm, in
     m.roughness = 0.8f;
     m.opacity = 1.0f;
     m.ior = 1.0f;
-    m.albedo_tex = -1;
+    m.albedo_tex = m.normal_tex = m.emissive_tex = m.specular_tex = -1;
     rm_scene_desc d{};
     d.n_tris = n;
     d.verts = verts;
@@ -607,7 +711,8 @@ extern "C" int32_t rm_scene_update(rm_scene *scene, const rm_material *materials
         for (size_t m = 0; m < h.mats.size(); ++m) {
             next[m] = mat_rec(materials[m], has_sky);
             const MatRec &o = h.mats[m], &n = next[m];
-            if (n.tex != o.tex) return fail(RM_ERR_INVALID, "rm_scene_update: material %zu changes its texture", m);
+            if (n.tex != o.tex || n.ntex != o.ntex || n.etex != o.etex || n.stex != o.stex)
+                return fail(RM_ERR_INVALID, "rm_scene_update: material %zu changes a texture binding", m);
             next[m].cutout = o.cutout;  // the alpha planes do not change
             bool e0 = o.emission[0] != 0.0f || o.emission[1] != 0.0f || o.emission[2] != 0.0f;
             bool e1 = n.emission[0] != 0.0f || n.emission[1] != 0.0f || n.emission[2] != 0.0f;
@@ -626,6 +731,9 @@ extern "C" int32_t rm_scene_update(rm_scene *scene, const rm_material *materials
         }
     }
     if (texels) {
+        for (const TexRec &T : h.texs)
+            if (T.channels)
+                return fail(RM_ERR_UNSUPPORTED, "rm_scene_update: texels of a scene with 8-bit textures");
         int64_t off = 0;
         for (size_t t = 0; t < h.texs.size(); ++t) {
             const TexRec &T = h.texs[t];

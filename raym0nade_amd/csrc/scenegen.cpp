@@ -92,7 +92,7 @@ rm_material lambert(float r, float g, float b) {
     m.metallic = 0.0f;
     m.opacity = 1.0f;
     m.ior = 1.0f;
-    m.albedo_tex = -1;
+    m.albedo_tex = m.normal_tex = m.emissive_tex = m.specular_tex = -1;
     return m;
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include "rm_layout.h"
 #include "rm_vec.h"
+#include "tex8.h"
 #include "traverse.h"
 
 namespace rm {
@@ -164,6 +165,11 @@ __device__ __forceinline__ bool transparent_at(const SceneView &S, V3 o, V3 d, f
     V2 u0{F.uv[0][0], F.uv[0][1]}, u1{F.uv[1][0], F.uv[1][1]}, u2{F.uv[2][0], F.uv[2][1]};
     V2 uv = (bc.x * u0 + bc.y * u1) + bc.z * u2;
     const TexRec &T = S.texs[M.tex];
+    if (T.channels) {  // getDiffuseColor(u, v, NaN)[3]: the level-0 sample's fourth component
+        float c[4];
+        tex8_get<4>(S.bytes, T, uv.x, uv.y, 0.0f, c);
+        return c[3] < T_EPS;
+    }
     return alpha_bilinear(S.texels + T.alpha, T.width, T.height, uv.x, 1.0f - uv.y) < T_EPS;
 }
 
@@ -175,8 +181,9 @@ __device__ __forceinline__ V2 get_duv(const TriShade &F, V3 dP) {
     return ((b.x - 1.0f) * u0 + b.y * u1) + b.z * u2;
 }
 
-// calcSurfaceNormal (src/model.cpp:279-298) with a zero normal map
-__device__ __forceinline__ void calc_surface_normal(const TriShade &F, V3 shapeN, V3 &surfN) {
+// calcSurfaceNormal (src/model.cpp:279-298): the normal map's x and y tilt
+// the normal in the face's uv tangent frame (its z is not used)
+__device__ __forceinline__ void calc_surface_normal(const TriShade &F, V3 nm, V3 shapeN, V3 &surfN) {
     V3 v0 = ld3(F.v[0]);
     V3 e1 = ld3(F.v[1]) - v0, e2 = ld3(F.v[2]) - v0;
     V2 d1{F.uv[1][0] - F.uv[0][0], F.uv[1][1] - F.uv[0][1]};
@@ -186,8 +193,20 @@ __device__ __forceinline__ void calc_surface_normal(const TriShade &F, V3 shapeN
     V3 t = normalize(tu - shapeN * dot(shapeN, tu));
     V3 b = normalize(tv - shapeN * dot(shapeN, tv) - t * dot(t, tv));
     V3 sav = surfN;
-    surfN = normalize(t * 0.0f + b * 0.0f + surfN);
+    surfN = normalize(t * nm.x + b * nm.y + surfN);
     if (!any_finite(surfN)) surfN = sav;
+}
+
+// gammaPow (src/material.cpp:337-347) of a sample's first three components:
+// powf(x, 2.2f), evaluated in fp64 and rounded once.
+__device__ __forceinline__ V3 gamma_pow3(const float c[4]) {
+    const double g = (double)2.2f;
+    return V3{(float)pow((double)c[0], g), (float)pow((double)c[1], g), (float)pow((double)c[2], g)};
+}
+
+// The mip depth Material's getters use: log2(duv * width), 0 for duv = NaN.
+__device__ __forceinline__ float tex_depth(float duv, int width) {
+    return isnan(duv) ? 0.0f : log2f(duv * float(width));
 }
 
 // Where a vertex's albedo came from: texture `tex` at (u, v) and mip depth
@@ -205,8 +224,16 @@ __device__ __forceinline__ void hit_material(const SceneView &S, const TriShade 
     V2 uv = (bc.x * u0 + bc.y * u1) + bc.z * u2;
     const MatRec &M = S.mats[F.material];
     h.id = F.material;
-    h.roughness = M.roughness;  // getSurfaceData without a specular map
-    h.metallic = M.metallic;
+    if (M.stex >= 0) {
+        // getSurfaceData (src/material.cpp:374-383): the level-0 sample
+        float c[4];
+        tex8_get<4>(S.bytes, S.texs[M.stex], uv.x, uv.y, 0.0f, c);
+        h.metallic = gmin(c[2], 0.99f);   // std::min(b, 0.99)
+        h.roughness = gmax(c[1], 1e-3f);  // std::max(g, 1e-3)
+    } else {
+        h.roughness = M.roughness;  // getSurfaceData without a specular map
+        h.metallic = M.metallic;
+    }
     h.opacity = M.opacity;
     h.eta = M.ior;
     if (h.opacity > 1.0f - T_EPS) h.entering = true;
@@ -217,14 +244,37 @@ __device__ __forceinline__ void hit_material(const SceneView &S, const TriShade 
         h.base = ld3(M.trans);
     } else if (M.tex >= 0) {
         const TexRec &T = S.texs[M.tex];
-        float depth = isnan(duv) ? 0.0f : log2f(duv * float(T.width));
-        h.base = tex_get(S, T, uv.x, uv.y, depth);
-        if (tf) *tf = TexFoot{M.tex, uv.x, uv.y, depth};
+        float depth = tex_depth(duv, T.width);
+        if (T.channels) {
+            // getDiffuseColor (src/material.cpp:349-356) on 8-bit texels
+            float c[4];
+            tex8_get<4>(S.bytes, T, uv.x, uv.y, depth, c);
+            h.base = gamma_pow3(c);
+        } else {
+            h.base = tex_get(S, T, uv.x, uv.y, depth);
+            if (tf) *tf = TexFoot{M.tex, uv.x, uv.y, depth};
+        }
     } else {
         h.base = ld3(M.base);
     }
-    h.emission = ld3(M.emission);
-    calc_surface_normal(F, h.shapeN, h.surfN);
+    if (M.etex >= 0) {
+        // getEmissiveColor (src/material.cpp:365-372)
+        const TexRec &T = S.texs[M.etex];
+        float c[4];
+        tex8_get<4>(S.bytes, T, uv.x, uv.y, tex_depth(duv, T.width), c);
+        h.emission = gamma_pow3(c);
+    } else {
+        h.emission = ld3(M.emission);
+    }
+    V3 nm = v3(0.0f);
+    if (M.ntex >= 0) {
+        // getNormal (src/material.cpp:358-363)
+        const TexRec &T = S.texs[M.ntex];
+        float c[4];
+        tex8_get<3>(S.bytes, T, uv.x, uv.y, tex_depth(duv, T.width), c);
+        nm = V3{c[0] * 2.0f - 1.0f, c[1] * 2.0f - 1.0f, c[2] * 2.0f - 1.0f};
+    }
+    calc_surface_normal(F, nm, h.shapeN, h.surfN);
 }
 
 // calc_dPdxy (src/render.cpp:44-50)

@@ -28,11 +28,39 @@ def gpu_render(scene, args, stream=None):
     return download_all(fr)
 
 
-def assert_gbuffer_equal(gb, ogb):
+def _tweaked(lo, hi):
+    """hi == lo with the reference's +4e-2 on the red channel of a grey base
+    colour (src/render.cpp:492-495): float32(lo + 0.04) in fp64."""
+    return np.float32(lo.astype(np.float64) + 4e-2) == hi
+
+
+def assert_gbuffer_equal(gb, ogb, max_untweaked=0.0, ulp_channels=(), ulps=0):
+    """The two G-buffers agree bit for bit, except that
+
+    * a fraction `max_untweaked` of the pixels may differ in base red by the
+      reference's +4e-2 tweak alone: renderPixel leaves the tweak in place
+      when no indirect sample was gathered (src/render.cpp:529-530), which
+      whole paths decide, so a path that a last-bit difference of a
+      transcendental function sends elsewhere can flip it;
+    * channels in `ulp_channels` may differ by `ulps` units in the last place
+      (values of powf, log2f)."""
     a = gb.view(np.uint32)
     b = ogb.view(np.uint32)
     nan_a, nan_b = np.isnan(gb), np.isnan(ogb)
     same = (a == b) | (nan_a & nan_b)
+    if ulp_channels:
+        ch = list(ulp_channels)
+        d = np.abs(gb[..., ch].view(np.int32).astype(np.int64) - ogb[..., ch].view(np.int32).astype(np.int64))
+        same[..., ch] |= d <= ulps
+    if max_untweaked:
+        bad = ~same
+        only_red = bad[..., 12] & (bad.sum(axis=-1) == 1)
+        x, y = gb[..., 12], ogb[..., 12]
+        tw = only_red & (_tweaked(x, y) | _tweaked(y, x))
+        n_pix = int(np.prod(gb.shape[:-1]))
+        if tw.sum() > max_untweaked * n_pix:
+            raise AssertionError(f"{int(tw.sum())} of {n_pix} pixels differ by the base-colour tweak alone")
+        same[..., 12] |= tw
     if not same.all():
         idx = np.argwhere(~same)[:5]
         raise AssertionError(f"G-buffer differs at {len(np.argwhere(~same))} values, e.g. {idx.tolist()}: "

@@ -28,7 +28,8 @@ def quad(z, half=1.0):
 
 def make_desc(verts, uvs, material, mesh, mats, textures, normals=None):
     """RmSceneDesc over numpy arrays.  mats: RmMaterial list; textures: list of
-    (rgb [h][w][3], alpha [h][w] or None).  Returns (desc, keepalive)."""
+    (rgb [h][w][3], alpha [h][w] or None) fp32 textures or uint8 [h][w][3|4]
+    arrays (8-bit textures).  Returns (desc, keepalive)."""
     verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
     uvs = np.ascontiguousarray(uvs, np.float32).reshape(-1, 6)
     material = np.ascontiguousarray(material, np.int32)
@@ -37,7 +38,14 @@ def make_desc(verts, uvs, material, mesh, mats, textures, normals=None):
     m_arr = (RmMaterial * len(mats))(*mats)
     t_arr = (RmTexture * max(1, len(textures)))()
     keep += [m_arr, t_arr]
-    for i, (rgb, alpha) in enumerate(textures):
+    for i, tex in enumerate(textures):
+        if isinstance(tex, np.ndarray) and tex.dtype == np.uint8:  # 8-bit [h][w][channels]
+            b = np.ascontiguousarray(tex)
+            keep.append(b)
+            t_arr[i].height, t_arr[i].width, t_arr[i].channels = b.shape
+            t_arr[i].bytes = b.ctypes.data_as(C.POINTER(C.c_uint8))
+            continue
+        rgb, alpha = tex
         rgb = np.ascontiguousarray(rgb, np.float32)
         keep.append(rgb)
         t_arr[i].height, t_arr[i].width = rgb.shape[:2]

@@ -9,7 +9,6 @@ MTL materials (d < 0.99) follow the reference's name table
 (src/material.cpp:300-333)."""
 import ctypes as C
 import struct
-import zlib
 
 import numpy as np
 import pytest
@@ -90,23 +89,18 @@ def test_cornell_obj_round_trip_is_the_same_scene(dev, tmp_path):
     assert np.array_equal(rad1.view(np.uint32), rad0.view(np.uint32))
 
 
-def png_bytes(rgb):
-    h, w = rgb.shape[:2]
-    raw = b"".join(b"\x00" + rgb[y].tobytes() for y in range(h))
-    def chunk(t, d):
-        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
-    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)) +
-            chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
-
-
 def test_textured_quad_and_hdr_sky(dev, tmp_path):
-    """A quad facing the camera with a constant PNG albedo (200, 40, 10) and
-    a constant HDR sky: hits carry the base colour (c / 255)^2.2, misses the
-    sky's radiance; glass by name."""
-    (tmp_path / "tex.png").write_bytes(png_bytes(np.tile(np.uint8([200, 40, 10]), (8, 8, 1))))
+    """A quad facing the camera with a constant RGBA DDS albedo (200, 40, 10,
+    255) and a constant HDR sky: hits carry the base colour (c / 255)^2.2,
+    misses the sky's radiance; glass by name.  (PNG maps: test_tex8_gpu.)"""
+    dds = bytearray(128)
+    dds[0:4] = b"DDS "
+    struct.pack_into("<IIII", dds, 4, 124, 0x1007, 8, 8)
+    struct.pack_into("<IIIIIIII", dds, 76, 32, 0x41, 0, 32, 0x00FF0000, 0x0000FF00, 0x000000FF, 0xFF000000)
+    (tmp_path / "tex.dds").write_bytes(bytes(dds) + bytes([10, 40, 200, 255]) * 64)
     hdr = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 4 +X 8\n" + bytes([128, 64, 32, 129]) * 32
     (tmp_path / "sky.hdr").write_bytes(hdr)
-    (tmp_path / "q.mtl").write_text("newmtl Paint\nmap_Kd tex.png\nnewmtl Water\nd 0.5\n")
+    (tmp_path / "q.mtl").write_text("newmtl Paint\nmap_Kd tex.dds\nnewmtl Water\nd 0.5\n")
     (tmp_path / "q.obj").write_text(
         "mtllib q.mtl\nv -1 -1 0\nv 1 -1 0\nv 1 1 0\nv -1 1 0\nv -1 -1 -3\nv 1 -1 -3\nv 1 1 -3\n"
         "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nvn 0 0 1\n"

@@ -27,7 +27,7 @@ def test_every_declared_function_is_exported():
 
 
 def test_abi_version():
-    assert lib().rm_abi_version() == 1
+    assert lib().rm_abi_version() == 2
 
 
 def test_null_arguments_are_errors():
