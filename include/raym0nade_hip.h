@@ -199,6 +199,19 @@ int32_t rm_trace_closest(const rm_scene *scene, const float *rays_dev, int64_t n
                          float tmin, float tmax, float *t_out_dev, int32_t *id_out_dev,
                          void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  rm_trace_closest with the same results, whose
+ * ray-reordering workspace is a static device array of the library (64 MiB per
+ * GPU, held until the process ends) instead of allocated memory; it calls no
+ * allocating routine and ignores RM_TUNE_KERNEL_TIMING.  At most
+ * RM_NOALLOC_MAX_RAYS rays per call (RM_ERR_INVALID beyond).  Every call on a
+ * GPU shares that GPU's array: calls that overlap in time on one GPU (on
+ * different streams or threads) race, so the caller serialises them. */
+#define RM_NOALLOC_MAX_RAYS 2097152
+int32_t rm_trace_closest_noalloc(const rm_scene *scene, const float *rays_dev, int64_t n, float tmin, float tmax,
+                                 float *t_out_dev, int32_t *id_out_dev, void *stream);
+
 /* Occlusion.  Replaces Model::rayHit_test (src/model.cpp:343-354): true iff the
  * closest hit on (tmin, aim+1e-4) has t < aim.  aim_dev (per ray, may be NULL)
  * overrides aim_depth.  Exits as soon as the running closest t drops below

@@ -153,6 +153,8 @@ SIGNATURES = [
     ("rm_scene_get_perm", C.c_int32, [C.c_void_p, i32p]),
     ("rm_bvh_build", C.c_int32, [f32p, C.c_int64, i32p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     ("rm_trace_closest", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rm_trace_closest_noalloc", C.c_int32,
+     [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rm_trace_any", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rm_trace_count", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
     ("rm_frame_create", C.c_int32, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

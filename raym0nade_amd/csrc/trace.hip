@@ -507,12 +507,21 @@ static TraceWork *trace_work(hipStream_t st) {
     return w;
 }
 
+// A caller-provided reordering workspace (rm_trace_closest_noalloc): the
+// call then allocates nothing, records no events, and fails when the
+// workspace is too small.
+struct FixedWorkspace {
+    char *base;
+    size_t bytes;
+};
+
 template <bool ANY>
 static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, float tmin, float t0, const float *aim_dev,
-                            float *t_out, int32_t *id_out, uint8_t *occ, hipStream_t st) {
+                            float *t_out, int32_t *id_out, uint8_t *occ, hipStream_t st,
+                            const FixedWorkspace *fixed = nullptr) {
     const Tuning &T = tuning();
     unsigned grid = (unsigned)((n + TRACE_BLOCK - 1) / TRACE_BLOCK);
-    const bool timing = T.kernel_timing != 0;
+    const bool timing = T.kernel_timing != 0 && !fixed;
     int32_t mrc;
     if (timing && (mrc = mark(0, st))) return mrc;
     // reordering workspace: keys[n], order[n], bins, flag
@@ -528,14 +537,22 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                                          (unsigned)(3 * CELL_BITS + 3), st));
         const size_t plane = ((size_t)n * 4 + 255) & ~(size_t)255;
         const size_t bytes = 4 * plane + 256 + tmp_bytes;
-        TraceWork *w = trace_work(st);
-        ws_lock = std::unique_lock<std::mutex>(w->mu);
-        if (w->buf.bytes < bytes) {
-            if (w->buf.p) RM_HIP(hipStreamSynchronize(st));  // earlier calls on this stream may still read it
-            int32_t rc = w->buf.alloc(bytes);
-            if (rc) return rc;
+        char *base = nullptr;
+        if (fixed) {
+            if (fixed->bytes < bytes)
+                return fail(RM_ERR_INVALID, "trace: %lld rays need %zu workspace bytes, %zu available", (long long)n,
+                            bytes, fixed->bytes);
+            base = fixed->base;
+        } else {
+            TraceWork *w = trace_work(st);
+            ws_lock = std::unique_lock<std::mutex>(w->mu);
+            if (w->buf.bytes < bytes) {
+                if (w->buf.p) RM_HIP(hipStreamSynchronize(st));  // earlier calls on this stream may still read it
+                int32_t rc = w->buf.alloc(bytes);
+                if (rc) return rc;
+            }
+            base = (char *)w->buf.p;
         }
-        char *base = (char *)w->buf.p;
         uint32_t *keys = (uint32_t *)base, *keys2 = (uint32_t *)(base + plane), *vals = (uint32_t *)(base + 2 * plane);
         order = (uint32_t *)(base + 3 * plane);
         flag = (int *)(base + 4 * plane);
@@ -547,14 +564,21 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                                          (size_t)n, 0u, (unsigned)(3 * CELL_BITS + 3), st));
     } else if (T.ray_sort != 0 && n >= 4096) {
         size_t bytes = (size_t)n * 8 + (size_t)NBINS * 4 + 256;
-        TraceWork *w = trace_work(st);
-        ws_lock = std::unique_lock<std::mutex>(w->mu);
-        if (w->buf.bytes < bytes) {
-            if (w->buf.p) RM_HIP(hipStreamSynchronize(st));  // earlier calls on this stream may still read it
-            int32_t rc = w->buf.alloc(bytes);
-            if (rc) return rc;
+        if (fixed) {
+            if (fixed->bytes < bytes)
+                return fail(RM_ERR_INVALID, "trace: %lld rays need %zu workspace bytes, %zu available", (long long)n,
+                            bytes, fixed->bytes);
+            ws = fixed->base;
+        } else {
+            TraceWork *w = trace_work(st);
+            ws_lock = std::unique_lock<std::mutex>(w->mu);
+            if (w->buf.bytes < bytes) {
+                if (w->buf.p) RM_HIP(hipStreamSynchronize(st));  // earlier calls on this stream may still read it
+                int32_t rc = w->buf.alloc(bytes);
+                if (rc) return rc;
+            }
+            ws = (char *)w->buf.p;
         }
-        ws = (char *)w->buf.p;
         uint32_t *keys = reinterpret_cast<uint32_t *>(ws);
         order = keys + n;
         uint32_t *hist = order + n;
@@ -634,6 +658,44 @@ extern "C" int32_t rm_trace_closest(const rm_scene *s, const float *rays, int64_
     if (n > 0 && !id_out) return fail(RM_ERR_INVALID, "rm_trace_closest: id_out is NULL");
     if (n == 0) return RM_OK;
     return launch_trace<false>(s, rays, n, tmin, tmax, nullptr, t_out, id_out, nullptr, (hipStream_t)stream);
+}
+
+// rm_trace_closest_noalloc's workspace: one static array per GPU, of which
+// host code only ever uses the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in, not a device
+// address).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static int32_t noalloc_scratch(char **out) {
+    static std::mutex mu;
+    static char *ptr[64] = {nullptr};
+    int dev = 0;
+    RM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(RM_ERR_UNSUPPORTED, "rm_trace_closest_noalloc: device index %d", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        ptr[dev] = static_cast<char *>(p);
+    }
+    *out = ptr[dev];
+    return RM_OK;
+}
+
+extern "C" int32_t rm_trace_closest_noalloc(const rm_scene *s, const float *rays, int64_t n, float tmin, float tmax,
+                                            float *t_out, int32_t *id_out, void *stream) {
+    int32_t st = check_trace_args(s, rays, n, t_out, "rm_trace_closest_noalloc");
+    if (st) return st;
+    if (n > 0 && !id_out) return fail(RM_ERR_INVALID, "rm_trace_closest_noalloc: id_out is NULL");
+    if (n > RM_NOALLOC_MAX_RAYS)
+        return fail(RM_ERR_INVALID, "rm_trace_closest_noalloc: %lld rays exceed the maximum of %d", (long long)n,
+                    (int)RM_NOALLOC_MAX_RAYS);
+    if (n == 0) return RM_OK;
+    FixedWorkspace ws{nullptr, NOALLOC_SCRATCH_BYTES};
+    if ((st = noalloc_scratch(&ws.base))) return st;
+    return launch_trace<false>(s, rays, n, tmin, tmax, nullptr, t_out, id_out, nullptr, (hipStream_t)stream, &ws);
 }
 
 extern "C" int32_t rm_trace_any(const rm_scene *s, const float *rays, int64_t n, float tmin, float aim_depth,
