@@ -456,7 +456,7 @@ enum rm_tune_key {
                                            (default; in a render launch: every walk of its kernels),
                                            4 as 3 with two walks per lane (traces; renders as 3) */
     RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variants 2, 3 (default 8) */
-    RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 16) */
+    RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 32) */
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive
                                            origins are far apart, 2 always, 3 as 1 with a radix sort
                                            of the bins (rocPRIM) instead of the counting sort
@@ -470,8 +470,8 @@ enum rm_tune_key {
                                            shading and NEE kernels are compiled for (4 caps them
                                            at 128 VGPRs, spilling the rest) */
     RM_TUNE_LDS_STACK_ENTRIES = 8,      /* trace variant 3: walk-stack entries per lane kept in LDS
-                                           (1..32, default 8; deeper entries go to scratch; 32 holds
-                                           the whole stack of any tree) */
+                                           (1..32, default 10: 160 KiB of LDS at 8 blocks/CU; deeper
+                                           entries go to scratch; 32 holds the whole stack of any tree) */
     RM_TUNE_TRIS_PER_STEP = 9           /* trace variant 3 and render walks: triangles one walk step
                                            tests (0 = a whole leaf; 1, 2 (default) or 4: a leaf over
                                            several steps; the render's shading kernels use 0 or 2) */,

@@ -47,7 +47,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
         candidates += [(1, 8, 16, s, 32, 0) for s in (0, 1)]
         candidates += [(2, b, f, s, 32, 0) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
         candidates += [(3, b, f, 1, e, 0) for b, f, e in itertools.product((4, 8, 16), (16, 32), (4, 6, 8, 12, 32))]
-        candidates += [(3, b, f, s, e, p) for b, f, e, p, s in itertools.product((8, 16), (16, 32), (6, 8), (1, 2, 4), (1, 3))]
+        candidates += [(3, b, f, s, e, p) for b, f, e, p, s in itertools.product((8, 16), (16, 32), (6, 8, 10), (1, 2, 4), (1, 3))]
         candidates += [(4, b, 32, 1, 6, 0) for b in (6, 8)]
 
     def run():
@@ -71,7 +71,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
 
 
 def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4, 8),
-                walks=((3, 8, 2), (3, 8, 0), (3, 8, 1), (3, 8, 4), (3, 6, 2), (3, 12, 2), (2, 32, 2)),
+                walks=((3, 10, 2), (3, 10, 0), (3, 10, 1), (3, 10, 4), (3, 8, 2), (3, 6, 2), (2, 32, 2)),
                 log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp

@@ -350,7 +350,7 @@ __device__ __forceinline__ void replay_path(const AdjArgs &G, const Surf &s0, V3
         V3 dPdx, dPdy;
         TexFoot tf;
         tf.tex = -1;
-        hit_info(S, slot, d, t, diff, dPdx, dPdy, s, AD ? &tf : nullptr);
+        hit_info<false>(S, slot, d, t, diff, dPdx, dPdy, s, AD ? &tf : nullptr);  // rm_adjoint refuses 8-bit maps
         V3 V = -d;
         if (rf < 1.0f * s.roughness) src = s.id;  // gmax keeps the first argument on ties
         rf = gmax(rf, 1.0f * s.roughness);
@@ -489,7 +489,7 @@ __device__ __forceinline__ void adjoint_pixel(const AdjArgs &G, int64_t p, int c
     s.pos = A.pos + t * Dir;
     V3 dPdx, dPdy;
     TexFoot tf0;
-    hit_info(S, slot, Dir, t, cam, dPdx, dPdy, s, &tf0);
+    hit_info<false>(S, slot, Dir, t, cam, dPdx, dPdy, s, &tf0);
     const int mat0 = s.id;
     if (chunk == 0) {
         rep += s.emission * ex;

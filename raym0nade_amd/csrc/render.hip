@@ -56,8 +56,8 @@ constexpr int NCOUNT = 32;   // queue counters per round (>= max_depth + 1)
 // max_depth <= 16 vertices continue, so 16 entries hold the reference's
 // stack exactly; an insert into a full stack (only possible at the last
 // vertex, where nothing reads the stack afterwards) is dropped and counted.
-// The entries live in registers: every loop below is unrolled, so each index
-// is a constant.
+// The loops below run over the live entries only, so a path outside every
+// medium never touches the array, which lives in per-lane scratch.
 constexpr int MED_CAP = 16;
 constexpr uint32_t MED_NUDGED = 0x80000000u;
 struct Med {
@@ -65,11 +65,7 @@ struct Med {
     uint32_t e[MED_CAP];
 };
 __device__ __forceinline__ int med_key(uint32_t e) { return (int)(e & ~MED_NUDGED); }
-__device__ __forceinline__ void med_clear(Med &m) {
-    m.n = 0;
-#pragma unroll
-    for (int i = 0; i < MED_CAP; ++i) m.e[i] = 0u;
-}
+__device__ __forceinline__ void med_clear(Med &m) { m.n = 0; }
 __device__ __forceinline__ V3 med_entry_absorb(const SceneView &S, uint32_t e) {
     V3 a = ld3(S.mats[med_key(e)].trans);
     if (e & MED_NUDGED) nudge_base(a);
@@ -77,56 +73,36 @@ __device__ __forceinline__ V3 med_entry_absorb(const SceneView &S, uint32_t e) {
 }
 __device__ __forceinline__ float med_ior(const SceneView &S, const Med &m) {
     float r = 1.0f;
-#pragma unroll
-    for (int i = 0; i < MED_CAP; ++i)
-        if (i < m.n) r = gmax(r, S.mats[med_key(m.e[i])].ior);  // std::max(ior, medium.ior)
+#pragma nounroll
+    for (int i = 0; i < m.n; ++i) r = gmax(r, S.mats[med_key(m.e[i])].ior);  // std::max(ior, medium.ior)
     return r;
 }
 __device__ __forceinline__ V3 med_absorb(const SceneView &S, const Med &m) {
     V3 a = v3(1.0f);
-#pragma unroll
-    for (int i = 0; i < MED_CAP; ++i)
-        if (i < m.n) a *= med_entry_absorb(S, m.e[i]);
+#pragma nounroll
+    for (int i = 0; i < m.n; ++i) a *= med_entry_absorb(S, m.e[i]);
     return a;
 }
-// multimap::erase(id): the entries of `id` form one run [lo, lo + c) of the
-// sorted stack; the tail moves down by c in shifts of 1, 2, 4, 8 and 16.
+// multimap::erase(id): every entry of `id` goes; the others keep their order.
 __device__ __forceinline__ void med_erase(Med &m, int id) {
-    int lo = 0, c = 0;
-#pragma unroll
-    for (int i = 0; i < MED_CAP; ++i)
-        if (i < m.n) {
-            const int k = med_key(m.e[i]);
-            lo += k < id ? 1 : 0;
-            c += k == id ? 1 : 0;
-        }
-    if (c == 0) return;
-#pragma unroll
-    for (int b = 0; b < 5; ++b) {
-        const int sft = 1 << b;
-        if (c & sft) {
-#pragma unroll
-            for (int i = 0; i < MED_CAP; ++i)
-                if (i >= lo) m.e[i] = i + sft < MED_CAP ? m.e[i + sft] : 0u;
-        }
-    }
-    m.n -= c;
+    int k = 0;
+#pragma nounroll
+    for (int i = 0; i < m.n; ++i)
+        if (med_key(m.e[i]) != id) m.e[k++] = m.e[i];
+    m.n = k;
 }
 // multimap::emplace(id, ...): after the entries with keys <= id.  Returns
 // false when the stack is full and the entry is dropped (counted as
 // RM_STAT_MEDIUM_OVERFLOW by the caller).
 __device__ __forceinline__ bool med_insert(Med &m, int id, bool nudged) {
     if (m.n >= MED_CAP) return false;
-    int pos = 0;
-#pragma unroll
-    for (int i = 0; i < MED_CAP; ++i)
-        if (i < m.n && med_key(m.e[i]) <= id) ++pos;
-    const uint32_t x = (uint32_t)id | (nudged ? MED_NUDGED : 0u);
-#pragma unroll
-    for (int i = MED_CAP - 1; i >= 0; --i) {
-        const uint32_t below = i > 0 ? m.e[i - 1] : 0u;
-        m.e[i] = i > pos ? below : (i == pos ? x : m.e[i]);
+    int pos = m.n;
+#pragma nounroll
+    while (pos > 0 && med_key(m.e[pos - 1]) > id) {
+        m.e[pos] = m.e[pos - 1];
+        --pos;
     }
+    m.e[pos] = (uint32_t)id | (nudged ? MED_NUDGED : 0u);
     ++m.n;
     return true;
 }
@@ -164,7 +140,7 @@ struct PathQ {
 // f0 o.xyz | pixel      f1 d.xyz | sample     f2 T.xyz | W        f3 p0.xyz | rf
 // f4 dPdx  | rng n      f5 dPdy  | set        f6 dDdx  | depth+exclude<<8
 // f7 dDdy  | medium n + job<<8              f8-f11 medium entries 0..15
-// (written and read only when the path is inside a medium)
+// (only the float4s holding live entries are written and read)
 
 struct Path {
     V3 o, d, T, p0;
@@ -188,12 +164,10 @@ __device__ __forceinline__ void store_path(const PathQ &Q, uint32_t q, const Pat
     Q.f[5][q] = make_float4(P.diff.dPdy.x, P.diff.dPdy.y, P.diff.dPdy.z, ub(P.set));
     Q.f[6][q] = make_float4(P.diff.dDdx.x, P.diff.dDdx.y, P.diff.dDdx.z, ub((uint32_t)P.depth | (P.exclude ? 256u : 0u)));
     Q.f[7][q] = make_float4(P.diff.dDdy.x, P.diff.dDdy.y, P.diff.dDdy.z, ub((uint32_t)P.med.n | ((uint32_t)P.job << 8)));
-    if (P.med.n > 0) {
-        const uint32_t *e = P.med.e;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            Q.f[8 + k][q] = make_float4(ub(e[4 * k]), ub(e[4 * k + 1]), ub(e[4 * k + 2]), ub(e[4 * k + 3]));
-    }
+#pragma nounroll
+    for (int k = 0; 4 * k < P.med.n; ++k)
+        Q.f[8 + k][q] = make_float4(ub(P.med.e[4 * k]), ub(P.med.e[4 * k + 1]), ub(P.med.e[4 * k + 2]),
+                                    ub(P.med.e[4 * k + 3]));
 }
 
 __device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
@@ -214,15 +188,13 @@ __device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
     med_clear(P.med);
     P.med.n = (int)(bu(h.w) & 255u);
     P.job = (int)(bu(h.w) >> 8);
-    if (P.med.n > 0) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float4 m = Q.f[8 + k][q];
-            P.med.e[4 * k] = bu(m.x);
-            P.med.e[4 * k + 1] = bu(m.y);
-            P.med.e[4 * k + 2] = bu(m.z);
-            P.med.e[4 * k + 3] = bu(m.w);
-        }
+#pragma nounroll
+    for (int k = 0; 4 * k < P.med.n; ++k) {
+        const float4 m = Q.f[8 + k][q];
+        P.med.e[4 * k] = bu(m.x);
+        P.med.e[4 * k + 1] = bu(m.y);
+        P.med.e[4 * k + 2] = bu(m.z);
+        P.med.e[4 * k + 3] = bu(m.w);
     }
     return P;
 }
@@ -296,6 +268,104 @@ __device__ __forceinline__ void job_stat_add(const Job *J, int myjob, uint32_t v
     }
 }
 
+// The largest v (< 2^BITS) of each job in the wave, one atomicMax per job
+// and wave.  Called by every lane of the wave together; lanes with v == 0
+// take no part.
+template <int BITS>
+__device__ __forceinline__ void job_stat_max(const Job *J, int myjob, uint32_t v, int k) {
+    uint64_t pend = __ballot(v != 0u);
+    while (pend) {
+        const int lead = (int)__ffsll((unsigned long long)pend) - 1;
+        const int j = __builtin_amdgcn_readfirstlane(__shfl(myjob, lead, 64));
+        const bool mine = v != 0u && myjob == j;
+        uint32_t m = 0;
+        for (int b = BITS - 1; b >= 0; --b) {
+            const uint32_t c = m | (1u << b);
+            if (__ballot(mine && v >= c)) m = c;
+        }
+        if ((int)(threadIdx.x & 63u) == lead) atomicMax(J[j].F.stats + k, (unsigned long long)m);
+        pend &= ~__ballot(mine);
+    }
+}
+
+// Per-lane render counters of a persistent kernel, kept in the block's LDS
+// (rows [N + 1][RB]: the lane's job + 1, then N counters; max-type where
+// bit k of MAXMASK is set, sums otherwise), so that they cost the kernel no
+// registers.  A lane whose next item belongs to another job first adds its
+// counts to the old job's RM_STAT_* keys[k] (note); the kernel ends with
+// flush_wave, which every lane of the wave calls together and which adds
+// each job's counts over the wave with one atomic per counter.  Counting per
+// item instead costs a contended atomic per 64 items.
+template <int N, uint32_t MAXMASK>
+struct LaneStats {
+    uint32_t *s;  // this lane's column
+    __device__ __forceinline__ void init(uint32_t *rows) {
+        s = rows + threadIdx.x;
+#pragma unroll
+        for (int k = 0; k <= N; ++k) s[k * RB] = 0u;
+    }
+    __device__ __forceinline__ void add_lane(const Job *J, const int (&keys)[N], int job) const {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const uint32_t x = s[(k + 1) * RB];
+            if (!x) continue;
+            if ((MAXMASK >> k) & 1u)
+                atomicMax(J[job].F.stats + keys[k], (unsigned long long)x);
+            else
+                atomicAdd(J[job].F.stats + keys[k], (unsigned long long)x);
+        }
+    }
+    __device__ __forceinline__ void note(const Job *J, const int (&keys)[N], int job, const uint32_t (&x)[N]) {
+        const uint32_t cur = s[0];
+        if (cur != (uint32_t)job + 1u) {
+            if (cur) add_lane(J, keys, (int)cur - 1);
+            s[0] = (uint32_t)job + 1u;
+#pragma unroll
+            for (int k = 0; k < N; ++k) s[(k + 1) * RB] = 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            uint32_t &c = s[(k + 1) * RB];
+            c = ((MAXMASK >> k) & 1u) ? (x[k] > c ? x[k] : c) : c + x[k];
+        }
+    }
+    __device__ __forceinline__ void flush_wave(const Job *J, const int (&keys)[N]) const {
+        const uint32_t lane = threadIdx.x & 63u;
+        const int job = (int)s[0] - 1;
+        uint32_t v[N];
+        bool has = false;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            v[k] = s[(k + 1) * RB];
+            has = has || v[k] != 0u;
+        }
+        has = has && job >= 0;
+        uint64_t pending = __ballot(has);
+        while (pending) {
+            const int lead = (int)__ffsll((unsigned long long)pending) - 1;
+            const int lj = __builtin_amdgcn_readfirstlane(__shfl(job, lead, 64));
+            const bool mine = has && job == lj;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                uint32_t x = mine ? v[k] : 0u;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const uint32_t y = __shfl_xor(x, off, 64);
+                    x = ((MAXMASK >> k) & 1u) ? (x > y ? x : y) : x + y;
+                }
+                if (lane == (uint32_t)lead && x) {
+                    if ((MAXMASK >> k) & 1u)
+                        atomicMax(J[lj].F.stats + keys[k], (unsigned long long)x);
+                    else
+                        atomicAdd(J[lj].F.stats + keys[k], (unsigned long long)x);
+                }
+            }
+            if (mine) has = false;
+            pending &= ~__ballot(mine);
+        }
+    }
+};
+
 // p: global pixel (work buffers), lp: job-local pixel (frame buffers)
 __device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F, int64_t p, int64_t lp) {
     Surf s = surf_default();
@@ -316,7 +386,8 @@ __device__ __forceinline__ Surf pixel_surf(const PixBufs &P, const FrameView &F,
 }
 
 // TPS (here and in k_direct, k_extend, k_nee): triangles per walk step.
-template <int TPS>
+// MAPS (here and in k_bounce): some job's scene has 8-bit texture maps.
+template <int TPS, bool MAPS>
 __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int nj, int64_t total_px, PixBufs P) {
     const int64_t gp = (int64_t)blockIdx.x * RB + threadIdx.x;
     if (gp >= total_px) return;
@@ -353,7 +424,7 @@ __global__ __launch_bounds__(RB) void k_primary(const Job *__restrict__ J, int n
     }
     s.pos = A.pos + t * Dir;
     V3 dPdx, dPdy;
-    hit_info(S, slot, Dir, t, base, dPdx, dPdy, s);
+    hit_info<MAPS>(S, slot, Dir, t, base, dPdx, dPdy, s);
     bool opaque = s.opacity > 1.0f - T_EPS;
     V3 sav = s.base;
     nudge_base(s.base);
@@ -653,7 +724,7 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
     if (want) store_path(Q, q, path);
     job_stat_add<1>(J, myjob, want ? 1u : 0u, RM_STAT_PATHS);
     job_stat_add<1>(J, myjob, ovf ? 1u : 0u, RM_STAT_MEDIUM_OVERFLOW);
-    if (want && path.med.n > 0) atomicMax(J[myjob].F.stats + RM_STAT_MEDIUM_MAX, (unsigned long long)path.med.n);
+    job_stat_max<5>(J, myjob, want ? (uint32_t)path.med.n : 0u, RM_STAT_MEDIUM_MAX);
 }
 
 // Light sample of a finished path -> its accumulator set.
@@ -784,11 +855,43 @@ __device__ __forceinline__ float4 *rec_slot(const Job &jb, uint32_t gp, uint32_t
 // LDS: the walk's first lds_entries pending entries in the block's dynamic
 // LDS (lds_entries * RB * 8 bytes), deeper ones in per-lane scratch.
 // COUNT: each walk's node visits and tests go to its frame's RM_STAT_WALK_*.
+// A lane sums the walks of one job in registers and adds them when its next
+// walk belongs to another job (add_walk_counts) and, summed over the wave per
+// job, when the kernel ends (flush_walk_counts).
 __device__ __forceinline__ void add_walk_counts(unsigned long long *stats, TraceCounts &c) {
-    atomicAdd(stats + RM_STAT_WALK_INTERIOR, (unsigned long long)c.interior);
-    atomicAdd(stats + RM_STAT_WALK_LEAVES, (unsigned long long)c.leaves);
-    atomicAdd(stats + RM_STAT_WALK_BOXES, (unsigned long long)c.boxes);
-    atomicAdd(stats + RM_STAT_WALK_TRIS, (unsigned long long)c.tris);
+    if (c.interior | c.leaves) {
+        atomicAdd(stats + RM_STAT_WALK_INTERIOR, (unsigned long long)c.interior);
+        atomicAdd(stats + RM_STAT_WALK_LEAVES, (unsigned long long)c.leaves);
+        atomicAdd(stats + RM_STAT_WALK_BOXES, (unsigned long long)c.boxes);
+        atomicAdd(stats + RM_STAT_WALK_TRIS, (unsigned long long)c.tris);
+    }
+    c = TraceCounts{};
+}
+
+// Called by every lane of the wave together.
+__device__ __forceinline__ void flush_walk_counts(const Job *__restrict__ J, int job, TraceCounts &c) {
+    const uint32_t lane = threadIdx.x & 63u;
+    bool has = (c.interior | c.leaves) != 0u;
+    uint64_t pending = __ballot(has);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lj = __shfl(job, leader, 64);
+        const bool mine = has && job == lj;
+        uint32_t v[4] = {mine ? c.interior : 0u, mine ? c.leaves : 0u, mine ? c.boxes : 0u, mine ? c.tris : 0u};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+        if (lane == (uint32_t)leader) {
+            unsigned long long *stats = J[lj].F.stats;
+            atomicAdd(stats + RM_STAT_WALK_INTERIOR, (unsigned long long)v[0]);
+            atomicAdd(stats + RM_STAT_WALK_LEAVES, (unsigned long long)v[1]);
+            atomicAdd(stats + RM_STAT_WALK_BOXES, (unsigned long long)v[2]);
+            atomicAdd(stats + RM_STAT_WALK_TRIS, (unsigned long long)v[3]);
+        }
+        if (mine) has = false;
+        pending &= ~__ballot(mine);
+    }
     c = TraceCounts{};
 }
 
@@ -827,7 +930,9 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
                 if (slot < n) {
                     q = slot;
                     float4 a = Q.f[0][slot], b = Q.f[1][slot];
-                    job = (int)(bu(Q.f[7][slot].w) >> 8);
+                    const int nj = (int)(bu(Q.f[7][slot].w) >> 8);
+                    if (COUNT && nj != job) add_walk_counts(J[job].F.stats, c);
+                    job = nj;
                     const SceneView &SJ = J[job].S;
                     S.nodes = SJ.nodes;
                     S.tris = SJ.tris;
@@ -839,20 +944,24 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
         }
         if (q >= 0 && w.step(S, r, T_EPS, -INFINITY, c)) {
             hit[q] = make_uint2(bu(w.tmax), (uint32_t)w.best);
-            if (COUNT) add_walk_counts(J[job].F.stats, c);
             q = -1;
         }
     }
+    if (COUNT) flush_walk_counts(J, job, c);
 }
 
 // MINW: minimum resident waves per SIMD the register allocation must allow
 // (1: no cap; 4: at most 128 VGPRs, the rest spilled).  Chosen by tuning.
-template <int MINW>
+template <int MINW, bool MAPS>
 __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, int nj, PixBufs P, AccBufs AB, PathQ Qin,
                                                const uint32_t *count_in, uint32_t *work, const uint2 *__restrict__ hit,
                                                PathQ Qout, uint32_t *count_out, NeeQ NQ, uint32_t *nee_count) {
   const uint32_t n_in = *count_in;
   const uint32_t lane = threadIdx.x & 63u;
+  const int keys[4] = {RM_STAT_EXTENSION_RAYS, RM_STAT_HITS, RM_STAT_MEDIUM_OVERFLOW, RM_STAT_MEDIUM_MAX};
+  __shared__ uint32_t stat_rows[5 * RB];
+  LaneStats<4, 8u> st;
+  st.init(stat_rows);
   while (true) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(work, 64u);
@@ -898,7 +1007,7 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
             Surf s = surf_default();
             s.pos = path.o + path.d * t;
             V3 dPdx, dPdy;
-            hit_info(S, slot, path.d, t, path.diff, dPdx, dPdy, s);
+            hit_info<MAPS>(S, slot, path.d, t, path.diff, dPdx, dPdy, s);
             V3 V = -path.d;
             path.rf = gmax(path.rf, 1.0f * s.roughness);
             s.roughness = gmax(s.roughness, path.rf);
@@ -989,15 +1098,14 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
     });
     uint32_t slot_out = append(count_out, cont);
     if (cont) store_path(Qout, slot_out, path);
-    if (cont && path.med.n > 1)  // rare: a path inside two or more media
-        atomicMax(J[path.job].F.stats + RM_STAT_MEDIUM_MAX, (unsigned long long)path.med.n);
     uint32_t slot_nee = append(nee_count, nee_out);
     if (nee_out) store_nee(NQ, slot_nee, path, nee_rec);
-    const int myjob = live ? path.job : -1;
-    job_stat_add<1>(J, myjob, live ? 1u : 0u, RM_STAT_EXTENSION_RAYS);
-    job_stat_add<1>(J, myjob, hit_any ? 1u : 0u, RM_STAT_HITS);
-    job_stat_add<2>(J, myjob, ovf, RM_STAT_MEDIUM_OVERFLOW);
+    if (live) {
+        const uint32_t x[4] = {1u, hit_any ? 1u : 0u, ovf, cont ? (uint32_t)path.med.n : 0u};
+        st.note(J, keys, path.job, x);
+    }
   }
+  st.flush_wave(J, keys);
 }
 
 // sampleDirectLight at the last vertex of every path that ended with
@@ -1009,6 +1117,10 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
                                             const uint32_t *count, uint32_t *work) {
   const uint32_t n = *count;
   const uint32_t lane = threadIdx.x & 63u;
+  const int keys[1] = {RM_STAT_SHADOW_RAYS};
+  __shared__ uint32_t stat_rows[2 * RB];
+  LaneStats<1, 0u> st;
+  st.init(stat_rows);
   while (true) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(work, 64u);
@@ -1058,8 +1170,12 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
             }
         }
     });
-    job_stat_add<3>(J, live ? (int)(flags >> 8) : -1, nrays, RM_STAT_SHADOW_RAYS);
+    if (live) {
+        const uint32_t x[1] = {nrays};
+        st.note(J, keys, (int)(flags >> 8), x);
+    }
   }
+  st.flush_wave(J, keys);
 }
 
 template <int MINW, int TPS>
@@ -1067,6 +1183,10 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
                                                 uint32_t *work, ShadowQ SQ, int queue_depth, uint32_t *queued) {
   const uint32_t n = *count;
   const uint32_t lane = threadIdx.x & 63u;
+  const int keys[1] = {RM_STAT_SHADOW_RAYS};
+  __shared__ uint32_t stat_rows[2 * RB];
+  LaneStats<1, 0u> st;
+  st.init(stat_rows);
   while (true) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(work, 64u);
@@ -1116,8 +1236,12 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
             }
         }
     });
-    job_stat_add<3>(J, live ? (int)(flags >> 8) : -1, nrays, RM_STAT_SHADOW_RAYS);
+    if (live) {
+        const uint32_t x[1] = {nrays};
+        st.note(J, keys, (int)(flags >> 8), x);
+    }
   }
+  st.flush_wave(J, keys);
 }
 
 // Any-hit walk of every shadow ray of the round (the k_extend scheme with
@@ -1157,7 +1281,9 @@ __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, Shadow
                 if (slot < n) {
                     q = slot;
                     const float4 oo = SQ.o[slot], dd = SQ.d[slot];
-                    job = bu(dd.w);
+                    const uint32_t nj = bu(dd.w);
+                    if (COUNT && nj != job) add_walk_counts(J[job].F.stats, c);
+                    job = nj;
                     const SceneView &SJ = J[job].S;
                     S.nodes = SJ.nodes;
                     S.tris = SJ.tris;
@@ -1170,10 +1296,10 @@ __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, Shadow
         }
         if (q >= 0 && w.step(S, r, T_EPS, aim, c)) {
             SQ.vis[q] = (w.tmax < aim) ? 0 : 1;
-            if (COUNT) add_walk_counts(J[job].F.stats, c);
             q = -1;
         }
     }
+    if (COUNT) flush_walk_counts(J, (int)job, c);
 }
 
 template <int MINW>
@@ -1422,10 +1548,12 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     for (int i = 0; i < nj; ++i)
         if (hj[i].args) all_px += (int64_t)std::max(0, hj[i].args->width) * std::max(0, hj[i].args->height);
     int64_t total_px = 0;
+    bool maps = false;  // the shading kernels with 8-bit texture lookups
     for (int i = 0; i < nj; ++i) {
         int r = 0, d = 0;
         int32_t rc = make_job(hj[i], J[i], r, d, all_px, nj > 1 ? "rm_render_batch" : "rm_render");
         if (rc) return rc;
+        maps = maps || hj[i].scene->host.maps8 != 0;
         J[i].pix_off = total_px;
         total_px += J[i].np;
         R = std::max(R, r);
@@ -1481,8 +1609,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     // triangles per walk step: any of 0, 1, 2, 4 for the extension kernel;
     // the shading kernels' walks take a whole leaf (0) or two triangles
     const int tps = tuning().tris_per_step;
-    hipLaunchKernelGGL(tps ? k_primary<2> : k_primary<0>, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj,
-                       total_px, P);
+    hipLaunchKernelGGL(maps ? (tps ? k_primary<2, true> : k_primary<0, true>)
+                            : (tps ? k_primary<2, false> : k_primary<0, false>),
+                       dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
     hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nblk), dim3(SB), 0, st, P.M, total_px, bsum);
     RM_HIP(hipGetLastError());
     std::vector<int64_t> hs(nblk);
@@ -1610,7 +1739,8 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
         const unsigned gb = (unsigned)(device_cu_count() * tuning().render_blocks_per_cu);
         hipLaunchKernelGGL(k_clear_sets, dim3(gs), dim3(RB), 0, st, AB, nsets);
         const unsigned ge = gpers;
-        auto bounce = capped ? k_bounce<4> : k_bounce<1>;
+        auto bounce = maps ? (capped ? k_bounce<4, true> : k_bounce<1, true>)
+                           : (capped ? k_bounce<4, false> : k_bounce<1, false>);
         auto nee = capped ? (tps ? k_nee<4, 2> : k_nee<4, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
         auto nee_gen = capped ? (tps ? k_nee_gen<4, 2> : k_nee_gen<4, 0>) : (tps ? k_nee_gen<1, 2> : k_nee_gen<1, 0>);
         auto nee_add = capped ? k_nee_add<4> : k_nee_add<1>;

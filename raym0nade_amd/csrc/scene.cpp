@@ -195,7 +195,7 @@ V3 emissive_avg_tex(const HostScene &hs, const TexRec &T, const TriShade &F) {
             float u = (a * F.uv[0][0] + b * F.uv[1][0]) + c * F.uv[2][0];
             float v = (a * F.uv[0][1] + b * F.uv[1][1]) + c * F.uv[2][1];
             float t[4];
-            tex8_get<4>(hs.bytes.data(), T, u, v, 0.0f, t);
+            tex8_get(hs.bytes.data(), T, u, v, 0.0f, 4, t);
             avg += V3{powf(t[0], 2.2f), powf(t[1], 2.2f), powf(t[2], 2.2f)};
         }
     return avg / static_cast<float>(g * g);
@@ -386,6 +386,7 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs) {
             // reference's bytes: integer 2x2 sums / 4 in the channel layout
             const int ch = X.channels;
             R.channels = ch;
+            hs.maps8 = 1;
             R.offset[0] = (int64_t)hs.bytes.size();
             R.level_bytes[0] = (int64_t)X.width * X.height * ch;
             hs.bytes.insert(hs.bytes.end(), X.bytes, X.bytes + R.level_bytes[0]);

@@ -152,6 +152,7 @@ __device__ __forceinline__ float alpha_bilinear(const float *data, int w, int h,
     return c0 * (1.0f - dy) + c1 * dy;
 }
 
+
 // TransparentTest (src/model.cpp:217-230): the hit (t, slot) of the ray
 // (o, d) lies on a fully transparent texel of an alpha cut-out.  The level-1
 // term of get() is multiplied by a blend of 0, so the level-0 value is the
@@ -167,7 +168,7 @@ __device__ __forceinline__ bool transparent_at(const SceneView &S, V3 o, V3 d, f
     const TexRec &T = S.texs[M.tex];
     if (T.channels) {  // getDiffuseColor(u, v, NaN)[3]: the level-0 sample's fourth component
         float c[4];
-        tex8_get<4>(S.bytes, T, uv.x, uv.y, 0.0f, c);
+        tex8_get(S.bytes, T, uv.x, uv.y, 0.0f, 4, c);
         return c[3] < T_EPS;
     }
     return alpha_bilinear(S.texels + T.alpha, T.width, T.height, uv.x, 1.0f - uv.y) < T_EPS;
@@ -197,12 +198,6 @@ __device__ __forceinline__ void calc_surface_normal(const TriShade &F, V3 nm, V3
     if (!any_finite(surfN)) surfN = sav;
 }
 
-// gammaPow (src/material.cpp:337-347) of a sample's first three components:
-// powf(x, 2.2f), evaluated in fp64 and rounded once.
-__device__ __forceinline__ V3 gamma_pow3(const float c[4]) {
-    const double g = (double)2.2f;
-    return V3{(float)pow((double)c[0], g), (float)pow((double)c[1], g), (float)pow((double)c[2], g)};
-}
 
 // The mip depth Material's getters use: log2(duv * width), 0 for duv = NaN.
 __device__ __forceinline__ float tex_depth(float duv, int width) {
@@ -218,61 +213,76 @@ struct TexFoot {
 
 // getHitMaterial (src/model.cpp:300-328).  `tf`, when given, receives the
 // albedo footprint (the adjoint scatters texel gradients through it).
+// MAPS: the scene may have 8-bit texture maps (the lookups are compiled in
+// only then: they would cost every other scene registers).
+template <bool MAPS = true>
 __device__ __forceinline__ void hit_material(const SceneView &S, const TriShade &F, V3 bc, V3 dPdx, V3 dPdy, Surf &h,
                                              TexFoot *tf = nullptr) {
     V2 u0{F.uv[0][0], F.uv[0][1]}, u1{F.uv[1][0], F.uv[1][1]}, u2{F.uv[2][0], F.uv[2][1]};
     V2 uv = (bc.x * u0 + bc.y * u1) + bc.z * u2;
     const MatRec &M = S.mats[F.material];
     h.id = F.material;
-    if (M.stex >= 0) {
-        // getSurfaceData (src/material.cpp:374-383): the level-0 sample
-        float c[4];
-        tex8_get<4>(S.bytes, S.texs[M.stex], uv.x, uv.y, 0.0f, c);
-        h.metallic = gmin(c[2], 0.99f);   // std::min(b, 0.99)
-        h.roughness = gmax(c[1], 1e-3f);  // std::max(g, 1e-3)
-    } else {
-        h.roughness = M.roughness;  // getSurfaceData without a specular map
-        h.metallic = M.metallic;
-    }
+    h.roughness = M.roughness;  // getSurfaceData without a specular map
+    h.metallic = M.metallic;
     h.opacity = M.opacity;
     h.eta = M.ior;
     if (h.opacity > 1.0f - T_EPS) h.entering = true;
     V2 ddx = get_duv(F, dPdx), ddy = get_duv(F, dPdy);
     float duv = any_finite2(ddx) ? (len2d(ddx) + len2d(ddy)) / 2.0f : NAN;
     if (tf) tf->tex = -1;
+    int dtex = -1;  // an 8-bit diffuse texture
     if (h.opacity < T_EPS) {
         h.base = ld3(M.trans);
     } else if (M.tex >= 0) {
         const TexRec &T = S.texs[M.tex];
-        float depth = tex_depth(duv, T.width);
         if (T.channels) {
-            // getDiffuseColor (src/material.cpp:349-356) on 8-bit texels
-            float c[4];
-            tex8_get<4>(S.bytes, T, uv.x, uv.y, depth, c);
-            h.base = gamma_pow3(c);
+            dtex = M.tex;
         } else {
+            float depth = tex_depth(duv, T.width);
             h.base = tex_get(S, T, uv.x, uv.y, depth);
             if (tf) *tf = TexFoot{M.tex, uv.x, uv.y, depth};
         }
     } else {
         h.base = ld3(M.base);
     }
-    if (M.etex >= 0) {
-        // getEmissiveColor (src/material.cpp:365-372)
-        const TexRec &T = S.texs[M.etex];
-        float c[4];
-        tex8_get<4>(S.bytes, T, uv.x, uv.y, tex_depth(duv, T.width), c);
-        h.emission = gamma_pow3(c);
-    } else {
-        h.emission = ld3(M.emission);
-    }
+    h.emission = ld3(M.emission);
     V3 nm = v3(0.0f);
-    if (M.ntex >= 0) {
-        // getNormal (src/material.cpp:358-363)
-        const TexRec &T = S.texs[M.ntex];
-        float c[4];
-        tex8_get<3>(S.bytes, T, uv.x, uv.y, tex_depth(duv, T.width), c);
-        nm = V3{c[0] * 2.0f - 1.0f, c[1] * 2.0f - 1.0f, c[2] * 2.0f - 1.0f};
+    // The 8-bit maps, in one loop so that the kernel holds one copy of the
+    // lookup: 0 getSurfaceData (the level-0 sample; metallic = min(b, 0.99),
+    // roughness = max(g, 1e-3)), 1 getDiffuseColor and 2 getEmissiveColor
+    // (gammaPow after filtering), 3 getNormal (a vec3 lookup, * 2 - 1)
+    // (src/material.cpp:337-383).
+    if (MAPS && (M.stex >= 0 || dtex >= 0 || M.etex >= 0 || M.ntex >= 0)) {
+#pragma nounroll
+        for (int k = 0; k < 4; ++k) {
+            const int ti = k == 0 ? M.stex : k == 1 ? dtex : k == 2 ? M.etex : M.ntex;
+            if (ti < 0) continue;
+            const TexRec &T = S.texs[ti];
+            float c[4];
+            tex8_get(S.bytes, T, uv.x, uv.y, k == 0 ? 0.0f : tex_depth(duv, T.width), k == 3 ? 3 : 4, c);
+            V3 x{c[0], c[1], c[2]};
+            if (k == 1 || k == 2) {
+                // powf(x, 2.2f), evaluated in fp64 and rounded once
+                const double g = (double)2.2f;
+#pragma nounroll
+                for (int j = 0; j < 3; ++j) {
+                    const float y = (float)pow((double)(j == 0 ? x.x : j == 1 ? x.y : x.z), g);
+                    x.x = j == 0 ? y : x.x;
+                    x.y = j == 1 ? y : x.y;
+                    x.z = j == 2 ? y : x.z;
+                }
+            }
+            if (k == 0) {
+                h.metallic = gmin(c[2], 0.99f);   // std::min(b, 0.99)
+                h.roughness = gmax(c[1], 1e-3f);  // std::max(g, 1e-3)
+            } else if (k == 1) {
+                h.base = x;
+            } else if (k == 2) {
+                h.emission = x;
+            } else {
+                nm = V3{x.x * 2.0f - 1.0f, x.y * 2.0f - 1.0f, x.z * 2.0f - 1.0f};
+            }
+        }
     }
     calc_surface_normal(F, nm, h.shapeN, h.surfN);
 }
@@ -295,6 +305,7 @@ __device__ __forceinline__ void calc_dDdxy(V3 dir, V3 n, const Diff &b, V3 &dDdx
 }
 
 // getHitInfo (src/render.cpp:62-79); h.pos must be set by the caller
+template <bool MAPS = true>
 __device__ __forceinline__ void hit_info(const SceneView &S, int32_t slot, V3 dir, float t, const Diff &base, V3 &dPdx,
                                          V3 &dPdy, Surf &h, TexFoot *tf = nullptr) {
     const TriShade &F = S.shade[slot];
@@ -302,7 +313,7 @@ __device__ __forceinline__ void hit_info(const SceneView &S, int32_t slot, V3 di
     hit_normals(F, dir, bc, h.shapeN, h.surfN, h.entering);
     V3 raw = h.surfN;
     calc_dPdxy(dir, t, h.shapeN, base, dPdx, dPdy);
-    hit_material(S, F, bc, dPdx, dPdy, h, tf);
+    hit_material<MAPS>(S, F, bc, dPdx, dPdy, h, tf);
     if (dot(h.surfN, dir) >= 0.0f) h.surfN = raw;
     if (dot(h.surfN, dir) >= 0.0f) h.surfN = h.shapeN;
 }
