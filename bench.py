@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="accepted for older profile commands; the 1 GiB calibration copy always runs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--render-tuning", default="",
+                    help="fix the 1080p render launch instead of autotuning it: "
+                         "'blocks_per_cu,min_waves,paths_per_round,walk_variant,lds_stack_entries,tris_per_step'")
     ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort[,lds_stack_entries[,tris_per_step]]'")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
@@ -635,7 +638,15 @@ def render_leg(args, torch, rm, dev, stream):
         f"{S.info.device_bytes / 2**30:.2f} GiB in HBM")
     fr = rm.Frame(a.width, a.height)
     tuned = None
-    if not args.no_autotune:
+    if args.render_tuning:
+        b, w, k, v, e, p = (int(x) for x in args.render_tuning.split(","))
+        for key, val in ((rm.TUNE_RENDER_BLOCKS_PER_CU, b), (rm.TUNE_RENDER_MIN_WAVES, w), (rm.TUNE_TRACE_VARIANT, v),
+                         (rm.TUNE_LDS_STACK_ENTRIES, e), (rm.TUNE_TRIS_PER_STEP, p)):
+            rm.set_tuning(key, val)
+        a.paths_per_round = k
+        tuned = {"blocks_per_cu": b, "min_waves": w, "paths_per_round": k, "walk_variant": v,
+                 "lds_stack_entries": e, "tris_per_step": p, "fixed": True}
+    elif not args.no_autotune:
         from raym0nade_amd import autotune
         tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 4, blocks=(4, 8), waves=(1, 4),
                                      paths_per_round=(0, 4, 8), log=log)
