@@ -28,6 +28,14 @@ __device__ __forceinline__ void load_ray(const float *rays, int64_t i, V3 &o, V3
     d = V3{b.y, c.x, c.y};
 }
 
+// What a traversal kernel writes for a closest hit in slot `slot` (BVH
+// order): the original triangle id, -1 for a miss; in scenes with alpha
+// cut-outs the slot itself, which k_trace_cutout continues from and maps.
+__device__ __forceinline__ int32_t out_id(const SceneView &S, int32_t slot) {
+    if (slot < 0) return -1;
+    return S.cutout ? slot : __float_as_int(S.tris[slot].e1[3]);
+}
+
 // KIND only distinguishes instantiations by name in profiles (0 = generic).
 // Ray processed by slot j: order[j] when the batch was reordered, else j.
 __device__ __forceinline__ int64_t ray_of(int64_t j, const uint32_t *order, const int *use_order) {
@@ -49,10 +57,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_closest(SceneView S, cons
     float tmax = tmax0;
     TraceCounts c{};
     int32_t best = trace<false, false>(S, r, tmin, tmax, -INFINITY, c);
-    int32_t id = -1;
-    if (best >= 0) id = __float_as_int(S.tris[best].e1[3]);
     t_out[i] = tmax;
-    id_out[i] = id;
+    id_out[i] = out_id(S, best);
 }
 
 template <int KIND>
@@ -75,9 +81,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_any(SceneView S, const fl
 }
 
 // Alpha cut-outs after the walk of a batch (TransparentTest,
-// src/model.cpp:217-230): a ray with a closest hit is walked again through
-// the whole of Model::rayHit (src/model.cpp:332-341); a blocked shadow ray
-// runs Model::rayHit_test (src/model.cpp:343-354).  Launched only for scenes
+// src/model.cpp:217-230): a ray whose closest hit is transparent continues
+// through the rest of Model::rayHit (src/model.cpp:332-341); a blocked
+// shadow ray runs Model::rayHit_test (src/model.cpp:343-354).  Launched only for scenes
 // with a cut-out material, so the traversal kernels stay as they are.
 template <bool ANY>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_cutout(SceneView S, const float *__restrict__ rays, int64_t n,
@@ -92,10 +98,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_cutout(SceneView S, const
         const float aim = aim_arr ? aim_arr[i] : t0;
         occ[i] = model_blocked<false, 0>(S, o, d, tmin, aim) ? 1 : 0;
     } else {
-        if (id_out[i] < 0) return;
-        float t = t0;
-        int32_t slot = trace_walk<false, false>(S, ray_pre(o, d), tmin, t, -INFINITY);
-        slot = model_hit_rest<false, 0>(S, o, d, t, slot, t0);
+        // the batch walk's closest hit (its slot): only a transparent one
+        // walks again
+        const int32_t first = id_out[i];
+        if (first < 0) return;
+        float t = t_out[i];
+        const int32_t slot = model_hit_rest<false, 0>(S, o, d, t, first, t0);
         t_out[i] = t;
         id_out[i] = slot >= 0 ? __float_as_int(S.tris[slot].e1[3]) : -1;
     }
@@ -126,7 +134,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const f
         occ[i] = (w.tmax < aim) ? 1 : 0;
     } else {
         t_out[i] = w.tmax;
-        id_out[i] = w.best >= 0 ? __float_as_int(S.tris[w.best].e1[3]) : -1;
+        id_out[i] = out_id(S, w.best);
     }
 }
 
@@ -189,7 +197,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                 occ[ray] = (w.tmax < aim) ? 1 : 0;
             } else {
                 t_out[ray] = w.tmax;
-                id_out[ray] = w.best >= 0 ? __float_as_int(S.tris[w.best].e1[3]) : -1;
+                id_out[ray] = out_id(S, w.best);
             }
             ray = -1;
         }
@@ -269,7 +277,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent2(SceneView S, 
                 occ[ya] = (wa.tmax < aa) ? 1 : 0;
             } else {
                 t_out[ya] = wa.tmax;
-                id_out[ya] = wa.best >= 0 ? __float_as_int(S.tris[wa.best].e1[3]) : -1;
+                id_out[ya] = out_id(S, wa.best);
             }
             ya = -1;
         }
@@ -278,7 +286,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent2(SceneView S, 
                 occ[yb] = (wb.tmax < ab) ? 1 : 0;
             } else {
                 t_out[yb] = wb.tmax;
-                id_out[yb] = wb.best >= 0 ? __float_as_int(S.tris[wb.best].e1[3]) : -1;
+                id_out[yb] = out_id(S, wb.best);
             }
             yb = -1;
         }
