@@ -219,6 +219,19 @@ def main():
                 "launch_ms": {k: round(float(np.mean(v)), 4) for k, v in kt.items()},
                 "reorder_ms": {k: round(float(np.mean(v)), 4) for k, v in rt.items()}}
 
+    # lane utilisation of the persistent LDS walk (untimed calls with the
+    # instrumented kernel; the results are unchanged)
+    if rm.get_tuning(rm.TUNE_TRACE_VARIANT) == 3:
+        util = {}
+        for name, r in (("coh", r_coh), ("inc", r_inc)):
+            t_out, id_out, _ = bufs[name]
+            rm.set_tuning(rm.TUNE_TRACE_UTIL, 1)
+            S.trace_closest(r, t_out=t_out, id_out=id_out)
+            util[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in rm.last_trace_util().items()}
+            rm.set_tuning(rm.TUNE_TRACE_UTIL, 0)
+        roofline["lane_utilisation"] = util
+        log("[bench] lane utilisation:", {k: (v["interior_util"], v["leaf_util"], v["util"]) for k, v in util.items()})
+
     # measured streaming bandwidth: a plain 1 GiB device copy (1 GiB read + 1 GiB
     # written per launch); also the known-size kernel that scales the
     # FETCH_SIZE / WRITE_SIZE counters

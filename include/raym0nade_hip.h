@@ -483,7 +483,10 @@ enum rm_tune_key {
                                            visits, box and triangle tests into the frame's
                                            RM_STAT_WALK_* counters (every shadow ray then goes
                                            through the queued pass; results are unchanged, the
-                                           render is slower); default 0 */
+                                           render is slower); default 0 */,
+    RM_TUNE_TRACE_UTIL = 12             /* 1: trace calls of variant 3 count the lanes that walk an
+                                           interior node or test triangles in each step of each wave
+                                           (rm_last_trace_util; results unchanged); default 0 */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);
@@ -491,6 +494,13 @@ int32_t rm_get_tuning(int32_t key, int32_t *value);
  * kernel timing on: its ray-reordering kernels and its traversal kernel (HIP
  * events on the call's stream; waits for them). */
 int32_t rm_last_trace_ms(float *reorder_ms, float *traverse_ms);
+/* Lane utilisation of the last trace call that ran with RM_TUNE_TRACE_UTIL
+ * on (waits for it): counts[0] wave steps in which some lane walked an
+ * interior node and counts[1] the lanes that did; counts[2] wave steps in
+ * which some lane tested triangles and counts[3] those lanes.  A wave step
+ * offers 64 lanes, so counts[1] / (64 counts[0]) is the interior steps'
+ * utilisation. */
+int32_t rm_last_trace_util(uint64_t *counts);
 
 /* ------------------------------------------------------------ utilities */
 

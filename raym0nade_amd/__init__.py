@@ -76,7 +76,19 @@ def philox(ctr, key) -> tuple:
 
 (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT,
  TUNE_RENDER_BLOCKS_PER_CU, TUNE_ADJOINT_SPLIT, TUNE_KERNEL_TIMING, TUNE_RENDER_MIN_WAVES,
- TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP, TUNE_NEE_SPLIT, TUNE_RENDER_COUNT) = range(12)
+ TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP, TUNE_NEE_SPLIT, TUNE_RENDER_COUNT, TUNE_TRACE_UTIL) = range(13)
+
+
+def last_trace_util() -> dict:
+    """Lane utilisation of the last trace call made with TUNE_TRACE_UTIL on
+    (waits for it): per kind of walk step (interior node, triangle tests) the
+    wave steps, the lanes that took them, and their share of the 64 lanes."""
+    c = (C.c_uint64 * 4)()
+    check(lib().rm_last_trace_util(c))
+    wi, li, wl, ll = (int(x) for x in c)
+    return {"interior_wave_steps": wi, "interior_lane_steps": li, "leaf_wave_steps": wl, "leaf_lane_steps": ll,
+            "interior_util": li / (64 * wi) if wi else 0.0, "leaf_util": ll / (64 * wl) if wl else 0.0,
+            "util": (li + ll) / (64 * (wi + wl)) if wi + wl else 0.0}
 
 
 def last_trace_ms() -> tuple:

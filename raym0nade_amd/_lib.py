@@ -184,6 +184,7 @@ SIGNATURES = [
     ("rm_set_tuning", C.c_int32, [C.c_int32, C.c_int32]),
     ("rm_get_tuning", C.c_int32, [C.c_int32, C.POINTER(C.c_int32)]),
     ("rm_last_trace_ms", C.c_int32, [C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("rm_last_trace_util", C.c_int32, [C.POINTER(C.c_uint64)]),
     ("rm_frame_get_stats", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64)]),
 ]
 

@@ -136,6 +136,7 @@ struct Tuning {
     int tris_per_step = 2;        // trace variant 3: triangles tested per step (0: a whole leaf)
     int nee_split = 2;            // render: shadow rays of light samples in their own passes (2: deep trees)
     int render_count = 0;         // render: count the extension and shadow walks (RM_STAT_WALK_*)
+    int trace_util = 0;           // trace variant 3: count lane utilisation (rm_last_trace_util)
 };
 Tuning &tuning();
 int device_cu_count();

@@ -146,7 +146,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const f
 // pending entries live in the block's dynamic LDS (lds_entries * TRACE_BLOCK *
 // 8 bytes), deeper ones in per-lane scratch.  TPS: triangles per step (0: a
 // whole leaf per step).
-template <bool ANY, bool LDS, int TPS>
+// UTIL: count lane utilisation into util[4]: loop iterations of a wave in
+// which some lane walks an interior node, the lanes doing so, iterations in
+// which some lane tests triangles, and those lanes.
+template <bool ANY, bool LDS, int TPS, bool UTIL = false>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, const float *__restrict__ rays, int64_t n,
                                                                   float tmin, float tmax0,
                                                                   const float *__restrict__ aim_arr,
@@ -154,7 +157,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                                                                   uint8_t *__restrict__ occ,
                                                                   unsigned long long *__restrict__ next, int fetch_min,
                                                                   const uint32_t *__restrict__ order,
-                                                                  const int *__restrict__ use_order, int lds_entries) {
+                                                                  const int *__restrict__ use_order, int lds_entries,
+                                                                  unsigned long long *__restrict__ util = nullptr) {
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
     Walker<ANY, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
@@ -167,6 +171,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
     int64_t ray = -1;
     bool exhausted = false;
     TraceCounts c{};
+    unsigned long long u[4] = {0, 0, 0, 0};
     while (true) {
         bool idle = ray < 0;
         uint64_t idle_mask = __ballot(idle);
@@ -192,6 +197,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                 }
             }
         }
+        if constexpr (UTIL) {
+            const bool leaf = ray >= 0 && (w.cur & 0x80000000u);
+            const uint64_t mi = __ballot(ray >= 0 && !leaf), ml = __ballot(leaf);
+            if (mi) { u[0] += 1; u[1] += (unsigned long long)__popcll(mi); }
+            if (ml) { u[2] += 1; u[3] += (unsigned long long)__popcll(ml); }
+        }
         if (ray >= 0 && w.step(S, r, tmin, aim, c)) {
             if (ANY) {
                 occ[ray] = (w.tmax < aim) ? 1 : 0;
@@ -201,6 +212,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
             }
             ray = -1;
         }
+    }
+    if constexpr (UTIL) {
+        if (lane == 0)
+            for (int k = 0; k < 4; ++k)
+                if (u[k]) atomicAdd(util + k, u[k]);
     }
 }
 
@@ -481,6 +497,14 @@ static int32_t ray_counter(unsigned long long **out) {
     return RM_OK;
 }
 
+// Lane-utilisation counters of the last trace call that ran with
+// RM_TUNE_TRACE_UTIL on (variant 3), one set per device.
+__device__ __align__(256) unsigned long long g_trace_util[4];
+__global__ void k_zero_util(unsigned long long *u) {
+    if (threadIdx.x < 4) u[threadIdx.x] = 0ull;
+}
+static bool g_util_valid = false;
+
 // HIP events bracketing the reordering kernels and the traversal kernel of
 // the last trace call (recorded when kernel timing is on; rm_last_trace_ms).
 struct TraceEvents {
@@ -613,13 +637,26 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                                (size_t)2 * nl * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
                                occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, nl);
         else if (T.trace_variant == 3) {
-            auto kern = T.tris_per_step == 1   ? k_trace_persistent<ANY, true, 1>
-                        : T.tris_per_step == 2 ? k_trace_persistent<ANY, true, 2>
-                        : T.tris_per_step == 4 ? k_trace_persistent<ANY, true, 4>
-                                               : k_trace_persistent<ANY, true, 0>;
+            const bool util = T.trace_util != 0 && !fixed;
+            unsigned long long *ud = nullptr;
+            if (util) {
+                void *p = nullptr;
+                RM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(g_trace_util)));
+                ud = static_cast<unsigned long long *>(p);
+                hipLaunchKernelGGL(k_zero_util, dim3(1), dim3(64), 0, st, ud);
+                g_util_valid = true;
+            }
+            auto kern = util ? (T.tris_per_step == 1   ? k_trace_persistent<ANY, true, 1, true>
+                                : T.tris_per_step == 2 ? k_trace_persistent<ANY, true, 2, true>
+                                : T.tris_per_step == 4 ? k_trace_persistent<ANY, true, 4, true>
+                                                       : k_trace_persistent<ANY, true, 0, true>)
+                             : (T.tris_per_step == 1   ? k_trace_persistent<ANY, true, 1>
+                                : T.tris_per_step == 2 ? k_trace_persistent<ANY, true, 2>
+                                : T.tris_per_step == 4 ? k_trace_persistent<ANY, true, 4>
+                                                       : k_trace_persistent<ANY, true, 0>);
             hipLaunchKernelGGL(kern, dim3(pgrid), dim3(TRACE_BLOCK), (size_t)nl * TRACE_BLOCK * 8, st, s->view, rays, n,
                                tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
-                               (const int *)flag, nl);
+                               (const int *)flag, nl, ud);
         } else
             hipLaunchKernelGGL((k_trace_persistent<ANY, false, 0>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays,
                                n, tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
@@ -656,6 +693,14 @@ extern "C" int32_t rm_last_trace_ms(float *reorder_ms, float *traverse_ms) {
     RM_HIP(hipEventElapsedTime(&b, ev.e[1], ev.e[2]));
     if (reorder_ms) *reorder_ms = a;
     if (traverse_ms) *traverse_ms = b;
+    return RM_OK;
+}
+
+extern "C" int32_t rm_last_trace_util(uint64_t *counts) {
+    if (!counts) return fail(RM_ERR_INVALID, "rm_last_trace_util: NULL");
+    if (!g_util_valid) return fail(RM_ERR_INVALID, "rm_last_trace_util: no trace call ran with RM_TUNE_TRACE_UTIL on");
+    RM_HIP(hipDeviceSynchronize());
+    RM_HIP(hipMemcpyFromSymbol(counts, HIP_SYMBOL(g_trace_util), 4 * sizeof(uint64_t), 0, hipMemcpyDeviceToHost));
     return RM_OK;
 }
 

@@ -180,3 +180,40 @@ def test_full_size_c1(dev, variant):
         c = S.trace_count(gpu(r[idx], dev)).cpu().numpy().astype(np.uint64)
         _, _, co = OS.closest(r[idx], counts=True)
         assert np.array_equal(c, co)
+
+
+@pytest.mark.parametrize("tps", [0, 1, 2])
+def test_lane_utilisation_counts_match_the_walk(soup, dev, tps):
+    """With TUNE_TRACE_UTIL on, the persistent LDS walk reports the lanes that
+    took interior and leaf steps: the interior lane steps are the walk's
+    interior node visits exactly (rm_trace_count), the leaf lane steps are the
+    leaf visits (a whole leaf per step), the triangle tests (one per step) or
+    between the two (two per step); results are unchanged."""
+    g, S, OS = soup
+    keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_TRIS_PER_STEP, rm.TUNE_TRACE_UTIL)
+    old = tuple(rm.get_tuning(k) for k in keys)
+    try:
+        r = gpu(rm.rays(1, 200000, seed=4), dev)
+        rm.set_tuning(rm.TUNE_TRACE_VARIANT, 3)
+        rm.set_tuning(rm.TUNE_TRIS_PER_STEP, tps)
+        t0, i0 = S.trace_closest(r)
+        rm.set_tuning(rm.TUNE_TRACE_UTIL, 1)
+        t1, i1 = S.trace_closest(r)
+        u = rm.last_trace_util()
+        rm.set_tuning(rm.TUNE_TRACE_UTIL, 0)
+        torch.cuda.synchronize()
+        assert torch.equal(i0, i1) and torch.equal(t0.view(torch.int32), t1.view(torch.int32))
+        interior, leaves, boxes, tris = (int(x) for x in S.trace_count(r).cpu().numpy())
+        print("utilisation", tps, u)
+        assert u["interior_lane_steps"] == interior
+        if tps == 0:
+            assert u["leaf_lane_steps"] == leaves
+        elif tps == 1:
+            assert u["leaf_lane_steps"] == tris
+        else:
+            assert (tris + 1) // 2 <= u["leaf_lane_steps"] <= tris and u["leaf_lane_steps"] >= leaves
+        assert 0.0 < u["interior_util"] <= 1.0 and 0.0 < u["leaf_util"] <= 1.0
+        assert u["interior_wave_steps"] * 64 >= interior
+    finally:
+        for k, v in zip(keys, old):
+            rm.set_tuning(k, v)
