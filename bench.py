@@ -72,8 +72,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--render-tuning", default="",
                     help="fix the 1080p render launch instead of autotuning it: "
-                         "'blocks_per_cu,min_waves,paths_per_round,walk_variant,lds_stack_entries,tris_per_step'")
-    ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort[,lds_stack_entries[,tris_per_step]]'")
+                         "'blocks_per_cu,min_waves,paths_per_round,walk_variant,lds_stack_entries,tris_per_step[,walk_phases]'")
+    ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort[,lds_stack_entries[,tris_per_step[,walk_phases]]]'")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
@@ -126,12 +126,14 @@ def main():
                                            for k, v in algo.items()})
     tuned = None
     if args.trace:
-        v, b, f, s, e, p = (int(x) for x in (args.trace + ",32,0").split(",")[:6])
+        fields = [int(x) for x in args.trace.split(",")]
+        v, b, f, s, e, p, ph = (fields + [32, 0, 0][len(fields) - 4:])[:7]
         for k, val in ((rm.TUNE_TRACE_VARIANT, v), (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, b), (rm.TUNE_FETCH_MIN, f),
-                       (rm.TUNE_RAY_SORT, s), (rm.TUNE_LDS_STACK_ENTRIES, e), (rm.TUNE_TRIS_PER_STEP, p)):
+                       (rm.TUNE_RAY_SORT, s), (rm.TUNE_LDS_STACK_ENTRIES, e), (rm.TUNE_TRIS_PER_STEP, p),
+                       (rm.TUNE_WALK_PHASES, ph)):
             rm.set_tuning(k, val)
         tuned = {"variant": v, "blocks_per_cu": b, "fetch_min": f, "ray_sort": s, "lds_stack_entries": e,
-                 "tris_per_step": p, "fixed": True}
+                 "tris_per_step": p, "walk_phases": ph, "fixed": True}
     elif not args.no_autotune:
         from raym0nade_amd import autotune
         tuned = autotune.tune_trace(S, [r_coh, r_inc], reps=2, log=log)
@@ -652,13 +654,14 @@ def render_leg(args, torch, rm, dev, stream):
     fr = rm.Frame(a.width, a.height)
     tuned = None
     if args.render_tuning:
-        b, w, k, v, e, p = (int(x) for x in args.render_tuning.split(","))
+        fields = [int(x) for x in args.render_tuning.split(",")]
+        b, w, k, v, e, p, ph = (fields + [0])[:7]
         for key, val in ((rm.TUNE_RENDER_BLOCKS_PER_CU, b), (rm.TUNE_RENDER_MIN_WAVES, w), (rm.TUNE_TRACE_VARIANT, v),
-                         (rm.TUNE_LDS_STACK_ENTRIES, e), (rm.TUNE_TRIS_PER_STEP, p)):
+                         (rm.TUNE_LDS_STACK_ENTRIES, e), (rm.TUNE_TRIS_PER_STEP, p), (rm.TUNE_WALK_PHASES, ph)):
             rm.set_tuning(key, val)
         a.paths_per_round = k
         tuned = {"blocks_per_cu": b, "min_waves": w, "paths_per_round": k, "walk_variant": v,
-                 "lds_stack_entries": e, "tris_per_step": p, "fixed": True}
+                 "lds_stack_entries": e, "tris_per_step": p, "walk_phases": ph, "fixed": True}
     elif not args.no_autotune:
         from raym0nade_amd import autotune
         tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 4, blocks=(4, 8), waves=(1, 4),

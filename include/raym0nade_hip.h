@@ -486,7 +486,13 @@ enum rm_tune_key {
                                            render is slower); default 0 */,
     RM_TUNE_TRACE_UTIL = 12             /* 1: trace calls of variant 3 count the lanes that walk an
                                            interior node or test triangles in each step of each wave
-                                           (rm_last_trace_util; results unchanged); default 0 */
+                                           (rm_last_trace_util; results unchanged); default 0 */,
+    RM_TUNE_WALK_PHASES = 13            /* p in 1..64: in each iteration of a persistent walk (trace
+                                           variant 3) only one kind of step is taken -- triangle tests
+                                           when at least p/64 of the walking lanes stand at a leaf,
+                                           else interior nodes -- so the wave runs one kind at a time;
+                                           each ray's steps and results are unchanged; 0: every
+                                           walking lane steps (default 0) */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

@@ -76,7 +76,8 @@ def philox(ctr, key) -> tuple:
 
 (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT,
  TUNE_RENDER_BLOCKS_PER_CU, TUNE_ADJOINT_SPLIT, TUNE_KERNEL_TIMING, TUNE_RENDER_MIN_WAVES,
- TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP, TUNE_NEE_SPLIT, TUNE_RENDER_COUNT, TUNE_TRACE_UTIL) = range(13)
+ TUNE_LDS_STACK_ENTRIES, TUNE_TRIS_PER_STEP, TUNE_NEE_SPLIT, TUNE_RENDER_COUNT, TUNE_TRACE_UTIL,
+ TUNE_WALK_PHASES) = range(14)
 
 
 def last_trace_util() -> dict:

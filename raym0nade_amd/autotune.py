@@ -6,8 +6,9 @@ on the caller's own workload and keeps the fastest:
 
   tune_trace   traversal variant (stack walk / step walk / persistent step
                walk), resident blocks per CU, the idle-lane threshold at which
-               a persistent wave fetches new rays, and ray reordering, on the
-               caller's ray batches;
+               a persistent wave fetches new rays, ray reordering, LDS stack
+               entries, triangles per step and walk phases, on the caller's
+               ray batches;
   tune_render  resident blocks per CU of the persistent bounce kernel (the
                ray-queue consumers) and paths per pixel per round (the depth of
                the path queues), on a copy of the caller's batch with the spp
@@ -21,10 +22,10 @@ import time
 from ._lib import RM_ERR_OOM, RmError
 from . import (TUNE_FETCH_MIN, TUNE_LDS_STACK_ENTRIES, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_RAY_SORT,
                TUNE_NEE_SPLIT, TUNE_RENDER_BLOCKS_PER_CU, TUNE_RENDER_MIN_WAVES, TUNE_TRACE_VARIANT, TUNE_TRIS_PER_STEP,
-               RmRenderArgs, get_tuning, render_batch, set_tuning)
+               TUNE_WALK_PHASES, RmRenderArgs, get_tuning, render_batch, set_tuning)
 
 TRACE_KEYS = (TUNE_TRACE_VARIANT, TUNE_PERSISTENT_BLOCKS_PER_CU, TUNE_FETCH_MIN, TUNE_RAY_SORT, TUNE_LDS_STACK_ENTRIES,
-              TUNE_TRIS_PER_STEP)
+              TUNE_TRIS_PER_STEP, TUNE_WALK_PHASES)
 
 
 def _time(fn, reps):
@@ -43,12 +44,14 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     """batches: list of device ray tensors [n, 6].  Sets and returns the
     fastest {key: value} for closest-hit traversal of all batches."""
     if candidates is None:
-        candidates = [(0, 8, 16, 1, 32, 0)]
-        candidates += [(1, 8, 16, s, 32, 0) for s in (0, 1)]
-        candidates += [(2, b, f, s, 32, 0) for b, f, s in itertools.product((4, 8, 16), (8, 16, 32), (0, 1))]
-        candidates += [(3, b, f, 1, e, 0) for b, f, e in itertools.product((4, 8, 16), (16, 32), (4, 6, 8, 12, 32))]
-        candidates += [(3, b, f, s, e, p) for b, f, e, p, s in itertools.product((8, 16), (16, 32), (6, 8, 10), (1, 2, 4), (1, 3))]
-        candidates += [(4, b, 32, 1, 6, 0) for b in (6, 8)]
+        candidates = [(0, 8, 16, 1, 32, 0, 0)]
+        candidates += [(1, 8, 16, s, 32, 0, 0) for s in (0, 1)]
+        candidates += [(2, b, f, s, 32, 0, 0) for b, f, s in itertools.product((4, 8), (16, 32), (0, 1))]
+        candidates += [(3, b, f, 1, e, 0, 0) for b, f, e in itertools.product((4, 8), (16, 32), (6, 8, 12, 32))]
+        candidates += [(3, b, f, s, e, p, 0) for b, f, e, p, s in itertools.product((8,), (16, 32), (8, 10), (1, 2, 4), (1, 3))]
+        candidates += [(3, b, f, s, e, p, ph) for b, f, e, p, s, ph in
+                       itertools.product((8,), (32, 48), (8, 10), (0, 2, 4), (1, 3), (24, 32, 40))]
+        candidates += [(4, b, 32, 1, 6, 0, 0) for b in (6, 8)]
 
     def run():
         for r in batches:
@@ -67,19 +70,22 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     for k, v in zip(TRACE_KEYS, best):
         set_tuning(k, v)
     return {"variant": best[0], "blocks_per_cu": best[1], "fetch_min": best[2], "ray_sort": best[3],
-            "lds_stack_entries": best[4], "tris_per_step": best[5], "seconds": t, "tried": len(results)}
+            "lds_stack_entries": best[4], "tris_per_step": best[5], "walk_phases": best[6], "seconds": t,
+            "tried": len(results)}
 
 
 def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4, 8),
-                walks=((3, 10, 2), (3, 10, 0), (3, 10, 1), (3, 10, 4), (3, 8, 2), (3, 6, 2), (2, 32, 2)),
+                walks=((3, 10, 2, 0), (3, 10, 0, 0), (3, 10, 1, 0), (3, 10, 4, 0), (3, 8, 2, 0), (3, 6, 2, 0),
+                       (2, 32, 2, 0), (3, 10, 0, 32), (3, 10, 0, 40), (3, 10, 1, 32), (3, 10, 2, 32)),
                 log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp
     scaled by spp_scale: first every (blocks per CU, register cap) pair with
     automatic paths per round and the walk settings of walks[0], then the
     other paths-per-round values at the best pair, then the other walk
-    settings (trace variant, LDS entries per lane, triangles per step: variant
-    2 keeps the walk stacks in scratch, 3 their first entries in LDS).  Sets
+    settings (trace variant, LDS entries per lane, triangles per step, walk
+    phases: variant 2 keeps the walk stacks in scratch, 3 their first entries
+    in LDS).  Sets
     the launch parameters and returns the choice; the caller applies
     paths_per_round (0 = automatic) to its jobs' arguments."""
     small = []
@@ -96,6 +102,7 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
         set_tuning(TUNE_TRACE_VARIANT, walk[0])
         set_tuning(TUNE_LDS_STACK_ENTRIES, walk[1])
         set_tuning(TUNE_TRIS_PER_STEP, walk[2])
+        set_tuning(TUNE_WALK_PHASES, walk[3] if len(walk) > 3 else 0)
         try:
             t = _time(lambda: render_batch([tuple(j) for j in small], use_graph=True), 1)
         except RmError as e:
@@ -117,9 +124,10 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
     set_tuning(TUNE_TRACE_VARIANT, walk[0])
     set_tuning(TUNE_LDS_STACK_ENTRIES, walk[1])
     set_tuning(TUNE_TRIS_PER_STEP, walk[2])
+    set_tuning(TUNE_WALK_PHASES, walk[3] if len(walk) > 3 else 0)
     return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "walk_variant": walk[0],
-            "lds_stack_entries": walk[1], "tris_per_step": walk[2], "seconds_scaled": t, "spp_scale": spp_scale,
-            "tried": len(results)}
+            "lds_stack_entries": walk[1], "tris_per_step": walk[2], "walk_phases": walk[3] if len(walk) > 3 else 0,
+            "seconds_scaled": t, "spp_scale": spp_scale, "tried": len(results)}
 
 
 def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> dict:
@@ -143,7 +151,7 @@ def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> di
 _KEYS = {"trace_variant": TUNE_TRACE_VARIANT, "persistent_blocks_per_cu": TUNE_PERSISTENT_BLOCKS_PER_CU,
          "fetch_min": TUNE_FETCH_MIN, "ray_sort": TUNE_RAY_SORT, "render_blocks_per_cu": TUNE_RENDER_BLOCKS_PER_CU,
          "render_min_waves": TUNE_RENDER_MIN_WAVES, "lds_stack_entries": TUNE_LDS_STACK_ENTRIES,
-         "tris_per_step": TUNE_TRIS_PER_STEP, "nee_split": TUNE_NEE_SPLIT}
+         "tris_per_step": TUNE_TRIS_PER_STEP, "nee_split": TUNE_NEE_SPLIT, "walk_phases": TUNE_WALK_PHASES}
 
 
 def current() -> dict:

@@ -100,6 +100,10 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
             if (value < 0 || value > 1) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: trace utilisation %d", value);
             t.trace_util = value;
             return RM_OK;
+        case RM_TUNE_WALK_PHASES:
+            if (value < 0 || value > 64) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: walk phases %d", value);
+            t.walk_phases = value;
+            return RM_OK;
         case RM_TUNE_LDS_STACK_ENTRIES:
             if (value < 1 || value > rm::MAX_STACK)
                 return rm::fail(RM_ERR_INVALID, "rm_set_tuning: LDS stack entries %d (1..%d)", value, rm::MAX_STACK);
@@ -127,6 +131,7 @@ extern "C" int32_t rm_get_tuning(int32_t key, int32_t *value) {
         case RM_TUNE_NEE_SPLIT: *value = t.nee_split; return RM_OK;
         case RM_TUNE_RENDER_COUNT: *value = t.render_count; return RM_OK;
         case RM_TUNE_TRACE_UTIL: *value = t.trace_util; return RM_OK;
+        case RM_TUNE_WALK_PHASES: *value = t.walk_phases; return RM_OK;
         default: return rm::fail(RM_ERR_INVALID, "rm_get_tuning: unknown key %d", key);
     }
 }

@@ -295,7 +295,9 @@ __device__ __forceinline__ void job_stat_max(const Job *J, int myjob, uint32_t v
 // counts to the old job's RM_STAT_* keys[k] (note); the kernel ends with
 // flush_wave, which every lane of the wave calls together and which adds
 // each job's counts over the wave with one atomic per counter.  Counting per
-// item instead costs a contended atomic per 64 items.
+// 64 items instead costs a contended atomic each time.  For launches of one
+// job; in a batch, whose queues interleave the jobs' items, the kernels add
+// per wave and item batch (job_stat_add), one atomic per job present.
 template <int N, uint32_t MAXMASK>
 struct LaneStats {
     uint32_t *s;  // this lane's column
@@ -898,7 +900,7 @@ __device__ __forceinline__ void flush_walk_counts(const Job *__restrict__ J, int
 template <bool LDS, int TPS, bool COUNT = false>
 __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ Q, const uint32_t *count_in,
                                                uint32_t *work, uint2 *__restrict__ hit, int fetch_min,
-                                               int lds_entries) {
+                                               int lds_entries, int phases) {
     const uint32_t n = *count_in;
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
@@ -942,7 +944,8 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
                 }
             }
         }
-        if (q >= 0 && w.step(S, r, T_EPS, -INFINITY, c)) {
+        const bool turn = walk_turn(w.cur, phases, q >= 0);
+        if (q >= 0 && turn && w.step(S, r, T_EPS, -INFINITY, c)) {
             hit[q] = make_uint2(bu(w.tmax), (uint32_t)w.best);
             q = -1;
         }
@@ -1100,7 +1103,13 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
     if (cont) store_path(Qout, slot_out, path);
     uint32_t slot_nee = append(nee_count, nee_out);
     if (nee_out) store_nee(NQ, slot_nee, path, nee_rec);
-    if (live) {
+    if (nj > 1) {
+        const int myjob = live ? path.job : -1;
+        job_stat_add<1>(J, myjob, live ? 1u : 0u, RM_STAT_EXTENSION_RAYS);
+        job_stat_add<1>(J, myjob, hit_any ? 1u : 0u, RM_STAT_HITS);
+        job_stat_add<2>(J, myjob, ovf, RM_STAT_MEDIUM_OVERFLOW);
+        job_stat_max<5>(J, cont ? path.job : -1, cont ? (uint32_t)path.med.n : 0u, RM_STAT_MEDIUM_MAX);
+    } else if (live) {
         const uint32_t x[4] = {1u, hit_any ? 1u : 0u, ovf, cont ? (uint32_t)path.med.n : 0u};
         st.note(J, keys, path.job, x);
     }
@@ -1113,7 +1122,7 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
 // src/sampling.cpp:467-527): up to sampleCount[depth] light samples, each
 // with its shadow ray, scaled by the branch factor, into the path's set.
 template <int MINW, int TPS>
-__global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, PixBufs P, AccBufs AB, NeeQ NQ,
+__global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, int nj, PixBufs P, AccBufs AB, NeeQ NQ,
                                             const uint32_t *count, uint32_t *work) {
   const uint32_t n = *count;
   const uint32_t lane = threadIdx.x & 63u;
@@ -1170,7 +1179,9 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
             }
         }
     });
-    if (live) {
+    if (nj > 1) {
+        job_stat_add<3>(J, live ? (int)(flags >> 8) : -1, nrays, RM_STAT_SHADOW_RAYS);
+    } else if (live) {
         const uint32_t x[1] = {nrays};
         st.note(J, keys, (int)(flags >> 8), x);
     }
@@ -1179,7 +1190,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee(const Job *__restrict__ J, Pix
 }
 
 template <int MINW, int TPS>
-__global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J, NeeQ NQ, const uint32_t *count,
+__global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J, int nj, NeeQ NQ, const uint32_t *count,
                                                 uint32_t *work, ShadowQ SQ, int queue_depth, uint32_t *queued) {
   const uint32_t n = *count;
   const uint32_t lane = threadIdx.x & 63u;
@@ -1236,7 +1247,9 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
             }
         }
     });
-    if (live) {
+    if (nj > 1) {
+        job_stat_add<3>(J, live ? (int)(flags >> 8) : -1, nrays, RM_STAT_SHADOW_RAYS);
+    } else if (live) {
         const uint32_t x[1] = {nrays};
         st.note(J, keys, (int)(flags >> 8), x);
     }
@@ -1248,7 +1261,7 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
 // ANY: it stops once the running closest t is below aim).
 template <bool LDS, int TPS, bool COUNT = false>
 __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, ShadowQ SQ, const uint32_t *count,
-                                               uint32_t *work, int fetch_min, int lds_entries) {
+                                               uint32_t *work, int fetch_min, int lds_entries, int phases) {
     const uint32_t n = *count;
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
@@ -1294,7 +1307,8 @@ __global__ __launch_bounds__(RB) void k_shadow(const Job *__restrict__ J, Shadow
                 }
             }
         }
-        if (q >= 0 && w.step(S, r, T_EPS, aim, c)) {
+        const bool turn = walk_turn(w.cur, phases, q >= 0);
+        if (q >= 0 && turn && w.step(S, r, T_EPS, aim, c)) {
             SQ.vis[q] = (w.tmax < aim) ? 0 : 1;
             q = -1;
         }
@@ -1698,6 +1712,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     const bool capped = tuning().render_min_waves == 4;
     const unsigned gpers = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
     const int fetch_min = tuning().fetch_min;
+    const int phases = tuning().walk_phases;
     auto shadow = counting ? (lds_walk ? (tps == 1 ? k_shadow<true, 1, true> : tps == 2 ? k_shadow<true, 2, true>
                                           : tps == 4 ? k_shadow<true, 4, true> : k_shadow<true, 0, true>)
                                        : (tps == 1 ? k_shadow<false, 1, true> : tps == 2 ? k_shadow<false, 2, true>
@@ -1723,7 +1738,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
             hipLaunchKernelGGL(direct_gen, dim3((unsigned)((n + RB - 1) / RB)), dim3(RB), walk_lds, st,
                                (const Job *)dJ, nj, npx, P, C, t0, SQ, queue_depth, claims + 1);
             hipLaunchKernelGGL(shadow, dim3(gpers), dim3(RB), walk_lds, st, (const Job *)dJ, SQ,
-                               (const uint32_t *)(claims + 1), claims, fetch_min, nl);
+                               (const uint32_t *)(claims + 1), claims, fetch_min, nl, phases);
             hipLaunchKernelGGL(k_direct_add, dim3((unsigned)((npx + RB - 1) / RB)), dim3(RB), 0, st, (const Job *)dJ, nj,
                                npx, P, C, t0, SQ);
             t0 += C;
@@ -1759,24 +1774,25 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
                 const uint32_t *n_in = counts + d - 1;
                 if (lds_walk)
                     hipLaunchKernelGGL(extend_lds, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ,
-                                       Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min, nl);
+                                       Q[(d - 1) & 1], n_in, counts + 2 * NCOUNT + d, hits, fetch_min, nl, phases);
                 else
                     hipLaunchKernelGGL(extend_scratch, dim3(ge), dim3(RB), 0, st, (const Job *)dJ, Q[(d - 1) & 1],
-                                       n_in, counts + 2 * NCOUNT + d, hits, fetch_min, 0);
+                                       n_in, counts + 2 * NCOUNT + d, hits, fetch_min, 0, phases);
                 hipLaunchKernelGGL(bounce, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, nj, P, AB, Q[(d - 1) & 1], n_in,
                                    counts + NCOUNT + d, (const uint2 *)hits, Q[d & 1], counts + d, NQ,
                                    counts + 3 * NCOUNT);
             }
             const uint32_t *n_nee = counts + 3 * NCOUNT;
             if (split) {
-                hipLaunchKernelGGL(nee_gen, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, NQ, n_nee,
+                hipLaunchKernelGGL(nee_gen, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, nj, NQ, n_nee,
                                    counts + 3 * NCOUNT + 1, SQ, queue_depth, counts + 3 * NCOUNT + 4);
                 hipLaunchKernelGGL(shadow, dim3(ge), dim3(RB), walk_lds, st, (const Job *)dJ, SQ,
-                                   (const uint32_t *)(counts + 3 * NCOUNT + 4), counts + 3 * NCOUNT + 2, fetch_min, nl);
+                                   (const uint32_t *)(counts + 3 * NCOUNT + 4), counts + 3 * NCOUNT + 2, fetch_min, nl,
+                                   phases);
                 hipLaunchKernelGGL(nee_add, dim3(gb), dim3(RB), 0, st, (const Job *)dJ, P, AB, NQ, n_nee,
                                    counts + 3 * NCOUNT + 3, SQ);
             } else {
-                hipLaunchKernelGGL(nee, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, P, AB, NQ, n_nee,
+                hipLaunchKernelGGL(nee, dim3(gb), dim3(RB), walk_lds, st, (const Job *)dJ, nj, P, AB, NQ, n_nee,
                                    counts + 3 * NCOUNT + 1);
             }
         }

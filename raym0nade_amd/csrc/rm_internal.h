@@ -137,6 +137,7 @@ struct Tuning {
     int nee_split = 2;            // render: shadow rays of light samples in their own passes (2: deep trees)
     int render_count = 0;         // render: count the extension and shadow walks (RM_STAT_WALK_*)
     int trace_util = 0;           // trace variant 3: count lane utilisation (rm_last_trace_util)
+    int walk_phases = 0;          // persistent walks: step only the majority kind of lanes per iteration
 };
 Tuning &tuning();
 int device_cu_count();

@@ -158,7 +158,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                                                                   unsigned long long *__restrict__ next, int fetch_min,
                                                                   const uint32_t *__restrict__ order,
                                                                   const int *__restrict__ use_order, int lds_entries,
-                                                                  unsigned long long *__restrict__ util = nullptr) {
+                                                                  unsigned long long *__restrict__ util = nullptr,
+                                                                  int phases = 0) {
     const uint32_t lane = threadIdx.x & 63u;
     ScratchEntries ent;
     Walker<ANY, false, typename std::conditional<LDS, LdsStack, PrivateStack>::type, TPS> w;
@@ -197,13 +198,15 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
                 }
             }
         }
+        const bool turn = walk_turn(w.cur, phases, ray >= 0);
+        const bool go = ray >= 0 && turn;
+        const bool leaf = ray >= 0 && (w.cur & 0x80000000u);
         if constexpr (UTIL) {
-            const bool leaf = ray >= 0 && (w.cur & 0x80000000u);
-            const uint64_t mi = __ballot(ray >= 0 && !leaf), ml = __ballot(leaf);
+            const uint64_t mi = __ballot(go && !leaf), ml = __ballot(go && leaf);
             if (mi) { u[0] += 1; u[1] += (unsigned long long)__popcll(mi); }
             if (ml) { u[2] += 1; u[3] += (unsigned long long)__popcll(ml); }
         }
-        if (ray >= 0 && w.step(S, r, tmin, aim, c)) {
+        if (go && w.step(S, r, tmin, aim, c)) {
             if (ANY) {
                 occ[ray] = (w.tmax < aim) ? 1 : 0;
             } else {
@@ -656,7 +659,7 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
                                                        : k_trace_persistent<ANY, true, 0>);
             hipLaunchKernelGGL(kern, dim3(pgrid), dim3(TRACE_BLOCK), (size_t)nl * TRACE_BLOCK * 8, st, s->view, rays, n,
                                tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
-                               (const int *)flag, nl, ud);
+                               (const int *)flag, nl, ud, T.walk_phases);
         } else
             hipLaunchKernelGGL((k_trace_persistent<ANY, false, 0>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays,
                                n, tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,

@@ -376,6 +376,19 @@ struct Walker {
     }
 };
 
+// Whether a walking lane, standing at `cur` (a walker's current node: bit 31
+// for a leaf), takes its step in this iteration of a persistent walk.  With
+// phases p in 1..64 a wave takes one kind of step at a time: triangle tests
+// when at least p/64 of its walking lanes stand at a leaf, else interior
+// nodes; each ray's own sequence of steps is unchanged.  Called by every lane
+// of the wave together; `walking` is false for lanes without a ray.
+__device__ __forceinline__ bool walk_turn(uint32_t cur, int phases, bool walking) {
+    if (!phases) return true;
+    const bool leaf = walking && (cur & 0x80000000u);
+    const uint32_t nlf = (uint32_t)__popcll(__ballot(leaf)), nact = (uint32_t)__popcll(__ballot(walking));
+    return leaf == (64u * nlf >= (uint32_t)phases * nact);
+}
+
 // The whole walk for one ray with the step-wise walker: the same result as
 // trace<ANY, false>, with far fewer registers, for kernels that do more than
 // traversal.  The pending entries live in scratch, or their first S.walk_lds
