@@ -201,8 +201,8 @@ def main():
     bytes_per_launch = (algo[("coh", False)] + algo[("inc", False)]) / 2
     dur_per_launch = (np.mean(kt["coh"]) + np.mean(kt["inc"])) / 2 * 1e-3
     achieved = bytes_per_launch / dur_per_launch / 1e9
-    kernel_name = {0: "k_trace_closest", 1: "k_trace_step", 2: "k_trace_persistent",
-                   3: "k_trace_persistent_lds", 4: "k_trace_persistent2"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
+    kernel_name = {0: "k_trace_closest", 2: "k_trace_persistent",
+                   3: "k_trace_persistent_lds"}[rm.get_tuning(rm.TUNE_TRACE_VARIANT)]
     # HBM bytes per launch from the committed PMC passes of this kernel (the
     # launch parameters they were taken at are reported beside them)
     traffic, traffic_launch = None, None

@@ -451,29 +451,31 @@ int32_t rm_generate_rays(int32_t kind, uint32_t seed, int64_t n, const rm_render
  * its thread count).  All settings give identical results; they trade speed.
  * Process-wide; set them before launching work. */
 enum rm_tune_key {
-    RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk, 1 step walk, 2 persistent step walk,
+    RM_TUNE_TRACE_VARIANT = 0,          /* 0 stack walk (one ray per lane), 2 persistent step walk,
                                            3 persistent step walk with the first stack entries in LDS
-                                           (default; in a render launch: every walk of its kernels),
-                                           4 as 3 with two walks per lane (traces; renders as 3) */
+                                           (default; in a render launch: every walk of its kernels);
+                                           1 and 4 (a one-ray step walk and two walks per lane) are
+                                           retired and rejected */
     RM_TUNE_PERSISTENT_BLOCKS_PER_CU = 1, /* resident 256-lane blocks per CU for variants 2, 3 (default 8) */
-    RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 32) */
+    RM_TUNE_FETCH_MIN = 2,              /* idle lanes before a wave claims new rays (default 48) */
     RM_TUNE_RAY_SORT = 3,               /* ray reordering for trace calls: 0 never, 1 when consecutive
                                            origins are far apart, 2 always, 3 as 1 with a radix sort
-                                           of the bins (rocPRIM) instead of the counting sort
-                                           (default) */
+                                           of the bins (rocPRIM) instead of the counting sort, 4 as 3
+                                           with the library's own radix sort, whose kernels do no work
+                                           on a batch left unsorted (default) */
     RM_TUNE_RENDER_BLOCKS_PER_CU = 4,   /* blocks per CU of the persistent bounce kernel (default 4) */
     RM_TUNE_ADJOINT_SPLIT = 5,          /* lanes per pixel of the adjoint, each replaying a share of
                                            the pixel's samples (default 4) */
     RM_TUNE_KERNEL_TIMING = 6,          /* 1: trace calls record HIP events around their reordering
                                            and traversal kernels (default 0) */
-    RM_TUNE_RENDER_MIN_WAVES = 7,       /* 1 or 4 (default): resident waves per SIMD the render's
+    RM_TUNE_RENDER_MIN_WAVES = 7,       /* 1 (default) or 4: resident waves per SIMD the render's
                                            shading and NEE kernels are compiled for (4 caps them
                                            at 128 VGPRs, spilling the rest) */
     RM_TUNE_LDS_STACK_ENTRIES = 8,      /* trace variant 3: walk-stack entries per lane kept in LDS
                                            (1..32, default 10: 160 KiB of LDS at 8 blocks/CU; deeper
                                            entries go to scratch; 32 holds the whole stack of any tree) */
     RM_TUNE_TRIS_PER_STEP = 9           /* trace variant 3 and render walks: triangles one walk step
-                                           tests (0 = a whole leaf; 1, 2 (default) or 4: a leaf over
+                                           tests (0 = a whole leaf (default); 1, 2 or 4: a leaf over
                                            several steps; the render's shading kernels use 0 or 2) */,
     RM_TUNE_NEE_SPLIT = 10,             /* render: 1 traces the shadow rays of light samples (the
                                            primary vertex's and the NEE records') in their own
@@ -492,7 +494,7 @@ enum rm_tune_key {
                                            when at least p/64 of the walking lanes stand at a leaf,
                                            else interior nodes -- so the wave runs one kind at a time;
                                            each ray's steps and results are unchanged; 0: every
-                                           walking lane steps (default 0) */
+                                           walking lane steps (default 32) */
 };
 int32_t rm_set_tuning(int32_t key, int32_t value);
 int32_t rm_get_tuning(int32_t key, int32_t *value);

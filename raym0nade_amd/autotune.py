@@ -4,8 +4,8 @@ Every setting gives the same results (traversal bit for bit; renders up to
 the summation order of accumulator sets), so the tuner only times candidates
 on the caller's own workload and keeps the fastest:
 
-  tune_trace   traversal variant (stack walk / step walk / persistent step
-               walk), resident blocks per CU, the idle-lane threshold at which
+  tune_trace   traversal variant (stack walk / persistent step walk with
+               the stack in scratch or partly in LDS), resident blocks per CU, the idle-lane threshold at which
                a persistent wave fetches new rays, ray reordering, LDS stack
                entries, triangles per step and walk phases, on the caller's
                ray batches;
@@ -44,14 +44,11 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     """batches: list of device ray tensors [n, 6].  Sets and returns the
     fastest {key: value} for closest-hit traversal of all batches."""
     if candidates is None:
-        candidates = [(0, 8, 16, 1, 32, 0, 0)]
-        candidates += [(1, 8, 16, s, 32, 0, 0) for s in (0, 1)]
-        candidates += [(2, b, f, s, 32, 0, 0) for b, f, s in itertools.product((4, 8), (16, 32), (0, 1))]
-        candidates += [(3, b, f, 1, e, 0, 0) for b, f, e in itertools.product((4, 8), (16, 32), (6, 8, 12, 32))]
-        candidates += [(3, b, f, s, e, p, 0) for b, f, e, p, s in itertools.product((8,), (16, 32), (8, 10), (1, 2, 4), (1, 3))]
+        candidates = [(0, 8, 16, s, 32, 0, 0) for s in (0, 4)]
+        candidates += [(2, b, f, s, 32, 0, 0) for b, f, s in itertools.product((4, 8), (32, 48), (0, 4))]
+        candidates += [(3, b, f, 4, e, 0, 0) for b, f, e in itertools.product((4, 8), (32, 48), (6, 8, 12, 32))]
         candidates += [(3, b, f, s, e, p, ph) for b, f, e, p, s, ph in
-                       itertools.product((8,), (32, 48), (8, 10), (0, 2, 4), (1, 3), (24, 32, 40))]
-        candidates += [(4, b, 32, 1, 6, 0, 0) for b in (6, 8)]
+                       itertools.product((8,), (32, 48), (8, 10), (0, 2, 4), (1, 3, 4), (24, 32, 40))]
 
     def run():
         for r in batches:

@@ -52,7 +52,8 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
     rm::Tuning &t = rm::tuning();
     switch (key) {
         case RM_TUNE_TRACE_VARIANT:
-            if (value < 0 || value > 4) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: trace variant %d", value);
+            if (value != 0 && value != 2 && value != 3)
+                return rm::fail(RM_ERR_INVALID, "rm_set_tuning: trace variant %d (0, 2 or 3)", value);
             t.trace_variant = value;
             return RM_OK;
         case RM_TUNE_PERSISTENT_BLOCKS_PER_CU:
@@ -64,7 +65,7 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
             t.fetch_min = value;
             return RM_OK;
         case RM_TUNE_RAY_SORT:
-            if (value < 0 || value > 3) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: ray sort %d", value);
+            if (value < 0 || value > 4) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: ray sort %d", value);
             t.ray_sort = value;
             return RM_OK;
         case RM_TUNE_RENDER_BLOCKS_PER_CU:

@@ -1600,7 +1600,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     // Trace variant 3 keeps the first pending entries of every walk in LDS:
     // each kernel of the launch that traverses gets nl entries per lane (at
     // most the deepest tree's depth), and the job table's views say so.
-    const bool lds_walk = tuning().trace_variant >= 3;
+    const bool lds_walk = tuning().trace_variant == 3;
     int max_depth = 1;
     for (int i = 0; i < nj; ++i) max_depth = std::max(max_depth, (int)J[i].S.depth);
     const int nl = lds_walk ? std::min(max_depth, tuning().lds_stack_entries) : 0;

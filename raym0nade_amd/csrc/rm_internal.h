@@ -123,21 +123,23 @@ int32_t build_host_scene(const rm_scene_desc *d, HostScene &hs);
 
 // Launch parameters chosen by the autotuner (rm_set_tuning).
 struct Tuning {
-    int trace_variant = 3;        // 0 stack walk, 1 step walk, 2 persistent step walk, 3 same with LDS stacks
+    int trace_variant = 3;        // 0 stack walk, 2 persistent step walk, 3 same with LDS stacks
     int persistent_blocks_per_cu = 8;
-    int fetch_min = 32;           // idle lanes before a wave claims new rays
-    int ray_sort = 3;             // 0 never, 1 when consecutive origins are far apart, 2 always,
-                                  // 3 as 1 with a radix sort (rocPRIM) instead of the counting sort
+    int fetch_min = 48;           // idle lanes before a wave claims new rays
+    int ray_sort = 4;             // 0 never, 1 when consecutive origins are far apart, 2 always,
+                                  // 3 as 1 with a radix sort (rocPRIM) instead of the counting sort,
+                                  // 4 as 3 with the library's own radix sort (skipped on the device
+                                  // for a batch left unsorted)
     int render_blocks_per_cu = 4; // resident blocks per CU of the persistent bounce kernel
     int adjoint_split = 4;        // lanes per pixel of the adjoint kernel
     int kernel_timing = 0;        // record HIP events around the trace kernels (rm_last_trace_ms)
-    int render_min_waves = 4;     // 1 or 4: register cap of the shading and NEE kernels
+    int render_min_waves = 1;     // 1 or 4: register cap of the shading and NEE kernels
     int lds_stack_entries = 10;   // trace variant 3: walk-stack entries per lane kept in LDS (160 KiB at 8 blocks/CU)
-    int tris_per_step = 2;        // trace variant 3: triangles tested per step (0: a whole leaf)
+    int tris_per_step = 0;        // trace variant 3: triangles tested per step (0: a whole leaf)
     int nee_split = 2;            // render: shadow rays of light samples in their own passes (2: deep trees)
     int render_count = 0;         // render: count the extension and shadow walks (RM_STAT_WALK_*)
     int trace_util = 0;           // trace variant 3: count lane utilisation (rm_last_trace_util)
-    int walk_phases = 0;          // persistent walks: step only the majority kind of lanes per iteration
+    int walk_phases = 32;         // persistent walks: one kind of step per iteration (RM_TUNE_WALK_PHASES)
 };
 Tuning &tuning();
 int device_cu_count();

@@ -109,35 +109,6 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_cutout(SceneView S, const
     }
 }
 
-// One ray per lane, step-wise walker (one node per loop iteration).
-template <bool ANY>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_step(SceneView S, const float *__restrict__ rays, int64_t n,
-                                                            float tmin, float tmax0, const float *__restrict__ aim_arr,
-                                                            float *__restrict__ t_out, int32_t *__restrict__ id_out,
-                                                            uint8_t *__restrict__ occ, const uint32_t *__restrict__ order,
-                                                            const int *__restrict__ use_order) {
-    int64_t j = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TRACE_BLOCK + threadIdx.x;
-    if (j >= n) return;
-    int64_t i = ray_of(j, order, use_order);
-    V3 o, d;
-    load_ray(rays, i, o, d);
-    RayPre r = ray_pre(o, d);
-    float aim = ANY ? (aim_arr ? aim_arr[i] : tmax0) : -INFINITY;
-    ScratchEntries ent;
-    Walker<ANY, false> w;
-    w.stk.bind(&ent);
-    w.init(S, r, ANY ? aim + T_EPS : tmax0);
-    TraceCounts c{};
-    while (!w.step(S, r, tmin, aim, c)) {
-    }
-    if (ANY) {
-        occ[i] = (w.tmax < aim) ? 1 : 0;
-    } else {
-        t_out[i] = w.tmax;
-        id_out[i] = out_id(S, w.best);
-    }
-}
-
 // Persistent lanes with dynamic ray fetch: a lane whose ray finished takes the
 // next unclaimed ray; a wave claims rays for its idle lanes with one atomic
 // once at least `fetch_min` lanes are idle (or all of its live lanes are).
@@ -223,95 +194,6 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent(SceneView S, c
     }
 }
 
-// Two walks per lane: the same persistent scheme with two ray slots per lane,
-// whose node loads are issued together before either is used, so each lane
-// keeps two loads in flight.  A wave claims rays for all its idle slots with
-// one atomic (first the idle A slots in lane order, then the idle B slots).
-// LDS: each slot's first lds_entries entries in its own rows of the block's
-// dynamic LDS (2 * lds_entries * TRACE_BLOCK * 8 bytes).
-template <bool ANY>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persistent2(SceneView S, const float *__restrict__ rays,
-                                                                   int64_t n, float tmin, float tmax0,
-                                                                   const float *__restrict__ aim_arr,
-                                                                   float *__restrict__ t_out,
-                                                                   int32_t *__restrict__ id_out,
-                                                                   uint8_t *__restrict__ occ,
-                                                                   unsigned long long *__restrict__ next,
-                                                                   int fetch_min, const uint32_t *__restrict__ order,
-                                                                   const int *__restrict__ use_order,
-                                                                   int lds_entries) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t below = (1ull << lane) - 1ull;
-    ScratchEntries ea, eb;
-    Walker<ANY, false, LdsStack> wa, wb;
-    wa.stk.bind(&ea, lds_entries, 0);
-    wb.stk.bind(&eb, lds_entries, lds_entries);
-    RayPre ra, rb;
-    float aa = -INFINITY, ab = -INFINITY;
-    int64_t ya = -1, yb = -1;
-    bool exhausted = false;
-    TraceCounts c{};
-    while (true) {
-        const bool ia = ya < 0, ib = yb < 0;
-        const uint64_t ma = __ballot(ia), mb = __ballot(ib);
-        const uint64_t busy = __ballot(!ia || !ib);
-        if (exhausted && busy == 0) break;
-        const uint32_t na = (uint32_t)__popcll(ma), nidle = na + (uint32_t)__popcll(mb);
-        if (!exhausted && nidle && (nidle >= (uint32_t)fetch_min || busy == 0)) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)(ma | mb)) - 1u;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(next, (unsigned long long)nidle);
-            base = __shfl(base, (int)leader, 64);
-            if (base + nidle >= (unsigned long long)n) exhausted = true;
-            if (ia) {
-                const int64_t slot = (int64_t)base + __popcll(ma & below);
-                if (slot < n) {
-                    const int64_t k = ray_of(slot, order, use_order);
-                    ya = k;
-                    V3 o, d;
-                    load_ray(rays, k, o, d);
-                    ra = ray_pre(o, d);
-                    aa = ANY ? (aim_arr ? aim_arr[k] : tmax0) : -INFINITY;
-                    wa.init(S, ra, ANY ? aa + T_EPS : tmax0);
-                }
-            }
-            if (ib) {
-                const int64_t slot = (int64_t)base + na + __popcll(mb & below);
-                if (slot < n) {
-                    const int64_t k = ray_of(slot, order, use_order);
-                    yb = k;
-                    V3 o, d;
-                    load_ray(rays, k, o, d);
-                    rb = ray_pre(o, d);
-                    ab = ANY ? (aim_arr ? aim_arr[k] : tmax0) : -INFINITY;
-                    wb.init(S, rb, ANY ? ab + T_EPS : tmax0);
-                }
-            }
-        }
-        typename Walker<ANY, false, LdsStack>::Node nda{}, ndb{};
-        if (ya >= 0) nda = wa.fetch(S);
-        if (yb >= 0) ndb = wb.fetch(S);
-        if (ya >= 0 && wa.step_with(S, ra, tmin, aa, c, nda)) {
-            if (ANY) {
-                occ[ya] = (wa.tmax < aa) ? 1 : 0;
-            } else {
-                t_out[ya] = wa.tmax;
-                id_out[ya] = out_id(S, wa.best);
-            }
-            ya = -1;
-        }
-        if (yb >= 0 && wb.step_with(S, rb, tmin, ab, c, ndb)) {
-            if (ANY) {
-                occ[yb] = (wb.tmax < ab) ? 1 : 0;
-            } else {
-                t_out[yb] = wb.tmax;
-                id_out[yb] = out_id(S, wb.best);
-            }
-            yb = -1;
-        }
-    }
-}
-
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 // ---------------------------------------------------------------- reordering
@@ -391,11 +273,13 @@ __global__ __launch_bounds__(256) void k_bin_hist(SceneView S, const float *__re
 
 // Keys for the radix-sort reordering (ray_sort 3): the bin when the batch is
 // incoherent, else 0 (a stable sort then keeps the batch's order).
+// skip: leave the pairs unwritten on a batch left unsorted (sort mode 4,
+// whose kernels then read nothing)
 __global__ __launch_bounds__(256) void k_bin_keys(SceneView S, const float *__restrict__ rays, int64_t n,
                                                   const int *__restrict__ flag, uint32_t *__restrict__ keys,
-                                                  uint32_t *__restrict__ vals) {
+                                                  uint32_t *__restrict__ vals, int skip = 0) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (skip && !*flag)) return;
     uint32_t b = 0;
     if (*flag) {
         V3 o, d;
@@ -437,6 +321,199 @@ __global__ __launch_bounds__(256) void k_bin_scatter(int64_t n, const int *__res
     if (i >= n) return;
     uint32_t pos = atomicAdd(hist + keys[i], 1u);
     order[pos] = (uint32_t)i;
+}
+
+// Sort mode 4: a stable least-significant-digit radix sort of the
+// (bin, index) pairs in two passes of 9 bits, whose kernels all return at
+// once on a batch that k_coherence left unsorted.  Tiles of RS_TILE pairs,
+// one per block: per pass a digit histogram of each tile (digit-major,
+// hist[d * nt + tile]), an exclusive scan of that array in chunks of
+// RS_CHUNK entries plus a scan of the chunk totals, and a scatter.  The
+// scatter ranks the tile stably (each wave a contiguous quarter, with
+// wave-private digit counters and ballots), sorts it by digit in LDS and
+// writes each digit's run contiguously; consecutive tiles go to blocks of
+// one XCD so that the runs of neighbouring tiles meet in one L2.
+constexpr int RS_BITS = 9;
+constexpr uint32_t RS_DIGITS = 1u << RS_BITS;
+constexpr int RS_TILE = 4096;      // 256 threads x 16 pairs
+constexpr int RS_CHUNK = 4096;     // scan chunk (256 threads x 16 entries)
+static_assert(2 * RS_BITS >= 3 * CELL_BITS + 3, "two radix passes cover the bin key");
+static_assert(RS_DIGITS == 2 * 256 && RS_TILE == 16 * 256, "k_rs_scatter: two digits and 16 pairs per lane");
+
+__global__ __launch_bounds__(256) void k_rs_hist(const int *__restrict__ flag, const uint32_t *__restrict__ keys,
+                                                 int64_t n, int shift, uint32_t nt, uint32_t *__restrict__ hist) {
+    if (!*flag) return;
+    __shared__ uint32_t cnt[RS_DIGITS];
+    for (uint32_t d = threadIdx.x; d < RS_DIGITS; d += 256) cnt[d] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int k = 0; k < RS_TILE / 256; ++k) {
+        const int64_t i = base + k * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (RS_DIGITS - 1)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < RS_DIGITS; d += 256) hist[(size_t)d * nt + blockIdx.x] = cnt[d];
+}
+
+// Exclusive scan of a[0, m) within chunks of RS_CHUNK entries (in place);
+// part[c] = the total of chunk c.
+__global__ __launch_bounds__(256) void k_rs_scan_chunks(const int *__restrict__ flag, uint32_t *__restrict__ a,
+                                                        size_t m, uint32_t *__restrict__ part) {
+    if (!*flag) return;
+    __shared__ uint32_t sh[256];
+    const size_t c0 = (size_t)blockIdx.x * RS_CHUNK, b = c0 + (size_t)threadIdx.x * 16;
+    uint32_t v[16], sum = 0;
+    for (int k = 0; k < 16; ++k) {
+        v[k] = b + k < m ? a[b + k] : 0u;
+        sum += v[k];
+    }
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t add = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = sh[threadIdx.x] - sum;
+    for (int k = 0; k < 16; ++k)
+        if (b + k < m) {
+            a[b + k] = run;
+            run += v[k];
+        }
+    if (threadIdx.x == 255) part[blockIdx.x] = sh[255];
+}
+
+// Exclusive scan of part[0, nc) in place, one block.
+__global__ __launch_bounds__(256) void k_rs_scan_parts(const int *__restrict__ flag, uint32_t *__restrict__ part,
+                                                       uint32_t nc) {
+    if (!*flag) return;
+    __shared__ uint32_t sh[256];
+    const uint32_t per = (nc + 255) / 256, b = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b + k < nc) sum += part[b + k];
+    sh[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t add = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = sh[threadIdx.x] - sum;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b + k < nc) {
+            const uint32_t c = part[b + k];
+            part[b + k] = run;
+            run += c;
+        }
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tile handled by block b of nt: blocks are dealt to the 8 XCDs round-robin,
+// so block b runs on XCD b % 8; the first 8 * (nt / 8) blocks take tiles
+// such that XCD x holds the consecutive tiles [x * q, (x + 1) * q).
+__device__ __forceinline__ uint32_t rs_tile(uint32_t b, uint32_t nt) {
+    const uint32_t q = nt >> 3;
+    return b < 8u * q ? (b & 7u) * q + (b >> 3) : b;
+}
+
+__global__ __launch_bounds__(256) void k_rs_scatter(const int *__restrict__ flag, const uint32_t *__restrict__ kin,
+                                                    const uint32_t *__restrict__ vin, int64_t n, int shift,
+                                                    uint32_t nt, const uint32_t *__restrict__ hist,
+                                                    const uint32_t *__restrict__ part, uint32_t *__restrict__ kout,
+                                                    uint32_t *__restrict__ vout) {
+    if (!*flag) return;
+    constexpr int PER = RS_TILE / 256;  // pairs per lane
+    __shared__ uint32_t cnt[4][RS_DIGITS];  // per wave: running count, then its first local slot
+    __shared__ int64_t gbase[RS_DIGITS];    // global slot of local slot 0 for each digit
+    __shared__ uint32_t sh[256];
+    __shared__ uint32_t lk[RS_TILE], lv[RS_TILE];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t tile = rs_tile(blockIdx.x, nt);
+    const int64_t base = (int64_t)tile * RS_TILE;
+    const int len = (int)(n - base < RS_TILE ? n - base : RS_TILE);
+    for (uint32_t d = threadIdx.x; d < RS_DIGITS; d += 256) cnt[0][d] = cnt[1][d] = cnt[2][d] = cnt[3][d] = 0u;
+    __syncthreads();
+    // rank within the wave's quarter: pair w * 1024 + k * 64 + lane
+    uint32_t key[PER], val[PER], rank[PER];
+    for (int k = 0; k < PER; ++k) {
+        const int j = (int)w * (RS_TILE / 4) + k * 64 + (int)lane;
+        const bool valid = j < len;
+        key[k] = valid ? kin[base + j] : 0u;
+        val[k] = valid ? vin[base + j] : 0u;
+    }
+    for (int k = 0; k < PER; ++k) {
+        const int j = (int)w * (RS_TILE / 4) + k * 64 + (int)lane;
+        const bool valid = j < len;
+        const uint32_t d = (key[k] >> shift) & (RS_DIGITS - 1);
+        uint64_t m = __ballot(valid);
+        for (int b = 0; b < RS_BITS; ++b) {
+            const uint64_t bb = __ballot(valid && ((d >> b) & 1u));
+            m &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t r = valid ? cnt[w][d] + (uint32_t)__popcll(m & below) : 0u;
+        wave_sync();
+        if (valid && (m >> lane) == 1ull) cnt[w][d] = r + 1u;  // the last lane of its digit
+        wave_sync();
+        rank[k] = r;
+    }
+    __syncthreads();
+    // tile-local digit offsets (exclusive scan over digits, two per thread)
+    const uint32_t d0 = 2u * threadIdx.x, d1 = d0 + 1u;
+    const uint32_t t0 = cnt[0][d0] + cnt[1][d0] + cnt[2][d0] + cnt[3][d0];
+    const uint32_t t1 = cnt[0][d1] + cnt[1][d1] + cnt[2][d1] + cnt[3][d1];
+    sh[threadIdx.x] = t0 + t1;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t add = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const uint32_t off0 = sh[threadIdx.x] - (t0 + t1), off1 = off0 + t0;
+    {
+        const size_t e0 = (size_t)d0 * nt + tile, e1 = (size_t)d1 * nt + tile;
+        gbase[d0] = (int64_t)hist[e0] + part[e0 / RS_CHUNK] - (int64_t)off0;
+        gbase[d1] = (int64_t)hist[e1] + part[e1 / RS_CHUNK] - (int64_t)off1;
+        uint32_t a0 = off0, a1 = off1;
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t c0 = cnt[v][d0], c1 = cnt[v][d1];
+            cnt[v][d0] = a0;
+            cnt[v][d1] = a1;
+            a0 += c0;
+            a1 += c1;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < PER; ++k) {
+        const int j = (int)w * (RS_TILE / 4) + k * 64 + (int)lane;
+        if (j < len) {
+            const uint32_t d = (key[k] >> shift) & (RS_DIGITS - 1);
+            const uint32_t p = cnt[w][d] + rank[k];
+            lk[p] = key[k];
+            lv[p] = val[k];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < PER; ++k) {
+        const int j = k * 256 + (int)threadIdx.x;
+        if (j < len) {
+            const uint32_t kk = lk[j];
+            const int64_t g = gbase[(kk >> shift) & (RS_DIGITS - 1)] + j;
+            if (g >= 0 && g < n) {
+                kout[g] = kk;
+                vout[g] = lv[j];
+            }
+        }
+    }
 }
 
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_count(SceneView S, const float *__restrict__ rays, int64_t n,
@@ -564,7 +641,50 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
     int *flag = nullptr;
     char *ws = nullptr;
     std::unique_lock<std::mutex> ws_lock;
-    if (T.ray_sort == 3 && n >= 4096) {
+    if (T.ray_sort == 4 && n >= 4096) {
+        // the library's radix sort: keys, keys2, values, values2, flag, tile histograms, chunk totals
+        const size_t plane = ((size_t)n * 4 + 255) & ~(size_t)255;
+        const uint32_t nt = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+        const size_t m = (size_t)RS_DIGITS * nt;
+        const uint32_t nc = (uint32_t)((m + RS_CHUNK - 1) / RS_CHUNK);
+        const size_t hbytes = (m * 4 + 255) & ~(size_t)255;
+        const size_t bytes = 4 * plane + 256 + hbytes + (size_t)nc * 4;
+        char *base = nullptr;
+        if (fixed) {
+            if (fixed->bytes < bytes)
+                return fail(RM_ERR_INVALID, "trace: %lld rays need %zu workspace bytes, %zu available", (long long)n,
+                            bytes, fixed->bytes);
+            base = fixed->base;
+        } else {
+            TraceWork *w = trace_work(st);
+            ws_lock = std::unique_lock<std::mutex>(w->mu);
+            if (w->buf.bytes < bytes) {
+                if (w->buf.p) RM_HIP(hipStreamSynchronize(st));  // earlier calls on this stream may still read it
+                int32_t rc = w->buf.alloc(bytes);
+                if (rc) return rc;
+            }
+            base = (char *)w->buf.p;
+        }
+        uint32_t *keys = (uint32_t *)base, *keys2 = (uint32_t *)(base + plane);
+        uint32_t *vals = (uint32_t *)(base + 2 * plane), *vals2 = (uint32_t *)(base + 3 * plane);
+        flag = (int *)(base + 4 * plane);
+        uint32_t *hist = (uint32_t *)(base + 4 * plane + 256);
+        uint32_t *part = (uint32_t *)(base + 4 * plane + 256 + hbytes);
+        const unsigned gn = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_coherence, dim3(1), dim3(256), 0, st, s->view, rays, n, 0.02f, flag);
+        hipLaunchKernelGGL(k_bin_keys, dim3(gn), dim3(256), 0, st, s->view, rays, n, (const int *)flag, keys, vals, 1);
+        for (int pass = 0; pass < 2; ++pass) {
+            const int shift = pass * RS_BITS;
+            const uint32_t *kin = pass ? keys2 : keys, *vin = pass ? vals2 : vals;
+            uint32_t *kout = pass ? keys : keys2, *vout = pass ? vals : vals2;
+            hipLaunchKernelGGL(k_rs_hist, dim3(nt), dim3(256), 0, st, (const int *)flag, kin, n, shift, nt, hist);
+            hipLaunchKernelGGL(k_rs_scan_chunks, dim3(nc), dim3(256), 0, st, (const int *)flag, hist, m, part);
+            hipLaunchKernelGGL(k_rs_scan_parts, dim3(1), dim3(256), 0, st, (const int *)flag, part, nc);
+            hipLaunchKernelGGL(k_rs_scatter, dim3(nt), dim3(256), 0, st, (const int *)flag, kin, vin, n, shift, nt,
+                               (const uint32_t *)hist, (const uint32_t *)part, kout, vout);
+        }
+        order = vals;  // after the second pass
+    } else if (T.ray_sort == 3 && n >= 4096) {
         // radix sort of (bin, index) pairs: keys, sorted keys, indices, order, flag, sort storage
         size_t tmp_bytes = 0;
         RM_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
@@ -635,11 +755,7 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
         unsigned pgrid = (unsigned)std::min<int64_t>(grid, (int64_t)device_cu_count() * T.persistent_blocks_per_cu);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, next);
         const int nl = std::min(std::max(1, (int)s->view.depth), T.lds_stack_entries);
-        if (T.trace_variant == 4)
-            hipLaunchKernelGGL((k_trace_persistent2<ANY>), dim3(pgrid), dim3(TRACE_BLOCK),
-                               (size_t)2 * nl * TRACE_BLOCK * 8, st, s->view, rays, n, tmin, t0, aim_dev, t_out, id_out,
-                               occ, next, T.fetch_min, (const uint32_t *)order, (const int *)flag, nl);
-        else if (T.trace_variant == 3) {
+        if (T.trace_variant == 3) {
             const bool util = T.trace_util != 0 && !fixed;
             unsigned long long *ud = nullptr;
             if (util) {
@@ -664,9 +780,6 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
             hipLaunchKernelGGL((k_trace_persistent<ANY, false, 0>), dim3(pgrid), dim3(TRACE_BLOCK), 0, st, s->view, rays,
                                n, tmin, t0, aim_dev, t_out, id_out, occ, next, T.fetch_min, (const uint32_t *)order,
                                (const int *)flag, 0);
-    } else if (T.trace_variant == 1) {
-        hipLaunchKernelGGL(k_trace_step<ANY>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev,
-                           t_out, id_out, occ, (const uint32_t *)order, (const int *)flag);
     } else if (ANY) {
         hipLaunchKernelGGL(k_trace_any<0>, dim3(grid), dim3(TRACE_BLOCK), 0, st, s->view, rays, n, tmin, t0, aim_dev, occ,
                            (const uint32_t *)order, (const int *)flag);

@@ -237,12 +237,10 @@ struct LdsStack {
     uint32_t *col;     // this lane's column
     int nl;            // entries held in LDS
     ScratchEntries *e; // the deeper ones
-    // Binds the launching kernel's dynamic LDS as this lane's first n entries,
-    // starting at entry row `first` of the block's LDS (a second walker of the
-    // same lane takes the rows after the first one's).
-    __device__ __forceinline__ void bind(ScratchEntries *m, int n, int first = 0) {
+    // Binds the launching kernel's dynamic LDS as this lane's first n entries.
+    __device__ __forceinline__ void bind(ScratchEntries *m, int n) {
         extern __shared__ uint32_t walk_lds[];
-        col = walk_lds + first * (2 * WALK_BLOCK) + threadIdx.x;
+        col = walk_lds + threadIdx.x;
         nl = n;
         e = m;
     }
@@ -288,9 +286,7 @@ struct Walker {
         li = 0;
     }
 
-    // The record of the interior node about to be visited (nothing for a leaf):
-    // a walker that loads it ahead of step_with lets a lane keep two walks'
-    // loads in flight.
+    // The record of the interior node about to be visited (nothing for a leaf).
     struct Node {
         float4 a, b, c;
         uint2 leaf;

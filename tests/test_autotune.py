@@ -11,7 +11,7 @@ from raym0nade_amd._lib import RM_ERR_INVALID, RM_ERR_OOM, RmError
 def test_tuning_keys_round_trip_and_reject_out_of_range():
     saved = autotune.current()
     try:
-        for key, good, bad in [(rm.TUNE_TRACE_VARIANT, (0, 4), (-1, 5)),
+        for key, good, bad in [(rm.TUNE_TRACE_VARIANT, (0, 2, 3), (-1, 1, 4, 5)),
                                (rm.TUNE_LDS_STACK_ENTRIES, (1, 32), (0, 33)),
                                (rm.TUNE_RENDER_MIN_WAVES, (1, 4), (2, 8)),
                                (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, (1, 64), (0, 65)),
@@ -20,7 +20,7 @@ def test_tuning_keys_round_trip_and_reject_out_of_range():
                                (rm.TUNE_RENDER_COUNT, (1, 0), (-1, 2)),
                                (rm.TUNE_TRACE_UTIL, (1, 0), (-1, 2)),
                                (rm.TUNE_WALK_PHASES, (64, 32, 0), (-1, 65)),
-                               (rm.TUNE_RAY_SORT, (0, 1, 2, 3), (-1, 4))]:
+                               (rm.TUNE_RAY_SORT, (0, 1, 2, 3, 4), (-1, 5))]:
             for v in good:
                 rm.set_tuning(key, v)
                 assert rm.get_tuning(key) == v

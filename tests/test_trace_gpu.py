@@ -29,20 +29,21 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
     return occ.cpu().numpy()
 
 
-@pytest.fixture(params=[(0, 0, 32, 0, 0), (0, 2, 32, 0, 0), (1, 1, 32, 0, 0), (2, 1, 32, 0, 0), (2, 2, 32, 0, 0),
-                        (3, 1, 32, 0, 0), (3, 0, 32, 0, 0), (3, 1, 2, 0, 0), (3, 1, 8, 1, 0), (3, 1, 2, 2, 0),
-                        (3, 0, 8, 4, 0), (4, 1, 8, 0, 0), (4, 0, 2, 0, 0), (3, 3, 8, 2, 0), (0, 3, 32, 0, 0),
-                        (3, 3, 10, 2, 32), (3, 0, 2, 1, 16), (3, 3, 10, 0, 48)],
-                ids=["stack", "stack-sorted", "step-adaptive", "persistent-adaptive", "persistent-sorted",
-                     "persistent-lds-adaptive", "persistent-lds", "persistent-lds2-spill", "lds8-1tri",
-                     "lds2-spill-2tri", "lds8-4tri", "two-walks-lds8", "two-walks-lds2-spill",
-                     "lds8-2tri-radix-sorted", "stack-radix-sorted", "lds10-2tri-phases", "lds2-spill-1tri-phases", "lds10-leaf-phases"])
+@pytest.fixture(params=[(0, 0, 32, 0, 0), (0, 2, 32, 0, 0), (2, 1, 32, 0, 0), (2, 2, 32, 0, 0),
+                        (3, 0, 32, 0, 0), (3, 1, 2, 0, 0), (3, 1, 8, 1, 0), (3, 1, 2, 2, 0),
+                        (3, 0, 8, 4, 0), (3, 3, 8, 2, 0), (0, 3, 32, 0, 0),
+                        (3, 3, 10, 2, 32), (3, 0, 2, 1, 16), (3, 4, 10, 0, 32), (0, 4, 32, 0, 0)],
+                ids=["stack", "stack-sorted", "persistent-adaptive", "persistent-sorted",
+                     "persistent-lds", "persistent-lds2-spill", "lds8-1tri", "lds2-spill-2tri", "lds8-4tri",
+                     "lds8-2tri-radix-sorted", "stack-radix-sorted", "lds10-2tri-phases", "lds2-spill-1tri-phases",
+                     "lds10-leaf-phases-own-radix", "stack-own-radix"])
 def variant(request):
-    """Every traversal kernel variant (one or two walks per lane), with and
-    without ray reordering, with the walk stack in scratch, in LDS, or split
-    between them (2 LDS entries per lane, the rest in scratch), testing a
-    whole leaf or 1, 2, 4 triangles per step, and stepping every lane or the
-    majority kind of lanes per iteration, must give the same bits."""
+    """Every traversal kernel variant, with and without ray reordering (the
+    counting sort, the rocPRIM radix sort, the library's own radix sort), with
+    the walk stack in scratch, in LDS, or split between them (2 LDS entries
+    per lane, the rest in scratch), testing a whole leaf or 1, 2, 4 triangles
+    per step, and stepping every lane or the majority kind of lanes per
+    iteration, must give the same bits."""
     keys = (rm.TUNE_TRACE_VARIANT, rm.TUNE_RAY_SORT, rm.TUNE_LDS_STACK_ENTRIES, rm.TUNE_TRIS_PER_STEP,
             rm.TUNE_WALK_PHASES)
     old = tuple(rm.get_tuning(k) for k in keys)
