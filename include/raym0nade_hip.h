@@ -161,16 +161,35 @@ int32_t rm_scene_get_info(const rm_scene *scene, rm_scene_info *info);
  * (the reference permutes `faces`, src/bvh.cpp:38). Host array of n_tris. */
 int32_t rm_scene_get_perm(const rm_scene *scene, int32_t *perm);
 
-/* Model::Model(model_folder, model_name, skyMap_name) (src/model.cpp:172-215)
- * for Wavefront OBJ models: reads model_folder + model_name with its MTL
- * libraries, PNG or DDS albedo maps (map_Kd) and, unless sky_name is "null", the
- * Radiance HDR sky model_folder + sky_name, then calls rm_scene_create.
- * Faces are fan-triangulated; meshes are the (group, material) pairs in order
- * of appearance.  MTL d / Tr below 0.99 makes the reference's glass
- * (src/material.cpp:300-333); Kd, Ke, Pr and Pm set the constant albedo,
- * emission, roughness and metallic (extensions).  Other model formats are
- * RM_ERR_UNSUPPORTED (the reference reads them through assimp). */
+/* Model::Model(model_folder, model_name, skyMap_name) (src/model.cpp:172-215):
+ * rm_model_read, then rm_scene_create. */
 int32_t rm_scene_load(const char *model_folder, const char *model_name, const char *sky_name, rm_scene **out);
+
+/* The host half of rm_scene_load (the reference's assimp import and
+ * Model::processMaterial / processMesh, src/model.cpp:88-170): reads
+ * model_folder + model_name, its texture maps and, unless sky_name is "null",
+ * the Radiance HDR sky model_folder + sky_name into host memory, and fills
+ * *desc with pointers into it, valid until rm_model_destroy.  Touches no
+ * device.
+ *   - Wavefront OBJ + MTL: faces fan-triangulated; meshes are the
+ *     (group, material) pairs in order of appearance; MTL d / Tr below 0.99
+ *     makes the reference's glass (src/material.cpp:300-333); Kd, Ke, Pr and
+ *     Pm set the constant albedo, emission, roughness and metallic
+ *     (extensions); map_Kd, norm / map_Kn, map_Ke / map_emissive and map_Ks
+ *     are the diffuse, normal, emissive and specular maps.
+ *   - FBX (binary, version 7100 and later), as assimp imports it with the
+ *     reference's flags (Triangulate | PreTransformVertices, hierarchy kept |
+ *     SortByPType | FixInfacingNormals): one mesh per (geometry, material)
+ *     in order of first use, vertices in world space, quads split at their
+ *     concave corner, inward-facing meshes flipped; the maps connected as
+ *     DiffuseColor, NormalMap, EmissiveColor / EmissiveFactor and
+ *     SpecularColor / SpecularFactor; a material Opacity below 0.99 makes
+ *     the glass.  ASCII FBX is RM_ERR_UNSUPPORTED.
+ * Other model formats are RM_ERR_UNSUPPORTED. */
+typedef struct rm_model rm_model;
+int32_t rm_model_read(const char *model_folder, const char *model_name, const char *sky_name, rm_scene_desc *desc,
+                      rm_model **out);
+int32_t rm_model_destroy(rm_model *model);
 
 /* An image as linear floats [h][w][3] (malloc'd; free with rm_image_free):
  * PNG as c / 255 after libpng's conversions in the reference's loader

@@ -12,7 +12,7 @@ import numpy as np
 
 from ._lib import (RM_BUF_BASE_COLOR, RM_BUF_DD, RM_BUF_DS, RM_BUF_EMISSION, RM_BUF_ID, RM_BUF_IS, RM_BUF_MATERIAL,
                    RM_BUF_PIXELS, RM_BUF_POSITION, RM_BUF_SHAPE_NORMAL, RM_BUF_SURFACE_NORMAL, RmError, RmGenOptions, RmRenderArgs,
-                   RmSceneDesc, RmSceneInfo, check, lib)
+                   RmMaterial, RmSceneDesc, RmSceneInfo, check, lib)
 
 __all__ = ["GeneratedScene", "Scene", "Frame", "generate", "rays", "philox", "RmError", "RmRenderArgs"]
 
@@ -136,7 +136,7 @@ class Scene:
     @classmethod
     def load(cls, model_folder: str, model_name: str, sky_name: str = "null") -> "Scene":
         """Model::Model(model_folder, model_name, skyMap_name) for Wavefront OBJ
-        models (rm_scene_load)."""
+        and binary FBX models (rm_scene_load)."""
         self = cls.__new__(cls)
         self._keep = None
         h = C.c_void_p()
@@ -191,6 +191,48 @@ class Scene:
             except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
                 pass
             self._h = None
+
+
+class Model:
+    """A model read into host memory (rm_model_read): the scene description
+    rm_scene_create takes, as numpy copies.  verts [n,3,3], uvs [n,3,2],
+    normals [n,3,3] or None, material [n], mesh [n], materials (RmMaterial
+    list), textures (list of dicts: width, height, channels, bytes or rgb)
+    and the sky ([h,w,3] or None)."""
+
+    def __init__(self, model_folder: str, model_name: str, sky_name: str = "null"):
+        desc = RmSceneDesc()
+        h = C.c_void_p()
+        check(lib().rm_model_read(model_folder.encode(), model_name.encode(), sky_name.encode(), C.byref(desc),
+                                  C.byref(h)))
+        try:
+            n = desc.n_tris
+
+            def arr(ptr, ctype, count, shape):
+                if not ptr or count == 0:
+                    return None
+                a = (ctype * count).from_address(C.cast(ptr, C.c_void_p).value)
+                return np.ctypeslib.as_array(a).copy().reshape(shape)
+
+            self.verts = arr(desc.verts, C.c_float, n * 9, (n, 3, 3))
+            self.uvs = arr(desc.uvs, C.c_float, n * 6, (n, 3, 2))
+            self.normals = arr(desc.normals, C.c_float, n * 9, (n, 3, 3))
+            self.material = arr(desc.material, C.c_int32, n, (n,))
+            self.mesh = arr(desc.mesh, C.c_int32, n, (n,))
+            self.materials = [RmMaterial.from_buffer_copy(desc.materials[i]) for i in range(desc.n_materials)]
+            self.textures = []
+            for i in range(desc.n_textures):
+                t = desc.textures[i]
+                d = {"width": t.width, "height": t.height, "channels": t.channels}
+                if t.channels:
+                    d["bytes"] = arr(t.bytes, C.c_uint8, t.width * t.height * t.channels, (t.height, t.width, t.channels))
+                else:
+                    d["rgb"] = arr(t.rgb, C.c_float, t.width * t.height * 3, (t.height, t.width, 3))
+                self.textures.append(d)
+            self.sky = arr(desc.sky_rgb, C.c_float, desc.sky_width * desc.sky_height * 3,
+                           (desc.sky_height, desc.sky_width, 3))
+        finally:
+            lib().rm_model_destroy(h)
 
 
 def render_batch(jobs, use_graph: bool = True, stream=None) -> None:

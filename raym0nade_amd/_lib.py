@@ -168,6 +168,8 @@ SIGNATURES = [
     ("rm_frame_upload", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("rm_spatial_clamp", C.c_int32, [C.c_void_p, C.c_void_p]),
     ("rm_scene_load", C.c_int32, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("rm_model_read", C.c_int32, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(RmSceneDesc), C.POINTER(C.c_void_p)]),
+    ("rm_model_destroy", C.c_int32, [C.c_void_p]),
     ("rm_image_load", C.c_int32, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.POINTER(C.c_float))]),
     ("rm_image_free", None, [C.POINTER(C.c_float)]),
     ("rm_frame_set_dof", C.c_int32, [C.c_void_p, C.c_float, C.c_float, C.POINTER(C.c_float)]),

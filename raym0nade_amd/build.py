@@ -21,7 +21,7 @@ OBJDIR = os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-SOURCES = ["api.cpp", "scene.cpp", "scenegen.cpp", "assets.cpp", "frame.cpp", "trace.hip", "render.hip", "post.hip", "dof.hip", "denoise.hip", "adjoint.hip"]
+SOURCES = ["api.cpp", "scene.cpp", "scenegen.cpp", "assets.cpp", "fbx.cpp", "frame.cpp", "trace.hip", "render.hip", "post.hip", "dof.hip", "denoise.hip", "adjoint.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -17,9 +17,11 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <new>
 #include <sstream>
 #include <string>
 #include <vector>
+#include "assets.h"
 #include "rm_internal.h"
 
 namespace rm {
@@ -430,15 +432,7 @@ std::string url_decode(const std::string &s) {
     return out;
 }
 
-// Texture maps of a material under the keywords assimp's OBJ importer maps
-// to the reference's texture types: map_Kd diffuse, norm / map_Kn normals,
-// map_Ke / map_emissive emissive, map_Ks specular.
-struct MtlDef {
-    std::string name;
-    rm_material m{};
-    std::string map_kd, map_norm, map_ke, map_ks;
-    bool has_emission = false;
-};
+}  // namespace
 
 rm_material default_material() {
     rm_material m{};
@@ -468,6 +462,11 @@ void glass_by_name(MtlDef &d) {
     if (n == "White_Wine") set(0.85f, 0.78f, 0.6f);
 }
 
+namespace {
+
+// Texture maps of a material under the keywords assimp's OBJ importer maps
+// to the reference's texture types: map_Kd diffuse, norm / map_Kn normals,
+// map_Ke / map_emissive emissive, map_Ks specular.
 int32_t parse_mtl(const std::string &path, std::vector<MtlDef> &mats) {
     std::ifstream in(path);
     if (!in) return fail(RM_ERR_INVALID, "cannot read material library %s", path.c_str());
@@ -491,7 +490,6 @@ int32_t parse_mtl(const std::string &path, std::vector<MtlDef> &mats) {
             ss >> cur->m.base_color[0] >> cur->m.base_color[1] >> cur->m.base_color[2];
         } else if (k == "Ke") {  // EXT: a constant emission (the reference uses only map_Ke)
             ss >> cur->m.emission[0] >> cur->m.emission[1] >> cur->m.emission[2];
-            cur->has_emission = true;
         } else if (k == "Pr") {  // EXT: constant roughness (PBR extension of MTL)
             ss >> cur->m.roughness;
         } else if (k == "Pm") {  // EXT: constant metallic
@@ -520,19 +518,9 @@ int32_t parse_mtl(const std::string &path, std::vector<MtlDef> &mats) {
     return RM_OK;
 }
 
-struct ObjScene {
-    std::vector<float> verts, uvs, normals;
-    std::vector<int32_t> material, mesh;
-    std::vector<rm_material> mats;
-    std::vector<std::vector<float>> tex_data, tex_alpha;
-    std::vector<std::vector<uint8_t>> tex_bytes;
-    std::vector<rm_texture> texs;
-    std::vector<float> sky;
-    int sky_w = 0, sky_h = 0;
-    bool all_normals = true;
-};
+}  // namespace
 
-int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &S) {
+int32_t parse_obj(const std::string &folder, const std::string &name, LoadedModel &S) {
     std::ifstream in(folder + name);
     if (!in) return fail(RM_ERR_INVALID, "cannot read model %s%s", folder.c_str(), name.c_str());
     std::vector<float> P, T, N;
@@ -638,25 +626,35 @@ int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &
             S.material.push_back(t.mat);
             S.mesh.push_back(t.mesh);
         }
-    // materials and their texture maps (Material::loadImageFromFile,
-    // src/material.cpp:290-298): PNG and DDS maps as the reference's 8-bit
-    // texels; an HDR diffuse map (EXT) as linear fp32 texels.  Each file is
-    // loaded once.
-    std::map<std::string, int> tex_of;
+    return load_material_maps(folder, defs, S);
+}
+
+// Materials and their texture maps (Material::loadImageFromFile,
+// src/material.cpp:290-298, as Model::processMaterial passes them,
+// src/model.cpp:146-152: model folder + path, backslashes made slashes, then
+// URL-decoded): PNG and DDS maps as the reference's 8-bit texels; an HDR
+// diffuse map (extension) as linear fp32 texels.  Each file is loaded once.
+int32_t load_material_maps(const std::string &folder, std::vector<MtlDef> &defs, LoadedModel &S) {
     auto texture = [&](const std::string &file, bool allow_hdr, int32_t &index) -> int32_t {
         index = -1;
         if (file.empty()) return RM_OK;
-        std::string path = url_decode(folder + file);
+        std::string path = folder + file;
         for (auto &c : path)
             if (c == '\\') c = '/';
-        auto it = tex_of.find(path);
-        if (it != tex_of.end()) {
-            index = it->second;
-            return RM_OK;
-        }
+        path = url_decode(path);
+        for (size_t i = 0; i < S.tex_paths.size(); ++i)
+            if (S.tex_paths[i] == path) {
+                index = (int)i;
+                return RM_OK;
+            }
+        // Material::loadImageFromFile reads "png", "PNG", "dds" and "DDS"
+        // files and leaves the map empty for any other extension
+        const size_t dot = path.find_last_of('.');
+        const std::string e = dot == std::string::npos ? path : path.substr(dot + 1);
+        const bool hdr_map = allow_hdr && ext_of(path) == "hdr";
+        if (!hdr_map && e != "png" && e != "PNG" && e != "dds" && e != "DDS") return RM_OK;
         int w = 0, h = 0;
-        if (ext_of(path) == "hdr") {
-            if (!allow_hdr) return fail(RM_ERR_UNSUPPORTED, "%s: an HDR file as a normal, emissive or specular map", path.c_str());
+        if (hdr_map) {
             bool hdr = false;
             std::vector<float> px;
             int32_t rc = load_image(path, w, h, px, hdr);
@@ -673,8 +671,8 @@ int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &
             S.tex_bytes.push_back(std::move(bytes));
             S.texs.push_back(rm_texture{w, h, nullptr, nullptr, nullptr, ch});
         }
+        S.tex_paths.push_back(path);
         index = (int)S.texs.size() - 1;
-        tex_of.emplace(path, index);
         return RM_OK;
     };
     for (auto &d : defs) {
@@ -683,7 +681,12 @@ int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &
             (rc = texture(d.map_ke, false, d.m.emissive_tex)) || (rc = texture(d.map_ks, false, d.m.specular_tex)))
             return rc;
         S.mats.push_back(d.m);
+        S.mat_names.push_back(d.name);
     }
+    return RM_OK;
+}
+
+void fill_desc(LoadedModel &S) {
     for (size_t i = 0; i < S.texs.size(); ++i) {
         if (S.texs[i].channels) {
             S.texs[i].bytes = S.tex_bytes[i].data();
@@ -691,10 +694,23 @@ int32_t parse_obj(const std::string &folder, const std::string &name, ObjScene &
             S.texs[i].rgb = S.tex_data[i].data();
         }
     }
-    return RM_OK;
+    rm_scene_desc &d = S.desc;
+    d = rm_scene_desc{};
+    d.n_tris = (int64_t)S.material.size();
+    d.verts = S.verts.data();
+    d.uvs = S.uvs.data();
+    d.normals = S.all_normals ? S.normals.data() : nullptr;
+    d.material = S.material.data();
+    d.mesh = S.mesh.data();
+    d.n_materials = (int32_t)S.mats.size();
+    d.materials = S.mats.data();
+    d.n_textures = (int32_t)S.texs.size();
+    d.textures = S.texs.empty() ? nullptr : S.texs.data();
+    d.sky_width = S.sky_w;
+    d.sky_height = S.sky_h;
+    d.sky_rgb = S.sky.empty() ? nullptr : S.sky.data();
 }
 
-}  // namespace
 }  // namespace rm
 
 using namespace rm;
@@ -718,33 +734,49 @@ extern "C" int32_t rm_image_load(const char *path, int32_t *width, int32_t *heig
 
 extern "C" void rm_image_free(float *rgb) { free(rgb); }
 
+struct rm_model {
+    rm::LoadedModel m;
+};
+
+extern "C" int32_t rm_model_read(const char *model_folder, const char *model_name, const char *sky_name,
+                                 rm_scene_desc *desc, rm_model **out) {
+    if (!model_folder || !model_name || !desc || !out) return fail(RM_ERR_INVALID, "rm_model_read: NULL argument");
+    *out = nullptr;
+    const std::string folder = model_folder, name = model_name, ext = ext_of(name);
+    if (ext != "obj" && ext != "fbx")
+        return fail(RM_ERR_UNSUPPORTED, "rm_model_read: %s: only Wavefront OBJ and binary FBX models are read",
+                    model_name);
+    rm_model *M = new (std::nothrow) rm_model;
+    if (!M) return fail(RM_ERR_OOM, "rm_model_read: host allocation");
+    LoadedModel &S = M->m;
+    int32_t rc = ext == "obj" ? parse_obj(folder, name, S) : parse_fbx(folder, name, S);
+    if (!rc && sky_name && strcmp(sky_name, "null") != 0) {
+        bool hdr = false;
+        rc = load_image(folder + sky_name, S.sky_w, S.sky_h, S.sky, hdr);
+    }
+    if (rc) {
+        delete M;
+        return rc;
+    }
+    fill_desc(S);
+    *desc = S.desc;
+    *out = M;
+    return RM_OK;
+}
+
+extern "C" int32_t rm_model_destroy(rm_model *model) {
+    delete model;
+    return RM_OK;
+}
+
 extern "C" int32_t rm_scene_load(const char *model_folder, const char *model_name, const char *sky_name, rm_scene **out) {
     if (!model_folder || !model_name || !out) return fail(RM_ERR_INVALID, "rm_scene_load: NULL argument");
     *out = nullptr;
-    const std::string folder = model_folder, name = model_name;
-    if (ext_of(name) != "obj")
-        return fail(RM_ERR_UNSUPPORTED, "rm_scene_load: %s: only Wavefront OBJ models are read by this build", model_name);
-    ObjScene S;
-    int32_t rc = parse_obj(folder, name, S);
-    if (rc) return rc;
-    if (sky_name && strcmp(sky_name, "null") != 0) {
-        bool hdr = false;
-        rc = load_image(folder + sky_name, S.sky_w, S.sky_h, S.sky, hdr);
-        if (rc) return rc;
-    }
     rm_scene_desc d{};
-    d.n_tris = (int64_t)S.material.size();
-    d.verts = S.verts.data();
-    d.uvs = S.uvs.data();
-    d.normals = S.all_normals ? S.normals.data() : nullptr;
-    d.material = S.material.data();
-    d.mesh = S.mesh.data();
-    d.n_materials = (int32_t)S.mats.size();
-    d.materials = S.mats.data();
-    d.n_textures = (int32_t)S.texs.size();
-    d.textures = S.texs.empty() ? nullptr : S.texs.data();
-    d.sky_width = S.sky_w;
-    d.sky_height = S.sky_h;
-    d.sky_rgb = S.sky.empty() ? nullptr : S.sky.data();
-    return rm_scene_create(&d, out);
+    rm_model *M = nullptr;
+    int32_t rc = rm_model_read(model_folder, model_name, sky_name, &d, &M);
+    if (rc) return rc;
+    rc = rm_scene_create(&d, out);
+    rm_model_destroy(M);
+    return rc;
 }

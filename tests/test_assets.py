@@ -254,7 +254,9 @@ def test_scene_load_rejects_bad_models_before_touching_a_device(tmp_path):
         return e.value.code
     assert load("missing.obj") == RM_ERR_INVALID
     (tmp_path / "m.fbx").write_bytes(b"")
-    assert load("m.fbx") == RM_ERR_UNSUPPORTED
+    assert load("m.fbx") == RM_ERR_INVALID             # not an FBX file (binary FBX is read)
+    (tmp_path / "m.3ds").write_bytes(b"")
+    assert load("m.3ds") == RM_ERR_UNSUPPORTED
     (tmp_path / "idx.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 4\n")
     assert load("idx.obj") == RM_ERR_INVALID          # vertex 4 does not exist
     (tmp_path / "zero.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 0 1 2\n")

@@ -1,0 +1,404 @@
+"""Binary FBX import (rm_model_read; the reference reads FBX through assimp
+in Model::Model, src/model.cpp:172-215, with aiProcess_Triangulate |
+aiProcess_PreTransformVertices (hierarchy kept) | aiProcess_SortByPType |
+aiProcess_FixInfacingNormals).  assimp is absent here, so the files come from
+an independent writer of the published record layout (fbx_writer.py), and
+each expectation is fixed by geometry: world positions from the FBX node
+transform written out with explicit matrices, normals by the inverse
+transpose, the quad split from the concave corner checked by area, the mesh
+and material order by first use, glass by the reference's name table
+(src/material.cpp:300-333).  No device is touched."""
+import math
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+import raym0nade_amd as rm
+from raym0nade_amd._lib import RM_ERR_EMPTY, RM_ERR_INVALID, RM_ERR_UNSUPPORTED, RmError
+
+import fbx_writer as W
+
+
+def read(tmp_path, data, name="m.fbx", sky="null"):
+    (tmp_path / name).write_bytes(data)
+    return rm.Model(str(tmp_path) + "/", name, sky)
+
+
+def code_of(tmp_path, data, name="m.fbx"):
+    with pytest.raises(RmError) as e:
+        read(tmp_path, data, name)
+    return e.value.code
+
+
+# -------- transforms written out from the FBX definition (degrees, column vectors)
+def T(v):
+    m = np.eye(4)
+    m[:3, 3] = v
+    return m
+
+
+def Sc(v):
+    return np.diag([v[0], v[1], v[2], 1.0])
+
+
+def Rx(a):
+    c, s = math.cos(math.radians(a)), math.sin(math.radians(a))
+    return np.array([[1, 0, 0, 0], [0, c, -s, 0], [0, s, c, 0], [0, 0, 0, 1.0]])
+
+
+def Ry(a):
+    c, s = math.cos(math.radians(a)), math.sin(math.radians(a))
+    return np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0], [0, 0, 0, 1.0]])
+
+
+def Rz(a):
+    c, s = math.cos(math.radians(a)), math.sin(math.radians(a))
+    return np.array([[c, -s, 0, 0], [s, c, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1.0]])
+
+
+def euler(r, order="XYZ"):
+    """The first axis of `order` turns first: XYZ -> Rz @ Ry @ Rx."""
+    m = np.eye(4)
+    for ax in order:
+        m = {"X": Rx, "Y": Ry, "Z": Rz}[ax](r["XYZ".index(ax)]) @ m
+    return m
+
+
+ORDERS = ["XYZ", "XZY", "YZX", "YXZ", "ZXY", "ZYX"]
+
+
+def apply(m, pts):
+    p = np.c_[np.asarray(pts, np.float64), np.ones(len(pts))]
+    return (p @ m.T)[:, :3]
+
+
+def normals_by(m, ns):
+    n = np.asarray(ns, np.float64) @ np.linalg.inv(m[:3, :3])  # (M^-T n) as rows
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+QUAD = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)]
+
+
+def one_model_doc(points, polys, props=(), normals=None, uvs=None, version=7400, compress=False, mats=None,
+                  material_objs=(), extra=(), extra_conns=()):
+    objs = [W.model(10, "M", props), W.geometry(20, "G", points, polys, normals, uvs, mats, compress)]
+    objs += list(material_objs) + list(extra)
+    conns = [W.oo(10, 0), W.oo(20, 10)] + [W.oo(m.props[0], 10) for m in material_objs] + list(extra_conns)
+    return W.document(objs, conns, version=version)
+
+
+@pytest.mark.parametrize("version,compress", [(7400, False), (7500, True), (7300, True)])
+def test_world_space_quad_and_triangle(tmp_path, version, compress):
+    """One model turned, scaled and moved; a quad and a triangle with
+    per-polygon-vertex normals (Direct) and uvs (IndexToDirect).  Versions
+    before and after 7500 (32- and 64-bit record fields), raw and deflated
+    arrays give the same model."""
+    pts = QUAD + [(2, 0, 0)]
+    polys = [[0, 1, 2, 3], [1, 4, 2]]
+    nv = [(0, 0, 1)] * 7
+    uvv = [(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0), (0.5, 0.25)]
+    uvi = [0, 1, 2, 3, 1, 4, 2]
+    r, t, s = (30.0, -45.0, 60.0), (1.0, 2.0, -3.0), (2.0, 0.5, 1.5)
+    props = [W.vec("Lcl Rotation", *r), W.vec("Lcl Translation", *t), W.vec("Lcl Scaling", *s)]
+    doc = one_model_doc(pts, polys, props, normals=([c for n in nv for c in n], "ByPolygonVertex", "Direct", None),
+                        uvs=([c for u in uvv for c in u], "ByPolygonVertex", "IndexToDirect", uvi),
+                        version=version, compress=compress)
+    m = read(tmp_path, doc)
+    M = T(t) @ euler(r) @ Sc(s)
+    corners = [[0, 1, 2], [0, 2, 3], [1, 4, 2]]  # the convex quad splits from corner 0
+    want = apply(M, [pts[i] for c in corners for i in c]).reshape(3, 3, 3)
+    assert m.verts.shape == (3, 3, 3)
+    np.testing.assert_allclose(m.verts, want, rtol=0, atol=2e-6)
+    pv_of = [[0, 1, 2], [0, 2, 3], [4, 5, 6]]  # polygon-vertex numbers of each corner
+    np.testing.assert_array_equal(m.uvs, np.array([[uvv[uvi[k]] for k in c] for c in pv_of], np.float32))
+    wn = normals_by(M, [(0, 0, 1)])[0]
+    np.testing.assert_allclose(m.normals, np.broadcast_to(wn, (3, 3, 3)), atol=1e-6)
+    assert list(m.material) == [0, 0, 0] and list(m.mesh) == [0, 0, 0]
+    assert len(m.materials) == 1  # the default material: white, roughness 0.8, opaque
+    d = m.materials[0]
+    assert list(d.base_color) == [1.0, 1.0, 1.0] and d.roughness == np.float32(0.8) and d.opacity == 1.0
+    assert d.albedo_tex == -1 and d.normal_tex == -1
+
+
+@pytest.mark.parametrize("order", range(6))
+def test_rotation_orders(tmp_path, order):
+    r = (20.0, 35.0, -50.0)
+    doc = one_model_doc([(1, 0.5, 0.25), (0, 1, 0), (0, 0, 1)], [[0, 1, 2]],
+                        [W.vec("Lcl Rotation", *r), W.P("RotationOrder", "enum", "", "", order)])
+    m = read(tmp_path, doc)
+    want = apply(euler(r, ORDERS[order]), [(1, 0.5, 0.25), (0, 1, 0), (0, 0, 1)])
+    np.testing.assert_allclose(m.verts[0], want, atol=2e-6)
+    for other in range(6):  # the orders really differ for these angles
+        if other != order:
+            assert not np.allclose(apply(euler(r, ORDERS[other]), [(1, 0.5, 0.25)]), want[:1], atol=1e-3)
+
+
+def test_pivots_offsets_hierarchy_and_geometric_transform(tmp_path):
+    """T * Roff * Rp * Rpre * R * Rpost^-1 * Rp^-1 * Soff * Sp * S * Sp^-1 per
+    node; a node's geometric transform applies to its own geometry and not to
+    its children."""
+    par = dict(T=(1, -2, 0.5), R=(10, 0, 25), Rpre=(0, 30, 0), G=(0.25, 0, 0))
+    ch = dict(T=(0.5, 0.25, 2), R=(-15, 40, 5), S=(1.5, 0.75, 2.0), Rp=(0.3, -0.2, 0.1), Roff=(0.05, 0.1, -0.2),
+              Sp=(-0.4, 0.2, 0.3), Soff=(0.2, 0.0, 0.1), Rpost=(5, -10, 20), Gs=(2, 2, 2), Gr=(0, 0, 90))
+    pprops = [W.vec("Lcl Translation", *par["T"]), W.vec("Lcl Rotation", *par["R"]), W.vec("PreRotation", *par["Rpre"]),
+              W.vec("GeometricTranslation", *par["G"])]
+    cprops = [W.vec("Lcl Translation", *ch["T"]), W.vec("Lcl Rotation", *ch["R"]), W.vec("Lcl Scaling", *ch["S"]),
+              W.vec("RotationPivot", *ch["Rp"]), W.vec("RotationOffset", *ch["Roff"]),
+              W.vec("ScalingPivot", *ch["Sp"]), W.vec("ScalingOffset", *ch["Soff"]),
+              W.vec("PostRotation", *ch["Rpost"]), W.vec("GeometricScaling", *ch["Gs"]),
+              W.vec("GeometricRotation", *ch["Gr"])]
+    tri = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    objs = [W.model(1, "parent", pprops), W.model(2, "child", cprops),
+            W.geometry(11, "gp", tri, [[0, 1, 2]]), W.geometry(12, "gc", tri, [[0, 1, 2]])]
+    conns = [W.oo(1, 0), W.oo(2, 1), W.oo(11, 1), W.oo(12, 2)]
+    m = read(tmp_path, W.document(objs, conns))
+    Lp = T(par["T"]) @ euler(par["Rpre"]) @ euler(par["R"])
+    Lc = (T(ch["T"]) @ T(ch["Roff"]) @ T(ch["Rp"]) @ euler(ch["R"]) @ np.linalg.inv(euler(ch["Rpost"])) @
+          T(-np.array(ch["Rp"])) @ T(ch["Soff"]) @ T(ch["Sp"]) @ Sc(ch["S"]) @ T(-np.array(ch["Sp"])))
+    Gc = euler(ch["Gr"]) @ Sc(ch["Gs"])
+    np.testing.assert_allclose(m.verts[0], apply(Lp @ T(par["G"]), tri), atol=3e-6)   # parent: own geometric shift
+    np.testing.assert_allclose(m.verts[1], apply(Lp @ Lc @ Gc, tri), atol=3e-6)       # child: not the parent's
+    assert list(m.mesh) == [0, 1]
+
+
+def test_concave_quad_splits_at_its_concave_corner(tmp_path):
+    """A dart whose corner 2 is reflex: both triangles start there, so they
+    cover the dart exactly (a split from corner 0 would overlap it)."""
+    dart = [(0, 0, 0), (2, 0, 0), (0.6, 0.6, 0), (0, 2, 0)]
+    m = read(tmp_path, one_model_doc(dart, [[0, 1, 2, 3]]))
+    P = np.float32(dart)
+    np.testing.assert_array_equal(m.verts, np.stack([P[[2, 3, 0]], P[[2, 0, 1]]]))
+
+    def area(t):
+        return 0.5 * np.cross(np.subtract(t[1], t[0]), np.subtract(t[2], t[0]))[2]
+    dart_area = 0.5 * abs(sum(dart[i][0] * dart[(i + 1) % 4][1] - dart[(i + 1) % 4][0] * dart[i][1] for i in range(4)))
+    assert all(area(t) > 0 for t in m.verts)  # same turning sense as the dart
+    assert math.isclose(sum(area(t) for t in m.verts), dart_area, rel_tol=1e-6)
+
+
+def test_polygons_with_more_corners_and_dropped_lines(tmp_path):
+    """A concave hexagon becomes four triangles covering exactly its area;
+    polygons of one or two corners (points, lines) are dropped."""
+    hexa = [(0, 0, 0), (4, 0, 0), (4, 3, 0), (2, 1, 0), (0, 3, 0), (-1, 1.5, 0)]
+    pts = hexa + [(9, 9, 9), (8, 8, 8)]
+    m = read(tmp_path, one_model_doc(pts, [[6], [0, 1, 2, 3, 4, 5], [6, 7]]))
+    assert m.verts.shape[0] == 4
+    area = lambda t: 0.5 * np.cross(np.subtract(t[1], t[0]), np.subtract(t[2], t[0]))[2]  # noqa: E731
+    poly_area = 0.5 * abs(sum(hexa[i][0] * hexa[(i + 1) % 6][1] - hexa[(i + 1) % 6][0] * hexa[i][1] for i in range(6)))
+    assert all(area(t) > 0 for t in m.verts)
+    assert math.isclose(sum(area(t) for t in m.verts), poly_area, rel_tol=1e-6)
+    assert set(map(tuple, m.verts.reshape(-1, 3).tolist())) <= set(map(tuple, np.float32(hexa).tolist()))
+
+
+def test_mirroring_transform_reverses_winding(tmp_path):
+    tri = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    m = read(tmp_path, one_model_doc(tri, [[0, 1, 2]], [W.vec("Lcl Scaling", -1, 1, 1)]))
+    np.testing.assert_array_equal(m.verts[0], np.float32([(0, 1, 0), (-1, 0, 0), (0, 0, 0)]))
+
+
+CUBE_PTS = [(x, y, z) for x in (0, 1) for y in (0, 1) for z in (0, 1)]
+# faces wound counter-clockwise seen from outside, with their outward normals
+CUBE_FACES = [([0, 1, 3, 2], (-1, 0, 0)), ([4, 6, 7, 5], (1, 0, 0)), ([0, 4, 5, 1], (0, -1, 0)),
+              ([2, 3, 7, 6], (0, 1, 0)), ([0, 2, 6, 4], (0, 0, -1)), ([1, 5, 7, 3], (0, 0, 1))]
+
+
+@pytest.mark.parametrize("inward", [False, True])
+def test_infacing_normals_are_flipped(tmp_path, inward):
+    """A closed 4-unit cube with smooth corner normals (along the diagonals)
+    pointing outward is left alone: moving each vertex by its normal grows
+    the box.  With every normal pointing inward the box shrinks, so the mesh
+    gets its normals negated and its triangles reversed."""
+    s = -1.0 if inward else 1.0
+    pts = [tuple(4 * c for c in p) for p in CUBE_PTS]
+    corner_n = {i: tuple(s * (2 * c - 1) / math.sqrt(3) for c in CUBE_PTS[i]) for i in range(8)}
+    nv = [c for f, _ in CUBE_FACES for i in f for c in corner_n[i]]
+    m = read(tmp_path, one_model_doc(pts, [f for f, _ in CUBE_FACES], normals=(nv, "ByPolygonVertex", "Direct", None)))
+    assert m.verts.shape[0] == 12
+    P = np.float32(pts)
+    for k, (f, n) in enumerate(CUBE_FACES):
+        tris = [[f[0], f[1], f[2]], [f[0], f[2], f[3]]]
+        for j, t in enumerate(tris):
+            want_idx = t[::-1] if inward else t
+            np.testing.assert_array_equal(m.verts[2 * k + j], P[want_idx])
+            want_n = np.float32([[abs(x) * (2 * c - 1) for x, c in zip(corner_n[i], CUBE_PTS[i])] for i in want_idx])
+            np.testing.assert_allclose(m.normals[2 * k + j], want_n, atol=1e-7)  # outward either way
+            geo = np.cross(m.verts[2 * k + j, 1] - m.verts[2 * k + j, 0], m.verts[2 * k + j, 2] - m.verts[2 * k + j, 0])
+            assert np.dot(geo, n) * (-1 if inward else 1) > 0
+
+
+def test_planar_mesh_with_inward_normals_is_not_flipped(tmp_path):
+    tri = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    m = read(tmp_path, one_model_doc(tri, [[0, 1, 2]], normals=([0, 0, -1] * 3, "ByPolygonVertex", "Direct", None)))
+    np.testing.assert_array_equal(m.normals[0], np.float32([(0, 0, -1)] * 3))
+    np.testing.assert_array_equal(m.verts[0], np.float32(tri))
+
+
+def test_layer_mappings(tmp_path):
+    """ByVertice Direct normals and ByVertice IndexToDirect uvs follow the
+    control point; ByPolygonVertex IndexToDirect -1 gives zeros; a layer
+    element no Layer record names is ignored."""
+    pts = QUAD
+    cp_normals = [(0, 0, 1), (0, 0.6, 0.8), (0.6, 0, 0.8), (0, -0.6, 0.8)]
+    uv_table = [(0.1, 0.2), (0.3, 0.4)]
+    uv_index = [1, 0, 1, 0]
+    objs = [W.model(10, "M"),
+            W.geometry(20, "G", pts, [[0, 1, 2, 3]], normals=([c for n in cp_normals for c in n], "ByVertice", "Direct", None),
+                       uvs=([c for u in uv_table for c in u], "ByVertice", "IndexToDirect", uv_index))]
+    m = read(tmp_path, W.document(objs, [W.oo(10, 0), W.oo(20, 10)]))
+    corners = [[0, 1, 2], [0, 2, 3]]
+    np.testing.assert_array_equal(m.normals, np.float32([[cp_normals[i] for i in c] for c in corners]))
+    np.testing.assert_array_equal(m.uvs, np.float32([[uv_table[uv_index[i]] for i in c] for c in corners]))
+    g = W.geometry(20, "G", pts, [[0, 1, 2, 3]], uvs=([0.5, 0.5], "ByPolygonVertex", "IndexToDirect", [0, -1, 0, -1]))
+    m = read(tmp_path, W.document([W.model(10, "M"), g], [W.oo(10, 0), W.oo(20, 10)]))
+    np.testing.assert_array_equal(m.uvs, np.float32([[(0.5, 0.5), (0, 0), (0.5, 0.5)], [(0.5, 0.5), (0.5, 0.5), (0, 0)]]))
+    g = W.geometry(20, "G", pts, [[0, 1, 2, 3]], normals=([0, 0, 1] * 4, "ByPolygonVertex", "Direct", None))
+    g.kids = [k for k in g.kids if k.name != "Layer"]
+    assert read(tmp_path, W.document([W.model(10, "M"), g], [W.oo(10, 0), W.oo(20, 10)])).normals is None
+
+
+def test_meshes_split_by_material_in_first_use_order(tmp_path):
+    """Polygon materials [1, 0, 1, 2] over a model with materials A, B, C
+    (connection order): the meshes are B (polygons 0, 2), A (1) and C (3);
+    the scene's materials follow first use: B, A, C.  B is named Water with
+    Opacity 0.5 (glass, clear); A has TransparentColor 0.9 x factor 1 and no
+    Opacity (opacity 0.1: glass, Beer's colour); C is opaque."""
+    tris = [[0, 1, 2], [0, 2, 3], [1, 2, 3], [0, 1, 3]]
+    mats = [W.material(31, "Beer", [W.vec("TransparentColor", 0.9, 0.9, 0.9, "Color"),
+                                    W.P("TransparencyFactor", "double", "Number", "", 1.0)]),
+            W.material(32, "Water", [W.P("Opacity", "double", "Number", "", 0.5)]),
+            W.material(33, "Wood", [W.P("Opacity", "double", "Number", "", 1.0)])]
+    pts = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0.5)]
+    m = read(tmp_path, one_model_doc(pts, tris, mats=[1, 0, 1, 2], material_objs=mats))
+    assert list(m.mesh) == [0, 0, 1, 2] and list(m.material) == [0, 0, 1, 2]
+    want_first = np.float32([[pts[i] for i in t] for t in (tris[0], tris[2], tris[1], tris[3])])
+    np.testing.assert_array_equal(m.verts, want_first)
+    water, beer, wood = m.materials
+    assert water.opacity == 0.0 and water.ior == np.float32(1.25) and water.roughness == np.float32(5e-3)
+    assert list(water.transmitting_color) == [1.0, 1.0, 1.0]
+    assert beer.opacity == 0.0 and np.allclose(list(beer.transmitting_color), [0.8, 0.7, 0.55])
+    assert wood.opacity == 1.0 and wood.ior == 1.0
+
+
+def test_material_index_without_a_material_takes_the_default(tmp_path):
+    mats = [W.material(31, "Only", [])]
+    m = read(tmp_path, one_model_doc(QUAD, [[0, 1, 2], [0, 2, 3]], mats=[0, 5], material_objs=mats))
+    assert list(m.material) == [0, 1] and len(m.materials) == 2
+    m = read(tmp_path, one_model_doc(QUAD, [[0, 1, 2], [0, 2, 3]], mats=("AllSame", [0]), material_objs=mats))
+    assert list(m.material) == [0, 0] and list(m.mesh) == [0, 0]
+
+
+def png_rgb(w, h, rgb):
+    raw = b"".join(b"\0" + bytes(rgb) * w for _ in range(h))
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
+
+
+def test_textures_connect_by_property(tmp_path):
+    """DiffuseColor / NormalMap / EmissiveColor then EmissiveFactor (the later
+    one wins) / SpecularFactor; RelativeFilename with a backslash and a %20
+    escape; a .tga map is left empty, as Material::loadImageFromFile does."""
+    (tmp_path / "tex").mkdir()
+    (tmp_path / "tex" / "a b.png").write_bytes(png_rgb(2, 2, (10, 20, 30)))
+    (tmp_path / "n.png").write_bytes(png_rgb(1, 1, (128, 128, 255)))
+    (tmp_path / "e1.png").write_bytes(png_rgb(1, 1, (1, 1, 1)))
+    (tmp_path / "e2.png").write_bytes(png_rgb(1, 1, (200, 100, 50)))
+    (tmp_path / "s.png").write_bytes(png_rgb(1, 1, (0, 128, 255)))
+    mats = [W.material(31, "Painted", []), W.material(32, "Other", [])]
+    texs = [W.texture(41, "d", "tex\\a%20b.png"), W.texture(42, "n", "n.png"), W.texture(43, "e1", "e1.png"),
+            W.texture(44, "e2", "e2.png"), W.texture(45, "s", "s.png"), W.texture(46, "t", "x.tga")]
+    conns = [W.op(41, 31, "DiffuseColor"), W.op(42, 31, "NormalMap"), W.op(44, 31, "EmissiveFactor"),
+             W.op(43, 31, "EmissiveColor"), W.op(45, 31, "SpecularFactor"), W.op(46, 32, "DiffuseColor"),
+             W.op(41, 32, "AmbientColor")]
+    m = read(tmp_path, one_model_doc(QUAD, [[0, 1, 2], [0, 2, 3]], mats=[0, 1], material_objs=mats, extra=texs,
+                                     extra_conns=conns))
+    painted, other = m.materials
+    tex = m.textures
+    assert tex[painted.albedo_tex]["bytes"].tolist() == [[[10, 20, 30]] * 2] * 2
+    assert tex[painted.normal_tex]["bytes"].tolist() == [[[128, 128, 255]]]
+    assert tex[painted.emissive_tex]["bytes"].tolist() == [[[200, 100, 50]]]
+    assert tex[painted.specular_tex]["bytes"].tolist() == [[[0, 128, 255]]]
+    assert all(t["channels"] == 3 for t in tex) and len(tex) == 4
+    assert other.albedo_tex == -1 and other.normal_tex == -1 and other.emissive_tex == -1
+
+
+def test_instanced_geometry_is_copied_per_transform(tmp_path):
+    """Geometry g1 under models A (at the origin) and C (moved), g2 under B:
+    the meshes are g1 at A, g2 at B, then g1's copy at C; a fourth model D
+    with A's transform shares A's mesh (no second copy)."""
+    tri = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    objs = [W.model(1, "A"), W.model(2, "B", [W.vec("Lcl Translation", 0, 0, 3)]),
+            W.model(3, "C", [W.vec("Lcl Translation", 5, 0, 0)]), W.model(4, "D"),
+            W.geometry(11, "g1", tri, [[0, 1, 2]]), W.geometry(12, "g2", [(0, 0, 0), (0, 1, 0), (0, 0, 1)], [[0, 1, 2]])]
+    conns = [W.oo(1, 0), W.oo(2, 0), W.oo(3, 0), W.oo(4, 0), W.oo(11, 1), W.oo(12, 2), W.oo(11, 3), W.oo(11, 4)]
+    m = read(tmp_path, W.document(objs, conns))
+    assert list(m.mesh) == [0, 1, 2]
+    np.testing.assert_array_equal(m.verts[0], np.float32(tri))
+    np.testing.assert_array_equal(m.verts[1], np.float32([(0, 0, 3), (0, 1, 3), (0, 0, 4)]))
+    np.testing.assert_array_equal(m.verts[2], np.float32([(5, 0, 0), (6, 0, 0), (5, 1, 0)]))
+
+
+def test_templates_supply_missing_properties(tmp_path):
+    """A Model template with Lcl Scaling 2 scales a model that leaves it out;
+    a FbxSurfacePhong template with Opacity 0.25 makes a phong material
+    without its own Opacity a glass."""
+    templates = [("Model", "FbxNode", [W.vec("Lcl Scaling", 2, 2, 2)]),
+                 ("Material", "FbxSurfaceLambert", []),
+                 ("Material", "FbxSurfacePhong", [W.P("Opacity", "double", "Number", "", 0.25)])]
+    tri = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    objs = [W.model(10, "M"), W.geometry(20, "G", tri, [[0, 1, 2]]), W.material(31, "Ice", [])]
+    m = read(tmp_path, W.document(objs, [W.oo(10, 0), W.oo(20, 10), W.oo(31, 10)], templates=templates))
+    np.testing.assert_array_equal(m.verts[0], np.float32(tri) * 2)
+    ice = m.materials[0]
+    assert ice.opacity == 0.0 and ice.roughness == np.float32(0.5) and list(ice.transmitting_color) == [1, 1, 1]
+
+
+def test_object_ids_beyond_double_precision(tmp_path):
+    """Ids 2^62 + 1 and 2^62 + 2 name different objects (a double would merge
+    them)."""
+    a, b = 2 ** 62 + 1, 2 ** 62 + 2
+    tri = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    objs = [W.model(a, "A"), W.model(b, "B", [W.vec("Lcl Translation", 0, 0, 1)]), W.geometry(20, "G", tri, [[0, 1, 2]])]
+    m = read(tmp_path, W.document(objs, [W.oo(a, 0), W.oo(b, 0), W.oo(20, b)]))
+    np.testing.assert_array_equal(m.verts[0], np.float32([(0, 0, 1), (1, 0, 1), (0, 1, 1)]))
+
+
+def test_sky_and_scene_load_dispatch(tmp_path):
+    rgbe = bytes([128, 64, 32, 129])
+    (tmp_path / "sky.hdr").write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 2\n" + rgbe * 2)
+    m = read(tmp_path, one_model_doc(QUAD, [[0, 1, 2, 3]]), sky="sky.hdr")
+    assert m.sky.shape == (1, 2, 3) and m.sky[0, 0, 0] == np.float32(128 * 2.0 ** (129 - 136))
+
+
+def test_errors(tmp_path):
+    good = one_model_doc(QUAD, [[0, 1, 2, 3]])
+    assert code_of(tmp_path, good[:200]) == RM_ERR_INVALID                      # truncated
+    assert code_of(tmp_path, b"not an fbx at all" * 4) == RM_ERR_INVALID
+    assert code_of(tmp_path, b"") == RM_ERR_INVALID
+    assert code_of(tmp_path, b"; FBX 7.4.0 project file\nFBXHeaderExtension:  {\n}\n") == RM_ERR_UNSUPPORTED
+    old = bytearray(good)
+    old[23:27] = struct.pack("<I", 6100)
+    assert code_of(tmp_path, bytes(old)) == RM_ERR_UNSUPPORTED
+    assert code_of(tmp_path, one_model_doc(QUAD, [[0, 1, 2, 7]])) == RM_ERR_INVALID   # no control point 7
+    bad = bytearray(one_model_doc(QUAD, [[0, 1, 2, 3]], compress=True))
+    k = bad.find(b"x\x9c")                                                        # corrupt a deflate stream
+    bad[k + 2:k + 6] = b"\xff\xff\xff\xff"
+    assert code_of(tmp_path, bytes(bad)) == RM_ERR_INVALID
+    grown = bytearray(good)
+    struct.pack_into("<I", grown, 27, len(good) + 1000)                           # first record ends past the file
+    assert code_of(tmp_path, bytes(grown)) == RM_ERR_INVALID
+    assert code_of(tmp_path, W.document([W.model(10, "M")], [W.oo(10, 0)])) == RM_ERR_EMPTY
+    nrm = ([0, 0, 1], "ByPolygonVertex", "IndexToDirect", [0, 0, 3, 0])
+    assert code_of(tmp_path, one_model_doc(QUAD, [[0, 1, 2, 3]], normals=nrm)) == RM_ERR_INVALID
+    g = W.geometry(20, "G", QUAD, [[0, 1, 2, 3]])
+    g.kids[0] = W.Node("Vertices", W.Arr("d", [0.0] * 10))
+    assert code_of(tmp_path, W.document([W.model(10, "M"), g], [W.oo(10, 0), W.oo(20, 10)])) == RM_ERR_INVALID
+    (tmp_path / "m.3ds").write_bytes(b"")
+    with pytest.raises(RmError) as e:
+        rm.Model(str(tmp_path) + "/", "m.3ds")
+    assert e.value.code == RM_ERR_UNSUPPORTED
