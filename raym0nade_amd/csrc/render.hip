@@ -142,6 +142,9 @@ struct PathQ {
 // f7 dDdy  | medium n + job<<8              f8-f11 medium entries 0..15
 // (only the float4s holding live entries are written and read)
 
+// The medium stack travels beside the path (load_path / store_path take it
+// separately): its entries are indexed dynamically and so stay in scratch,
+// and inside Path they would keep the whole record there with them.
 struct Path {
     V3 o, d, T, p0;
     float W, rf;
@@ -149,13 +152,12 @@ struct Path {
     int depth, job;
     bool exclude;
     Diff diff;
-    Med med;
 };
 
 __device__ __forceinline__ float ub(uint32_t x) { return __uint_as_float(x); }
 __device__ __forceinline__ uint32_t bu(float x) { return __float_as_uint(x); }
 
-__device__ __forceinline__ void store_path(const PathQ &Q, uint32_t q, const Path &P) {
+__device__ __forceinline__ void store_path(const PathQ &Q, uint32_t q, const Path &P, const Med &med) {
     Q.f[0][q] = make_float4(P.o.x, P.o.y, P.o.z, ub(P.pix));
     Q.f[1][q] = make_float4(P.d.x, P.d.y, P.d.z, ub(P.smp));
     Q.f[2][q] = make_float4(P.T.x, P.T.y, P.T.z, P.W);
@@ -163,15 +165,13 @@ __device__ __forceinline__ void store_path(const PathQ &Q, uint32_t q, const Pat
     Q.f[4][q] = make_float4(P.diff.dPdx.x, P.diff.dPdx.y, P.diff.dPdx.z, ub(P.n));
     Q.f[5][q] = make_float4(P.diff.dPdy.x, P.diff.dPdy.y, P.diff.dPdy.z, ub(P.set));
     Q.f[6][q] = make_float4(P.diff.dDdx.x, P.diff.dDdx.y, P.diff.dDdx.z, ub((uint32_t)P.depth | (P.exclude ? 256u : 0u)));
-    Q.f[7][q] = make_float4(P.diff.dDdy.x, P.diff.dDdy.y, P.diff.dDdy.z, ub((uint32_t)P.med.n | ((uint32_t)P.job << 8)));
+    Q.f[7][q] = make_float4(P.diff.dDdy.x, P.diff.dDdy.y, P.diff.dDdy.z, ub((uint32_t)med.n | ((uint32_t)P.job << 8)));
 #pragma nounroll
-    for (int k = 0; 4 * k < P.med.n; ++k)
-        Q.f[8 + k][q] = make_float4(ub(P.med.e[4 * k]), ub(P.med.e[4 * k + 1]), ub(P.med.e[4 * k + 2]),
-                                    ub(P.med.e[4 * k + 3]));
+    for (int k = 0; 4 * k < med.n; ++k)
+        Q.f[8 + k][q] = make_float4(ub(med.e[4 * k]), ub(med.e[4 * k + 1]), ub(med.e[4 * k + 2]), ub(med.e[4 * k + 3]));
 }
 
-__device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
-    Path P;
+__device__ __forceinline__ void load_path(const PathQ &Q, uint32_t q, Path &P, Med &med) {
     float4 a = Q.f[0][q], b = Q.f[1][q], c = Q.f[2][q], d = Q.f[3][q];
     float4 e = Q.f[4][q], f = Q.f[5][q], g = Q.f[6][q], h = Q.f[7][q];
     P.o = V3{a.x, a.y, a.z}; P.pix = bu(a.w);
@@ -185,18 +185,16 @@ __device__ __forceinline__ Path load_path(const PathQ &Q, uint32_t q) {
     P.depth = (int)(dx & 255u);
     P.exclude = (dx & 256u) != 0;
     P.diff.dDdy = V3{h.x, h.y, h.z};
-    med_clear(P.med);
-    P.med.n = (int)(bu(h.w) & 255u);
+    med.n = (int)(bu(h.w) & 255u);
     P.job = (int)(bu(h.w) >> 8);
 #pragma nounroll
-    for (int k = 0; 4 * k < P.med.n; ++k) {
+    for (int k = 0; 4 * k < med.n; ++k) {
         const float4 m = Q.f[8 + k][q];
-        P.med.e[4 * k] = bu(m.x);
-        P.med.e[4 * k + 1] = bu(m.y);
-        P.med.e[4 * k + 2] = bu(m.z);
-        P.med.e[4 * k + 3] = bu(m.w);
+        med.e[4 * k] = bu(m.x);
+        med.e[4 * k + 1] = bu(m.y);
+        med.e[4 * k + 2] = bu(m.z);
+        med.e[4 * k + 3] = bu(m.w);
     }
-    return P;
 }
 
 // Per-pixel primary-vertex state kept between kernels.
@@ -647,6 +645,8 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
     uint32_t g = blockIdx.x * RB + threadIdx.x;
     bool want = false;
     Path path;
+    Med med;
+    med_clear(med);
     bool active = false;
     int64_t gp = 0;
     uint32_t s_idx = 0;
@@ -682,8 +682,6 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
                 P_reflect = 0.24f + (1.0f - 0.24f) * Fr;
             }
             Stream rng{A.seed, TAG_INDIRECT, (uint32_t)p, s_idx, 0u};
-            Med med;
-            med_clear(med);
             V3 nd, bp;
             int fails = 0;
             Diff next;
@@ -719,14 +717,13 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
                 path.depth = 1;
                 path.exclude = true;
                 path.diff = next;
-                path.med = med;
             }
     });
     uint32_t q = append(count, want);
-    if (want) store_path(Q, q, path);
+    if (want) store_path(Q, q, path, med);
     job_stat_add<1>(J, myjob, want ? 1u : 0u, RM_STAT_PATHS);
     job_stat_add<1>(J, myjob, ovf ? 1u : 0u, RM_STAT_MEDIUM_OVERFLOW);
-    job_stat_max<5>(J, myjob, want ? (uint32_t)path.med.n : 0u, RM_STAT_MEDIUM_MAX);
+    job_stat_max<5>(J, myjob, want ? (uint32_t)med.n : 0u, RM_STAT_MEDIUM_MAX);
 }
 
 // Light sample of a finished path -> its accumulator set.
@@ -955,8 +952,10 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
 
 // MINW: minimum resident waves per SIMD the register allocation must allow
 // (1: no cap; 4: at most 128 VGPRs, the rest spilled).  Chosen by tuning.
+// The 8-bit map code would take the uncapped variant one register past 256
+// (one wave per SIMD); it is held to two waves.
 template <int MINW, bool MAPS>
-__global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, int nj, PixBufs P, AccBufs AB, PathQ Qin,
+__global__ __launch_bounds__(RB, (MAPS && MINW < 2) ? 2 : MINW) void k_bounce(const Job *__restrict__ J, int nj, PixBufs P, AccBufs AB, PathQ Qin,
                                                const uint32_t *count_in, uint32_t *work, const uint2 *__restrict__ hit,
                                                PathQ Qout, uint32_t *count_out, NeeQ NQ, uint32_t *nee_count) {
   const uint32_t n_in = *count_in;
@@ -974,9 +973,11 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
     bool live = q < n_in;
     bool cont = false;
     Path path;
+    Med med;
+    med_clear(med);
     uint2 hq = make_uint2(0u, 0u);
     if (live) {
-        path = load_path(Qin, q);
+        load_path(Qin, q, path, med);
         hq = hit[q];
     }
     bool nee_out = false;
@@ -1016,16 +1017,16 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
             s.roughness = gmax(s.roughness, path.rf);
             const float P_RR = 1.0f + (0.5f - 1.0f) * sqrtf(s.roughness);
             bool doNEE = P_RR < 0.9f && depth >= A.rr_min;
-            V3 absorb = get_absorb(med_absorb(S, path.med), t);
+            V3 absorb = get_absorb(med_absorb(S, med), t);
             if (length(s.emission) > T_EPS) {
                 if (!path.exclude) emit(LSample{absorb, s.emission, 1.0f});
             } else {
                 float P_reflect = 1.0f, Fr = 0.0f;
                 if (s.opacity < T_EPS) {
-                    if (!calc_eta(S, path.med, s)) ++ovf;
+                    if (!calc_eta(S, med, s)) ++ovf;
                     V3 rd;
                     precise_refraction(s, V, rd, Fr);
-                    P_reflect = path.med.n == 0 ? 0.24f + (1.0f - 0.24f) * Fr : Fr;
+                    P_reflect = med.n == 0 ? 0.24f + (1.0f - 0.24f) * Fr : Fr;
                     P_reflect = gmax(P_reflect - 1e-3f, 0.0f);
                 }
                 V3 nd = vnan(), bp = vnan();
@@ -1034,7 +1035,7 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
                 bool nee = false;
                 float factor = 1.0f;
                 if (rng.next() <= P_reflect) {
-                    doNEE = doNEE && s.entering && path.med.n == 0;
+                    doNEE = doNEE && s.entering && med.n == 0;
                     if (doNEE && (depth >= maxD || rng.next() > P_RR)) {
                         nee = true;
                         factor = 1.0f / P_reflect;
@@ -1048,7 +1049,7 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
                         calc_dDdxy(path.d, s.surfN, path.diff, next.dDdx, next.dDdy);
                     }
                 } else {
-                    doNEE = doNEE && !s.entering && path.med.n == 1;
+                    doNEE = doNEE && !s.entering && med.n == 1;
                     if (doNEE && (depth >= maxD || rng.next() > P_RR)) {
                         nee = true;
                         factor = 1.0f / (1.0f - P_reflect);
@@ -1067,9 +1068,9 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
                         bp /= (1.0f - P_reflect);
                         if (doNEE) bp /= P_RR;
                         if (s.entering) {
-                            if (!med_insert(path.med, s.id, false)) ++ovf;
+                            if (!med_insert(med, s.id, false)) ++ovf;
                         } else {
-                            med_erase(path.med, s.id);
+                            med_erase(med, s.id);
                         }
                     }
                 }
@@ -1100,7 +1101,7 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
         }
     });
     uint32_t slot_out = append(count_out, cont);
-    if (cont) store_path(Qout, slot_out, path);
+    if (cont) store_path(Qout, slot_out, path, med);
     uint32_t slot_nee = append(nee_count, nee_out);
     if (nee_out) store_nee(NQ, slot_nee, path, nee_rec);
     if (nj > 1) {
@@ -1108,9 +1109,9 @@ __global__ __launch_bounds__(RB, MINW) void k_bounce(const Job *__restrict__ J, 
         job_stat_add<1>(J, myjob, live ? 1u : 0u, RM_STAT_EXTENSION_RAYS);
         job_stat_add<1>(J, myjob, hit_any ? 1u : 0u, RM_STAT_HITS);
         job_stat_add<2>(J, myjob, ovf, RM_STAT_MEDIUM_OVERFLOW);
-        job_stat_max<5>(J, cont ? path.job : -1, cont ? (uint32_t)path.med.n : 0u, RM_STAT_MEDIUM_MAX);
+        job_stat_max<5>(J, cont ? path.job : -1, cont ? (uint32_t)med.n : 0u, RM_STAT_MEDIUM_MAX);
     } else if (live) {
-        const uint32_t x[4] = {1u, hit_any ? 1u : 0u, ovf, cont ? (uint32_t)path.med.n : 0u};
+        const uint32_t x[4] = {1u, hit_any ? 1u : 0u, ovf, cont ? (uint32_t)med.n : 0u};
         st.note(J, keys, path.job, x);
     }
   }

@@ -66,18 +66,25 @@ __device__ __forceinline__ V3 barycentric(V3 v0, V3 v1, V3 v2, V3 P) {
 
 // getTangentSpace / getTangentSpaceWithInDir (src/geometry.cpp:105-123)
 __device__ __forceinline__ void tangent_space(V3 n, V3 &t, V3 &b) {
-    V3 v0 = fabsf(n.x) < 0.8f ? V3{1.0f, 0.0f, 0.0f} : V3{0.0f, 1.0f, 0.0f};
+    // per-component selects: a select between two vector temporaries would
+    // be made in memory (per-lane scratch)
+    const bool ax = fabsf(n.x) < 0.8f;
+    const V3 v0{ax ? 1.0f : 0.0f, ax ? 0.0f : 1.0f, 0.0f};
     t = normalize(cross(v0, n));
     b = cross(n, t);
 }
+// Both frames are computed and the result selected per component: with a
+// branch, the compiler merges the two branches' final stores (into t in one,
+// b in the other) into one store through a selected address, which keeps
+// the frames in per-lane scratch.  The selected values are the branch's.
 __device__ __forceinline__ void tangent_space_in(V3 n, V3 in, V3 &t, V3 &b) {
-    float c = dot(n, in);
-    if (c < -1.0f + 1e-3f) {
-        tangent_space(n, t, b);
-        return;
-    }
-    b = normalize(cross(n, in));
-    t = cross(b, n);
+    const bool opposite = dot(n, in) < -1.0f + 1e-3f;
+    V3 t1, b1;
+    tangent_space(n, t1, b1);
+    const V3 b2 = normalize(cross(n, in));
+    const V3 t2 = cross(b2, n);
+    t = V3{opposite ? t1.x : t2.x, opposite ? t1.y : t2.y, opposite ? t1.z : t2.z};
+    b = V3{opposite ? b1.x : b2.x, opposite ? b1.y : b2.y, opposite ? b1.z : b2.z};
 }
 
 // reverseFix (src/model.cpp:232-238)
@@ -895,27 +902,36 @@ __device__ __forceinline__ void acc_basic(Acc4 &a, V3 inr, float w) {
     a.r += x.x; a.g += x.y; a.b += x.z;
     a.v += dot(inr, inr) * w;
 }
+// The increments are found first and added by two calls with fixed targets:
+// one call whose target depends on the branch would address d or s through
+// a selected pointer, which keeps them in per-lane scratch.
 __device__ __forceinline__ void accumulate(V3 base, const LSample &smp, Acc4 &d, Acc4 &s) {
     if (length(smp.light) < T_EPS) return;
     V3 b0 = normalize(base);
+    bool to_d = true, to_s = true;
+    V3 xd = v3(0.0f), xs = v3(0.0f);
     if (length(base) < T_EPS) {
-        acc_basic(s, smp.light * smp.bsdf, smp.weight);
-        return;
+        to_d = false;
+        xs = smp.light * smp.bsdf;
+    } else {
+        const V3 white = normalize(v3(1.0f));
+        float xy = dot(b0, white);
+        if (xy > 0.99f) {
+            to_s = false;
+            xd = smp.light * smp.bsdf / base;
+        } else {
+            V3 perp = normalize(cross(b0, white));
+            V3 bp = smp.bsdf - perp * dot(perp, smp.bsdf);
+            float d1 = dot(bp, white), d2 = dot(bp, b0);
+            float apb = (d1 + d2) / (1 + xy), amb = (d1 - d2) / (1 - xy);
+            float B = (apb - amb) / 2.0f;
+            V3 bb = B * b0;
+            xd = smp.light * B / length(base);
+            xs = smp.light * (smp.bsdf - bb);
+        }
     }
-    const V3 white = normalize(v3(1.0f));
-    float xy = dot(b0, white);
-    if (xy > 0.99f) {
-        acc_basic(d, smp.light * smp.bsdf / base, smp.weight);
-        return;
-    }
-    V3 perp = normalize(cross(b0, white));
-    V3 bp = smp.bsdf - perp * dot(perp, smp.bsdf);
-    float d1 = dot(bp, white), d2 = dot(bp, b0);
-    float apb = (d1 + d2) / (1 + xy), amb = (d1 - d2) / (1 - xy);
-    float B = (apb - amb) / 2.0f;
-    V3 bb = B * b0;
-    acc_basic(d, smp.light * B / length(base), smp.weight);
-    acc_basic(s, smp.light * (smp.bsdf - bb), smp.weight);
+    if (to_d) acc_basic(d, xd, smp.weight);
+    if (to_s) acc_basic(s, xs, smp.weight);
 }
 
 }  // namespace rm
