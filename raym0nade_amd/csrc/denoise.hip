@@ -17,8 +17,13 @@
 //                 the G-buffer terms of a tap (normal power, depth slope,
 //                 material distance) are computed once and shared, only the
 //                 radiance-distance term is per buffer.
-// Every arithmetic step keeps the reference's operation order; the only
-// rounding difference left is the device expf against the host's.
+// Every arithmetic step keeps the reference's operation order.  spatialClamp
+// and filterVar round as the reference does.  In the a-trous taps the square
+// roots, divisions and exponentials use the hardware's approximate
+// instructions (v_sqrt, v_rsq, v_rcp, v_exp: about 1 ulp each, where the
+// exactly rounded sequences cost ten times as many instructions): the tap
+// weights move by about 1e-6 relative, and the filter is compared with the
+// reference within 1e-4 relative.
 #include <hip/hip_runtime.h>
 #include "rm_frame.h"
 #include "rm_vec.h"
@@ -129,9 +134,19 @@ __global__ __launch_bounds__(DB) void k_filter_var(Quad in, Quad out, int w, int
     out.b[b][i] = make_float4(c.x, c.y, c.z, V + E2 - dot(E, E));
 }
 
+// Approximate square root, reciprocal square root, reciprocal and e^x of the
+// a-trous taps (about 1 ulp; e^x as 2^(x log2 e)).
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
 // src/image.cpp:109-189 for the four buffers at once.  Buffers 1 and 3 (Ds,
 // Is) are the specular ones.
+// The taps' sums may use fused multiply-adds (one rounding fewer per term,
+// inside the same tolerance as the approximate instructions).
 __global__ __launch_bounds__(DB) void k_atrous(GView G, Quad in, Quad out, int w, int hgt, uint32_t n, int step) {
+#pragma clang fp contract(fast)
     uint32_t i = xcd_block(blockIdx.x, gridDim.x) * DB + threadIdx.x;
     if (i >= n) return;
     const V3 pp = ld3g(G.pos, i);
@@ -145,12 +160,12 @@ __global__ __launch_bounds__(DB) void k_atrous(GView G, Quad in, Quad out, int w
     const float mp = G.mat[(size_t)i * 3], rp = G.mat[(size_t)i * 3 + 1], op = G.mat[(size_t)i * 3 + 2];
     const float rw = (rp < 0.04f) ? 0.04f : rp;  // std::max(Gp.roughness, eps_r)
     float4 Lp[4];
-    float den[4], ws[4], va[4];
+    float inv_den[4], ws[4], va[4];
     V3 acc[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         Lp[b] = in.b[b][i];
-        den[b] = 1.0f * sqrtf(Lp[b].w) + 1e-2f;  // sigma_l * sqrt(Lp.Var) + eps
+        inv_den[b] = 1.0f / (1.0f * sqrtf(Lp[b].w) + 1e-2f);  // 1 / (sigma_l * sqrt(Lp.Var) + eps)
         ws[b] = 0.0f;
         va[b] = 0.0f;
         acc[b] = v3(0.0f);
@@ -178,23 +193,26 @@ __global__ __launch_bounds__(DB) void k_atrous(GView G, Quad in, Quad out, int w
                     const float c = dot(un, ld3g(G.surfN, q));
                     const float pw = pow1024((0.0f < c) ? c : 0.0f);
                     if (!(pw < 1e-6f)) {
-                        const V3 dir = normalize(pq - pp);
+                        const V3 dd = pq - pp;
+                        const V3 dir = dd * frsq(dot(dd, dd));
                         const float s = fabsf(dot(sn, dir));
-                        const float t = s / (sqrtf(1.0f - s * s) + 1e-4f);
+                        const float t = s * frcp(fsqrt(1.0f - s * s) + 1e-4f);
                         const float mq = G.mat[(size_t)q * 3], oq = G.mat[(size_t)q * 3 + 2];
-                        const float md = length(V3{mp - mq, 0.04f - 0.04f, op - oq});  // specular is 0.04 everywhere
+                        const V3 mv{mp - mq, 0.04f - 0.04f, op - oq};  // specular is 0.04 everywhere
+                        const float md = fsqrt(dot(mv, mv));
                         const float k1 = 0.0f + -t / 1.0f;
                         // k = (k1 - rd) - md <= k1 - md for rd >= 0 (rounding is monotonic): when
                         // k1 - md is already below the cut every buffer's weight is 0.
                         if (!(k1 + -md / 1.0f < -7.5f)) {
 #pragma unroll
                             for (int b = 0; b < 4; ++b) {
-                                const float rd = length(rgb(Lp[b]) - rgb(Lq[b])) / den[b];
+                                const V3 dl = rgb(Lp[b]) - rgb(Lq[b]);
+                                const float rd = fsqrt(dot(dl, dl)) * inv_den[b];
                                 float k = k1;
                                 k += -rd;
                                 k += -md / 1.0f;
                                 if (k < -7.5f) continue;
-                                float wgt = pw * expf(k);
+                                float wgt = pw * fexp(k);
                                 if (b & 1) wgt *= rw;
                                 g[b] = isfinite(wgt) ? wgt : 0.0f;
                             }
