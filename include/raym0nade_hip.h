@@ -177,14 +177,14 @@ int32_t rm_scene_load(const char *model_folder, const char *model_name, const ch
  *     Pm set the constant albedo, emission, roughness and metallic
  *     (extensions); map_Kd, norm / map_Kn, map_Ke / map_emissive and map_Ks
  *     are the diffuse, normal, emissive and specular maps.
- *   - FBX (binary, version 7100 and later), as assimp imports it with the
+ *   - FBX (binary or ASCII, version 7100 and later), as assimp imports it with the
  *     reference's flags (Triangulate | PreTransformVertices, hierarchy kept |
  *     SortByPType | FixInfacingNormals): one mesh per (geometry, material)
  *     in order of first use, vertices in world space, quads split at their
  *     concave corner, inward-facing meshes flipped; the maps connected as
  *     DiffuseColor, NormalMap, EmissiveColor / EmissiveFactor and
  *     SpecularColor / SpecularFactor; a material Opacity below 0.99 makes
- *     the glass.  ASCII FBX is RM_ERR_UNSUPPORTED.
+ *     the glass.
  * Other model formats are RM_ERR_UNSUPPORTED. */
 typedef struct rm_model rm_model;
 int32_t rm_model_read(const char *model_folder, const char *model_name, const char *sky_name, rm_scene_desc *desc,

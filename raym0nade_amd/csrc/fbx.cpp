@@ -1,4 +1,4 @@
-// Autodesk FBX models (binary FBX, version 7100 and later) as the reference
+// Autodesk FBX models (binary or ASCII FBX, version 7100 and later) as the reference
 // loads them through assimp (Model::Model, src/model.cpp:172-215): the
 // importer runs aiProcess_Triangulate | aiProcess_PreTransformVertices (with
 // AI_CONFIG_PP_PTV_KEEP_HIERARCHY = 1) | aiProcess_SortByPType |
@@ -16,6 +16,11 @@
 //     property bytes, name) with 32-bit fields before version 7500 and
 //     64-bit ones from 7500, nested lists closed by an all-zero record,
 //     typed properties, arrays raw or zlib-deflated;
+//   - the ASCII form of the same tree: "Name: value, value, ... { children }"
+//     with ';' comments, numbers, quoted strings, bare words and arrays
+//     "*count { a: v, v, ... }", object names "Class::Name" where the binary
+//     form has "Name\x00\x01Class", and the version in
+//     FBXHeaderExtension / FBXVersion;
 //   - objects and connections: a Model's geometries, its materials (in
 //     connection order) and its child models; a Material's textures by the
 //     property they connect to;
@@ -52,6 +57,9 @@
 // four corners can differ as well.
 #include <zlib.h>
 #include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cstdlib>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -95,12 +103,8 @@ class FbxReader {
 
     int32_t read(const std::string &path, std::vector<FbxNode> &top) {
         static const char magic[] = "Kaydara FBX Binary  ";
-        if (buf.size() < 27 || memcmp(buf.data(), magic, 21) != 0) {
-            const std::string head(buf.begin(), buf.begin() + std::min<size_t>(buf.size(), 4096));
-            if (head.find("FBXHeaderExtension") != std::string::npos || head.rfind("; FBX", 0) == 0)
-                return fail(RM_ERR_UNSUPPORTED, "%s: ASCII FBX is not read (binary FBX only)", path.c_str());
-            return fail(RM_ERR_INVALID, "%s: not an FBX file", path.c_str());
-        }
+        if (buf.size() < 27 || memcmp(buf.data(), magic, 21) != 0)
+            return fail(RM_ERR_INVALID, "%s: not a binary FBX file", path.c_str());
         uint32_t version;
         memcpy(&version, buf.data() + 23, 4);
         if (version < 7100)
@@ -272,6 +276,175 @@ class FbxReader {
     }
 };
 
+// The ASCII form: the same node tree as text.
+class FbxAsciiReader {
+  public:
+    explicit FbxAsciiReader(const std::vector<uint8_t> &b) : s((const char *)b.data()), n(b.size()) {}
+
+    int32_t read(const std::string &path, std::vector<FbxNode> &top) {
+        int32_t rc = list(top, false, 0);
+        if (rc) {
+            const std::string why = rm_last_error();  // fail() rewrites that buffer
+            return fail(rc, "%s: line %zu: %s", path.c_str(), line, why.c_str());
+        }
+        return RM_OK;
+    }
+
+  private:
+    const char *s;
+    size_t n, pos = 0, line = 1;
+
+    bool eof() const { return pos >= n; }
+    // spaces, tabs and (with nl) line ends and ';' comments
+    void skip(bool nl) {
+        while (pos < n) {
+            const char c = s[pos];
+            if (c == ' ' || c == '\t' || c == '\r') {
+                ++pos;
+            } else if (nl && c == '\n') {
+                ++pos;
+                ++line;
+            } else if (nl && c == ';') {
+                while (pos < n && s[pos] != '\n') ++pos;
+            } else {
+                break;
+            }
+        }
+    }
+    static bool word_char(char c) { return isalnum((unsigned char)c) || c == '_' || c == '|' || c == '-' || c == '.'; }
+
+    // one number as a property: integers exactly (ids are 64-bit), others as double
+    int32_t number(FbxProp &p) {
+        const char *b = s + pos;
+        char *e = nullptr;
+        size_t k = pos;
+        bool real = false;
+        if (k < n && (s[k] == '-' || s[k] == '+')) ++k;
+        while (k < n && (isdigit((unsigned char)s[k]) || s[k] == '.' || s[k] == 'e' || s[k] == 'E' ||
+                         ((s[k] == '-' || s[k] == '+') && (s[k - 1] == 'e' || s[k - 1] == 'E')))) {
+            if (!isdigit((unsigned char)s[k])) real = true;
+            ++k;
+        }
+        if (k == pos) return fail(RM_ERR_INVALID, "expected a value");
+        const std::string tok(b, k - pos);
+        if (!real) {
+            errno = 0;
+            const long long v = strtoll(tok.c_str(), &e, 10);
+            if (*e || errno) return fail(RM_ERR_INVALID, "bad integer '%s'", tok.c_str());
+            p.type = 'L';
+            p.iv = v;
+            p.num = (double)v;
+        } else {
+            const double v = strtod(tok.c_str(), &e);
+            if (*e) return fail(RM_ERR_INVALID, "bad number '%s'", tok.c_str());
+            p.type = 'D';
+            p.num = v;
+        }
+        pos = k;
+        return RM_OK;
+    }
+
+    int32_t value(FbxProp &p) {
+        if (eof()) return fail(RM_ERR_INVALID, "value past the end of the file");
+        const char c = s[pos];
+        if (c == '"') {
+            const size_t e = std::string(s + pos + 1, n - pos - 1).find('"');
+            if (e == std::string::npos) return fail(RM_ERR_INVALID, "unterminated string");
+            p.type = 'S';
+            p.str.assign(s + pos + 1, e);
+            pos += e + 2;
+            return RM_OK;
+        }
+        if (c == '*') {  // array: *count { a: v, v, ... }
+            ++pos;
+            FbxProp cnt;
+            int32_t rc = number(cnt);
+            if (rc) return rc;
+            if (cnt.type != 'L' || cnt.iv < 0 || cnt.iv > (1ll << 30)) return fail(RM_ERR_INVALID, "bad array size");
+            skip(true);
+            if (eof() || s[pos] != '{') return fail(RM_ERR_INVALID, "array without '{'");
+            ++pos;
+            skip(true);
+            if (pos + 1 < n && s[pos] == 'a' && s[pos + 1] == ':') {
+                pos += 2;
+                skip(true);
+            }
+            p.type = 'd';
+            bool all_int = true;
+            p.f.reserve((size_t)cnt.iv);
+            while (!eof() && s[pos] != '}') {
+                FbxProp v;
+                int32_t r2 = number(v);
+                if (r2) return r2;
+                if (v.type != 'L') all_int = false;
+                p.f.push_back(v.num);
+                p.i.push_back(v.iv);
+                skip(true);
+                if (!eof() && s[pos] == ',') {
+                    ++pos;
+                    skip(true);
+                }
+            }
+            if (eof()) return fail(RM_ERR_INVALID, "unterminated array");
+            ++pos;
+            if ((int64_t)p.f.size() != cnt.iv)
+                return fail(RM_ERR_INVALID, "array of %lld values holds %zu", (long long)cnt.iv, p.f.size());
+            if (!all_int) p.i.clear();  // an index array must hold integers
+            return RM_OK;
+        }
+        if (c == '-' || c == '+' || c == '.' || isdigit((unsigned char)c)) return number(p);
+        size_t k = pos;
+        while (k < n && word_char(s[k])) ++k;
+        if (k == pos) return fail(RM_ERR_INVALID, "unexpected '%c'", c);
+        p.type = 'S';  // a bare word (T, Y, W, ...)
+        p.str.assign(s + pos, k - pos);
+        pos = k;
+        return RM_OK;
+    }
+
+    int32_t node(FbxNode &nd, int depth) {
+        size_t k = pos;
+        while (k < n && word_char(s[k])) ++k;
+        if (k == pos || k >= n || s[k] != ':') return fail(RM_ERR_INVALID, "expected 'Name:'");
+        nd.name.assign(s + pos, k - pos);
+        pos = k + 1;
+        skip(false);
+        while (!eof() && s[pos] != '\n' && s[pos] != '{' && s[pos] != '}' && s[pos] != ';') {
+            FbxProp p;
+            int32_t rc = value(p);
+            if (rc) return rc;
+            nd.props.push_back(std::move(p));
+            skip(false);
+            if (!eof() && s[pos] == ',') {
+                ++pos;
+                skip(true);  // values go on after a comma, across lines
+            }
+        }
+        if (!eof() && s[pos] == '{') {
+            ++pos;
+            return list(nd.kids, true, depth + 1);
+        }
+        return RM_OK;
+    }
+
+    int32_t list(std::vector<FbxNode> &out, bool braced, int depth) {
+        if (depth > FBX_MAX_DEPTH) return fail(RM_ERR_INVALID, "records nested deeper than %d", FBX_MAX_DEPTH);
+        while (true) {
+            skip(true);
+            if (eof()) return braced ? fail(RM_ERR_INVALID, "unterminated '{'") : RM_OK;
+            if (s[pos] == '}') {
+                if (!braced) return fail(RM_ERR_INVALID, "unmatched '}'");
+                ++pos;
+                return RM_OK;
+            }
+            FbxNode nd;
+            int32_t rc = node(nd, depth);
+            if (rc) return rc;
+            out.push_back(std::move(nd));
+        }
+    }
+};
+
 // ---------------------------------------------------------------- transforms
 using M4 = std::array<double, 16>;  // row-major; points are columns: p' = M p
 
@@ -424,12 +597,14 @@ struct Scene {
 
 int64_t id_of(const FbxNode &n) { return n.props.empty() ? 0 : n.props[0].iv; }
 
-// "Name\x00\x01Class" -> "Name"
+// "Name\x00\x01Class" (binary) or "Class::Name" (ASCII) -> "Name"
 std::string object_name(const FbxNode &n) {
     if (n.props.size() < 2) return std::string();
     const std::string &s = n.props[1].str;
     const size_t k = s.find(std::string("\x00\x01", 2));
-    return k == std::string::npos ? s : s.substr(0, k);
+    if (k != std::string::npos) return s.substr(0, k);
+    const size_t c = s.find("::");
+    return c == std::string::npos ? s : s.substr(c + 2);
 }
 
 struct Converter {
@@ -834,7 +1009,26 @@ int32_t parse_fbx(const std::string &folder, const std::string &name, LoadedMode
         fclose(f);
     }
     std::vector<FbxNode> top;
-    int32_t rc = FbxReader(bytes).read(path, top);
+    static const char magic[] = "Kaydara FBX Binary  ";
+    int32_t rc;
+    if (bytes.size() >= 21 && memcmp(bytes.data(), magic, 21) == 0) {
+        rc = FbxReader(bytes).read(path, top);
+    } else {
+        const std::string head(bytes.begin(), bytes.begin() + std::min<size_t>(bytes.size(), 4096));
+        if (head.find("FBXHeaderExtension") == std::string::npos && head.rfind("; FBX", 0) != 0)
+            return fail(RM_ERR_INVALID, "%s: not an FBX file", path.c_str());
+        rc = FbxAsciiReader(bytes).read(path, top);
+        if (!rc) {  // FBXHeaderExtension / FBXVersion
+            int64_t version = 0;
+            for (const FbxNode &t : top)
+                if (t.name == "FBXHeaderExtension")
+                    if (const FbxNode *v = t.kid("FBXVersion"))
+                        if (!v->props.empty()) version = v->props[0].iv;
+            if (version < 7100)
+                return fail(RM_ERR_UNSUPPORTED, "%s: FBX version %lld (7100 or later is read)", path.c_str(),
+                            (long long)version);
+        }
+    }
     if (rc) return rc;
     std::vector<uint8_t>().swap(bytes);
     Scene sc;

@@ -12,7 +12,11 @@ bytes, and arrays f d l i b as u32 count, u32 encoding (0 raw, 1 zlib), u32
 byte length, data.
 
 Plain Python values: int -> I, float -> D, str / bytes -> S; wrap with L(),
-F(), Y(), C() or Arr() for the other types."""
+F(), Y(), C() or Arr() for the other types.
+
+fbx_ascii writes the same tree in the ASCII form: "Name: v, v { children }",
+';' comments, arrays as "*count { a: v,v,... }" over several lines, object
+names "Class::Name" for the binary "Name\x00\x01Class"."""
 import struct
 import zlib
 
@@ -112,6 +116,44 @@ def fbx_bytes(top, version=7400):
     return out
 
 
+def ascii_value(p):
+    if isinstance(p, Arr):
+        vals = [repr(float(v)) if p.kind in "fd" else str(int(v)) for v in p.values]
+        # values wrap over lines after a comma, eight to a line, as exporters write them
+        body = ",\n\t\t".join(",".join(vals[i:i + 8]) for i in range(0, len(vals), 8))
+        return f"*{len(p.values)} {{\n\t\ta: {body}\n\t}}"
+    if isinstance(p, C):
+        return "T" if p.v else "F"
+    if isinstance(p, (L, Y)) or (isinstance(p, int) and not isinstance(p, bool)):
+        return str(int(p))
+    if isinstance(p, bool):
+        return "T" if p else "F"
+    if isinstance(p, (F, float)):
+        return repr(float(p))
+    if isinstance(p, (bytes, bytearray)):
+        p = bytes(p).decode("latin-1")
+    if "\x00\x01" in p:  # binary object name "Name\x00\x01Class" -> "Class::Name"
+        name, cls = p.split("\x00\x01")
+        p = f"{cls}::{name}"
+    return '"' + p + '"'
+
+
+def ascii_node(node, depth):
+    pad = "\t" * depth
+    vals = ", ".join(ascii_value(p) for p in node.props)
+    out = f"{pad}{node.name}: {vals}"
+    if node.kids:
+        out += " {\n" + "".join(ascii_node(k, depth + 1) for k in node.kids) + f"{pad}}}\n"
+    else:
+        out += "\n"
+    return out
+
+
+def fbx_ascii(top, version=7400):
+    out = f"; FBX {version // 1000}.{version // 100 % 10}.{version % 100} project file\n; written by the tests\n\n"
+    return (out + "".join(ascii_node(n, 0) for n in top)).encode("latin-1")
+
+
 # ------------------------------------------------------------ scene helpers
 def obj_name(name, cls):
     return name.encode() + b"\x00\x01" + cls.encode()
@@ -192,7 +234,7 @@ def op(src, dst, prop):
     return Node("C", "OP", L(src), L(dst), prop)
 
 
-def document(objects, connections, templates=(), version=7400):
+def document(objects, connections, templates=(), version=7400, ascii=False):
     """templates: [(object type, template name, [P entries])]."""
     defs = [Node("Version", 100)]
     for otype, tname, entries in templates:
@@ -202,4 +244,4 @@ def document(objects, connections, templates=(), version=7400):
            Node("GlobalSettings", kids=[Node("Version", 1000), props70(P("UpAxis", "int", "Integer", "", 1))]),
            Node("Definitions", kids=defs), Node("Objects", kids=list(objects)),
            Node("Connections", kids=list(connections))]
-    return fbx_bytes(top, version)
+    return fbx_ascii(top, version) if ascii else fbx_bytes(top, version)

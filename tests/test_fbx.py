@@ -1,4 +1,4 @@
-"""Binary FBX import (rm_model_read; the reference reads FBX through assimp
+"""FBX import, binary and ASCII (rm_model_read; the reference reads FBX through assimp
 in Model::Model, src/model.cpp:172-215, with aiProcess_Triangulate |
 aiProcess_PreTransformVertices (hierarchy kept) | aiProcess_SortByPType |
 aiProcess_FixInfacingNormals).  assimp is absent here, so the files come from
@@ -7,7 +7,8 @@ each expectation is fixed by geometry: world positions from the FBX node
 transform written out with explicit matrices, normals by the inverse
 transpose, the quad split from the concave corner checked by area, the mesh
 and material order by first use, glass by the reference's name table
-(src/material.cpp:300-333).  No device is touched."""
+(src/material.cpp:300-333); the ASCII form of a tree reads to the same model
+as its binary form.  No device is touched."""
 import math
 import struct
 import zlib
@@ -402,3 +403,48 @@ def test_errors(tmp_path):
     with pytest.raises(RmError) as e:
         rm.Model(str(tmp_path) + "/", "m.3ds")
     assert e.value.code == RM_ERR_UNSUPPORTED
+
+
+def _ascii_docs():
+    """(name, binary, ascii): three documents (per-polygon materials with a
+    glass, a turned and pivoted model with a geometric shift, a concave quad)
+    in both encodings."""
+    tri = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    mats = [W.material(31, "Water", [W.P("Opacity", "double", "Number", "", 0.5)]), W.material(32, "Wood", [])]
+    docs = []
+    for name, kw in [("materials", dict(points=[(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0.5)],
+                                        polys=[[0, 1, 2], [0, 2, 3], [1, 2, 3]], mats=[1, 0, 1], material_objs=mats)),
+                     ("transform", dict(points=tri, polys=[[0, 1, 2]],
+                                        props=[W.vec("Lcl Rotation", 20, 35, -50), W.vec("RotationPivot", 0.3, -0.2, 0.1),
+                                               W.vec("Lcl Scaling", 1.5, 0.75, 2), W.vec("GeometricTranslation", 0.25, 0, 0)])),
+                     ("concave", dict(points=[(0, 0, 0), (2, 0, 0), (0.6, 0.6, 0), (0, 2, 0)], polys=[[0, 1, 2, 3]]))]:
+        args = dict(kw)
+        pts, polys = args.pop("points"), args.pop("polys")
+        objs = [W.model(2 ** 62 + 10, "M", args.pop("props", ())),
+                W.geometry(20, "G", pts, polys, None, None, args.get("mats"))]
+        objs += list(args.get("material_objs", ()))
+        conns = [W.oo(2 ** 62 + 10, 0), W.oo(20, 2 ** 62 + 10)] + [W.oo(m.props[0], 2 ** 62 + 10)
+                                                                   for m in args.get("material_objs", ())]
+        docs.append((name, W.document(objs, conns), W.document(objs, conns, ascii=True)))
+    return docs
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_ascii_fbx_reads_as_the_binary_form(tmp_path, case):
+    """The same tree written in the ASCII form (comments, values wrapped over
+    lines, Class::Name object names, a 64-bit model id) gives the same model
+    as its binary form, bit for bit."""
+    name, binary, text = _ascii_docs()[case]
+    a = read(tmp_path, binary, "b.fbx")
+    b = read(tmp_path, text, "a.fbx")
+    for k in ("verts", "uvs", "material", "mesh"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), (name, k)
+    assert [(m.opacity, m.ior, m.roughness) for m in a.materials] == [(m.opacity, m.ior, m.roughness) for m in b.materials]
+
+
+def test_ascii_fbx_errors(tmp_path):
+    _, _, text = _ascii_docs()[0]
+    assert code_of(tmp_path, text.replace(b"FBXVersion: 7400", b"FBXVersion: 6100")) == RM_ERR_UNSUPPORTED
+    assert code_of(tmp_path, text[:text.rfind(b"}")]) == RM_ERR_INVALID           # unterminated block
+    assert code_of(tmp_path, text.replace(b"*9 {", b"*10 {", 1)) == RM_ERR_INVALID  # count against values
+    assert code_of(tmp_path, text.replace(b"a: 0,1,-3", b"a: 0,1,x", 1)) == RM_ERR_INVALID
