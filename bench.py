@@ -337,13 +337,18 @@ def batch_leg(args, torch, rm):
             a.paths_per_round = tuned["paths_per_round"]
         log(f"[bench] batch autotune: {tuned}")
     rm.render_batch(jobs[:2], use_graph=True)  # warm-up
-    # every frame's own work buffers exist before the sequential timing (a
-    # low-spp render sizes them: the sizes do not depend on spp)
-    for S, a, fr, post in jobs:
-        b = rm.RmRenderArgs.from_buffer_copy(a)
-        b.spp = 4
-        fr.render(S, b)
-    torch.cuda.synchronize()
+
+    def size_buffers():
+        # every frame's own work buffers exist before a sequential timing, at
+        # that timing's launch parameters (a low-spp render sizes them: the
+        # sizes depend on the paths per round, not on spp)
+        for S, a, fr, post in jobs:
+            b = rm.RmRenderArgs.from_buffer_copy(a)
+            b.spp = 4
+            fr.render(S, b)
+        torch.cuda.synchronize()
+
+    size_buffers()
     t0 = time.perf_counter()
     rm.render_batch(jobs, use_graph=True)
     torch.cuda.synchronize()
@@ -368,6 +373,7 @@ def batch_leg(args, torch, rm):
     autotune.restore(LIBRARY_DEFAULTS)
     for _, a, _, _ in jobs:
         a.paths_per_round = 0
+    size_buffers()
     per_job = []
     ts = sequential(per_job)
     autotune.restore(saved)
@@ -665,7 +671,7 @@ def render_leg(args, torch, rm, dev, stream):
     elif not args.no_autotune:
         from raym0nade_amd import autotune
         tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 4, blocks=(4, 8), waves=(1, 4),
-                                     paths_per_round=(0, 4, 8), log=log)
+                                     paths_per_round=(0, 4, 8), full_paths=(8, 16), log=log)
         a.paths_per_round = tuned["paths_per_round"]
     times = []
     counted = None

@@ -74,7 +74,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
 def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4, 8),
                 walks=((3, 10, 2, 0), (3, 10, 0, 0), (3, 10, 1, 0), (3, 10, 4, 0), (3, 8, 2, 0), (3, 6, 2, 0),
                        (2, 32, 2, 0), (3, 10, 0, 32), (3, 10, 0, 40), (3, 10, 1, 32), (3, 10, 2, 32)),
-                log=None) -> dict:
+                full_paths=(4, 8), log=None) -> dict:
     """jobs: list of (Scene, RmRenderArgs, Frame, post_options) as for
     render_batch.  Times the batch (hipGraph launch) with every job's spp
     scaled by spp_scale: first every (blocks per CU, register cap) pair with
@@ -82,9 +82,12 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
     other paths-per-round values at the best pair, then the other walk
     settings (trace variant, LDS entries per lane, triangles per step, walk
     phases: variant 2 keeps the walk stacks in scratch, 3 their first entries
-    in LDS).  Sets
-    the launch parameters and returns the choice; the caller applies
-    paths_per_round (0 = automatic) to its jobs' arguments."""
+    in LDS).  Then paths per round once more, at the full spp, among the
+    scaled choice and full_paths: with fewer samples per pixel a round of
+    many paths is often the last, partly empty one, so the scaled batch
+    prefers fewer paths per round than the full one.  Sets the launch
+    parameters and returns the choice; the caller applies paths_per_round
+    (0 = automatic) to its jobs' arguments."""
     small = []
     for S, a, fr, post in jobs:
         b = RmRenderArgs.from_buffer_copy(a)
@@ -116,6 +119,15 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
     _, bpc, w, k, _ = min(results)
     results += [trial(bpc, w, k, walk) for walk in walks[1:]]
     t, bpc, w, k, walk = min(results)
+    full = []
+    for kk in dict.fromkeys((k,) + tuple(full_paths)):
+        for (S, a, fr, post), j in zip(jobs, small):
+            j[1].spp = a.spp
+        full.append((trial(bpc, w, kk, walk)[0], kk))
+    for (S, a, fr, post), j in zip(jobs, small):
+        j[1].spp = max(4, int(round(a.spp * spp_scale)))
+    if full:
+        k = min(full)[1]
     set_tuning(TUNE_RENDER_BLOCKS_PER_CU, bpc)
     set_tuning(TUNE_RENDER_MIN_WAVES, w)
     set_tuning(TUNE_TRACE_VARIANT, walk[0])
@@ -124,7 +136,8 @@ def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_pe
     set_tuning(TUNE_WALK_PHASES, walk[3] if len(walk) > 3 else 0)
     return {"blocks_per_cu": bpc, "min_waves": w, "paths_per_round": k, "walk_variant": walk[0],
             "lds_stack_entries": walk[1], "tris_per_step": walk[2], "walk_phases": walk[3] if len(walk) > 3 else 0,
-            "seconds_scaled": t, "spp_scale": spp_scale, "tried": len(results)}
+            "seconds_scaled": t, "spp_scale": spp_scale, "tried": len(results),
+            "full_spp_seconds": {str(kk): round(tt, 4) for tt, kk in full}}
 
 
 def tune_adjoint(renderer, params, d_image, splits=(1, 2, 4, 8), log=None) -> dict:

@@ -1473,7 +1473,9 @@ struct HostJob {
 };
 
 // all_px: pixels of the whole launch; the automatic paths-per-round keeps
-// the live accumulator sets of a launch near 4M (468 B each).
+// the live accumulator sets of a launch near 16M (a single 1080p frame: 8
+// paths per round; more rounds of fewer paths cost a launch sequence each,
+// and every path slot holds about 1 KB of work buffers).
 static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, int64_t all_px, const char *fn) {
     const rm_render_args *a = h.args;
     rm_frame *f = h.frame;
@@ -1493,7 +1495,7 @@ static int32_t make_job(const HostJob &h, Job &J, int &rounds, int &maxd, int64_
     RArgs &A = J.A;
     A = rargs_from(a);
     int K = a->paths_per_round;
-    if (K <= 0) K = (int)std::min<int64_t>(16, std::max<int64_t>(1, (int64_t)(4 << 20) / std::max<int64_t>(1, all_px)));
+    if (K <= 0) K = (int)std::min<int64_t>(16, std::max<int64_t>(1, (int64_t)(16 << 20) / std::max<int64_t>(1, all_px)));
     A.K = K;
     const int spp_direct = (int)(float(a->spp) * a->p_direct);
     const int n_opaque = a->spp - spp_direct;

@@ -57,7 +57,13 @@ struct DevBuf {
         window_calls = 0;
         if (n == 0) n = 16;
         hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) { p = nullptr; return hip_fail(e, "hipMalloc"); }
+        if (e != hipSuccess) {
+            // the runtime keeps a failed allocation as its last error; take it
+            // so that the next launch check does not report it again
+            (void)hipGetLastError();
+            p = nullptr;
+            return hip_fail(e, "hipMalloc");
+        }
         bytes = n;
         return RM_OK;
     }

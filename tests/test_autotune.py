@@ -81,3 +81,35 @@ def test_tune_render_skips_candidates_that_run_out_of_memory(monkeypatch):
                                  paths_per_round=(0,), walks=((3, 8, 0),))
     finally:
         autotune.restore(saved)
+
+
+def test_tune_render_decides_paths_per_round_at_the_full_spp(monkeypatch):
+    """At the scaled spp 4 paths per round are fastest, at the job's own spp
+    8: the choice is made at the full spp (and the trials there really run
+    with the job's spp)."""
+    calls = []
+
+    def fake_batch(jobs, use_graph=True):
+        calls.append((jobs[0][1].paths_per_round, jobs[0][1].spp))
+
+    def fake_time(fn, reps):
+        fn()
+        k, spp = calls[-1]
+        if spp == 64:
+            return {4: 2.0, 8: 1.5}.get(k, 3.0)
+        return {4: 0.5}.get(k, 1.0)
+
+    monkeypatch.setattr(autotune, "render_batch", fake_batch)
+    monkeypatch.setattr(autotune, "_time", fake_time)
+    saved = autotune.current()
+    try:
+        a = rm.RmRenderArgs()
+        a.spp = 64
+        choice = autotune.tune_render([(None, a, None, 0)], blocks=(4,), waves=(1,), paths_per_round=(0, 4, 8),
+                                      walks=((3, 8, 0),), full_paths=(4, 8))
+        assert choice["paths_per_round"] == 8
+        assert {k for k, spp in calls if spp == 64} == {4, 8}
+        assert {spp for _, spp in calls} == {8, 64}  # 64 x 0.125, and the full spp
+        assert choice["full_spp_seconds"] == {"4": 2.0, "8": 1.5}
+    finally:
+        autotune.restore(saved)
