@@ -481,7 +481,9 @@ enum rm_tune_key {
                                            origins are far apart, 2 always, 3 as 1 with a radix sort
                                            of the bins (rocPRIM) instead of the counting sort, 4 as 3
                                            with the library's own radix sort, whose kernels do no work
-                                           on a batch left unsorted (default) */
+                                           on a batch left unsorted, 5 as 4 with a closest-hit batch of
+                                           shared origins (a camera's) sorted by direction into small
+                                           tiles instead (default) */
     RM_TUNE_RENDER_BLOCKS_PER_CU = 4,   /* blocks per CU of the persistent bounce kernel (default 4) */
     RM_TUNE_ADJOINT_SPLIT = 5,          /* lanes per pixel of the adjoint, each replaying a share of
                                            the pixel's samples (default 4) */

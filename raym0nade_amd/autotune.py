@@ -46,9 +46,9 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
     if candidates is None:
         candidates = [(0, 8, 16, s, 32, 0, 0) for s in (0, 4)]
         candidates += [(2, b, f, s, 32, 0, 0) for b, f, s in itertools.product((4, 8), (32, 48), (0, 4))]
-        candidates += [(3, b, f, 4, e, 0, 0) for b, f, e in itertools.product((4, 8), (32, 48), (6, 8, 12, 32))]
+        candidates += [(3, b, f, 5, e, 0, 0) for b, f, e in itertools.product((4, 8), (32, 48), (6, 8, 12, 32))]
         candidates += [(3, b, f, s, e, p, ph) for b, f, e, p, s, ph in
-                       itertools.product((8,), (32, 48), (8, 10), (0, 2, 4), (1, 3, 4), (24, 32, 40))]
+                       itertools.product((8,), (32, 48), (8, 10), (0, 2, 4), (1, 3, 4, 5), (24, 32, 40))]
 
     def run():
         for r in batches:

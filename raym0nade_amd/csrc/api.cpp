@@ -65,7 +65,7 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
             t.fetch_min = value;
             return RM_OK;
         case RM_TUNE_RAY_SORT:
-            if (value < 0 || value > 4) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: ray sort %d", value);
+            if (value < 0 || value > 5) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: ray sort %d", value);
             t.ray_sort = value;
             return RM_OK;
         case RM_TUNE_RENDER_BLOCKS_PER_CU:

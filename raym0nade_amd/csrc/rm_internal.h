@@ -132,10 +132,11 @@ struct Tuning {
     int trace_variant = 3;        // 0 stack walk, 2 persistent step walk, 3 same with LDS stacks
     int persistent_blocks_per_cu = 8;
     int fetch_min = 48;           // idle lanes before a wave claims new rays
-    int ray_sort = 4;             // 0 never, 1 when consecutive origins are far apart, 2 always,
+    int ray_sort = 5;             // 0 never, 1 when consecutive origins are far apart, 2 always,
                                   // 3 as 1 with a radix sort (rocPRIM) instead of the counting sort,
                                   // 4 as 3 with the library's own radix sort (skipped on the device
-                                  // for a batch left unsorted)
+                                  // for a batch left unsorted), 5 as 4 with closest-hit batches
+                                  // of shared origins sorted by direction
     int render_blocks_per_cu = 4; // resident blocks per CU of the persistent bounce kernel
     int adjoint_split = 4;        // lanes per pixel of the adjoint kernel
     int kernel_timing = 0;        // record HIP events around the trace kernels (rm_last_trace_ms)

@@ -228,12 +228,41 @@ __device__ __forceinline__ uint32_t ray_bin(const SceneView &S, V3 o, V3 d) {
     return (m << 3) | oct;
 }
 
+// Octahedral map of a direction into [-1, 1]^2.
+__device__ __forceinline__ void oct_map(V3 d, float &u, float &v) {
+    const float a = fabsf(d.x) + fabsf(d.y) + fabsf(d.z);
+    u = a > 0.0f ? d.x / a : 0.0f;
+    v = a > 0.0f ? d.y / a : 0.0f;
+    if (d.z < 0.0f) {
+        const float uu = (1.0f - fabsf(v)) * (u < 0.0f ? -1.0f : 1.0f);
+        v = (1.0f - fabsf(u)) * (v < 0.0f ? -1.0f : 1.0f);
+        u = uu;
+    }
+}
+
+// Direction bin of a ray whose batch shares its origins: the Morton code of
+// its octahedral map over the batch's range `box` (u0, v0, 512 / width,
+// 512 / height), 9 bits per axis, so that consecutive bins are neighbouring
+// directions (a camera batch falls into small square tiles).
+__device__ __forceinline__ uint32_t dir_bin(V3 d, const float *box) {
+    float u, v;
+    oct_map(d, u, v);
+    int qu = (int)((u - box[0]) * box[2]), qv = (int)((v - box[1]) * box[3]);
+    qu = qu < 0 ? 0 : (qu > 511 ? 511 : qu);
+    qv = qv < 0 ? 0 : (qv > 511 ? 511 : qv);
+    uint32_t m = 0;
+    for (int b = 0; b < 9; ++b) m |= (((uint32_t)qu >> b) & 1u) << (2 * b) | (((uint32_t)qv >> b) & 1u) << (2 * b + 1);
+    return m;
+}
+
 // flag = 1 when the mean distance between consecutive ray origins (4096
-// evenly spaced pairs) exceeds `frac` of the root box diagonal.
+// evenly spaced pairs) exceeds `frac` of the root box diagonal; otherwise 2
+// when `box` is given (the batch is then sorted by direction: box receives
+// the range of the sampled directions' octahedral map for dir_bin), else 0.
 __global__ __launch_bounds__(256) void k_coherence(SceneView S, const float *__restrict__ rays, int64_t n, float frac,
-                                                   int *__restrict__ flag) {
-    __shared__ float part[4];
-    float acc = 0.0f;
+                                                   int *__restrict__ flag, float *__restrict__ box = nullptr) {
+    __shared__ float part[4], lo_u[4], hi_u[4], lo_v[4], hi_v[4];
+    float acc = 0.0f, u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY;
     if (n >= 2) {
         for (int s = threadIdx.x; s < 4096; s += 256) {
             int64_t k = (int64_t)s * (n - 2) / 4095;
@@ -241,15 +270,44 @@ __global__ __launch_bounds__(256) void k_coherence(SceneView S, const float *__r
             load_ray(rays, k, o0, d0);
             load_ray(rays, k + 1, o1, d1);
             acc += length(o1 - o0);
+            if (box) {
+                float u, v;
+                oct_map(d0, u, v);
+                u0 = fminf(u0, u), u1 = fmaxf(u1, u), v0 = fminf(v0, v), v1 = fmaxf(v1, v);
+                oct_map(d1, u, v);
+                u0 = fminf(u0, u), u1 = fmaxf(u1, u), v0 = fminf(v0, v), v1 = fmaxf(v1, v);
+            }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    for (int off = 32; off > 0; off >>= 1) {
+        acc += __shfl_down(acc, off, 64);
+        u0 = fminf(u0, __shfl_down(u0, off, 64));
+        u1 = fmaxf(u1, __shfl_down(u1, off, 64));
+        v0 = fminf(v0, __shfl_down(v0, off, 64));
+        v1 = fmaxf(v1, __shfl_down(v1, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        part[w] = acc, lo_u[w] = u0, hi_u[w] = u1, lo_v[w] = v0, hi_v[w] = v1;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         float mean = (part[0] + part[1] + part[2] + part[3]) / 4096.0f;
         V3 ext{S.hi[0] - S.lo[0], S.hi[1] - S.lo[1], S.hi[2] - S.lo[2]};
-        *flag = (n >= 2 && mean > frac * length(ext)) ? 1 : 0;
+        const bool spread = n >= 2 && mean > frac * length(ext);
+        *flag = spread ? 1 : (box && n >= 2 ? 2 : 0);
+        if (box) {
+            for (int w = 1; w < 4; ++w) {
+                lo_u[0] = fminf(lo_u[0], lo_u[w]), hi_u[0] = fmaxf(hi_u[0], hi_u[w]);
+                lo_v[0] = fminf(lo_v[0], lo_v[w]), hi_v[0] = fmaxf(hi_v[0], hi_v[w]);
+            }
+            // 512 cells over the sampled range; a degenerate range gets one cell
+            const float du = hi_u[0] - lo_u[0], dv = hi_v[0] - lo_v[0];
+            box[0] = lo_u[0];
+            box[1] = lo_v[0];
+            box[2] = du > 0.0f ? 512.0f / du : 0.0f;
+            box[3] = dv > 0.0f ? 512.0f / dv : 0.0f;
+        }
     }
 }
 
@@ -277,14 +335,15 @@ __global__ __launch_bounds__(256) void k_bin_hist(SceneView S, const float *__re
 // whose kernels then read nothing)
 __global__ __launch_bounds__(256) void k_bin_keys(SceneView S, const float *__restrict__ rays, int64_t n,
                                                   const int *__restrict__ flag, uint32_t *__restrict__ keys,
-                                                  uint32_t *__restrict__ vals, int skip = 0) {
+                                                  uint32_t *__restrict__ vals, int skip = 0,
+                                                  const float *__restrict__ box = nullptr) {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n || (skip && !*flag)) return;
     uint32_t b = 0;
     if (*flag) {
         V3 o, d;
         load_ray(rays, i, o, d);
-        b = ray_bin(S, o, d);
+        b = *flag == 2 ? dir_bin(d, box) : ray_bin(S, o, d);
     }
     keys[i] = b;
     vals[i] = (uint32_t)i;
@@ -509,7 +568,7 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const int *__restrict__ flag
             const uint32_t kk = lk[j];
             const int64_t g = gbase[(kk >> shift) & (RS_DIGITS - 1)] + j;
             if (g >= 0 && g < n) {
-                kout[g] = kk;
+                if (kout) kout[g] = kk;  // the last pass needs the values only
                 vout[g] = lv[j];
             }
         }
@@ -641,7 +700,7 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
     int *flag = nullptr;
     char *ws = nullptr;
     std::unique_lock<std::mutex> ws_lock;
-    if (T.ray_sort == 4 && n >= 4096) {
+    if ((T.ray_sort == 4 || T.ray_sort == 5) && n >= 4096) {
         // the library's radix sort: keys, keys2, values, values2, flag, tile histograms, chunk totals
         const size_t plane = ((size_t)n * 4 + 255) & ~(size_t)255;
         const uint32_t nt = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
@@ -671,12 +730,16 @@ static int32_t launch_trace(const rm_scene *s, const float *rays, int64_t n, flo
         uint32_t *hist = (uint32_t *)(base + 4 * plane + 256);
         uint32_t *part = (uint32_t *)(base + 4 * plane + 256 + hbytes);
         const unsigned gn = (unsigned)((n + 255) / 256);
-        hipLaunchKernelGGL(k_coherence, dim3(1), dim3(256), 0, st, s->view, rays, n, 0.02f, flag);
-        hipLaunchKernelGGL(k_bin_keys, dim3(gn), dim3(256), 0, st, s->view, rays, n, (const int *)flag, keys, vals, 1);
+        // mode 5: flag, then the direction range of a coherent batch, in the flag's 256-byte slot;
+        // closest-hit calls only (any-hit walks stop at the first hit and do not pay for the sort)
+        float *box = (T.ray_sort == 5 && !ANY) ? reinterpret_cast<float *>(flag + 1) : nullptr;
+        hipLaunchKernelGGL(k_coherence, dim3(1), dim3(256), 0, st, s->view, rays, n, 0.02f, flag, box);
+        hipLaunchKernelGGL(k_bin_keys, dim3(gn), dim3(256), 0, st, s->view, rays, n, (const int *)flag, keys, vals, 1,
+                           (const float *)box);
         for (int pass = 0; pass < 2; ++pass) {
             const int shift = pass * RS_BITS;
             const uint32_t *kin = pass ? keys2 : keys, *vin = pass ? vals2 : vals;
-            uint32_t *kout = pass ? keys : keys2, *vout = pass ? vals : vals2;
+            uint32_t *kout = pass ? nullptr : keys2, *vout = pass ? vals : vals2;
             hipLaunchKernelGGL(k_rs_hist, dim3(nt), dim3(256), 0, st, (const int *)flag, kin, n, shift, nt, hist);
             hipLaunchKernelGGL(k_rs_scan_chunks, dim3(nc), dim3(256), 0, st, (const int *)flag, hist, m, part);
             hipLaunchKernelGGL(k_rs_scan_parts, dim3(1), dim3(256), 0, st, (const int *)flag, part, nc);

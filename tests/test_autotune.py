@@ -20,7 +20,7 @@ def test_tuning_keys_round_trip_and_reject_out_of_range():
                                (rm.TUNE_RENDER_COUNT, (1, 0), (-1, 2)),
                                (rm.TUNE_TRACE_UTIL, (1, 0), (-1, 2)),
                                (rm.TUNE_WALK_PHASES, (64, 32, 0), (-1, 65)),
-                               (rm.TUNE_RAY_SORT, (0, 1, 2, 3, 4), (-1, 5))]:
+                               (rm.TUNE_RAY_SORT, (0, 1, 2, 3, 4, 5), (-1, 6))]:
             for v in good:
                 rm.set_tuning(key, v)
                 assert rm.get_tuning(key) == v

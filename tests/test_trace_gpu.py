@@ -32,14 +32,17 @@ def run_any(S, r, dev, aim=np.inf, aims=None):
 @pytest.fixture(params=[(0, 0, 32, 0, 0), (0, 2, 32, 0, 0), (2, 1, 32, 0, 0), (2, 2, 32, 0, 0),
                         (3, 0, 32, 0, 0), (3, 1, 2, 0, 0), (3, 1, 8, 1, 0), (3, 1, 2, 2, 0),
                         (3, 0, 8, 4, 0), (3, 3, 8, 2, 0), (0, 3, 32, 0, 0),
-                        (3, 3, 10, 2, 32), (3, 0, 2, 1, 16), (3, 4, 10, 0, 32), (0, 4, 32, 0, 0)],
+                        (3, 3, 10, 2, 32), (3, 0, 2, 1, 16), (3, 4, 10, 0, 32), (0, 4, 32, 0, 0),
+                        (3, 5, 10, 0, 32), (3, 5, 2, 2, 0)],
                 ids=["stack", "stack-sorted", "persistent-adaptive", "persistent-sorted",
                      "persistent-lds", "persistent-lds2-spill", "lds8-1tri", "lds2-spill-2tri", "lds8-4tri",
                      "lds8-2tri-radix-sorted", "stack-radix-sorted", "lds10-2tri-phases", "lds2-spill-1tri-phases",
-                     "lds10-leaf-phases-own-radix", "stack-own-radix"])
+                     "lds10-leaf-phases-own-radix", "stack-own-radix", "lds10-leaf-phases-dir-sorted",
+                     "lds2-spill-2tri-dir-sorted"])
 def variant(request):
     """Every traversal kernel variant, with and without ray reordering (the
-    counting sort, the rocPRIM radix sort, the library's own radix sort), with
+    counting sort, the rocPRIM radix sort, the library's own radix sort, and
+    that with shared-origin batches sorted by direction), with
     the walk stack in scratch, in LDS, or split between them (2 LDS entries
     per lane, the rest in scratch), testing a whole leaf or 1, 2, 4 triangles
     per step, and stepping every lane or the majority kind of lanes per

@@ -45,8 +45,8 @@ def both(S, rays):
     return (t0.cpu().numpy(), i0.cpu().numpy()), (t1.cpu().numpy(), i1.cpu().numpy())
 
 
-@pytest.fixture(params=[3, 4, 1, 2, 0], ids=["radix-sorted", "own-radix-sorted", "counting-sorted", "always-sorted",
-                                              "unsorted"])
+@pytest.fixture(params=[3, 4, 5, 1, 2, 0], ids=["radix-sorted", "own-radix-sorted", "direction-sorted",
+                                                 "counting-sorted", "always-sorted", "unsorted"])
 def ray_sort(request):
     old = rm.get_tuning(rm.TUNE_RAY_SORT)
     rm.set_tuning(rm.TUNE_RAY_SORT, request.param)
