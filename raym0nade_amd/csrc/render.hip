@@ -529,6 +529,54 @@ __device__ __forceinline__ uint32_t append(uint32_t *counter, bool want) {
     return base + rank;
 }
 
+// Block-wide append: one atomic per block instead of one per wave (appends
+// to one counter from every wave of the launch serialise on its address).
+// Every thread of the block calls it together; s holds RB / 64 + 2 words of
+// LDS, and keeps the block's base and total in its last two.  Returns the
+// slot of a lane that wants one.
+__device__ __forceinline__ uint32_t append_block(uint32_t *counter, bool want, uint32_t *s) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t mask = __ballot(want);
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) s[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < RB / 64; ++w) {
+            const uint32_t c = s[w];
+            s[w] = t;
+            t += c;
+        }
+        s[RB / 64] = t ? atomicAdd(counter, t) : 0u;
+        s[RB / 64 + 1] = t;
+    }
+    __syncthreads();
+    return want ? s[RB / 64] + s[wave] + rank : 0xffffffffu;
+}
+
+// Work claims of the persistent item kernels: a wave takes 64 queue items
+// per iteration, claimed `per` batches at a time with one atomic (per up to
+// 4 when the queue holds that many batches for every wave of the launch, so
+// short queues still spread over all waves).  Wave-uniform.
+struct Claims {
+    uint32_t next = 0, end = 0, per;
+    __device__ __forceinline__ explicit Claims(uint32_t n) {
+        const uint32_t waves = gridDim.x * (RB / 64u);
+        per = (uint32_t)max(1, min(4, (int)(n / (64u * waves))));
+    }
+    __device__ __forceinline__ uint32_t take(uint32_t *work) {
+        if (next >= end) {
+            uint32_t b = 0;
+            if ((threadIdx.x & 63u) == 0) b = atomicAdd(work, 64u * per);
+            next = __shfl(b, 0, 64);
+            end = next + 64u * per;
+        }
+        const uint32_t r = next;
+        next += 64u;
+        return r;
+    }
+};
+
 // Queues the shadow ray of light-sample entry e, or traces it here for a
 // shallow tree; sets ref[e].
 template <int TPS>
@@ -558,39 +606,71 @@ __device__ __forceinline__ void shadow_entry(const ShadowQ &Q, int64_t e, const 
 template <int MINW, int TPS>
 __global__ __launch_bounds__(RB, MINW) void k_direct_gen(const Job *__restrict__ J, int nj, int64_t npx, PixBufs P,
                                                    int C, int t0, ShadowQ SQ, int queue_depth, uint32_t *queued) {
+    __shared__ uint32_t s_app[RB / 64 + 2];
     const int64_t g = (int64_t)blockIdx.x * RB + threadIdx.x;
-    if (g >= npx * C) return;
-    const int64_t gp = g / C;
-    const int T = t0 + (int)(g - gp * C);
-    const Job &jb = J[job_of(J, nj, gp)];
+    const bool in = g < npx * C;
+    const int64_t gp = in ? g / C : 0;
+    const int jid = job_of(J, nj, gp);
+    const Job &jb = J[jid];
     const SceneView &S = jb.S;
-    const RArgs &A = jb.A;
-    const FrameView &F = jb.F;
-    const int64_t p = gp - jb.pix_off;
-    const int cnt = (int)(float(A.spp) * A.p_direct);
     bool ok = false;
-    if (T < cnt) {
-        V3 pos = ld3(F.pos + p * 3);
-        V3 em = ld3(F.emis + p * 3);
-        if (any_finite(pos) && !(length(em) > 0.0f)) {
-            Surf s = pixel_surf(P, F, gp, p);
-            V3 V = -normalize(s.pos - A.pos);
-            float total = direct_total(S, s, V);
-            if (total != 0.0f) {
-                Stream rng{A.seed, TAG_DIRECT, (uint32_t)p, (uint32_t)T, 0u};
-                ShadowRay sr;
-                LSample ls;
-                if (direct_ray(S, s, V, total, cnt, rng, sr, ls)) {
-                    SQ.x[g] = make_float4(ls.bsdf.x, ls.bsdf.y, ls.bsdf.z, ls.weight);
-                    SQ.x2[g] = make_float4(ls.light.x, ls.light.y, ls.light.z, 0.0f);
-                    shadow_entry<TPS>(SQ, g, S, (int)(&jb - J), s.pos, sr, queue_depth, queued);
-                    ok = true;
+    ShadowRay sr;
+    V3 o;
+    if (in) {
+        const int T = t0 + (int)(g - gp * C);
+        const RArgs &A = jb.A;
+        const FrameView &F = jb.F;
+        const int64_t p = gp - jb.pix_off;
+        const int cnt = (int)(float(A.spp) * A.p_direct);
+        if (T < cnt) {
+            V3 pos = ld3(F.pos + p * 3);
+            V3 em = ld3(F.emis + p * 3);
+            if (any_finite(pos) && !(length(em) > 0.0f)) {
+                Surf s = pixel_surf(P, F, gp, p);
+                V3 V = -normalize(s.pos - A.pos);
+                float total = direct_total(S, s, V);
+                if (total != 0.0f) {
+                    Stream rng{A.seed, TAG_DIRECT, (uint32_t)p, (uint32_t)T, 0u};
+                    LSample ls;
+                    if (direct_ray(S, s, V, total, cnt, rng, sr, ls)) {
+                        SQ.x[g] = make_float4(ls.bsdf.x, ls.bsdf.y, ls.bsdf.z, ls.weight);
+                        SQ.x2[g] = make_float4(ls.light.x, ls.light.y, ls.light.z, 0.0f);
+                        o = s.pos;
+                        ok = true;
+                    }
                 }
             }
         }
     }
-    if (!ok) SQ.ref[g] = SH_NONE;
-    job_stat_add<1>(J, (int)(&jb - J), ok ? 1u : 0u, RM_STAT_SHADOW_RAYS);
+    // k_shadow carries only the walk's part of the scene, so shadow rays of a
+    // scene with alpha cut-outs are traced here, through Model::rayHit_test.
+    const bool q = ok && S.depth >= queue_depth && !S.cutout;
+    const uint32_t slot = append_block(queued, q, s_app);
+    if (q) {
+        SQ.o[slot] = make_float4(o.x, o.y, o.z, sr.aim);
+        SQ.d[slot] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, ub((uint32_t)jid));
+        SQ.ref[g] = slot;
+    } else if (ok) {
+        RayPre r = ray_pre(o, sr.dir);
+        SQ.ref[g] = model_clear<true, TPS>(S, o, sr.dir, r, sr.aim) ? SH_CLEAR : SH_BLOCKED;
+    } else if (in) {
+        SQ.ref[g] = SH_NONE;
+    }
+    if (nj == 1) {
+        // one job: its shadow-ray count once per block (the block's appended
+        // total when every ray is queued, else a second block sum)
+        __syncthreads();
+        const uint64_t m = __ballot(ok);
+        if ((threadIdx.x & 63u) == 0) s_app[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < RB / 64; ++w) t += s_app[w];
+            if (t) atomicAdd(J[0].F.stats + RM_STAT_SHADOW_RAYS, (unsigned long long)t);
+        }
+    } else {
+        job_stat_add<1>(J, jid, ok ? 1u : 0u, RM_STAT_SHADOW_RAYS);
+    }
 }
 
 __global__ __launch_bounds__(RB) void k_direct_add(const Job *__restrict__ J, int nj, int64_t npx, PixBufs P, int C,
@@ -642,6 +722,7 @@ __global__ __launch_bounds__(RB) void k_direct_final(const Job *__restrict__ J, 
 // The depth-0 vertex (sampleIndirectLightFromFirstIntersection, :314-397).
 __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj, PixBufs P, int round,
                                               uint32_t total_sets, PathQ Q, uint32_t *count) {
+    __shared__ uint32_t s_app[RB / 64 + 2], s_ovf[RB / 64], s_max[RB / 64];
     uint32_t g = blockIdx.x * RB + threadIdx.x;
     bool want = false;
     Path path;
@@ -719,11 +800,39 @@ __global__ __launch_bounds__(RB) void k_spawn(const Job *__restrict__ J, int nj,
                 path.diff = next;
             }
     });
-    uint32_t q = append(count, want);
+    // one append and, for a launch of one job, one atomic per counter per
+    // block: counters that every wave of the launch updates serialise
+    const uint32_t q = append_block(count, want, s_app);
     if (want) store_path(Q, q, path, med);
-    job_stat_add<1>(J, myjob, want ? 1u : 0u, RM_STAT_PATHS);
-    job_stat_add<1>(J, myjob, ovf ? 1u : 0u, RM_STAT_MEDIUM_OVERFLOW);
-    job_stat_max<5>(J, myjob, want ? (uint32_t)med.n : 0u, RM_STAT_MEDIUM_MAX);
+    if (nj == 1) {
+        const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        const uint32_t ov = (uint32_t)__popcll(__ballot(ovf));
+        uint32_t mx = 0;  // med.n <= MED_CAP = 16: five bits
+        for (int b = 4; b >= 0; --b) {
+            const uint32_t c = mx | (1u << b);
+            if (__ballot(want && (uint32_t)med.n >= c)) mx = c;
+        }
+        if (lane == 0) {
+            s_ovf[wave] = ov;
+            s_max[wave] = mx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t o = 0, m = 0;
+            for (int w = 0; w < RB / 64; ++w) {
+                o += s_ovf[w];
+                m = s_max[w] > m ? s_max[w] : m;
+            }
+            unsigned long long *stats = J[0].F.stats;
+            if (s_app[RB / 64 + 1]) atomicAdd(stats + RM_STAT_PATHS, (unsigned long long)s_app[RB / 64 + 1]);
+            if (o) atomicAdd(stats + RM_STAT_MEDIUM_OVERFLOW, (unsigned long long)o);
+            if (m) atomicMax(stats + RM_STAT_MEDIUM_MAX, (unsigned long long)m);
+        }
+    } else {
+        job_stat_add<1>(J, myjob, want ? 1u : 0u, RM_STAT_PATHS);
+        job_stat_add<1>(J, myjob, ovf ? 1u : 0u, RM_STAT_MEDIUM_OVERFLOW);
+        job_stat_max<5>(J, myjob, want ? (uint32_t)med.n : 0u, RM_STAT_MEDIUM_MAX);
+    }
 }
 
 // Light sample of a finished path -> its accumulator set.
@@ -964,10 +1073,9 @@ __global__ __launch_bounds__(RB, (MAPS && MINW < 2) ? 2 : MINW) void k_bounce(co
   __shared__ uint32_t stat_rows[5 * RB];
   LaneStats<4, 8u> st;
   st.init(stat_rows);
+  Claims cl(n_in);
   while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(work, 64u);
-    base = __shfl(base, 0, 64);
+    const uint32_t base = cl.take(work);
     if (base >= n_in) break;
     uint32_t q = base + lane;
     bool live = q < n_in;
@@ -1197,19 +1305,23 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
   const uint32_t lane = threadIdx.x & 63u;
   const int keys[1] = {RM_STAT_SHADOW_RAYS};
   __shared__ uint32_t stat_rows[2 * RB];
+  // the queued rays of this lane's record, (direction, aim) per sample: the
+  // wave appends them all with one atomic after the record's samples are drawn
+  // (appends to one counter from every wave serialise on its address)
+  __shared__ float4 rbuf[NEE_MAX * RB];
   LaneStats<1, 0u> st;
   st.init(stat_rows);
+  Claims cl(n);
   while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(work, 64u);
-    base = __shfl(base, 0, 64);
+    const uint32_t base = cl.take(work);
     if (base >= n) break;
     const uint32_t q = base + lane;
     const bool live = q < n;
     float4 f5 = make_float4(0, 0, 0, 0);
     if (live) f5 = NQ.f[5][q];
     const uint32_t flags = bu(f5.w);
-    uint32_t nrays = 0;
+    uint32_t nrays = 0, qmask = 0;
+    V3 org = v3(0.0f);
     for_each_job(live ? (int)(flags >> 8) : -1, live, [&](int jid) {
         const Job &jb = J[jid];
         const SceneView &S = jb.S;
@@ -1231,6 +1343,10 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
         const uint32_t pix = bu(f0.w), smp = bu(f2.w);
         const int cnt = (int)((flags >> 2) & 63u);
         const float factor = NQ.f[7][q].w;
+        // k_shadow carries only the walk's part of the scene, so shadow rays of
+        // a scene with alpha cut-outs are traced here, through Model::rayHit_test
+        const bool to_queue = S.depth >= queue_depth && !S.cutout;
+        org = s.pos;
         Stream rng{A.seed, TAG_INDIRECT, (uint32_t)(pix - jb.pix_off), smp, bu(f3.w)};
         const float total = direct_total(S, s, V);
         for (int Ti = 0; Ti < NEE_MAX; ++Ti) {
@@ -1241,13 +1357,44 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_gen(const Job *__restrict__ J,
                 x.bsdf *= factor;  // scaling(samples, factor)
                 const V3 L = x.light * x.bsdf;
                 SQ.x[e] = make_float4(L.x, L.y, L.z, x.weight);
-                shadow_entry<TPS>(SQ, e, S, jid, s.pos, sr, queue_depth, queued);
+                if (to_queue) {
+                    rbuf[Ti * RB + threadIdx.x] = make_float4(sr.dir.x, sr.dir.y, sr.dir.z, sr.aim);
+                    qmask |= 1u << Ti;
+                } else {
+                    RayPre r = ray_pre(s.pos, sr.dir);
+                    SQ.ref[e] = model_clear<true, TPS>(S, s.pos, sr.dir, r, sr.aim) ? SH_CLEAR : SH_BLOCKED;
+                }
                 ++nrays;
             } else {
                 SQ.ref[e] = SH_NONE;
             }
         }
     });
+    // one append for the wave: lane l's rays take slots base + (rays of the
+    // lanes below l), in sample order
+    const uint32_t c = (uint32_t)__popc(qmask);
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t rank = 0, wtotal = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const uint64_t m = __ballot((c >> b) & 1u);
+        rank += (uint32_t)__popcll(m & below) << b;
+        wtotal += (uint32_t)__popcll(m) << b;
+    }
+    if (wtotal) {
+        uint32_t wb = 0;
+        if (lane == 0) wb = atomicAdd(queued, wtotal);
+        uint32_t slot = __shfl(wb, 0, 64) + rank;
+        const uint32_t job = flags >> 8;
+        for (int Ti = 0; Ti < NEE_MAX; ++Ti) {
+            if (!((qmask >> Ti) & 1u)) continue;
+            const float4 r = rbuf[Ti * RB + threadIdx.x];
+            SQ.o[slot] = make_float4(org.x, org.y, org.z, r.w);
+            SQ.d[slot] = make_float4(r.x, r.y, r.z, ub(job));
+            SQ.ref[(int64_t)q * NEE_MAX + Ti] = slot;
+            ++slot;
+        }
+    }
     if (nj > 1) {
         job_stat_add<3>(J, live ? (int)(flags >> 8) : -1, nrays, RM_STAT_SHADOW_RAYS);
     } else if (live) {
@@ -1322,10 +1469,9 @@ __global__ __launch_bounds__(RB, MINW) void k_nee_add(const Job *__restrict__ J,
                                                 const uint32_t *count, uint32_t *work, ShadowQ SQ) {
   const uint32_t n = *count;
   const uint32_t lane = threadIdx.x & 63u;
+  Claims cl(n);
   while (true) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(work, 64u);
-    base = __shfl(base, 0, 64);
+    const uint32_t base = cl.take(work);
     if (base >= n) break;
     const uint32_t q = base + lane;
     const bool live = q < n;
