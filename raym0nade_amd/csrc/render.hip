@@ -221,7 +221,21 @@ struct Job {
     RArgs A;
     FrameView F;
     int64_t pix_off, np;
+    // launches of several jobs: the job's counters in NSHARD shards, xstride
+    // counters apart, added into F.stats when the launch ends (every wave of
+    // a batch updating one job's counters would serialise on their addresses)
+    unsigned long long *xs = nullptr;
+    int xstride = 0;
 };
+constexpr int NSHARD = 16;
+
+// Where a wave adds to counter k of job jb: the frame's own counter, or the
+// wave's shard of it.
+__device__ __forceinline__ unsigned long long *stat_at(const Job &jb, int k) {
+    if (!jb.xs) return jb.F.stats + k;
+    const uint32_t w = (blockIdx.x * (RB / 64u) + (threadIdx.x >> 6)) & (uint32_t)(NSHARD - 1);
+    return jb.xs + (size_t)w * (size_t)jb.xstride + k;
+}
 
 // Runs body(j) for the lanes whose job is j, once per distinct job among the
 // active lanes of the wave, with j wave-uniform (readfirstlane), so the job's
@@ -261,7 +275,7 @@ __device__ __forceinline__ void job_stat_add(const Job *J, int myjob, uint32_t v
         unsigned long long sum = 0;
         for (int b = 0; b < BITS; ++b)
             sum += (unsigned long long)__popcll(__ballot(mine && ((v >> b) & 1u))) << b;
-        if ((int)(threadIdx.x & 63u) == lead) atomicAdd(J[j].F.stats + k, sum);
+        if ((int)(threadIdx.x & 63u) == lead) atomicAdd(stat_at(J[j], k), sum);
         pend &= ~__ballot(mine);
     }
 }
@@ -281,7 +295,7 @@ __device__ __forceinline__ void job_stat_max(const Job *J, int myjob, uint32_t v
             const uint32_t c = m | (1u << b);
             if (__ballot(mine && v >= c)) m = c;
         }
-        if ((int)(threadIdx.x & 63u) == lead) atomicMax(J[j].F.stats + k, (unsigned long long)m);
+        if ((int)(threadIdx.x & 63u) == lead) atomicMax(stat_at(J[j], k), (unsigned long long)m);
         pend &= ~__ballot(mine);
     }
 }
@@ -310,9 +324,9 @@ struct LaneStats {
             const uint32_t x = s[(k + 1) * RB];
             if (!x) continue;
             if ((MAXMASK >> k) & 1u)
-                atomicMax(J[job].F.stats + keys[k], (unsigned long long)x);
+                atomicMax(stat_at(J[job], keys[k]), (unsigned long long)x);
             else
-                atomicAdd(J[job].F.stats + keys[k], (unsigned long long)x);
+                atomicAdd(stat_at(J[job], keys[k]), (unsigned long long)x);
         }
     }
     __device__ __forceinline__ void note(const Job *J, const int (&keys)[N], int job, const uint32_t (&x)[N]) {
@@ -355,9 +369,9 @@ struct LaneStats {
                 }
                 if (lane == (uint32_t)lead && x) {
                     if ((MAXMASK >> k) & 1u)
-                        atomicMax(J[lj].F.stats + keys[k], (unsigned long long)x);
+                        atomicMax(stat_at(J[lj], keys[k]), (unsigned long long)x);
                     else
-                        atomicAdd(J[lj].F.stats + keys[k], (unsigned long long)x);
+                        atomicAdd(stat_at(J[lj], keys[k]), (unsigned long long)x);
                 }
             }
             if (mine) has = false;
@@ -1516,6 +1530,24 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Adds job blockIdx.x's counter shards into its frame's counters (the
+// maximum for RM_STAT_MEDIUM_MAX).
+__global__ void k_stats_reduce(const Job *__restrict__ J) {
+    const Job &jb = J[blockIdx.x];
+    const int k = (int)threadIdx.x;
+    if (!jb.xs || k >= RM_STAT_COUNT) return;
+    unsigned long long v = 0;
+    for (int w = 0; w < NSHARD; ++w) {
+        const unsigned long long x = jb.xs[(size_t)w * (size_t)jb.xstride + k];
+        v = k == RM_STAT_MEDIUM_MAX ? (x > v ? x : v) : v + x;
+    }
+    if (!v) return;
+    if (k == RM_STAT_MEDIUM_MAX)
+        atomicMax(jb.F.stats + k, v);
+    else
+        atomicAdd(jb.F.stats + k, v);
+}
+
 __global__ void k_zero_u32(uint32_t *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -1724,8 +1756,10 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     }
     const int64_t nblk = (total_px + SB - 1) / SB;
     auto pad = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const bool shard = nj > 1;
+    const size_t nshard = shard ? (size_t)NSHARD * nj * RM_STAT_COUNT : 0;
     const size_t pix_bytes = pad(nj * sizeof(Job)) + 3 * pad(total_px * 16) + 2 * pad(total_px * 4) + pad(nblk * 8) +
-                             pad(4 * NCOUNT * 4);
+                             pad(4 * NCOUNT * 4) + pad(nshard * 8);
     if (work.bytes < pix_bytes) {
         int32_t rc = work.alloc(pix_bytes);
         if (rc) return rc;
@@ -1746,6 +1780,11 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     // queued shadow rays; counts[3 NCOUNT + 8, 9]: the same for each chunk of
     // the primary-vertex pass
     uint32_t *counts = carve<uint32_t>(cur, 4 * NCOUNT);
+    unsigned long long *shards = carve<unsigned long long>(cur, nshard);
+    for (int i = 0; i < nj; ++i) {
+        J[i].xs = shard ? shards + (size_t)i * RM_STAT_COUNT : nullptr;
+        J[i].xstride = nj * RM_STAT_COUNT;
+    }
     // Trace variant 3 keeps the first pending entries of every walk in LDS:
     // each kernel of the launch that traverses gets nl entries per lane (at
     // most the deepest tree's depth), and the job table's views say so.
@@ -1763,6 +1802,8 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     RM_HIP(hipMemcpyAsync(dJ, J.data(), nj * sizeof(Job), hipMemcpyHostToDevice, st));
     for (int i = 0; i < nj; ++i)
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(RM_STAT_COUNT), 0, st, J[i].F.stats, (int)RM_STAT_COUNT);
+    if (shard)
+        hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((nshard + 255) / 256)), dim3(256), 0, st, shards, (int)nshard);
     for (int i = 0; i < nj; ++i)
         if (J[i].F.rec) {
             const int64_t n4 = J[i].np * J[i].F.rec_stride * 3;
@@ -1947,6 +1988,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
         }
     }
     hipLaunchKernelGGL(k_finish, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px, P, AB);
+    if (shard) hipLaunchKernelGGL(k_stats_reduce, dim3(nj), dim3(RM_STAT_COUNT), 0, st, (const Job *)dJ);
     for (int i = 0; i < nj; ++i)
         if (hj[i].post) {
             int32_t rc = post_launch(hj[i].frame, hj[i].post, hj[i].args->exposure, st);
