@@ -569,14 +569,15 @@ __device__ __forceinline__ uint32_t append_block(uint32_t *counter, bool want, u
 }
 
 // Work claims of the persistent item kernels: a wave takes 64 queue items
-// per iteration, claimed `per` batches at a time with one atomic (per up to
-// 4 when the queue holds that many batches for every wave of the launch, so
-// short queues still spread over all waves).  Wave-uniform.
+// per iteration, claimed `per` batches at a time with one atomic.  per is up
+// to 4 when the queue holds 16 such claims for every wave of the launch, so
+// that the last claims, which end the launch, stay a small part of it;
+// shorter queues claim one batch at a time.  Wave-uniform.
 struct Claims {
     uint32_t next = 0, end = 0, per;
     __device__ __forceinline__ explicit Claims(uint32_t n) {
         const uint32_t waves = gridDim.x * (RB / 64u);
-        per = (uint32_t)max(1, min(4, (int)(n / (64u * waves))));
+        per = (uint32_t)max(1, min(4, (int)(n / (64u * 16u * waves))));
     }
     __device__ __forceinline__ uint32_t take(uint32_t *work) {
         if (next >= end) {
