@@ -75,6 +75,8 @@ def main():
                          "'blocks_per_cu,min_waves,paths_per_round,walk_variant,lds_stack_entries,tris_per_step[,walk_phases]'")
     ap.add_argument("--trace", default="", help="fix the traversal launch: 'variant,blocks_per_cu,fetch_min,ray_sort[,lds_stack_entries[,tris_per_step[,walk_phases]]]'")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
+    ap.add_argument("--batch-tuning", default="",
+                    help="fix the batch's launch instead of autotuning it (fields as --render-tuning)")
     ap.add_argument("--batch-jobs", type=int, default=64, help="jobs in the batched-serving leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rendering leg")
     ap.add_argument("--adam-steps", type=int, default=200, help="inverse-rendering Adam steps (0 skips the loop)")
@@ -331,7 +333,18 @@ def batch_leg(args, torch, rm):
     tris = [S.info.n_tris for _, S, _ in built]
     log(f"[bench] batch: {total / 1e6:.0f}M samples, {min(tris)}..{max(tris)} triangles per scene")
     tuned = None
-    if not args.no_autotune:
+    if args.batch_tuning:
+        fields = [int(x) for x in args.batch_tuning.split(",")]
+        b, w, k, v, e, p, ph = (fields + [0])[:7]
+        for key, val in ((rm.TUNE_RENDER_BLOCKS_PER_CU, b), (rm.TUNE_RENDER_MIN_WAVES, w), (rm.TUNE_TRACE_VARIANT, v),
+                         (rm.TUNE_LDS_STACK_ENTRIES, e), (rm.TUNE_TRIS_PER_STEP, p), (rm.TUNE_WALK_PHASES, ph)):
+            rm.set_tuning(key, val)
+        for _, a, _, _ in jobs:
+            a.paths_per_round = k
+        tuned = {"blocks_per_cu": b, "min_waves": w, "paths_per_round": k, "walk_variant": v,
+                 "lds_stack_entries": e, "tris_per_step": p, "walk_phases": ph, "fixed": True}
+        log(f"[bench] batch launch fixed: {tuned}")
+    elif not args.no_autotune:
         tuned = autotune.tune_render(jobs, log=log)
         for _, a, _, _ in jobs:
             a.paths_per_round = tuned["paths_per_round"]

@@ -1830,11 +1830,14 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     }
     if (total > 0x7fffffffLL) return fail(RM_ERR_UNSUPPORTED, "%lld path slots per round", (long long)total);
     // The split shadow passes' queue: NEE_MAX entries per path slot for the
-    // NEE pass, C per pixel for each chunk of the primary-vertex pass (C such
-    // that a chunk holds about 16M entries).  k_shadow counts in 32 bits.
+    // NEE pass, C per pixel for each chunk of the primary-vertex pass (C up to
+    // 32, such that a chunk fits the NEE pass's entries, or 16M entries when
+    // those are fewer: fewer, larger chunks spare launches and walk tails).
+    // k_shadow counts in 32 bits.
     int max_direct = 0;
     for (int i = 0; i < nj; ++i) max_direct = std::max(max_direct, (int)(float(J[i].A.spp) * J[i].A.p_direct));
-    const int dC = (int)std::max<int64_t>(1, std::min<int64_t>(32, ((int64_t)16 << 20) / std::max<int64_t>(1, total_px)));
+    const int64_t dcap = std::max<int64_t>((int64_t)16 << 20, total * NEE_MAX);
+    const int dC = (int)std::max<int64_t>(1, std::min<int64_t>(32, dcap / std::max<int64_t>(1, total_px)));
     // split: 1 queues every shadow ray; 2 queues those of jobs whose tree has
     // SPLIT_DEPTH levels or more and traces the others where they are drawn
     // (on shallow trees a walk costs less than queueing it), and keeps the
