@@ -589,7 +589,11 @@ __device__ __forceinline__ void adjoint_pixel(const AdjArgs &G, int64_t p, int c
     }
 }
 
-__global__ __launch_bounds__(ADJ_B) void k_adjoint(AdjArgs G) {
+// At most 128 VGPRs (four waves per SIMD): uncapped, the kernel takes 256
+// VGPRs and 2 AGPRs, one wave per SIMD, and its loads and atomics wait
+// exposed.  The spill the cap costs is cheaper than that latency (512x512,
+// 128 spp: 50 ms with one wave, 28 with two, 24 with four).
+__global__ __launch_bounds__(ADJ_B, 4) void k_adjoint(AdjArgs G) {
     extern __shared__ float sh[];
     const int nm4 = G.S.n_mats * 4;
     for (int i = threadIdx.x; i < nm4; i += ADJ_B) sh[i] = 0.0f;
