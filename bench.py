@@ -418,8 +418,12 @@ def adjoint_leg(args, torch, rm):
     """Differentiable rendering at 512^2, 128 spp on the textured Cornell box
     (five 256^2 fp32 albedo textures, per-wall GGX roughness): time of the
     forward render (rm_render + linear shade) and of the path-replay adjoint
-    (rm_adjoint, gradients for every texel, roughness and emission)."""
+    (rm_adjoint, gradients for every texel, roughness and emission).  The
+    renders run at the library's default launch parameters, not at those an
+    earlier leg tuned for its own workload."""
+    from raym0nade_amd import autotune as _at
     from raym0nade_amd.diff import DiffRenderer
+    _at.restore(LIBRARY_DEFAULTS)
     g = rm.generate(4, 1)
     R = DiffRenderer(g)
     tex, rough, emis = R.initial_params()
@@ -683,7 +687,7 @@ def render_leg(args, torch, rm, dev, stream):
                  "lds_stack_entries": e, "tris_per_step": p, "walk_phases": ph, "fixed": True}
     elif not args.no_autotune:
         from raym0nade_amd import autotune
-        tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 4, blocks=(4, 8), waves=(1, 4),
+        tuned = autotune.tune_render([(S, a, fr, 63 | 512)], spp_scale=1 / 4, blocks=(4, 8), waves=(1, 3),
                                      paths_per_round=(0, 4, 8), full_paths=(8, 16), log=log)
         a.paths_per_round = tuned["paths_per_round"]
     times = []

@@ -489,9 +489,9 @@ enum rm_tune_key {
                                            the pixel's samples (default 4) */
     RM_TUNE_KERNEL_TIMING = 6,          /* 1: trace calls record HIP events around their reordering
                                            and traversal kernels (default 0) */
-    RM_TUNE_RENDER_MIN_WAVES = 7,       /* 1 (default) or 4: resident waves per SIMD the render's
-                                           shading and NEE kernels are compiled for (4 caps them
-                                           at 128 VGPRs, spilling the rest) */
+    RM_TUNE_RENDER_MIN_WAVES = 7,       /* 1 (default), 3 or 4: resident waves per SIMD the render's
+                                           shading and NEE kernels are compiled for (3 caps them
+                                           at 168 VGPRs, 4 at 128, spilling the rest) */
     RM_TUNE_LDS_STACK_ENTRIES = 8,      /* trace variant 3: walk-stack entries per lane kept in LDS
                                            (1..32, default 10: 160 KiB of LDS at 8 blocks/CU; deeper
                                            entries go to scratch; 32 holds the whole stack of any tree) */

@@ -71,7 +71,7 @@ def tune_trace(scene, batches, reps=3, candidates=None, log=None) -> dict:
             "tried": len(results)}
 
 
-def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 4), paths_per_round=(0, 2, 4, 8),
+def tune_render(jobs, spp_scale=0.125, blocks=(4, 8, 16), waves=(1, 3), paths_per_round=(0, 2, 4, 8),
                 walks=((3, 10, 2, 0), (3, 10, 0, 0), (3, 10, 1, 0), (3, 10, 4, 0), (3, 8, 2, 0), (3, 6, 2, 0),
                        (2, 32, 2, 0), (3, 10, 0, 32), (3, 10, 0, 40), (3, 10, 1, 32), (3, 10, 2, 32)),
                 full_paths=(4, 8), log=None) -> dict:

@@ -81,7 +81,8 @@ extern "C" int32_t rm_set_tuning(int32_t key, int32_t value) {
             t.kernel_timing = value;
             return RM_OK;
         case RM_TUNE_RENDER_MIN_WAVES:
-            if (value != 1 && value != 4) return rm::fail(RM_ERR_INVALID, "rm_set_tuning: render min waves %d (1 or 4)", value);
+            if (value != 1 && value != 3 && value != 4)
+                return rm::fail(RM_ERR_INVALID, "rm_set_tuning: render min waves %d (1, 3 or 4)", value);
             t.render_min_waves = value;
             return RM_OK;
         case RM_TUNE_TRIS_PER_STEP:

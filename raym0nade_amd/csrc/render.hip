@@ -1075,7 +1075,8 @@ __global__ __launch_bounds__(RB) void k_extend(const Job *__restrict__ J, PathQ 
 }
 
 // MINW: minimum resident waves per SIMD the register allocation must allow
-// (1: no cap; 4: at most 128 VGPRs, the rest spilled).  Chosen by tuning.
+// (1: no cap; 3: at most 168 VGPRs; 4: at most 128, the rest spilled).
+// Chosen by tuning.
 // The 8-bit map code would take the uncapped variant one register past 256
 // (one wave per SIMD); it is held to two waves.
 template <int MINW, bool MAPS>
@@ -1903,7 +1904,7 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     if (graph) RM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    const bool capped = tuning().render_min_waves == 4;
+    const int mw = tuning().render_min_waves;  // 1, 3 or 4
     const unsigned gpers = (unsigned)(device_cu_count() * tuning().persistent_blocks_per_cu);
     const int fetch_min = tuning().fetch_min;
     const int phases = tuning().walk_phases;
@@ -1917,8 +1918,9 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
                                  : tps == 4 ? k_shadow<false, 4> : k_shadow<false, 0>));
     if (split) {
         uint32_t *claims = counts + 3 * NCOUNT + 8;  // and [9]: queued rays; both zeroed for each chunk
-        auto direct_gen = capped ? (tps ? k_direct_gen<4, 2> : k_direct_gen<4, 0>)
-                                 : (tps ? k_direct_gen<1, 2> : k_direct_gen<1, 0>);
+        auto direct_gen = mw == 4 ? (tps ? k_direct_gen<4, 2> : k_direct_gen<4, 0>)
+                          : mw == 3 ? (tps ? k_direct_gen<3, 2> : k_direct_gen<3, 0>)
+                                    : (tps ? k_direct_gen<1, 2> : k_direct_gen<1, 0>);
         // samples [t0, t0 + C) of the pixels [0, npx) whose jobs have more than
         // t0 samples (jobs are in order of falling sample counts)
         for (int t0 = 0; t0 < max_direct;) {
@@ -1939,7 +1941,8 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
         }
         hipLaunchKernelGGL(k_direct_final, dim3(gp), dim3(RB), 0, st, (const Job *)dJ, nj, total_px);
     } else {
-        auto direct = capped ? (tps ? k_direct<4, 2> : k_direct<4, 0>) : (tps ? k_direct<1, 2> : k_direct<1, 0>);
+        auto direct = mw == 4 ? (tps ? k_direct<4, 2> : k_direct<4, 0>)
+                      : mw == 3 ? (tps ? k_direct<3, 2> : k_direct<3, 0>) : (tps ? k_direct<1, 2> : k_direct<1, 0>);
         hipLaunchKernelGGL(direct, dim3(gp), dim3(RB), walk_lds, st, (const Job *)dJ, nj, total_px, P);
     }
     const uint32_t nsets = (uint32_t)total;
@@ -1948,11 +1951,13 @@ static int32_t render_jobs(const std::vector<HostJob> &hj_in, DevBuf &work, hipS
         const unsigned gb = (unsigned)(device_cu_count() * tuning().render_blocks_per_cu);
         hipLaunchKernelGGL(k_clear_sets, dim3(gs), dim3(RB), 0, st, AB, nsets);
         const unsigned ge = gpers;
-        auto bounce = maps ? (capped ? k_bounce<4, true> : k_bounce<1, true>)
-                           : (capped ? k_bounce<4, false> : k_bounce<1, false>);
-        auto nee = capped ? (tps ? k_nee<4, 2> : k_nee<4, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
-        auto nee_gen = capped ? (tps ? k_nee_gen<4, 2> : k_nee_gen<4, 0>) : (tps ? k_nee_gen<1, 2> : k_nee_gen<1, 0>);
-        auto nee_add = capped ? k_nee_add<4> : k_nee_add<1>;
+        auto bounce = maps ? (mw == 4 ? k_bounce<4, true> : mw == 3 ? k_bounce<3, true> : k_bounce<1, true>)
+                           : (mw == 4 ? k_bounce<4, false> : mw == 3 ? k_bounce<3, false> : k_bounce<1, false>);
+        auto nee = mw == 4 ? (tps ? k_nee<4, 2> : k_nee<4, 0>)
+                   : mw == 3 ? (tps ? k_nee<3, 2> : k_nee<3, 0>) : (tps ? k_nee<1, 2> : k_nee<1, 0>);
+        auto nee_gen = mw == 4 ? (tps ? k_nee_gen<4, 2> : k_nee_gen<4, 0>)
+                       : mw == 3 ? (tps ? k_nee_gen<3, 2> : k_nee_gen<3, 0>) : (tps ? k_nee_gen<1, 2> : k_nee_gen<1, 0>);
+        auto nee_add = mw == 4 ? k_nee_add<4> : mw == 3 ? k_nee_add<3> : k_nee_add<1>;
         auto extend_lds = counting ? (tps == 1 ? k_extend<true, 1, true> : tps == 2 ? k_extend<true, 2, true>
                                       : tps == 4 ? k_extend<true, 4, true> : k_extend<true, 0, true>)
                                    : (tps == 1 ? k_extend<true, 1> : tps == 2 ? k_extend<true, 2>

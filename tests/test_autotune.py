@@ -13,7 +13,7 @@ def test_tuning_keys_round_trip_and_reject_out_of_range():
     try:
         for key, good, bad in [(rm.TUNE_TRACE_VARIANT, (0, 2, 3), (-1, 1, 4, 5)),
                                (rm.TUNE_LDS_STACK_ENTRIES, (1, 32), (0, 33)),
-                               (rm.TUNE_RENDER_MIN_WAVES, (1, 4), (2, 8)),
+                               (rm.TUNE_RENDER_MIN_WAVES, (1, 3, 4), (2, 8)),
                                (rm.TUNE_PERSISTENT_BLOCKS_PER_CU, (1, 64), (0, 65)),
                                (rm.TUNE_TRIS_PER_STEP, (0, 1, 2, 4), (3, 8)),
                                (rm.TUNE_NEE_SPLIT, (0, 1, 2), (-1, 3)),

@@ -181,3 +181,25 @@ def test_lds_stack_walks_are_bit_identical(dev, scene, entries, tps, split):
             rm.set_tuning(k, v)
     assert np.array_equal(gb2.view(np.uint32), gb3.view(np.uint32))
     assert np.array_equal(r2.view(np.uint32), r3.view(np.uint32))
+
+
+@pytest.mark.parametrize("scene", ["cornell", "spheres"])
+@pytest.mark.parametrize("waves", [3, 4])
+def test_register_capped_kernels_are_bit_identical(dev, scene, waves):
+    # the shading and light-sample kernels compiled for 3 or 4 resident waves
+    # per SIMD (168 or 128 VGPRs, the rest spilled) compute what the uncapped
+    # ones do; Cornell takes the single-kernel NEE passes, the spheres the
+    # split generate / shadow / add passes
+    g = rm.generate(0, 1) if scene == "cornell" else rm.generate(2, 3, grid=2, freq=12, sky_width=256, sky_height=128)
+    S = rm.Scene(g)
+    a = small_args(g, 80, 48, 12, max_depth=16)
+    old = rm.get_tuning(rm.TUNE_RENDER_MIN_WAVES)
+    try:
+        rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, 1)
+        gb1, r1 = RC.gpu_render(S, a)
+        rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, waves)
+        gbw, rw = RC.gpu_render(S, a)
+    finally:
+        rm.set_tuning(rm.TUNE_RENDER_MIN_WAVES, old)
+    assert np.array_equal(gb1.view(np.uint32), gbw.view(np.uint32))
+    assert np.array_equal(r1.view(np.uint32), rw.view(np.uint32))
