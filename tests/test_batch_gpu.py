@@ -58,7 +58,7 @@ def test_serving_batch_with_autotune_equals_single_renders(dev):
     saved = autotune.current()
     try:
         choice = autotune.tune_render(jobs, spp_scale=0.25)
-        assert choice["blocks_per_cu"] in (4, 8, 16) and choice["min_waves"] in (1, 4)
+        assert choice["blocks_per_cu"] in (4, 8, 16) and choice["min_waves"] in (1, 3, 4)
         assert choice["walk_variant"] in (2, 3)
         rm.render_batch(jobs, use_graph=True)
         torch.cuda.synchronize()
