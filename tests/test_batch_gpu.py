@@ -161,3 +161,25 @@ def test_batch_post_with_clamp_filter_and_bloom(dev, use_graph):
         assert np.allclose(got_rad, want_rad, rtol=1e-4, atol=1e-6)
         px, want = fr.download(rm.RM_BUF_PIXELS), ref.download(rm.RM_BUF_PIXELS)
         assert np.isclose(px, want, rtol=1e-4, atol=1e-5).all(axis=-1).mean() >= 0.999
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_counters_equal_single_renders(dev, use_graph):
+    """Each job's render counters from a batched launch (kept per job in
+    shards picked by wave and added into the frame when the launch ends)
+    equal those of its single render: the same primary, extension and
+    shadow rays, hits, paths and medium-stack figures."""
+    gens = [rm.generate(0, 1), rm.generate(2, 7, grid=2, freq=40, sky_width=128, sky_height=64)]
+    scenes = [rm.Scene(g) for g in gens]
+    specs = [(0, 64, 48, 8, 8, 3), (1, 72, 40, 32, 16, 4), (0, 48, 48, 64, 12, 5), (1, 40, 56, 16, 16, 6)]
+    jobs = [(scenes[si], job_args(gens[si], w, h, spp, depth, seed), rm.Frame(w, h), 63 | 512)
+            for si, w, h, spp, depth, seed in specs]
+    rm.render_batch(jobs, use_graph=use_graph)
+    torch.cuda.synchronize()
+    for S, a, fr, _ in jobs:
+        one = rm.Frame(a.width, a.height)
+        one.render(S, a)
+        torch.cuda.synchronize()
+        got, want = fr.stats(), one.stats()
+        assert got["paths"] > 0 and got["shadow_rays"] > 0
+        assert got == want
