@@ -672,8 +672,8 @@ __global__ __launch_bounds__(RB, MINW) void k_direct_gen(const Job *__restrict__
         SQ.ref[g] = SH_NONE;
     }
     if (nj == 1) {
-        // one job: its shadow-ray count once per block (the block's appended
-        // total when every ray is queued, else a second block sum)
+        // one job: its shadow-ray count summed over the block (queued and
+        // traced here alike), one atomic per block
         __syncthreads();
         const uint64_t m = __ballot(ok);
         if ((threadIdx.x & 63u) == 0) s_app[threadIdx.x >> 6] = (uint32_t)__popcll(m);
