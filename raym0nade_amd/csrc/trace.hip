@@ -240,19 +240,34 @@ __device__ __forceinline__ void oct_map(V3 d, float &u, float &v) {
     }
 }
 
-// Direction bin of a ray whose batch shares its origins: the Morton code of
+// Direction bin of a ray whose batch shares its origins: the Hilbert index of
 // its octahedral map over the batch's range `box` (u0, v0, 512 / width,
-// 512 / height), 9 bits per axis, so that consecutive bins are neighbouring
-// directions (a camera batch falls into small square tiles).
+// 512 / height) on a 512 x 512 grid, so that consecutive bins are
+// neighbouring directions (a camera batch falls into small square tiles).
 __device__ __forceinline__ uint32_t dir_bin(V3 d, const float *box) {
     float u, v;
     oct_map(d, u, v);
     int qu = (int)((u - box[0]) * box[2]), qv = (int)((v - box[1]) * box[3]);
     qu = qu < 0 ? 0 : (qu > 511 ? 511 : qu);
     qv = qv < 0 ? 0 : (qv > 511 ? 511 : qv);
-    uint32_t m = 0;
-    for (int b = 0; b < 9; ++b) m |= (((uint32_t)qu >> b) & 1u) << (2 * b) | (((uint32_t)qv >> b) & 1u) << (2 * b + 1);
-    return m;
+    // Hilbert index on the 512 x 512 grid: consecutive bins always share an
+    // edge (a Morton order jumps at every power-of-two boundary), so the
+    // 64 rays of a wave, two or three bins, form a compact tile
+    uint32_t x = (uint32_t)qu, y = (uint32_t)qv, h = 0;
+    for (uint32_t sq = 256; sq > 0; sq >>= 1) {
+        const uint32_t rx = (x & sq) ? 1u : 0u, ry = (y & sq) ? 1u : 0u;
+        h += sq * sq * ((3u * rx) ^ ry);
+        if (!ry) {
+            if (rx) {
+                x = 511u - x;
+                y = 511u - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return h;
 }
 
 // flag = 1 when the mean distance between consecutive ray origins (4096
