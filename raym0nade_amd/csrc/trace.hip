@@ -225,6 +225,9 @@ __device__ __forceinline__ uint32_t ray_bin(const SceneView &S, V3 o, V3 d) {
     }
     uint32_t m = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
     uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    // the octants of a cell in Gray-code order (0, 1, 3, 2, 6, 7, 5, 4):
+    // consecutive ones differ in one direction sign
+    oct = oct ^ (oct >> 1) ^ (oct >> 2);
     return (m << 3) | oct;
 }
 
