@@ -46,10 +46,15 @@ def frame_at(fr, pix):
     return gb.reshape(h * w, -1)[pix], rad.reshape(4, h * w, 4)[:, pix]
 
 
-def test_1080p_workload_grid_matches_oracle(dev):
+@pytest.fixture(scope="module")
+def scene_1080p(dev):
     say("[c2] generating the 10.4M-triangle scene")
     g = rm.generate(2, 1)
-    S = rm.Scene(g)
+    return g, rm.Scene(g)
+
+
+def test_1080p_workload_grid_matches_oracle(scene_1080p):
+    g, S = scene_1080p
     assert S.info.n_tris == 10368002 and g.args.width == 1920 and g.args.height == 1080
     a = rm.RmRenderArgs.from_buffer_copy(g.args)
     a.spp = 16
@@ -139,3 +144,27 @@ def test_batched_serving_set_at_full_size(dev):
         m = RC.image_metrics(gb[rows], rad[:, rows], ogb[rows], orad[:, rows], a.spp, a.p_direct, a.exposure)
         say(f"[c3] job {k} metrics", m)
         assert m["max_abs_z"] < 4.0
+
+
+def test_deepest_lds_walk_stacks_on_the_1080p_tree(scene_1080p):
+    """The 21-level tree with every pending walk entry in LDS (32 per lane:
+    21 x 256 lanes x 8 B = 42 KiB of dynamic LDS per block, beside the
+    light-sample kernel's 26 KiB of staged rays; gfx950 allows 160 KiB per
+    block) renders what the default 10 entries render, bit for bit."""
+    g, S = scene_1080p
+    assert S.info.depth >= 20
+    a = rm.RmRenderArgs.from_buffer_copy(g.args)
+    a.accuracy = np.float32(a.accuracy * a.width / 64)
+    a.width, a.height, a.spp = 64, 36, 4
+    old = rm.get_tuning(rm.TUNE_LDS_STACK_ENTRIES)
+    try:
+        out = []
+        for e in (10, 32):
+            rm.set_tuning(rm.TUNE_LDS_STACK_ENTRIES, e)
+            out.append(RC.gpu_render(S, a))
+    finally:
+        rm.set_tuning(rm.TUNE_LDS_STACK_ENTRIES, old)
+    (gb0, r0), (gb1, r1) = out
+    assert np.array_equal(gb0.view(np.uint32), gb1.view(np.uint32))
+    assert np.array_equal(r0.view(np.uint32), r1.view(np.uint32))
+    assert np.isfinite(r0).all() and r0.max() > 0
